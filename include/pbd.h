@@ -1,0 +1,159 @@
+/*
+ * pbd.h -- C ABI of the MI355X-native position-based-dynamics cloth solver.
+ *
+ * One shared library (libpbd.so) exports these functions; plain pointers and
+ * sizes only, no C++ or torch types.  Host arrays are caller-owned and copied;
+ * device memory is owned by the context.  Every function returns PBD_OK (0)
+ * or a negative PBD_ERR_* code, and pbd_last_error() describes the last
+ * failure of the calling thread.  A context uses one HIP stream and must be
+ * driven by one host thread at a time.
+ *
+ * The reference (vivienldy/PositionBasedDynamicsOld) exposes this path as
+ * C++ classes only; each entry point below names the member it replaces.
+ * Paths are relative to PositionBasedDynamicsCUDA/PositionBasedDynamicsCUDA/.
+ */
+#ifndef PBD_H
+#define PBD_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define PBD_API __attribute__((visibility("default")))
+#else
+#define PBD_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pbd_ctx pbd_ctx;
+
+enum {
+    PBD_OK = 0,
+    PBD_ERR_ARG = -1,       /* bad argument (null pointer, index or size out of range) */
+    PBD_ERR_STATE = -2,     /* call out of order (e.g. step before init) */
+    PBD_ERR_HIP = -3,       /* HIP runtime error */
+    PBD_ERR_LIMIT = -4,     /* a size limit of the implementation was exceeded */
+    PBD_ERR_IO = -5,        /* file could not be read or parsed */
+    PBD_ERR_COLOR = -6      /* colouring did not converge in the allowed rounds */
+};
+
+/* ConstraintType values, PBD_Basic.cuh:38-43. */
+enum { PBD_DISTANCE = 0x001, PBD_BENDING = 0x010, PBD_ANCHOR = 0x100 };
+
+/* Which state buffer pbd_cloth_get_state / pbd_cloth_set_state address. */
+enum { PBD_STATE_X = 0, PBD_STATE_V = 1, PBD_STATE_P = 2 };
+
+/* Projection path. */
+enum {
+    PBD_PATH_AUTO = 0,      /* LDS tiles (whole-cloth or dependency-cone tiles) */
+    PBD_PATH_GLOBAL = 1     /* one launch per colour over global memory */
+};
+
+/* Procedural grid cloth (PBDObject(damping, g, resX, resY, sizeX, sizeY, ht)
+ * + PBDObject::Init(), PBD_Basic.cuh:155-158, PBD_cuda.cu:644-648,831-869). */
+typedef struct {
+    int res_x, res_y;
+    float size_x, size_y;
+    float origin[3];        /* translation added after generation (reference: 0) */
+    float jitter;           /* uniform +-jitter per coordinate from `seed` (0 = none) */
+    uint64_t seed;
+} pbd_grid_desc;
+
+/* Per-cloth physical parameters (PBDObject::dampingRate/gravity,
+ * SolverPBD sphere, PBD_Basic.cuh:190-192,242-244). */
+typedef struct {
+    float damping;          /* dampingRate; velocity scaled by damping^dt each substep */
+    float gravity[3];
+    float wind[3];          /* wind velocity (m/s) */
+    float wind_drag;        /* drag coefficient (1/s) toward the wind velocity; 0 = off */
+    int sphere_on;
+    float sphere_c[3];
+    float sphere_r;
+    int box_on;
+    float box_lo[3];
+    float box_hi[3];
+} pbd_cloth_params;
+
+/* Solver parameters (SolverPBD::ProjectConstraint(st, iterations), PBD_Basic.cuh:226). */
+typedef struct {
+    float dt;               /* substep length */
+    int iterations;         /* sweeps per substep (tol < 0) or sweep cap (tol >= 0) */
+    double tol;             /* stop when max |d-L|/L <= tol after a check; < 0 = fixed sweeps */
+    int check_every;        /* sweeps between residual checks (chunk length) */
+    int path;               /* PBD_PATH_* */
+    int tile_sweeps;        /* sweeps per launch for multi-tile cloths (0 = check_every) */
+    int tile_max_verts;     /* LDS vertices per tile (0 = from the device's LDS size) */
+    int block_threads;      /* threads per workgroup of the tile kernel (0 = 1024) */
+} pbd_solver_params;
+
+typedef struct {
+    int nv, nt;             /* vertices, triangles */
+    int ne, nb;             /* distance prims, bending prims */
+    int nprims;             /* ne + nb (coloured prim set) */
+    int ncolors;
+    int color_rounds;       /* assign/resolve rounds the colouring took */
+    int adj_len;            /* total prim->prims neighbour entries */
+    int ntiles;             /* LDS tiles planned for this cloth */
+    int whole;              /* 1 if the cloth is one LDS-resident tile */
+    int max_loaded;         /* largest loaded vertex set over its tiles */
+} pbd_cloth_info;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+PBD_API const char* pbd_last_error(void);
+PBD_API int pbd_create(int device, pbd_ctx** out);
+PBD_API int pbd_destroy(pbd_ctx* ctx);
+PBD_API int pbd_synchronize(pbd_ctx* ctx);
+
+/* ---- model build (PBDObject / ConstraintPBD) ------------------------------ */
+/* PBDObject(...,resX,resY,sizeX,sizeY) + initMeshTopol (PBD_cuda.cu:675-705,831-869) */
+PBD_API int pbd_cloth_create_grid(pbd_ctx* ctx, const pbd_grid_desc* desc, int* cloth);
+/* PBDObject(damping, g, ht) + IO::ReadTopolFromTxt (PBD_Basic.cuh:532-601), from memory */
+PBD_API int pbd_cloth_create_mesh(pbd_ctx* ctx, const float* pos, int nv, const int* tris, int nt, int* cloth);
+/* PBDObject::Init(topolFile, distConstrFile) (PBD_cuda.cu:650-658): the reference
+ * text formats, IO::ReadTopolFromTxt + IO::ReadDistConstrFromTxt (PBD_Basic.cuh:532-688) */
+PBD_API int pbd_cloth_load_txt(pbd_ctx* ctx, const char* topol_path, const char* constr_path, int* cloth);
+/* Explicit distance prims (IO::ReadDistConstrFromTxt pairs, PBD_Basic.cuh:654-675);
+ * without it a grid cloth uses InitDistanceIndices order (PBD_cuda.cu:19-54) and a
+ * mesh cloth uses its triangle edges in first-sighting order. */
+PBD_API int pbd_cloth_set_distance_pairs(pbd_ctx* ctx, int cloth, const int* pairs, int ne);
+/* PBDObject::SetConstrOption(ct, stiffness[DISTANCE, BENDING, ANCHOR]) (PBD_cuda.cu:669-673) */
+PBD_API int pbd_cloth_set_constraint_option(pbd_ctx* ctx, int cloth, unsigned ct, const float* stiffness3);
+/* Pins (replaces InitAnchorConstr, PBD_cuda.cu:125-153): weight 0 for the listed vertices */
+PBD_API int pbd_cloth_set_pins(pbd_ctx* ctx, int cloth, const int* verts, int n);
+PBD_API int pbd_cloth_set_params(pbd_ctx* ctx, int cloth, const pbd_cloth_params* params);
+/* initConstr: constraints, GenePoint2PrimsMap/GenePrim2PrimsMap, InitGPUBuffers,
+ * EdgeColoring(color_rounds) on the GPU, SortEdgesColors + EvalWorksets
+ * (PBD_cuda.cu:798-829,155-202,707-750,487-625), for every cloth created so far. */
+PBD_API int pbd_init(pbd_ctx* ctx, int color_rounds);
+PBD_API int pbd_cloth_get_info(pbd_ctx* ctx, int cloth, pbd_cloth_info* info);
+/* ConstraintPBD::topol.primList/indices + restLengthBuffer/rest angle (PBD_Basic.cuh:97-108) */
+PBD_API int pbd_cloth_get_prims(pbd_ctx* ctx, int cloth, int* prim_list, int* indices, float* rest);
+/* ConstraintPBD::Prim2PrimsMap (startNumList, indices) (PBD_cuda.cu:170-202) */
+PBD_API int pbd_cloth_get_adjacency(pbd_ctx* ctx, int cloth, int* start_num, int* adj);
+/* ConstraintPBD::color, sortedPrimId, colorWorksets (PBD_cuda.cu:546-625) */
+PBD_API int pbd_cloth_get_colors(pbd_ctx* ctx, int cloth, int* color, int* sorted_prim, int* worksets);
+
+/* ---- solver (SolverPBD) ---------------------------------------------------- */
+PBD_API int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* params);
+/* SolverPBD::Advect(dt) (PBD_cuda.cu:981-1052) */
+PBD_API int pbd_advect(pbd_ctx* ctx, float dt);
+/* SolverPBD::ProjectConstraint(GAUSSSEIDEL, iterations) (PBD_cuda.cu:1054-1070,1308-1435) */
+PBD_API int pbd_project(pbd_ctx* ctx, int iterations);
+/* SolverPBD::Integration(dt) (PBD_cuda.cu:1437-1518) */
+PBD_API int pbd_integrate(pbd_ctx* ctx, float dt);
+/* The driver loop body Advect -> ProjectConstraint -> Integration
+ * (PositionBasedDynamics.cpp:263-290), `substeps` times, fused. */
+PBD_API int pbd_step(pbd_ctx* ctx, int substeps);
+PBD_API int pbd_cloth_get_state(pbd_ctx* ctx, int cloth, int which, float* xyz);
+PBD_API int pbd_cloth_set_state(pbd_ctx* ctx, int cloth, int which, const float* xyz);
+/* Convergence residual over distance prims: max and RMS of |d - L| / L (FP64). */
+PBD_API int pbd_cloth_residual(pbd_ctx* ctx, int cloth, double* max_rel, double* rms);
+/* Sweeps the last projection of this cloth ran. */
+PBD_API int pbd_cloth_last_sweeps(pbd_ctx* ctx, int cloth, int* sweeps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

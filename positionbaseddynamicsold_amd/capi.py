@@ -1,0 +1,274 @@
+"""ctypes binding of libpbd.so (the C ABI declared in include/pbd.h).
+
+The library is loaded from this package directory; a missing or unloadable
+library raises immediately -- there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libpbd.so"
+
+PBD_OK = 0
+PBD_ERR_ARG = -1
+PBD_ERR_STATE = -2
+PBD_ERR_HIP = -3
+PBD_ERR_LIMIT = -4
+PBD_ERR_IO = -5
+PBD_ERR_COLOR = -6
+
+DISTANCE = 0x001
+BENDING = 0x010
+ANCHOR = 0x100
+
+STATE_X, STATE_V, STATE_P = 0, 1, 2
+PATH_AUTO, PATH_GLOBAL = 0, 1
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("res_x", C.c_int), ("res_y", C.c_int), ("size_x", C.c_float), ("size_y", C.c_float),
+                ("origin", C.c_float * 3), ("jitter", C.c_float), ("seed", C.c_uint64)]
+
+
+class ClothParams(C.Structure):
+    _fields_ = [("damping", C.c_float), ("gravity", C.c_float * 3), ("wind", C.c_float * 3),
+                ("wind_drag", C.c_float), ("sphere_on", C.c_int), ("sphere_c", C.c_float * 3),
+                ("sphere_r", C.c_float), ("box_on", C.c_int), ("box_lo", C.c_float * 3),
+                ("box_hi", C.c_float * 3)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("iterations", C.c_int), ("tol", C.c_double), ("check_every", C.c_int),
+                ("path", C.c_int), ("tile_sweeps", C.c_int), ("tile_max_verts", C.c_int),
+                ("block_threads", C.c_int)]
+
+
+class ClothInfo(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nv", "nt", "ne", "nb", "nprims", "ncolors", "color_rounds", "adj_len",
+                                       "ntiles", "whole", "max_loaded")]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+# name -> argtypes (all return int except pbd_last_error)
+SIGNATURES = {
+    "pbd_create": [_I, C.POINTER(_P)],
+    "pbd_destroy": [_P],
+    "pbd_synchronize": [_P],
+    "pbd_cloth_create_grid": [_P, C.POINTER(GridDesc), _ip],
+    "pbd_cloth_create_mesh": [_P, _fp, _I, _ip, _I, _ip],
+    "pbd_cloth_load_txt": [_P, C.c_char_p, C.c_char_p, _ip],
+    "pbd_cloth_set_distance_pairs": [_P, _I, _ip, _I],
+    "pbd_cloth_set_constraint_option": [_P, _I, C.c_uint, _fp],
+    "pbd_cloth_set_pins": [_P, _I, _ip, _I],
+    "pbd_cloth_set_params": [_P, _I, C.POINTER(ClothParams)],
+    "pbd_init": [_P, _I],
+    "pbd_cloth_get_info": [_P, _I, C.POINTER(ClothInfo)],
+    "pbd_cloth_get_prims": [_P, _I, _ip, _ip, _fp],
+    "pbd_cloth_get_adjacency": [_P, _I, _ip, _ip],
+    "pbd_cloth_get_colors": [_P, _I, _ip, _ip, _ip],
+    "pbd_set_solver_params": [_P, C.POINTER(SolverParams)],
+    "pbd_advect": [_P, _F],
+    "pbd_project": [_P, _I],
+    "pbd_integrate": [_P, _F],
+    "pbd_step": [_P, _I],
+    "pbd_cloth_get_state": [_P, _I, _I, _fp],
+    "pbd_cloth_set_state": [_P, _I, _I, _fp],
+    "pbd_cloth_residual": [_P, _I, _dp, _dp],
+    "pbd_cloth_last_sweeps": [_P, _I, _ip],
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libpbd.so once; raises if it is missing (build it first)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run positionbaseddynamicsold_amd.build.build() first")
+        lib = C.CDLL(str(LIB_PATH))
+        lib.pbd_last_error.restype = C.c_char_p
+        lib.pbd_last_error.argtypes = []
+        for name, args in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+class PbdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pbd error {code}: {msg}")
+        self.code = code
+
+
+def check(code: int) -> None:
+    if code != PBD_OK:
+        raise PbdError(code, load().pbd_last_error().decode(errors="replace"))
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+def default_cloth_params(**kw) -> ClothParams:
+    p = ClothParams()
+    p.damping = 1.0
+    p.gravity[:] = (0.0, -9.8, 0.0)
+    for k, v in kw.items():
+        if isinstance(getattr(p, k), C.Array):
+            getattr(p, k)[:] = v
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Context:
+    """One pbd_ctx: a batch of cloths on one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        check(self.lib.pbd_create(device, C.byref(self.h)))
+        self.cloths: list[int] = []
+
+    def close(self):
+        if self.h:
+            check(self.lib.pbd_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- model ---------------------------------------------------------
+    def add_grid(self, rx, ry, sx=1.0, sy=1.0, origin=(0.0, 0.0, 0.0), jitter=0.0, seed=0) -> int:
+        d = GridDesc(rx, ry, sx, sy, (C.c_float * 3)(*origin), jitter, seed)
+        cid = C.c_int()
+        check(self.lib.pbd_cloth_create_grid(self.h, C.byref(d), C.byref(cid)))
+        self.cloths.append(cid.value)
+        return cid.value
+
+    def add_mesh(self, pos: np.ndarray, tris: np.ndarray) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        cid = C.c_int()
+        check(self.lib.pbd_cloth_create_mesh(self.h, fptr(pos), len(pos), iptr(tris), len(tris), C.byref(cid)))
+        self.cloths.append(cid.value)
+        return cid.value
+
+    def load_txt(self, topol: str, constr: str | None = None) -> int:
+        cid = C.c_int()
+        check(self.lib.pbd_cloth_load_txt(self.h, topol.encode(), (constr or "").encode(), C.byref(cid)))
+        self.cloths.append(cid.value)
+        return cid.value
+
+    def set_distance_pairs(self, cloth: int, pairs: np.ndarray):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        check(self.lib.pbd_cloth_set_distance_pairs(self.h, cloth, iptr(pairs), len(pairs)))
+
+    def set_constraint_option(self, cloth: int, ct: int, stiffness=(1.0, 1.0, 1.0)):
+        st = np.asarray(stiffness, dtype=np.float32)
+        check(self.lib.pbd_cloth_set_constraint_option(self.h, cloth, ct, fptr(st)))
+
+    def set_pins(self, cloth: int, pins):
+        pins = np.ascontiguousarray(pins, dtype=np.int32)
+        check(self.lib.pbd_cloth_set_pins(self.h, cloth, iptr(pins), len(pins)))
+
+    def set_params(self, cloth: int, params: ClothParams | None = None, **kw):
+        p = params if params is not None else default_cloth_params(**kw)
+        check(self.lib.pbd_cloth_set_params(self.h, cloth, C.byref(p)))
+
+    def init(self, color_rounds: int = 20000):
+        check(self.lib.pbd_init(self.h, color_rounds))
+
+    def info(self, cloth: int) -> ClothInfo:
+        i = ClothInfo()
+        check(self.lib.pbd_cloth_get_info(self.h, cloth, C.byref(i)))
+        return i
+
+    def prims(self, cloth: int):
+        inf = self.info(cloth)
+        pl = np.zeros((inf.nprims, 2), np.int32)
+        idx = np.zeros(2 * inf.ne + 4 * inf.nb, np.int32)
+        rest = np.zeros(inf.nprims, np.float32)
+        check(self.lib.pbd_cloth_get_prims(self.h, cloth, iptr(pl), iptr(idx), fptr(rest)))
+        return pl, idx, rest
+
+    def adjacency(self, cloth: int):
+        inf = self.info(cloth)
+        sn = np.zeros((inf.nprims, 2), np.int32)
+        adj = np.zeros(max(1, inf.adj_len), np.int32)
+        check(self.lib.pbd_cloth_get_adjacency(self.h, cloth, iptr(sn), iptr(adj)))
+        return sn, adj[: inf.adj_len]
+
+    def colors(self, cloth: int):
+        inf = self.info(cloth)
+        col = np.zeros(inf.nprims, np.int32)
+        srt = np.zeros(inf.nprims, np.int32)
+        ws = np.zeros((max(1, inf.ncolors), 2), np.int32)
+        check(self.lib.pbd_cloth_get_colors(self.h, cloth, iptr(col), iptr(srt), iptr(ws)))
+        return col, srt, ws[: inf.ncolors]
+
+    # ---- solver --------------------------------------------------------
+    def set_solver(self, dt=1.0 / 600.0, iterations=20, tol=-1.0, check_every=1, path=PATH_AUTO,
+                   tile_sweeps=0, tile_max_verts=0, block_threads=0):
+        p = SolverParams(dt, iterations, tol, check_every, path, tile_sweeps, tile_max_verts, block_threads)
+        check(self.lib.pbd_set_solver_params(self.h, C.byref(p)))
+
+    def advect(self, dt: float):
+        check(self.lib.pbd_advect(self.h, dt))
+
+    def project(self, iterations: int):
+        check(self.lib.pbd_project(self.h, iterations))
+
+    def integrate(self, dt: float):
+        check(self.lib.pbd_integrate(self.h, dt))
+
+    def step(self, substeps: int = 1):
+        check(self.lib.pbd_step(self.h, substeps))
+
+    def sync(self):
+        check(self.lib.pbd_synchronize(self.h))
+
+    def get(self, cloth: int, which: int = STATE_X) -> np.ndarray:
+        n = self.info(cloth).nv
+        out = np.zeros((n, 3), np.float32)
+        check(self.lib.pbd_cloth_get_state(self.h, cloth, which, fptr(out)))
+        return out
+
+    def set(self, cloth: int, which: int, xyz: np.ndarray):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        check(self.lib.pbd_cloth_set_state(self.h, cloth, which, fptr(xyz)))
+
+    def residual(self, cloth: int):
+        m, r = C.c_double(), C.c_double()
+        check(self.lib.pbd_cloth_residual(self.h, cloth, C.byref(m), C.byref(r)))
+        return m.value, r.value
+
+    def last_sweeps(self, cloth: int) -> int:
+        s = C.c_int()
+        check(self.lib.pbd_cloth_last_sweeps(self.h, cloth, C.byref(s)))
+        return s.value

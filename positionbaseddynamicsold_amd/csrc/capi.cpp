@@ -1,0 +1,839 @@
+// extern "C" boundary of libpbd.so: context, model build, init pipeline and
+// solver orchestration.  Exceptions never cross this file's functions.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "pbd_internal.hpp"
+
+using namespace pbd;
+
+static thread_local std::string g_err;
+
+struct pbd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Cloth> cloths;
+    bool inited = false;
+    pbd_solver_params sp{};
+    int lds_max = 65536;
+    // state
+    DBuf<float4> x, v, pbuf[2];
+    int pcur = 0;
+    DBuf<int> vcloth;
+    std::vector<ClothDev> cl_h;
+    DBuf<ClothDev> cl;
+    int nv_total = 0;
+    // global-path prims, batch colour-major
+    DBuf<int4> gdist, gbidx, rdist;
+    DBuf<float2> gbrc;
+    std::vector<int> gd_off, gb_off;
+    int ncolors_max = 0;
+    // stiffness tables
+    DBuf<float> ktab_d, ktab_b;
+    int kmax = 0;
+    // tiles
+    TilePlan plan;
+    int plan_sweeps = 0, plan_max_load = 0;
+    DBuf<TileDev> tiles;
+    DBuf<int> tverts, tcols, order_whole, order_partial;
+    DBuf<uint32_t> tdist, tbend, town;
+    DBuf<float> tdist_rest, tbend_rest, town_rest;
+    bool any_whole = false, any_partial = false;
+    // residual / sweep bookkeeping
+    DBuf<unsigned long long> res_slots, rmax;
+    DBuf<double> rsum;
+    DBuf<int> sweeps;
+    int nslots = 0;
+};
+
+namespace {
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return PBD_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return PBD_ERR_LIMIT;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PBD_ERR_ARG;
+    }
+}
+
+Cloth& cloth_of(pbd_ctx* c, int id) {
+    PBD_CHECK(c, PBD_ERR_ARG, "null context");
+    PBD_CHECK(id >= 0 && id < (int)c->cloths.size(), PBD_ERR_ARG, "cloth id out of range");
+    return c->cloths[id];
+}
+
+void set_device(pbd_ctx* c) { PBD_HIP(hipSetDevice(c->device)); }
+
+pbd_cloth_params default_params() {
+    pbd_cloth_params p{};
+    p.damping = 1.0f;
+    p.gravity[1] = -9.8f;
+    return p;
+}
+
+// Per-cloth kernel constants for substep length dt.
+void fill_cloth_dev(pbd_ctx* c) {
+    float dt = c->sp.dt;
+    c->cl_h.resize(c->cloths.size());
+    for (size_t i = 0; i < c->cloths.size(); i++) {
+        const Cloth& k = c->cloths[i];
+        ClothDev& d = c->cl_h[i];
+        d = ClothDev{};
+        for (int a = 0; a < 3; a++) {
+            d.gdt[a] = k.prm.gravity[a] * dt;
+            d.wind[a] = k.prm.wind[a];
+            d.sphere_c[a] = k.prm.sphere_c[a];
+            d.box_lo[a] = k.prm.box_lo[a];
+            d.box_hi[a] = k.prm.box_hi[a];
+        }
+        d.damp = powf(k.prm.damping, dt);
+        d.kd = k.prm.wind_drag * dt;
+        d.wind_drag = k.prm.wind_drag;
+        d.sphere_on = k.prm.sphere_on;
+        d.sphere_r = k.prm.sphere_r;
+        d.box_on = k.prm.box_on;
+        d.vbase = k.vbase;
+        d.nv = k.nv;
+        d.ktab = (int)i * c->kmax;
+        d.whole = k.whole ? 1 : 0;
+    }
+    c->cl.upload(c->cl_h, c->stream);
+}
+
+// Stiffness per sweep: distance prims follow the reference kernel rule
+// k = 1 - powf(1 - s, 1/(iter+1)) (PBD_cuda.cu:1282); bending prims the
+// constant k' = 1 - (1 - s)^(1/I) of Mueller et al. 2007 with I = iterations.
+void fill_ktabs(pbd_ctx* c) {
+    int I = std::max(1, c->sp.iterations);
+    c->kmax = I;
+    std::vector<float> kd((size_t)c->cloths.size() * I), kb((size_t)c->cloths.size() * I);
+    for (size_t i = 0; i < c->cloths.size(); i++) {
+        float sd = c->cloths[i].stiff[0], sb = c->cloths[i].stiff[1];
+        float kbb = 1 - powf(1 - sb, 1.0 / I);
+        for (int s = 0; s < I; s++) {
+            kd[i * I + s] = 1 - powf(1 - sd, 1.0 / (s + 1));
+            kb[i * I + s] = kbb;
+        }
+    }
+    c->ktab_d.upload(kd, c->stream);
+    c->ktab_b.upload(kb, c->stream);
+}
+
+int effective_tile_sweeps(const pbd_ctx* c) {
+    int s = c->sp.tile_sweeps > 0 ? c->sp.tile_sweeps : c->sp.check_every;
+    return std::max(1, s);
+}
+
+int effective_max_load(const pbd_ctx* c) {
+    int lim = (c->lds_max - 256) / 16;
+    if (c->sp.tile_max_verts > 0) lim = std::min(lim, c->sp.tile_max_verts);
+    return std::max(64, lim);
+}
+
+void upload_plan(pbd_ctx* c) {
+    TilePlan& p = c->plan;
+    c->tiles.upload(p.tiles, c->stream);
+    c->tverts.upload(p.verts, c->stream);
+    c->tcols.upload(p.cols, c->stream);
+    c->tdist.upload(p.dist, c->stream);
+    c->tdist_rest.upload(p.dist_rest, c->stream);
+    c->tbend.upload(p.bend, c->stream);
+    c->tbend_rest.upload(p.bend_rest, c->stream);
+    c->town.upload(p.own, c->stream);
+    c->town_rest.upload(p.own_rest, c->stream);
+    c->order_whole.upload(p.whole_order, c->stream);
+    c->order_partial.upload(p.partial_order, c->stream);
+    c->any_whole = false;
+    c->any_partial = false;
+    for (auto& t : p.tiles) (t.whole ? c->any_whole : c->any_partial) = true;
+}
+
+void replan_if_needed(pbd_ctx* c) {
+    int s = effective_tile_sweeps(c), ml = effective_max_load(c);
+    if (s == c->plan_sweeps && ml == c->plan_max_load) return;
+    plan_tiles(c->cloths, ml, s, c->plan);
+    c->plan_sweeps = s;
+    c->plan_max_load = ml;
+    upload_plan(c);
+    fill_cloth_dev(c);
+}
+
+void ensure_slots(pbd_ctx* c, int nchunks) {
+    int need = std::max(2, nchunks + 1);
+    if (need > c->nslots) {
+        c->res_slots.alloc((size_t)need * c->cloths.size());
+        c->nslots = need;
+    }
+}
+
+// ---- model init -----------------------------------------------------------
+void do_init(pbd_ctx* c, int color_rounds) {
+    PBD_CHECK(!c->cloths.empty(), PBD_ERR_STATE, "no cloth created");
+    PBD_CHECK(color_rounds > 0, PBD_ERR_ARG, "color_rounds must be positive");
+    set_device(c);
+    hipStream_t s = c->stream;
+    // constraints + adjacency per cloth, packed offsets
+    int vbase = 0, pbase = 0;
+    long long adj_total = 0;
+    for (auto& k : c->cloths) {
+        build_prims(k);
+        build_adjacency(k);
+        k.vbase = vbase;
+        k.pbase = pbase;
+        vbase += k.nv;
+        pbase += k.np();
+        adj_total += (long long)k.adj.size();
+    }
+    PBD_CHECK(adj_total < (1ll << 31), PBD_ERR_LIMIT, "adjacency exceeds 2^31 entries");
+    c->nv_total = vbase;
+    int np = pbase;
+    // ---- GPU colouring (EdgeColoringGPU, PBD_cuda.cu:487-544)
+    std::vector<int> npts(np), adj_sn((size_t)2 * np), adj;
+    adj.reserve((size_t)adj_total);
+    for (auto& k : c->cloths) {
+        // cloth-local CSR rebased to global prim ids and offsets
+        int base = (int)adj.size();
+        for (int p = 0; p < k.np(); p++) {
+            npts[k.pbase + p] = k.prim_list[2 * p + 1];
+            adj_sn[2 * (size_t)(k.pbase + p)] = base + k.adj_start_num[2 * p];
+            adj_sn[2 * (size_t)(k.pbase + p) + 1] = k.adj_start_num[2 * p + 1];
+        }
+        for (int q : k.adj) adj.push_back(q + k.pbase);
+    }
+    std::vector<int> color;
+    int rounds = 0;
+    if (np > 0) {
+        DBuf<int> d_npts, d_sn, d_adj, d_color, d_prd, d_unc;
+        d_npts.upload(npts, s);
+        d_sn.upload(adj_sn, s);
+        d_adj.upload(adj, s);
+        std::vector<int> minus1(np, -1);
+        d_color.upload(minus1, s);
+        d_prd.upload(minus1, s);
+        d_unc.alloc(1);
+        int unc = np;
+        while (unc > 0) {
+            PBD_CHECK(rounds < color_rounds, PBD_ERR_COLOR, "colouring did not converge in the allowed rounds");
+            PBD_HIP(hipMemsetAsync(d_unc.ptr, 0, sizeof(int), s));
+            launch_color_round(s, d_npts.ptr, d_sn.ptr, d_adj.ptr, d_color.ptr, d_prd.ptr, d_unc.ptr, np);
+            PBD_HIP(hipMemcpyAsync(&unc, d_unc.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+            PBD_HIP(hipStreamSynchronize(s));
+            rounds++;
+        }
+        // ---- device stable colour sort (SortEdgesColors, PBD_cuda.cu:546-587)
+        std::vector<uint32_t> ids(np);
+        for (int i = 0; i < np; i++) ids[i] = (uint32_t)i;
+        d_color.download(color, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        int cmax = 0;
+        for (int q : color) cmax = std::max(cmax, q + 1);
+        int bits = 1;
+        while ((1 << bits) < cmax) bits++;
+        DBuf<uint32_t> kk, vv, k2, v2;
+        DBuf<int> hist;
+        kk.upload(std::vector<uint32_t>(color.begin(), color.end()), s);
+        vv.upload(ids, s);
+        k2.alloc(np);
+        v2.alloc(np);
+        hist.alloc((size_t)256 * ((np + 4095) / 4096));
+        radix_sort_pairs(s, kk.ptr, vv.ptr, k2.ptr, v2.ptr, np, bits, hist.ptr);
+        std::vector<uint32_t> sorted_ids;
+        vv.download(sorted_ids, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        c->ncolors_max = cmax;
+        // per-cloth colour data
+        for (auto& k : c->cloths) {
+            k.color.assign(color.begin() + k.pbase, color.begin() + k.pbase + k.np());
+            k.sorted.clear();
+            k.color_rounds = rounds;
+        }
+        for (uint32_t g : sorted_ids) {
+            // prims of one cloth are contiguous in global id
+            auto it = std::upper_bound(c->cloths.begin(), c->cloths.end(), (int)g,
+                                       [](int val, const Cloth& k) { return val < k.pbase; });
+            Cloth& k = *(it - 1);
+            k.sorted.push_back((int)g - k.pbase);
+        }
+        for (auto& k : c->cloths) {
+            k.ncolors = 0;
+            for (int q : k.color) k.ncolors = std::max(k.ncolors, q + 1);
+            k.worksets.assign((size_t)2 * k.ncolors, 0);
+            for (int q : k.color) k.worksets[2 * q + 1]++;
+            int run = 0;
+            for (int q = 0; q < k.ncolors; q++) {
+                k.worksets[2 * q] = run;
+                run += k.worksets[2 * q + 1];
+            }
+        }
+    } else {
+        for (auto& k : c->cloths) {
+            k.color.clear();
+            k.sorted.clear();
+            k.worksets.clear();
+            k.ncolors = 0;
+        }
+    }
+    // ---- batch colour-major prim arrays (global path) and cloth-ordered residual list
+    std::vector<int4> gd, rd, gb;
+    std::vector<float2> gbr;
+    c->gd_off.assign(c->ncolors_max + 1, 0);
+    c->gb_off.assign(c->ncolors_max + 1, 0);
+    for (int col = 0; col < c->ncolors_max; col++) {
+        c->gd_off[col] = (int)gd.size();
+        c->gb_off[col] = (int)gb.size();
+        for (size_t ci = 0; ci < c->cloths.size(); ci++) {
+            Cloth& k = c->cloths[ci];
+            if (col >= k.ncolors) continue;
+            int st = k.worksets[2 * col], n = k.worksets[2 * col + 1];
+            for (int j = st; j < st + n; j++) {
+                int p = k.sorted[j];
+                const int* ix = &k.indices[k.prim_list[2 * p]];
+                if (k.prim_list[2 * p + 1] == 2) {
+                    float r = k.rest[p];
+                    int rb;
+                    memcpy(&rb, &r, 4);
+                    gd.push_back(make_int4(ix[0] + k.vbase, ix[1] + k.vbase, rb, (int)ci));
+                } else {
+                    gb.push_back(make_int4(ix[0] + k.vbase, ix[1] + k.vbase, ix[2] + k.vbase, ix[3] + k.vbase));
+                    float cb;
+                    int ci_i = (int)ci;
+                    memcpy(&cb, &ci_i, 4);
+                    gbr.push_back(make_float2(k.rest[p], cb));
+                }
+            }
+        }
+    }
+    c->gd_off[c->ncolors_max] = (int)gd.size();
+    c->gb_off[c->ncolors_max] = (int)gb.size();
+    for (size_t ci = 0; ci < c->cloths.size(); ci++) {
+        Cloth& k = c->cloths[ci];
+        for (int p = 0; p < k.np(); p++) {
+            if (k.prim_list[2 * p + 1] != 2) continue;
+            const int* ix = &k.indices[k.prim_list[2 * p]];
+            float r = k.rest[p];
+            int rb;
+            memcpy(&rb, &r, 4);
+            rd.push_back(make_int4(ix[0] + k.vbase, ix[1] + k.vbase, rb, (int)ci));
+        }
+    }
+    c->gdist.upload(gd, s);
+    c->gbidx.upload(gb, s);
+    c->gbrc.upload(gbr, s);
+    c->rdist.upload(rd, s);
+    // ---- state buffers (InitGPUBuffers, PBD_cuda.cu:707-750)
+    std::vector<float4> X(c->nv_total), V(c->nv_total, make_float4(0, 0, 0, 0));
+    std::vector<int> vc(c->nv_total);
+    for (size_t ci = 0; ci < c->cloths.size(); ci++) {
+        Cloth& k = c->cloths[ci];
+        for (int i = 0; i < k.nv; i++) {
+            X[k.vbase + i] = make_float4(k.pos0[3 * i], k.pos0[3 * i + 1], k.pos0[3 * i + 2], k.w[i]);
+            vc[k.vbase + i] = (int)ci;
+        }
+    }
+    c->x.upload(X, s);
+    c->v.upload(V, s);
+    c->pbuf[0].upload(X, s);
+    c->pbuf[1].upload(X, s);
+    c->pcur = 0;
+    c->vcloth.upload(vc, s);
+    c->rmax.alloc(c->cloths.size());
+    c->rsum.alloc(c->cloths.size());
+    c->sweeps.alloc(c->cloths.size());
+    PBD_HIP(hipMemsetAsync(c->sweeps.ptr, 0, sizeof(int) * c->cloths.size(), s));
+    fill_ktabs(c);
+    c->plan_sweeps = 0;
+    c->plan_max_load = 0;
+    replan_if_needed(c);
+    PBD_HIP(hipStreamSynchronize(s));
+    c->inited = true;
+}
+
+// ---- solver orchestration ---------------------------------------------------
+void advect_all(pbd_ctx* c, float dt) {
+    // the kernel constants hold g*dt for sp.dt; recompute when dt differs
+    if (dt != c->sp.dt) {
+        c->sp.dt = dt;
+        fill_cloth_dev(c);
+    }
+    launch_advect(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, 0,
+                  c->nv_total, dt, 0);
+}
+
+// Sweeps on the current p.  Whole-cloth tiles run unfused (p_in -> p_out).
+void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
+    hipStream_t s = c->stream;
+    int ncl = (int)c->cloths.size();
+    double tol = c->sp.tol;
+    int ce = std::max(1, c->sp.check_every);
+    PBD_HIP(hipMemsetAsync(c->sweeps.ptr, 0, sizeof(int) * ncl, s));
+    if (c->sp.path == PBD_PATH_GLOBAL) {
+        PBD_CHECK(!fused_whole, PBD_ERR_STATE, "internal: global path is never fused");
+        float4* p = c->pbuf[c->pcur].ptr;
+        bool coll = false;
+        for (auto& k : c->cloths) coll |= (k.prm.sphere_on || k.prm.box_on);
+        int sw = 0;
+        while (sw < iterations) {
+            for (int col = 0; col < c->ncolors_max; col++)
+                launch_project_colour(s, c->gdist.ptr, c->gd_off[col], c->gd_off[col + 1], c->gbidx.ptr, c->gbrc.ptr,
+                                      c->gb_off[col], c->gb_off[col + 1], c->ktab_d.ptr, c->ktab_b.ptr, c->kmax, sw, p);
+            if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, p, 0, c->nv_total);
+            sw++;
+            if (tol >= 0.0 && sw % ce == 0) {
+                launch_fill_u64(s, c->rmax.ptr, ncl, 0ull);
+                PBD_HIP(hipMemsetAsync(c->rsum.ptr, 0, sizeof(double) * ncl, s));
+                launch_residual(s, c->rdist.ptr, (int)c->rdist.n, p, c->rmax.ptr, c->rsum.ptr, nullptr);
+                std::vector<unsigned long long> r;
+                c->rmax.download(r, s);
+                PBD_HIP(hipStreamSynchronize(s));
+                double m = 0.0;
+                for (auto b : r) {
+                    double dv;
+                    memcpy(&dv, &b, 8);
+                    m = std::max(m, dv);
+                }
+                if (m <= tol) break;
+            }
+        }
+        for (auto& k : c->cloths) k.last_sweeps = sw;
+        std::vector<int> sv(ncl, sw);
+        PBD_HIP(hipMemcpyAsync(c->sweeps.ptr, sv.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, s));
+        return;
+    }
+    replan_if_needed(c);
+    int ts = effective_tile_sweeps(c);
+    int nchunks = (iterations + ts - 1) / ts;
+    ensure_slots(c, nchunks);
+    launch_fill_u64(s, c->res_slots.ptr, (size_t)c->nslots * ncl, 0ull);
+    size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
+    int threads = c->sp.block_threads > 0 ? c->sp.block_threads : 1024;
+    int cur = c->pcur;
+    SolveArgs a{};
+    a.dt = dt;
+    a.check_every = ce;
+    a.tol = tol;
+    a.kmax = c->kmax;
+    if (c->any_whole) {
+        a.sweep_base = 0;
+        a.nsweeps = iterations;
+        a.fused = fused_whole ? 1 : 0;
+        a.chunk = 0;
+        // whole tiles write pbuf[cur ^ 1]; copied below if the partial chain
+        // ends in the other buffer
+        launch_tiles(s, (int)c->order_whole.n, c->order_whole.ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr,
+                     c->tdist_rest.ptr, c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr,
+                     c->cl.ptr, c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->pbuf[cur].ptr,
+                     c->pbuf[cur ^ 1].ptr, c->res_slots.ptr, ncl, c->sweeps.ptr, a, threads, lds);
+    }
+    if (c->any_partial) {
+        int sw = 0;
+        for (int ch = 0; ch < nchunks; ch++) {
+            a.sweep_base = sw;
+            a.nsweeps = std::min(ts, iterations - sw);
+            a.fused = 0;
+            a.chunk = ch;
+            launch_tiles(s, (int)c->order_partial.n, c->order_partial.ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr,
+                         c->tdist_rest.ptr, c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr,
+                         c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr,
+                         c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, c->res_slots.ptr, ncl, c->sweeps.ptr, a, threads,
+                         lds);
+            cur ^= 1;
+            sw += a.nsweeps;
+            if (tol >= 0.0 && ch + 1 < nchunks) {
+                // stop once every multi-tile cloth met the tolerance
+                std::vector<unsigned long long> r(ncl);
+                PBD_HIP(hipMemcpyAsync(r.data(), c->res_slots.ptr + (size_t)ch * ncl, sizeof(unsigned long long) * ncl,
+                                       hipMemcpyDeviceToHost, s));
+                PBD_HIP(hipStreamSynchronize(s));
+                bool all = true;
+                for (int ci = 0; ci < ncl; ci++) {
+                    if (c->cloths[ci].whole) continue;
+                    double dv;
+                    memcpy(&dv, &r[ci], 8);
+                    if (dv > tol) all = false;
+                }
+                if (all) break;
+            }
+        }
+    } else {
+        cur ^= 1;
+    }
+    // whole tiles wrote pbuf[pcur^1]; make the final index hold them too
+    if (c->any_whole && c->any_partial && cur != (c->pcur ^ 1)) {
+        // copy whole-cloth vertices into the final buffer
+        for (auto& k : c->cloths)
+            if (k.whole)
+                PBD_HIP(hipMemcpyAsync(c->pbuf[cur].ptr + k.vbase, c->pbuf[c->pcur ^ 1].ptr + k.vbase,
+                                       sizeof(float4) * k.nv, hipMemcpyDeviceToDevice, s));
+    }
+    c->pcur = cur;
+    std::vector<int> sv;
+    c->sweeps.download(sv, s);
+    PBD_HIP(hipStreamSynchronize(s));
+    for (int ci = 0; ci < ncl; ci++) c->cloths[ci].last_sweeps = sv[ci];
+}
+
+void integrate_all(pbd_ctx* c, float dt, bool skip_whole) {
+    launch_integrate(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, 0, c->nv_total,
+                     dt, skip_whole ? 1 : 0);
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char* pbd_last_error(void) { return g_err.c_str(); }
+
+int pbd_create(int device, pbd_ctx** out) {
+    return guard([&] {
+        PBD_CHECK(out, PBD_ERR_ARG, "null output pointer");
+        *out = nullptr;
+        int n = 0;
+        PBD_HIP(hipGetDeviceCount(&n));
+        PBD_CHECK(device >= 0 && device < n, PBD_ERR_ARG, "device index out of range");
+        auto* c = new pbd_ctx();
+        c->device = device;
+        try {
+            PBD_HIP(hipSetDevice(device));
+            PBD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            int lds = 0;
+            PBD_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+            c->lds_max = lds > 0 ? lds : 65536;
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        c->sp.dt = 1.0f / 600.0f;
+        c->sp.iterations = 20;
+        c->sp.tol = -1.0;
+        c->sp.check_every = 1;
+        c->sp.path = PBD_PATH_AUTO;
+        *out = c;
+    });
+}
+
+int pbd_destroy(pbd_ctx* ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        hipStream_t s = ctx->stream;
+        delete ctx;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int pbd_synchronize(pbd_ctx* ctx) {
+    return guard([&] {
+        PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_cloth_create_grid(pbd_ctx* ctx, const pbd_grid_desc* d, int* cloth) {
+    return guard([&] {
+        PBD_CHECK(ctx && d && cloth, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(!ctx->inited, PBD_ERR_STATE, "cloths must be created before pbd_init");
+        PBD_CHECK(d->res_x >= 2 && d->res_y >= 2, PBD_ERR_ARG, "grid resolution must be >= 2");
+        PBD_CHECK((long long)d->res_x * d->res_y < (1ll << 30), PBD_ERR_LIMIT, "grid too large");
+        Cloth c;
+        c.rx = d->res_x;
+        c.ry = d->res_y;
+        make_grid(*d, c.pos0, c.tris);
+        c.nv = c.rx * c.ry;
+        c.nt = (int)c.tris.size() / 3;
+        c.prm = default_params();
+        ctx->cloths.push_back(std::move(c));
+        *cloth = (int)ctx->cloths.size() - 1;
+    });
+}
+
+int pbd_cloth_create_mesh(pbd_ctx* ctx, const float* pos, int nv, const int* tris, int nt, int* cloth) {
+    return guard([&] {
+        PBD_CHECK(ctx && cloth && (pos || nv == 0) && (tris || nt == 0), PBD_ERR_ARG, "null argument");
+        PBD_CHECK(!ctx->inited, PBD_ERR_STATE, "cloths must be created before pbd_init");
+        PBD_CHECK(nv >= 0 && nt >= 0, PBD_ERR_ARG, "negative size");
+        Cloth c;
+        c.pos0.assign(pos, pos + (size_t)3 * nv);
+        c.tris.assign(tris, tris + (size_t)3 * nt);
+        for (int i : c.tris) PBD_CHECK(i >= 0 && i < nv, PBD_ERR_ARG, "triangle index out of range");
+        c.nv = nv;
+        c.nt = nt;
+        c.prm = default_params();
+        ctx->cloths.push_back(std::move(c));
+        *cloth = (int)ctx->cloths.size() - 1;
+    });
+}
+
+int pbd_cloth_load_txt(pbd_ctx* ctx, const char* topol, const char* constr, int* cloth) {
+    return guard([&] {
+        PBD_CHECK(ctx && topol && cloth, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(!ctx->inited, PBD_ERR_STATE, "cloths must be created before pbd_init");
+        Cloth c;
+        read_txt_cloth(topol, constr, c.pos0, c.tris, c.user_pairs, c.has_user_pairs);
+        c.nv = (int)c.pos0.size() / 3;
+        c.nt = (int)c.tris.size() / 3;
+        for (int i : c.tris) PBD_CHECK(i >= 0 && i < c.nv, PBD_ERR_IO, "triangle index out of range");
+        c.prm = default_params();
+        ctx->cloths.push_back(std::move(c));
+        *cloth = (int)ctx->cloths.size() - 1;
+    });
+}
+
+int pbd_cloth_set_distance_pairs(pbd_ctx* ctx, int id, const int* pairs, int ne) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(!ctx->inited, PBD_ERR_STATE, "constraints are fixed after pbd_init");
+        PBD_CHECK(ne >= 0 && (pairs || ne == 0), PBD_ERR_ARG, "bad pair array");
+        c.user_pairs.assign(pairs, pairs + (size_t)2 * ne);
+        c.has_user_pairs = true;
+    });
+}
+
+int pbd_cloth_set_constraint_option(pbd_ctx* ctx, int id, unsigned ct, const float* st) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(!ctx->inited, PBD_ERR_STATE, "constraints are fixed after pbd_init");
+        PBD_CHECK(!(ct & ~(unsigned)(PBD_DISTANCE | PBD_BENDING | PBD_ANCHOR)), PBD_ERR_ARG, "unknown constraint bits");
+        c.ct = ct;
+        if (st)
+            for (int k = 0; k < 3; k++) {
+                PBD_CHECK(st[k] >= 0.0f && st[k] <= 1.0f, PBD_ERR_ARG, "stiffness must be in [0,1]");
+                c.stiff[k] = st[k];
+            }
+    });
+}
+
+int pbd_cloth_set_pins(pbd_ctx* ctx, int id, const int* verts, int n) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(!ctx->inited, PBD_ERR_STATE, "pins are fixed after pbd_init");
+        PBD_CHECK(n >= 0 && (verts || n == 0), PBD_ERR_ARG, "bad pin array");
+        for (int i = 0; i < n; i++) PBD_CHECK(verts[i] >= 0 && verts[i] < c.nv, PBD_ERR_ARG, "pin out of range");
+        c.pins.assign(verts, verts + n);
+    });
+}
+
+int pbd_cloth_set_params(pbd_ctx* ctx, int id, const pbd_cloth_params* p) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(p, PBD_ERR_ARG, "null params");
+        PBD_CHECK(p->damping > 0.0f, PBD_ERR_ARG, "damping must be positive");
+        PBD_CHECK(!p->sphere_on || p->sphere_r > 0.0f, PBD_ERR_ARG, "sphere radius must be positive");
+        c.prm = *p;
+        if (ctx->inited) {
+            set_device(ctx);
+            fill_cloth_dev(ctx);
+        }
+    });
+}
+
+int pbd_init(pbd_ctx* ctx, int color_rounds) {
+    return guard([&] {
+        PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
+        PBD_CHECK(!ctx->inited, PBD_ERR_STATE, "pbd_init already called");
+        do_init(ctx, color_rounds);
+    });
+}
+
+int pbd_cloth_get_info(pbd_ctx* ctx, int id, pbd_cloth_info* info) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(info, PBD_ERR_ARG, "null info");
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        info->nv = c.nv;
+        info->nt = c.nt;
+        info->ne = c.ne;
+        info->nb = c.nb;
+        info->nprims = c.np();
+        info->ncolors = c.ncolors;
+        info->color_rounds = c.color_rounds;
+        info->adj_len = (int)c.adj.size();
+        info->ntiles = c.tile_end - c.tile_begin;
+        info->whole = c.whole ? 1 : 0;
+        info->max_loaded = c.max_loaded;
+    });
+}
+
+int pbd_cloth_get_prims(pbd_ctx* ctx, int id, int* prim_list, int* indices, float* rest) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        if (prim_list) std::copy(c.prim_list.begin(), c.prim_list.end(), prim_list);
+        if (indices) std::copy(c.indices.begin(), c.indices.end(), indices);
+        if (rest) std::copy(c.rest.begin(), c.rest.end(), rest);
+    });
+}
+
+int pbd_cloth_get_adjacency(pbd_ctx* ctx, int id, int* start_num, int* adj) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        if (start_num) std::copy(c.adj_start_num.begin(), c.adj_start_num.end(), start_num);
+        if (adj) std::copy(c.adj.begin(), c.adj.end(), adj);
+    });
+}
+
+int pbd_cloth_get_colors(pbd_ctx* ctx, int id, int* color, int* sorted, int* worksets) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        if (color) std::copy(c.color.begin(), c.color.end(), color);
+        if (sorted) std::copy(c.sorted.begin(), c.sorted.end(), sorted);
+        if (worksets) std::copy(c.worksets.begin(), c.worksets.end(), worksets);
+    });
+}
+
+int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* p) {
+    return guard([&] {
+        PBD_CHECK(ctx && p, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(p->dt > 0.0f, PBD_ERR_ARG, "dt must be positive");
+        PBD_CHECK(p->iterations >= 1 && p->iterations <= 1000000, PBD_ERR_ARG, "iterations out of range");
+        PBD_CHECK(p->check_every >= 1, PBD_ERR_ARG, "check_every must be >= 1");
+        PBD_CHECK(p->path == PBD_PATH_AUTO || p->path == PBD_PATH_GLOBAL, PBD_ERR_ARG, "unknown path");
+        PBD_CHECK(p->tile_sweeps >= 0 && p->tile_max_verts >= 0, PBD_ERR_ARG, "negative tile setting");
+        PBD_CHECK(p->block_threads == 0 || (p->block_threads % 64 == 0 && p->block_threads <= 1024), PBD_ERR_ARG,
+                  "block_threads must be a multiple of 64 up to 1024");
+        ctx->sp = *p;
+        if (ctx->inited) {
+            set_device(ctx);
+            fill_ktabs(ctx);
+            fill_cloth_dev(ctx);
+        }
+    });
+}
+
+int pbd_advect(pbd_ctx* ctx, float dt) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(dt > 0.0f, PBD_ERR_ARG, "dt must be positive");
+        set_device(ctx);
+        advect_all(ctx, dt);
+    });
+}
+
+int pbd_project(pbd_ctx* ctx, int iterations) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(iterations >= 1, PBD_ERR_ARG, "iterations must be >= 1");
+        set_device(ctx);
+        if (iterations > ctx->kmax) {
+            ctx->sp.iterations = iterations;
+            fill_ktabs(ctx);
+            fill_cloth_dev(ctx);
+        }
+        project(ctx, iterations, false, ctx->sp.dt);
+    });
+}
+
+int pbd_integrate(pbd_ctx* ctx, float dt) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(dt > 0.0f, PBD_ERR_ARG, "dt must be positive");
+        set_device(ctx);
+        integrate_all(ctx, dt, false);
+    });
+}
+
+int pbd_step(pbd_ctx* ctx, int substeps) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(substeps >= 0, PBD_ERR_ARG, "substeps must be >= 0");
+        set_device(ctx);
+        float dt = ctx->sp.dt;
+        for (int s = 0; s < substeps; s++) {
+            if (ctx->sp.path == PBD_PATH_GLOBAL) {
+                advect_all(ctx, dt);
+                project(ctx, ctx->sp.iterations, false, dt);
+                integrate_all(ctx, dt, false);
+            } else {
+                // multi-tile cloths advect separately; whole cloths fuse it
+                if (ctx->any_partial)
+                    launch_advect(ctx->stream, ctx->cl.ptr, ctx->vcloth.ptr, ctx->x.ptr, ctx->v.ptr,
+                                  ctx->pbuf[ctx->pcur].ptr, 0, ctx->nv_total, dt, 1);
+                project(ctx, ctx->sp.iterations, true, dt);
+                integrate_all(ctx, dt, true);
+            }
+        }
+    });
+}
+
+int pbd_cloth_get_state(pbd_ctx* ctx, int id, int which, float* xyz) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(xyz, PBD_ERR_ARG, "null output");
+        PBD_CHECK(which >= PBD_STATE_X && which <= PBD_STATE_P, PBD_ERR_ARG, "unknown state buffer");
+        set_device(ctx);
+        DBuf<float4>& b = which == PBD_STATE_X ? ctx->x : which == PBD_STATE_V ? ctx->v : ctx->pbuf[ctx->pcur];
+        std::vector<float4> h(c.nv);
+        PBD_HIP(hipMemcpyAsync(h.data(), b.ptr + c.vbase, sizeof(float4) * c.nv, hipMemcpyDeviceToHost, ctx->stream));
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < c.nv; i++) {
+            xyz[3 * i] = h[i].x;
+            xyz[3 * i + 1] = h[i].y;
+            xyz[3 * i + 2] = h[i].z;
+        }
+    });
+}
+
+int pbd_cloth_set_state(pbd_ctx* ctx, int id, int which, const float* xyz) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(xyz, PBD_ERR_ARG, "null input");
+        PBD_CHECK(which >= PBD_STATE_X && which <= PBD_STATE_P, PBD_ERR_ARG, "unknown state buffer");
+        set_device(ctx);
+        DBuf<float4>& b = which == PBD_STATE_X ? ctx->x : which == PBD_STATE_V ? ctx->v : ctx->pbuf[ctx->pcur];
+        std::vector<float4> h(c.nv);
+        for (int i = 0; i < c.nv; i++)
+            h[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], which == PBD_STATE_V ? 0.0f : c.w[i]);
+        PBD_HIP(hipMemcpyAsync(b.ptr + c.vbase, h.data(), sizeof(float4) * c.nv, hipMemcpyHostToDevice, ctx->stream));
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_cloth_residual(pbd_ctx* ctx, int id, double* max_rel, double* rms) {
+    return guard([&] {
+        cloth_of(ctx, id);
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        int ncl = (int)ctx->cloths.size();
+        launch_fill_u64(s, ctx->rmax.ptr, ncl, 0ull);
+        PBD_HIP(hipMemsetAsync(ctx->rsum.ptr, 0, sizeof(double) * ncl, s));
+        launch_residual(s, ctx->rdist.ptr, (int)ctx->rdist.n, ctx->pbuf[ctx->pcur].ptr, ctx->rmax.ptr, ctx->rsum.ptr,
+                        nullptr);
+        unsigned long long m = 0;
+        double sum = 0.0;
+        PBD_HIP(hipMemcpyAsync(&m, ctx->rmax.ptr + id, 8, hipMemcpyDeviceToHost, s));
+        PBD_HIP(hipMemcpyAsync(&sum, ctx->rsum.ptr + id, 8, hipMemcpyDeviceToHost, s));
+        PBD_HIP(hipStreamSynchronize(s));
+        double mv;
+        memcpy(&mv, &m, 8);
+        if (max_rel) *max_rel = mv;
+        int ne = ctx->cloths[id].ne;
+        if (rms) *rms = ne > 0 ? std::sqrt(sum / ne) : 0.0;
+    });
+}
+
+int pbd_cloth_last_sweeps(pbd_ctx* ctx, int id, int* sweeps) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(sweeps, PBD_ERR_ARG, "null output");
+        *sweeps = c.last_sweeps;
+    });
+}
+
+}  // extern "C"

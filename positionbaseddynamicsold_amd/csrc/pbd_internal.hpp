@@ -1,0 +1,210 @@
+// Internal declarations shared by the host model (model.cpp), the C ABI
+// (capi.cpp) and the HIP kernels (*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pbd.h"
+
+namespace pbd {
+
+// Error carried from anywhere in the library to the C ABI boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PBD_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::pbd::Error(PBD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define PBD_CHECK(cond, code, msg)                       \
+    do {                                                 \
+        if (!(cond)) throw ::pbd::Error((code), (msg));  \
+    } while (0)
+
+// Owning device buffer.
+template <class T>
+struct DBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && ptr) return;
+        release();
+        if (count) PBD_HIP(hipMalloc(&ptr, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        alloc(h.size());
+        if (!h.empty()) PBD_HIP(hipMemcpyAsync(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(std::vector<T>& h, hipStream_t s) const {
+        h.resize(n);
+        if (n) PBD_HIP(hipMemcpyAsync(h.data(), ptr, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Device-side records
+// ---------------------------------------------------------------------------
+
+// Per-cloth constants read by the kernels.
+struct ClothDev {
+    float gdt[3];        // gravity * dt
+    float damp;          // powf(damping, dt)
+    float wind[3];
+    float kd;            // wind_drag * dt
+    float wind_drag;
+    int sphere_on;
+    float sphere_c[3];
+    float sphere_r;
+    int box_on;
+    float box_lo[3];
+    float box_hi[3];
+    int vbase, nv;       // vertex range in the packed arrays
+    int ktab;            // offset of this cloth's per-sweep stiffness tables
+    int whole;           // 1 if the cloth is a single LDS tile
+};
+
+// One LDS tile: a set of loaded vertices (interior first) and the prims whose
+// vertices are all loaded, grouped by colour.
+struct TileDev {
+    int cloth;
+    int whole;           // 1: the tile is the whole cloth (runs complete substeps)
+    int load_off;        // offset into tile_verts
+    int n_load;
+    int n_int;           // interior vertices = first n_int of the loaded list
+    int dist_off;        // offset into tile_dist of colour 0
+    int bend_off;        // offset into tile_bend of colour 0
+    int own_off;         // offset into tile_own (distance prims owned by the tile)
+    int n_own;
+    int col_off;         // offset into tile_cols: (ncolors+1) dist offsets then (ncolors+1) bend offsets
+    int ncolors;
+    int pad;
+};
+
+// Kernel-wide parameters of one solver launch.
+struct SolveArgs {
+    float dt;
+    int sweep_base;      // global sweep index of this launch's first sweep
+    int nsweeps;         // sweeps this launch (partial tiles) / cap (whole tiles)
+    int check_every;
+    double tol;
+    int kmax;            // length of each stiffness table
+    int fused;           // whole tiles: 1 = advect on load + integrate on store
+    int chunk;           // chunk index (selects the residual slot)
+};
+
+// ---------------------------------------------------------------------------
+// Host model of one cloth
+// ---------------------------------------------------------------------------
+struct Cloth {
+    // input
+    int rx = 0, ry = 0;                   // grid dims (0 = general mesh)
+    std::vector<float> pos0;              // 3*nv
+    std::vector<int> tris;                // 3*nt
+    std::vector<int> user_pairs;          // explicit distance prims
+    bool has_user_pairs = false;
+    std::vector<int> pins;
+    unsigned ct = PBD_DISTANCE;
+    float stiff[3] = {1.0f, 1.0f, 1.0f};
+    pbd_cloth_params prm{};
+    // built
+    int nv = 0, nt = 0, ne = 0, nb = 0;
+    std::vector<int> prim_list;           // 2*np (start,count)
+    std::vector<int> indices;
+    std::vector<float> rest;
+    std::vector<float> w;
+    std::vector<int> adj_start_num;       // 2*np
+    std::vector<int> adj;
+    std::vector<int> color;
+    std::vector<int> sorted;
+    std::vector<int> worksets;            // 2*ncolors
+    int ncolors = 0;
+    int color_rounds = 0;
+    int vbase = 0;                        // offset in packed vertex arrays
+    int pbase = 0;                        // offset in packed prim arrays
+    int tile_begin = 0, tile_end = 0;     // range in the tile table
+    bool whole = false;
+    int max_loaded = 0;
+    int last_sweeps = 0;
+    int np() const { return ne + nb; }
+};
+
+// Host-side helpers shared by model.cpp and the tests of the C ABI.
+uint64_t splitmix64(uint64_t* s);
+float uniform_pm1(uint64_t* s);
+float distance_powf(const float* a, const float* b);
+void make_grid(const pbd_grid_desc& d, std::vector<float>& pos, std::vector<int>& tris);
+void grid_distance_pairs(int rx, int ry, std::vector<int>& pairs);
+void mesh_edge_pairs(const std::vector<int>& tris, std::vector<int>& pairs);
+void bending_quads(const std::vector<int>& tris, std::vector<int>& quads);
+float atan2_poly(float y, float x);
+float dihedral_angle(const float* e0, const float* e1, const float* o0, const float* o1);
+void build_prims(Cloth& c);
+void build_adjacency(Cloth& c);
+void read_txt_cloth(const char* topol, const char* constr, std::vector<float>& pos,
+                    std::vector<int>& tris, std::vector<int>& pairs, bool& has_pairs);
+
+// Tile plan for the whole batch.
+struct TilePlan {
+    std::vector<TileDev> tiles;
+    std::vector<int> verts;               // global vertex ids
+    std::vector<uint32_t> dist;           // packed (i0 | i1 << 16), local indices
+    std::vector<float> dist_rest;
+    std::vector<uint32_t> bend;           // 2 words per prim: (i0|i1<<16), (i2|i3<<16)
+    std::vector<float> bend_rest;
+    std::vector<uint32_t> own;            // owned distance prims (local pair)
+    std::vector<float> own_rest;
+    std::vector<int> cols;                // per tile colour offsets
+    std::vector<int> partial_order;       // launch order of the partial tiles only
+    std::vector<int> whole_order;         // launch order of the whole tiles only
+};
+void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps_per_launch, TilePlan& plan);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (implemented in the .hip files)
+// ---------------------------------------------------------------------------
+int launch_color_round(hipStream_t s, const int* prim_npts, const int* adj_sn, const int* adj,
+                       int* color, int* prd, int* uncolored, int np);
+void radix_sort_pairs(hipStream_t s, uint32_t* k, uint32_t* v, uint32_t* k2, uint32_t* v2, int n,
+                      int bits, int* hist);
+void launch_advect(hipStream_t s, const ClothDev* cloths, const int* vcloth, const float4* x,
+                   float4* v, float4* p, int nv_begin, int nv_end, float dt, int skip_whole);
+void launch_integrate(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* x, float4* v,
+                      const float4* p, int nv_begin, int nv_end, float dt, int skip_whole);
+void launch_project_colour(hipStream_t s, const int4* dist, int d_begin, int d_end,
+                           const int4* bend_idx, const float2* bend_rk, int b_begin, int b_end,
+                           const float* ktab_d, const float* ktab_b, int kmax, int sweep, float4* p);
+void launch_colliders(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* p,
+                      int nv_begin, int nv_end);
+void launch_residual(hipStream_t s, const int4* dist, int n, const float4* p,
+                     unsigned long long* rmax, double* rsum, int* rcnt);
+void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* tiles,
+                  const int* verts, const uint32_t* dist, const float* dist_rest,
+                  const uint32_t* bend, const float* bend_rest, const uint32_t* own,
+                  const float* own_rest, const int* cols, const ClothDev* cloths,
+                  const float* ktab_d, const float* ktab_b, float4* x, float4* v,
+                  const float4* p_in, float4* p_out, unsigned long long* res_slots,
+                  int nslots_cloth, int* sweeps_out, const SolveArgs& a, int threads,
+                  size_t lds_bytes);
+void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
+
+}  // namespace pbd

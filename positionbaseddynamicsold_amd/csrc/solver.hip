@@ -1,0 +1,372 @@
+// Substep kernels for gfx950: advect, colour-batched Gauss-Seidel projection
+// (global per-colour launches and LDS tiles), analytic colliders, FP64
+// residual reduction, integrate.
+//
+// Vertex state is float4 (x, y, z, w) with w the weight the reference keeps in
+// massBuffer (1 free, 0 pinned), so one 16-byte load brings a position and
+// its weight.  Reference paths are relative to
+// PositionBasedDynamicsCUDA/PositionBasedDynamicsCUDA/.
+#include "pbd_internal.hpp"
+#include "pbd_math.hpp"
+
+namespace pbd {
+
+// AdvectGPUKernel (PBD_cuda.cu:949-979): v += g*dt*m ; v *= damping^dt ;
+// p = x + v*dt, with m -> w; optional wind drag v += ((wind - v)*(drag*dt))*w.
+__device__ __forceinline__ float4 advect_one(const ClothDev& c, float4 X, float4 V, float dt, float4* Vout) {
+    float w = X.w;
+    float vx = V.x + c.gdt[0] * w, vy = V.y + c.gdt[1] * w, vz = V.z + c.gdt[2] * w;
+    if (c.wind_drag != 0.0f) {
+        vx = vx + ((c.wind[0] - vx) * c.kd) * w;
+        vy = vy + ((c.wind[1] - vy) * c.kd) * w;
+        vz = vz + ((c.wind[2] - vz) * c.kd) * w;
+    }
+    vx = vx * c.damp;
+    vy = vy * c.damp;
+    vz = vz * c.damp;
+    *Vout = make_float4(vx, vy, vz, 0.0f);
+    return make_float4(X.x + vx * dt, X.y + vy * dt, X.z + vz * dt, w);
+}
+
+__global__ __launch_bounds__(256) void k_advect(const ClothDev* __restrict__ cl,
+                                                const int* __restrict__ vcloth,
+                                                const float4* __restrict__ x, float4* __restrict__ v,
+                                                float4* __restrict__ p, int b, int e, float dt,
+                                                int skip_whole) {
+    int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    const ClothDev& c = cl[vcloth[i]];
+    if (skip_whole && c.whole) return;
+    float4 V;
+    p[i] = advect_one(c, x[i], v[i], dt, &V);
+    v[i] = V;
+}
+
+void launch_advect(hipStream_t s, const ClothDev* cloths, const int* vcloth, const float4* x,
+                   float4* v, float4* p, int b, int e, float dt, int skip_whole) {
+    if (e <= b) return;
+    hipLaunchKernelGGL(k_advect, dim3((e - b + 255) / 256), dim3(256), 0, s, cloths, vcloth, x, v, p, b, e, dt,
+                       skip_whole);
+    PBD_HIP(hipGetLastError());
+}
+
+// IntegrationGPUKernel (PBD_cuda.cu:1467-1482): v = (p - x)/dt ; x = p.
+__global__ __launch_bounds__(256) void k_integrate(const ClothDev* __restrict__ cl,
+                                                   const int* __restrict__ vcloth, float4* __restrict__ x,
+                                                   float4* __restrict__ v, const float4* __restrict__ p, int b,
+                                                   int e, float dt, int skip_whole) {
+    int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    if (skip_whole && cl[vcloth[i]].whole) return;
+    float4 P = p[i], X = x[i];
+    v[i] = make_float4((P.x - X.x) / dt, (P.y - X.y) / dt, (P.z - X.z) / dt, 0.0f);
+    x[i] = make_float4(P.x, P.y, P.z, X.w);
+}
+
+void launch_integrate(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* x, float4* v,
+                      const float4* p, int b, int e, float dt, int skip_whole) {
+    if (e <= b) return;
+    hipLaunchKernelGGL(k_integrate, dim3((e - b + 255) / 256), dim3(256), 0, s, cloths, vcloth, x, v, p, b, e, dt,
+                       skip_whole);
+    PBD_HIP(hipGetLastError());
+}
+
+// One colour of ProjectContraintsGPUKernel (PBD_cuda.cu:1240-1306) over the
+// whole batch: distance records (i0, i1, rest bits, cloth) and bending records
+// (idx4, (rest, cloth bits)).  Unlike the reference launch (PBD_cuda.cu:1332-1337)
+// the grid covers every prim of the colour.
+__global__ __launch_bounds__(256) void k_project_colour(const int4* __restrict__ dist, int db, int de,
+                                                        const int4* __restrict__ bidx,
+                                                        const float2* __restrict__ brc, int bb, int be,
+                                                        const float* __restrict__ ktab_d,
+                                                        const float* __restrict__ ktab_b, int kmax,
+                                                        int sweep, float4* __restrict__ p) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int nd = de - db;
+    int ks = sweep < kmax ? sweep : kmax - 1;
+    if (j < nd) {
+        int4 r = dist[db + j];
+        float k = ktab_d[r.w * kmax + ks];
+        float4 a = p[r.x], b = p[r.y];
+        project_distance(a, b, __int_as_float(r.z), k);
+        p[r.x] = a;
+        p[r.y] = b;
+        return;
+    }
+    j -= nd;
+    if (j < be - bb) {
+        int4 q = bidx[bb + j];
+        float2 rc = brc[bb + j];
+        float k = ktab_b[__float_as_int(rc.y) * kmax + ks];
+        float4 q0 = p[q.x], q1 = p[q.y], q2 = p[q.z], q3 = p[q.w];
+        project_bending(q0, q1, q2, q3, rc.x, k);
+        p[q.x] = q0;
+        p[q.y] = q1;
+        p[q.z] = q2;
+        p[q.w] = q3;
+    }
+}
+
+void launch_project_colour(hipStream_t s, const int4* dist, int db, int de, const int4* bidx,
+                           const float2* brc, int bb, int be, const float* ktab_d,
+                           const float* ktab_b, int kmax, int sweep, float4* p) {
+    int n = (de - db) + (be - bb);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_project_colour, dim3((n + 255) / 256), dim3(256), 0, s, dist, db, de, bidx, brc, bb,
+                       be, ktab_d, ktab_b, kmax, sweep, p);
+    PBD_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void k_colliders(const ClothDev* __restrict__ cl,
+                                                   const int* __restrict__ vcloth, float4* __restrict__ p,
+                                                   int b, int e) {
+    int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    const ClothDev& c = cl[vcloth[i]];
+    if (!c.sphere_on && !c.box_on) return;
+    float4 q = p[i];
+    collide_point(q, c.sphere_on, c.sphere_c, c.sphere_r, c.box_on, c.box_lo, c.box_hi);
+    p[i] = q;
+}
+
+void launch_colliders(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* p, int b, int e) {
+    if (e <= b) return;
+    hipLaunchKernelGGL(k_colliders, dim3((e - b + 255) / 256), dim3(256), 0, s, cloths, vcloth, p, b, e);
+    PBD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual r = |d - L| / L per distance prim, d from the FP32 positions
+// widened to double.  Max per cloth via 64-bit atomicMax on the bit pattern
+// (r >= 0, so IEEE order equals unsigned order); sum of r^2 per cloth.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rel_residual(float4 a, float4 b, float L) {
+    double dx = (double)a.x - (double)b.x;
+    double dy = (double)a.y - (double)b.y;
+    double dz = (double)a.z - (double)b.z;
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    double l = (double)L;
+    return fabs(d - l) / l;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_residual(const int4* __restrict__ dist, int n,
+                                                  const float4* __restrict__ p,
+                                                  unsigned long long* __restrict__ rmax,
+                                                  double* __restrict__ rsum) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int cl = -1;
+    double r = 0.0;
+    if (j < n) {
+        int4 q = dist[j];
+        cl = q.w;
+        r = rel_residual(p[q.x], p[q.y], __int_as_float(q.z));
+    }
+    int c0 = __shfl(cl, 0);
+    unsigned long long same = __ballot(cl == c0 || cl < 0);
+    if (same == ~0ull) {
+        double m = wave_max(r), s = wave_sum(r * r);
+        if ((threadIdx.x & 63) == 0 && c0 >= 0) {
+            atomicMax(&rmax[c0], (unsigned long long)__double_as_longlong(m));
+            atomicAdd(&rsum[c0], s);
+        }
+    } else if (cl >= 0) {
+        atomicMax(&rmax[cl], (unsigned long long)__double_as_longlong(r));
+        atomicAdd(&rsum[cl], r * r);
+    }
+}
+
+void launch_residual(hipStream_t s, const int4* dist, int n, const float4* p, unsigned long long* rmax,
+                     double* rsum, int*) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_residual, dim3((n + 255) / 256), dim3(256), 0, s, dist, n, p, rmax, rsum);
+    PBD_HIP(hipGetLastError());
+}
+
+__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    PBD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// LDS tile kernel.  One workgroup per tile: the tile's loaded vertices live
+// in LDS as float4 for the whole launch; each colour phase processes the
+// tile's prims of that colour and ends with a workgroup barrier.
+//
+// whole tiles (the tile is the entire cloth): optional fused advect on load,
+//   sweeps until the residual check passes or the cap, fused integrate on store.
+// partial tiles: exactly nsweeps sweeps on p_in -> p_out (interior only),
+//   skipped (interior copied) when the cloth already met the tolerance in the
+//   previous chunk; the residual of the owned prims is max-reduced into the
+//   cloth's slot for this chunk.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_max(double v, double* red) {
+    v = wave_max(v);
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double m = 0.0;
+    for (int i = 0; i < nw; i++) m = fmax(m, red[i]);
+    return m;
+}
+
+__global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
+                                                const TileDev* __restrict__ tiles,
+                                                const int* __restrict__ verts,
+                                                const uint32_t* __restrict__ dist,
+                                                const float* __restrict__ dist_rest,
+                                                const uint32_t* __restrict__ bend,
+                                                const float* __restrict__ bend_rest,
+                                                const uint32_t* __restrict__ own,
+                                                const float* __restrict__ own_rest,
+                                                const int* __restrict__ cols,
+                                                const ClothDev* __restrict__ cloths,
+                                                const float* __restrict__ ktab_d,
+                                                const float* __restrict__ ktab_b, float4* __restrict__ x,
+                                                float4* __restrict__ v, const float4* __restrict__ p_in,
+                                                float4* __restrict__ p_out,
+                                                unsigned long long* __restrict__ res_slots, int ncl,
+                                                int* __restrict__ sweeps_out, SolveArgs a) {
+    extern __shared__ float4 P[];
+    __shared__ double red[16];
+    int slot = order[blockIdx.x];
+    if (slot < 0) return;
+    const TileDev td = tiles[slot];
+    const ClothDev& cd = cloths[td.cloth];
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int* vl = verts + td.load_off;
+
+    if (!td.whole && a.chunk > 0 && a.tol >= 0.0) {
+        double prev = __longlong_as_double((long long)res_slots[(size_t)(a.chunk - 1) * ncl + td.cloth]);
+        if (prev <= a.tol) {
+            for (int i = tid; i < td.n_int; i += T) p_out[vl[i]] = p_in[vl[i]];
+            return;
+        }
+    }
+    if (td.whole && a.fused) {
+        for (int i = tid; i < td.n_load; i += T) {
+            int g = vl[i];
+            float4 V;
+            P[i] = advect_one(cd, x[g], v[g], a.dt, &V);
+        }
+    } else {
+        for (int i = tid; i < td.n_load; i += T) P[i] = p_in[vl[i]];
+    }
+    __syncthreads();
+
+    const int C = td.ncolors;
+    const int* dc = cols + td.col_off;
+    const int* bc = dc + C + 1;
+    const uint32_t* D = dist + td.dist_off;
+    const float* DR = dist_rest + td.dist_off;
+    const uint32_t* B = bend + 2 * (size_t)td.bend_off;
+    const float* BR = bend_rest + td.bend_off;
+    const bool coll = cd.sphere_on || cd.box_on;
+    int sw = 0;
+    double resid = -1.0;
+    for (; sw < a.nsweeps;) {
+        int gs = a.sweep_base + sw;
+        int ks = gs < a.kmax ? gs : a.kmax - 1;
+        float kd = ktab_d[cd.ktab + ks];
+        float kb = ktab_b[cd.ktab + ks];
+        for (int c = 0; c < C; c++) {
+            for (int j = dc[c] + tid; j < dc[c + 1]; j += T) {
+                uint32_t pr = D[j];
+                int i0 = pr & 0xffffu, i1 = pr >> 16;
+                float4 p0 = P[i0], p1 = P[i1];
+                project_distance(p0, p1, DR[j], kd);
+                P[i0] = p0;
+                P[i1] = p1;
+            }
+            for (int j = bc[c] + tid; j < bc[c + 1]; j += T) {
+                uint32_t w0 = B[2 * j], w1 = B[2 * j + 1];
+                int i0 = w0 & 0xffffu, i1 = w0 >> 16, i2 = w1 & 0xffffu, i3 = w1 >> 16;
+                float4 q0 = P[i0], q1 = P[i1], q2 = P[i2], q3 = P[i3];
+                project_bending(q0, q1, q2, q3, BR[j], kb);
+                P[i0] = q0;
+                P[i1] = q1;
+                P[i2] = q2;
+                P[i3] = q3;
+            }
+            __syncthreads();
+        }
+        if (coll) {
+            for (int i = tid; i < td.n_load; i += T) {
+                float4 q = P[i];
+                collide_point(q, cd.sphere_on, cd.sphere_c, cd.sphere_r, cd.box_on, cd.box_lo, cd.box_hi);
+                P[i] = q;
+            }
+            __syncthreads();
+        }
+        sw++;
+        if (td.whole && a.tol >= 0.0 && sw % a.check_every == 0) {
+            double m = 0.0;
+            for (int j = tid; j < td.n_own; j += T) {
+                uint32_t pr = own[td.own_off + j];
+                m = fmax(m, rel_residual(P[pr & 0xffffu], P[pr >> 16], own_rest[td.own_off + j]));
+            }
+            resid = block_max(m, red);
+            if (resid <= a.tol) break;
+        }
+    }
+    if (!td.whole || resid < 0.0 || (sw % a.check_every) != 0) {
+        double m = 0.0;
+        for (int j = tid; j < td.n_own; j += T) {
+            uint32_t pr = own[td.own_off + j];
+            m = fmax(m, rel_residual(P[pr & 0xffffu], P[pr >> 16], own_rest[td.own_off + j]));
+        }
+        resid = block_max(m, red);
+    }
+    if (tid == 0) {
+        atomicMax(&res_slots[(size_t)a.chunk * ncl + td.cloth], (unsigned long long)__double_as_longlong(resid));
+        atomicMax(&sweeps_out[td.cloth], a.sweep_base + sw);
+    }
+    if (td.whole && a.fused) {
+        for (int i = tid; i < td.n_int; i += T) {
+            int g = vl[i];
+            float4 Q = P[i], X = x[g];
+            v[g] = make_float4((Q.x - X.x) / a.dt, (Q.y - X.y) / a.dt, (Q.z - X.z) / a.dt, 0.0f);
+            x[g] = make_float4(Q.x, Q.y, Q.z, X.w);
+            p_out[g] = Q;
+        }
+    } else {
+        for (int i = tid; i < td.n_int; i += T) p_out[vl[i]] = P[i];
+    }
+}
+
+void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* tiles, const int* verts,
+                  const uint32_t* dist, const float* dist_rest, const uint32_t* bend, const float* bend_rest,
+                  const uint32_t* own, const float* own_rest, const int* cols, const ClothDev* cloths,
+                  const float* ktab_d, const float* ktab_b, float4* x, float4* v, const float4* p_in,
+                  float4* p_out, unsigned long long* res_slots, int ncl, int* sweeps_out,
+                  const SolveArgs& a, int threads, size_t lds_bytes) {
+    if (nblocks <= 0) return;
+    static bool attr_set = false;
+    if (!attr_set) {
+        // not every runtime needs or supports the opt-in; the launch reports real failures
+        (void)hipFuncSetAttribute((const void*)k_tiles, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+        (void)hipGetLastError();
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_tiles, dim3(nblocks), dim3(threads), lds_bytes, s, order, tiles, verts, dist, dist_rest,
+                       bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v, p_in, p_out, res_slots,
+                       ncl, sweeps_out, a);
+    PBD_HIP(hipGetLastError());
+}
+
+}  // namespace pbd

@@ -1,0 +1,261 @@
+"""GPU parity of the model build and the colour-batched substep against the
+CPU oracle, through the C ABI.  Bit-exact is the bar for integer work
+(adjacency, colours, order) and for positions on the distance/bending path
+(both sides evaluate the same IEEE operation sequence); the stated tolerance
+1e-5 x cloth extent is asserted as well."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from positionbaseddynamicsold_amd import capi
+
+
+def lib_grid(ctx, n, jitter=1e-4, seed=0, pins=(0, 31), ct=capi.DISTANCE, stiff=(1.0, 1.0, 1.0), **params):
+    cid = ctx.add_grid(n, n, 1.0, 1.0, jitter=jitter, seed=seed)
+    ctx.set_constraint_option(cid, ct, stiff)
+    if pins:
+        ctx.set_pins(cid, list(pins))
+    if params:
+        ctx.set_params(cid, **params)
+    return cid
+
+
+def orc_grid(orc, n, jitter=1e-4, seed=0, pins=(0, 31), bending=False, stiff=(1.0, 1.0, 1.0), iterations=20):
+    pos, tris = orc.grid_mesh(n, n, 1.0, 1.0, jitter=jitter, seed=seed)
+    return orc.OracleCloth(pos, tris, orc.grid_pairs(n, n), pins=pins, bending=bending, stiffness=stiff,
+                           iterations=iterations)
+
+
+def print_case(msg):
+    print(msg, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# model build: prims, adjacency, colouring, colour order
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,bending", [(32, False), (32, True), (77, True)])
+def test_build_bit_exact(orc, n, bending):
+    print_case(f"build n={n} bending={bending}")
+    ct = capi.DISTANCE | (capi.BENDING if bending else 0)
+    with capi.Context() as ctx:
+        cid = lib_grid(ctx, n, ct=ct)
+        ctx.init()
+        oc = orc_grid(orc, n, bending=bending)
+        pl, idx, rest = ctx.prims(cid)
+        assert np.array_equal(pl, oc.prim_list)
+        assert np.array_equal(idx, oc.indices)
+        assert np.array_equal(rest.view(np.uint32), oc.rest.view(np.uint32))
+        sn, adj = ctx.adjacency(cid)
+        assert np.array_equal(sn, oc.sn) and np.array_equal(adj, oc.adj)
+        col, srt, ws = ctx.colors(cid)
+        assert np.array_equal(col, oc.color)
+        assert np.array_equal(srt, oc.sorted)
+        assert np.array_equal(ws, oc.worksets)
+        assert ctx.info(cid).color_rounds == oc.rounds
+        x = ctx.get(cid, capi.STATE_X)
+        assert np.array_equal(x, oc.x)
+
+
+def test_ground_truth_graph_colours(orc):
+    # PBDObject::groundTruthTest (PBD_cuda.cu:871-881) through the C ABI
+    pos = np.arange(12, dtype=np.float32).reshape(4, 3)
+    tris = np.array([[0, 1, 2]], np.int32)
+    with capi.Context() as ctx:
+        cid = ctx.add_mesh(pos, tris)
+        ctx.set_distance_pairs(cid, np.array([[0, 3], [0, 1], [0, 2], [1, 3], [2, 3]]))
+        ctx.init()
+        col, srt, ws = ctx.colors(cid)
+        assert col.tolist() == [0, 1, 2, 2, 1]
+        assert srt.tolist() == [0, 1, 4, 2, 3]
+        assert ws.tolist() == [[0, 1], [1, 2], [3, 2]]
+        assert ctx.info(cid).color_rounds == 3
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_random_mesh_colouring(orc, seed):
+    rng = np.random.default_rng(seed)
+    nv = 400
+    pos = rng.standard_normal((nv, 3)).astype(np.float32)
+    tris = np.array([rng.choice(nv, 3, replace=False) for _ in range(700)], np.int32)
+    with capi.Context() as ctx:
+        cid = ctx.add_mesh(pos, tris)
+        ctx.set_constraint_option(cid, capi.DISTANCE | capi.BENDING)
+        ctx.init()
+        pl, idx, rest = ctx.prims(cid)
+        sn, adj = orc.adjacency(pl, idx, nv)
+        col_o, rounds = orc.color(pl, sn, adj)
+        col, srt, ws = ctx.colors(cid)
+        assert np.array_equal(col, col_o)
+        assert np.array_equal(srt, np.argsort(col_o, kind="stable"))
+
+
+# ---------------------------------------------------------------------------
+# substep parity
+# ---------------------------------------------------------------------------
+def run_lib(ctx, cid, substeps):
+    ctx.step(substeps)
+    return ctx.get(cid, capi.STATE_X), ctx.get(cid, capi.STATE_V)
+
+
+def assert_close(a, b, extent=1.0):
+    err = np.abs(a.astype(np.float64) - b.astype(np.float64)).max()
+    assert err <= 1e-5 * extent, err
+    return err
+
+
+@pytest.mark.parametrize("path", [capi.PATH_AUTO, capi.PATH_GLOBAL])
+def test_config1_120_frames_bit_exact(orc, path):
+    # 32x32, spacing 1/31, corners pinned, jitter +-1e-4 (seed 0), g=-9.8,
+    # dt = 1/600 (10 substeps per 1/60 s frame), 20 sweeps, s = 1, 120 frames
+    print_case(f"config1 path={path}")
+    with capi.Context() as ctx:
+        cid = lib_grid(ctx, 32)
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=20, path=path)
+        oc = orc_grid(orc, 32)
+        prm = oc.params(dt=1.0 / 600.0)
+        for frame in range(120):
+            ctx.step(10)
+            for _ in range(10):
+                oc.substep(prm, cap=20)
+            if frame in (0, 9, 59, 119):
+                x = ctx.get(cid, capi.STATE_X)
+                assert_close(x, oc.x)
+                assert np.array_equal(x, oc.x), f"frame {frame}"
+        assert np.array_equal(ctx.get(cid, capi.STATE_V), oc.v)
+        assert ctx.info(cid).whole == 1
+        # pins never move
+        x = ctx.get(cid, capi.STATE_X)
+        assert np.array_equal(x[[0, 31]], oc.x[[0, 31]])
+
+
+def test_split_api_matches_step(orc):
+    with capi.Context() as a, capi.Context() as b:
+        ca, cb = lib_grid(a, 32), lib_grid(b, 32)
+        a.init()
+        b.init()
+        for c in (a, b):
+            c.set_solver(dt=1.0 / 600.0, iterations=20)
+        for _ in range(5):
+            a.step(1)
+            b.advect(1.0 / 600.0)
+            b.project(20)
+            b.integrate(1.0 / 600.0)
+        assert np.array_equal(a.get(ca), b.get(cb))
+        assert np.array_equal(a.get(ca, capi.STATE_V), b.get(cb, capi.STATE_V))
+
+
+@pytest.mark.parametrize("tile_sweeps,max_verts", [(1, 1024), (2, 2048), (3, 4096)])
+def test_multi_tile_bit_exact(orc, tile_sweeps, max_verts):
+    # a 96x96 cloth split into dependency-cone tiles must equal the global
+    # colour order exactly
+    n = 96
+    print_case(f"multi-tile ts={tile_sweeps} mv={max_verts}")
+    with capi.Context() as ctx:
+        cid = lib_grid(ctx, n, pins=(0, n - 1))
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=6, check_every=tile_sweeps, tile_sweeps=tile_sweeps,
+                       tile_max_verts=max_verts)
+        ctx.step(1)
+        inf = ctx.info(cid)
+        assert inf.whole == 0 and inf.ntiles > 1 and inf.max_loaded <= max_verts
+        oc = orc_grid(orc, n, pins=(0, n - 1), iterations=6)
+        prm = oc.params()
+        oc.substep(prm, cap=6)
+        for _ in range(7):
+            ctx.step(1)
+            oc.substep(prm, cap=6)
+        x = ctx.get(cid)
+        assert_close(x, oc.x)
+        assert np.array_equal(x, oc.x)
+
+
+def test_residual_driven_sweeps_match(orc):
+    n = 48
+    with capi.Context() as ctx:
+        cid = lib_grid(ctx, n, pins=(0, n - 1))
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=300, tol=1e-4, check_every=5)
+        oc = orc_grid(orc, n, pins=(0, n - 1), iterations=300)
+        prm = oc.params()
+        for _ in range(30):
+            ctx.step(1)
+            ns, r = oc.substep(prm, cap=300, check_every=5, tol=1e-4)
+            assert ctx.last_sweeps(cid) == ns
+        assert np.array_equal(ctx.get(cid), oc.x)
+        m, _ = ctx.residual(cid)
+        mo, _ = oc.residual(oc.x)
+        assert m == mo
+
+
+def test_residual_driven_multi_tile(orc):
+    n = 80
+    with capi.Context() as ctx:
+        cid = lib_grid(ctx, n, pins=(0, n - 1))
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=60, tol=2e-4, check_every=2, tile_max_verts=2048)
+        oc = orc_grid(orc, n, pins=(0, n - 1), iterations=60)
+        prm = oc.params()
+        for _ in range(12):
+            ctx.step(1)
+            ns, _ = oc.substep(prm, cap=60, check_every=2, tol=2e-4)
+            assert ctx.last_sweeps(cid) == ns
+        assert ctx.info(cid).whole == 0
+        assert np.array_equal(ctx.get(cid), oc.x)
+
+
+def test_bending_sphere_bit_exact(orc):
+    # distance + dihedral bending (stiffness 0.1) dropped on a sphere
+    n = 40
+    sphere = ((0.5, -0.3, 0.5), 0.3)
+    with capi.Context() as ctx:
+        cid = lib_grid(ctx, n, pins=(), ct=capi.DISTANCE | capi.BENDING, stiff=(1.0, 0.1, 1.0),
+                       sphere_on=1, sphere_c=sphere[0], sphere_r=sphere[1])
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=20)
+        oc = orc_grid(orc, n, pins=(), bending=True, stiff=(1.0, 0.1, 1.0))
+        prm = oc.params(sphere=sphere)
+        for _ in range(60):
+            oc.substep(prm, cap=20)
+        ctx.step(60)
+        x = ctx.get(cid)
+        assert_close(x, oc.x)
+        assert np.array_equal(x, oc.x)
+        # the cloth actually reached the sphere and stayed outside it
+        d = np.linalg.norm(x - np.array(sphere[0], np.float32), axis=1)
+        assert d.min() >= sphere[1] - 1e-3
+
+
+def test_batch_of_cloths_independent(orc):
+    sizes = [(20, 0), (33, 1), (40, 2)]
+    with capi.Context() as ctx:
+        ids = [lib_grid(ctx, n, seed=s, pins=(0, n - 1)) for n, s in sizes]
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=15)
+        ctx.step(20)
+        for cid, (n, s) in zip(ids, sizes):
+            oc = orc_grid(orc, n, seed=s, pins=(0, n - 1), iterations=15)
+            prm = oc.params()
+            for _ in range(20):
+                oc.substep(prm, cap=15)
+            assert np.array_equal(ctx.get(cid), oc.x)
+
+
+def test_error_codes():
+    with capi.Context() as ctx:
+        with pytest.raises(capi.PbdError) as e:
+            ctx.step(1)
+        assert e.value.code == capi.PBD_ERR_STATE
+        with pytest.raises(capi.PbdError) as e:
+            ctx.add_grid(1, 5)
+        assert e.value.code == capi.PBD_ERR_ARG
+        cid = ctx.add_grid(4, 4)
+        with pytest.raises(capi.PbdError):
+            ctx.set_pins(cid, [99])
+        with pytest.raises(capi.PbdError):
+            ctx.set_solver(dt=-1.0)
+        ctx.init()
+        with pytest.raises(capi.PbdError) as e:
+            ctx.init()
+        assert e.value.code == capi.PBD_ERR_STATE

@@ -1,0 +1,337 @@
+"""Pins the CPU oracle against checks that do not come from the oracle itself:
+closed forms, the reference's own in-code colouring fixture, brute force and
+invariants.  CPU only."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+
+def F(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+# ---------------------------------------------------------------------------
+# mesh / constraint build
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [2, 3, 32, 256, 512])
+def test_grid_counts_closed_form(orc, n):
+    # E = 2n(n-1) + (n-1)^2 and T = 2(n-1)^2 (reference index order
+    # PBD_cuda.cu:19-54, 854-869); BASELINE config 1 quotes 2945 / 1922.
+    pairs = orc.grid_pairs(n, n)
+    _, tris = orc.grid_mesh(n, n)
+    assert len(pairs) == 2 * n * (n - 1) + (n - 1) ** 2
+    assert len(tris) == 2 * (n - 1) ** 2
+    if n == 32:
+        assert (len(pairs), len(tris)) == (2945, 1922)
+    quads = orc.bending_quads(tris)
+    # interior triangle edges = all triangle edges - boundary edges
+    assert len(quads) == 2 * n * (n - 1) + (n - 1) ** 2 - 4 * (n - 1)
+    if n == 256:
+        assert len(quads) == 194565
+
+
+def test_grid_pairs_set_brute_force(orc):
+    rx, ry = 7, 5
+    pairs = orc.grid_pairs(rx, ry)
+    expect = set()
+    for i in range(ry):
+        for j in range(rx):
+            v = i * rx + j
+            if j + 1 < rx:
+                expect.add((v, v + 1))
+            if i + 1 < ry:
+                expect.add((v, v + rx))
+                if j > 0:
+                    expect.add((v, v + rx - 1))
+    assert set(map(tuple, pairs.tolist())) == expect
+    assert len(pairs) == len(expect)
+    # first-vertex order is nondecreasing, as InitDistanceIndices emits it
+    assert np.all(np.diff(pairs[:, 0]) >= 0)
+
+
+def test_grid_positions_and_jitter(orc):
+    pos, _ = orc.grid_mesh(4, 3, 1.0, 2.0)
+    assert np.allclose(pos[:, 1], 0.0)
+    assert np.allclose(pos[5], [1.0 / 3.0, 0.0, 1.0])
+    pj, _ = orc.grid_mesh(4, 3, 1.0, 2.0, jitter=1e-4, seed=7)
+    d = pj - pos
+    assert np.all(np.abs(d) <= 1e-4 + 1e-7) and np.abs(d).max() > 1e-5
+    # same seed -> same jitter; different seed -> different
+    pj2, _ = orc.grid_mesh(4, 3, 1.0, 2.0, jitter=1e-4, seed=7)
+    pj3, _ = orc.grid_mesh(4, 3, 1.0, 2.0, jitter=1e-4, seed=8)
+    assert np.array_equal(pj, pj2) and not np.array_equal(pj, pj3)
+
+
+def test_splitmix64_known_answer(orc):
+    # splitmix64 reference outputs for seed 0 (Vigna's published generator)
+    s = C.c_uint64(0)
+    L = orc.lib()
+    out = [L.orc_splitmix64(C.byref(s)) for _ in range(3)]
+    assert out == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_rest_lengths_closed_form(orc):
+    n = 32
+    pos, _ = orc.grid_mesh(n, n)
+    pairs = orc.grid_pairs(n, n)
+    _, _, rest = orc.build_prims(pos, pairs)
+    h = np.float32(1.0) / np.float32(n - 1)
+    diag = (pairs[:, 1] - pairs[:, 0]) == n - 1
+    # coordinates j*h carry rounding of order ulp(1.0); the bound is absolute
+    assert np.allclose(rest[~diag], h, rtol=0, atol=2.5e-7)
+    assert np.allclose(rest[diag], h * math.sqrt(2.0), rtol=0, atol=4e-7)
+    assert abs(rest[~diag].mean() - h) < 1e-8
+
+
+def test_powf_distance_equals_sqrtf_form(orc):
+    # the reference's powf Distance (PBD_cuda.cu:3-6) and the solver's sqrtf form
+    rng = np.random.default_rng(1)
+    L = orc.lib()
+    a = rng.standard_normal((5000, 3)).astype(np.float32)
+    b = rng.standard_normal((5000, 3)).astype(np.float32)
+    diff = 0
+    for i in range(len(a)):
+        x, y = L.orc_distance_powf(F(a[i]), F(b[i])), L.orc_distance(F(a[i]), F(b[i]))
+        diff += x != y
+        assert abs(x - y) <= 1.2e-7 * abs(x)
+    assert diff <= 5
+
+
+# ---------------------------------------------------------------------------
+# colouring
+# ---------------------------------------------------------------------------
+def test_ground_truth_colouring_kat(orc):
+    # PBDObject::groundTruthTest (PBD_cuda.cu:871-881): edges (0,3),(0,1),(0,2),(1,3),(2,3).
+    # Worked by hand with the bulk-synchronous assign/resolve rules (ties to the
+    # lower prim id): round 1 commits (0,3)=0; round 2 (0,1)=1; round 3 the rest.
+    pairs = np.array([[0, 3], [0, 1], [0, 2], [1, 3], [2, 3]], np.int32)
+    pos = np.zeros((4, 3), np.float32)
+    pl, idx, _ = orc.build_prims(pos + np.arange(4)[:, None], pairs)
+    sn, adj = orc.adjacency(pl, idx, 4)
+    col, rounds = orc.color(pl, sn, adj)
+    assert col.tolist() == [0, 1, 2, 2, 1]
+    assert rounds == 3
+    # reference adjacency order: per point, prims in ascending id, self skipped
+    assert adj[sn[0, 0]:sn[0, 0] + sn[0, 1]].tolist() == [1, 2, 3, 4]
+    assert adj[sn[3, 0]:sn[3, 0] + sn[3, 1]].tolist() == [1, 0, 4]
+
+
+def _brute_adjacency(pl, idx):
+    sets = []
+    verts = [set(idx[s:s + c].tolist()) for s, c in pl]
+    for p in range(len(pl)):
+        sets.append({q for q in range(len(pl)) if q != p and verts[p] & verts[q]})
+    return sets
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_colouring_valid_first_fit_random_mesh(orc, seed):
+    rng = np.random.default_rng(seed)
+    nv = 60
+    tris = np.array([rng.choice(nv, 3, replace=False) for _ in range(90)], np.int32)
+    pos = rng.standard_normal((nv, 3)).astype(np.float32)
+    pairs = np.unique(np.sort(np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]]), 1), axis=0)
+    quads = orc.bending_quads(tris)
+    pl, idx, _ = orc.build_prims(pos, pairs, quads)
+    sn, adj = orc.adjacency(pl, idx, nv)
+    brute = _brute_adjacency(pl, idx)
+    for p in range(len(pl)):
+        got = adj[sn[p, 0]:sn[p, 0] + sn[p, 1]]
+        assert set(got.tolist()) == brute[p]
+    col, rounds = orc.color(pl, sn, adj)
+    assert rounds > 0 and np.all(col >= 0)
+    for p in range(len(pl)):
+        nb = brute[p]
+        assert all(col[q] != col[p] for q in nb)                     # proper
+        assert set(range(col[p])) <= {int(col[q]) for q in nb}       # first fit
+    srt, ws = orc.color_sort(col)
+    assert np.array_equal(srt, np.argsort(col, kind="stable"))
+    assert ws[:, 1].tolist() == np.bincount(col).tolist()
+    assert ws[:, 0].tolist() == np.concatenate([[0], np.cumsum(np.bincount(col))[:-1]]).tolist()
+
+
+def test_colour_sort_single_prim(orc):
+    # EvalWorksets emits nothing for one prim (PBD_cuda.cu:593-615); the oracle does
+    srt, ws = orc.color_sort(np.array([0], np.int32))
+    assert srt.tolist() == [0] and ws.tolist() == [[0, 1]]
+
+
+# ---------------------------------------------------------------------------
+# substep pieces
+# ---------------------------------------------------------------------------
+def test_free_fall_closed_form(orc):
+    # Start at y = 0: PBD re-derives v = (p - x)/dt each substep, so position
+    # rounding (ulp(y)) divided by dt feeds back into v; near 0 it is tiny.
+    pos = np.array([[0.0, 0.0, 0.0], [5.0, 1.0, 0.0]], np.float32)
+    cl = orc.OracleCloth(pos, np.zeros((0, 3), np.int32), np.zeros((0, 2), np.int32), pins=[1])
+    dt, g, N = 1.0 / 600.0, -9.8, 60
+    prm = cl.params(dt=dt, gravity=(0.0, g, 0.0))
+    for _ in range(N):
+        cl.substep(prm, cap=1)
+    # symplectic Euler: y_N = y0 + g dt^2 N(N+1)/2, v_N = g dt N
+    assert cl.x[0, 1] == pytest.approx(g * dt * dt * N * (N + 1) / 2, rel=2e-5)
+    assert cl.v[0, 1] == pytest.approx(g * dt * N, rel=2e-4)
+    assert np.array_equal(cl.x[1], pos[1])  # pinned: bit-identical
+
+
+def test_damping_closed_form(orc):
+    pos = np.zeros((1, 3), np.float32)
+    cl = orc.OracleCloth(pos, np.zeros((0, 3), np.int32), np.zeros((0, 2), np.int32))
+    cl.v[0] = [1.0, 0.0, 0.0]
+    prm = cl.params(dt=0.1, gravity=(0, 0, 0), damping=0.5)
+    cl.substep(prm, cap=1)
+    assert cl.v[0, 0] == pytest.approx(0.5 ** 0.1, rel=1e-6)
+
+
+@pytest.mark.parametrize("w", [(1.0, 1.0), (0.0, 1.0), (1.0, 0.0)])
+def test_distance_projection_two_vertex(orc, w):
+    L = orc.lib()
+    p = np.array([[0.0, 0.0, 0.0], [0.3, 0.4, 0.0]], np.float32)     # d = 0.5
+    ww = np.array(w, np.float32)
+    ids = np.array([0, 1], np.int32)
+    mid0 = (p[0] * ww[1] + p[1] * ww[0]) / (ww[0] + ww[1])
+    L.orc_project_distance(ids.ctypes.data_as(C.POINTER(C.c_int)), 0.25, 1.0, F(ww), F(p))
+    assert np.linalg.norm(p[0] - p[1]) == pytest.approx(0.25, rel=1e-6)
+    # the weighted centre is preserved (momentum conservation)
+    assert np.allclose((p[0] * ww[1] + p[1] * ww[0]) / (ww[0] + ww[1]), mid0, atol=1e-7)
+    if ww[0] == 0:
+        assert np.array_equal(p[0], [0, 0, 0])
+    # stiffness k scales the correction
+    p2 = np.array([[0.0, 0.0, 0.0], [0.3, 0.4, 0.0]], np.float32)
+    L.orc_project_distance(ids.ctypes.data_as(C.POINTER(C.c_int)), 0.25, 0.5, F(ww), F(p2))
+    assert np.linalg.norm(p2[0] - p2[1]) == pytest.approx(0.375, rel=1e-6)
+
+
+def test_stiffness_tables(orc):
+    L = orc.lib()
+    k = np.zeros(5, np.float32)
+    L.orc_stiffness_table_ref(0.1, 5, F(k))
+    # 1 - powf(.) cancels: relative error up to ulp(1)/k ~ 3e-6
+    assert np.allclose(k, [1 - 0.9 ** (1.0 / (i + 1)) for i in range(5)], rtol=1e-5)
+    L.orc_stiffness_table_ref(1.0, 5, F(k))
+    assert np.all(k == 1.0)
+    L.orc_stiffness_table_const(0.1, 20, 5, F(k))
+    assert np.allclose(k, 1 - 0.9 ** (1.0 / 20), rtol=2e-5)
+
+
+def test_sphere_collider_closed_form(orc):
+    L = orc.lib()
+    prm = orc.OracleCloth.params(sphere=((0.0, 0.0, 0.0), 1.0))
+    p = np.array([[0.3, 0.4, 0.0], [0.0, 2.0, 0.0], [0.0, 0.0, 0.0]], np.float32)
+    w = np.array([1, 1, 0], np.float32)
+    L.orc_colliders(C.byref(prm), 3, F(w), F(p))
+    assert np.linalg.norm(p[0]) == pytest.approx(1.0, rel=1e-6)
+    assert np.allclose(p[0] / np.linalg.norm(p[0]), [0.6, 0.8, 0.0], atol=1e-6)
+    assert np.array_equal(p[1], [0, 2, 0]) and np.array_equal(p[2], [0, 0, 0])
+
+
+def test_box_collider_closed_form(orc):
+    L = orc.lib()
+    prm = orc.OracleCloth.params(box=((-1, -1, -1), (1, 1, 1)))
+    p = np.array([[0.9, 0.0, 0.2], [0.0, -0.95, 0.0], [2.0, 0.0, 0.0]], np.float32)
+    w = np.ones(3, np.float32)
+    L.orc_colliders(C.byref(prm), 3, F(w), F(p))
+    assert p[0].tolist() == pytest.approx([1.0, 0.0, 0.2])
+    assert p[1].tolist() == pytest.approx([0.0, -1.0, 0.0])
+    assert p[2].tolist() == [2.0, 0.0, 0.0]
+
+
+def test_residual_brute_force(orc):
+    n = 8
+    pos, tris = orc.grid_mesh(n, n)
+    cl = orc.OracleCloth(pos, tris, orc.grid_pairs(n, n))
+    rng = np.random.default_rng(3)
+    p = (pos + 0.01 * rng.standard_normal(pos.shape)).astype(np.float32)
+    m, rms = cl.residual(p)
+    pairs = orc.grid_pairs(n, n)
+    d = np.linalg.norm(p[pairs[:, 0]].astype(np.float64) - p[pairs[:, 1]].astype(np.float64), axis=1)
+    L = cl.rest[: len(pairs)].astype(np.float64)
+    r = np.abs(d - L) / L
+    assert m == pytest.approx(r.max(), rel=1e-12)
+    assert rms == pytest.approx(np.sqrt(np.mean(r * r)), rel=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# bending
+# ---------------------------------------------------------------------------
+def test_atan2_against_libm(orc):
+    L = orc.lib()
+    rng = np.random.default_rng(0)
+    pts = np.concatenate([rng.standard_normal((20000, 2)), [[0, 1], [1, 0], [0, -1], [-1, 0], [1, 1], [-1e-30, -1]]])
+    for y, x in pts.astype(np.float32):
+        got = L.orc_atan2f(float(y), float(x))
+        assert abs(got - math.atan2(float(y), float(x))) <= 3e-7
+    assert L.orc_atan2f(0.0, 0.0) == 0.0
+
+
+def _dih64(L, q):
+    th = C.c_double()
+    g = np.zeros(12)
+    qs = [np.ascontiguousarray(q[i], np.float64) for i in range(4)]
+    L.orc_dihedral_grad_f64(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in qs], C.byref(th),
+                            g.ctypes.data_as(C.POINTER(C.c_double)))
+    return th.value, g
+
+
+def test_dihedral_gradient_finite_difference(orc):
+    L = orc.lib()
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        q = np.array([[0, 0, 0], [1, 0, 0], [0.4, 0.9, 0.1], [0.6, -0.8, 0.2]], np.float64)
+        q += 0.2 * rng.standard_normal(q.shape)
+        th, g = _dih64(L, q)
+        fd = np.zeros(12)
+        h = 1e-6
+        for k in range(12):
+            qp, qm = q.copy(), q.copy()
+            qp.flat[k] += h
+            qm.flat[k] -= h
+            fd[k] = (_dih64(L, qp)[0] - _dih64(L, qm)[0]) / (2 * h)
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-6)
+        assert np.allclose(g.reshape(4, 3).sum(0), 0.0, atol=1e-9)  # translation invariance
+        # FP32 angle and gradient agree with FP64
+        q32 = q.astype(np.float32)
+        a32 = L.orc_dihedral(*[F(np.ascontiguousarray(q32[i])) for i in range(4)])
+        assert abs(a32 - th) < 2e-5
+
+
+def test_dihedral_flat_zero_and_sign(orc):
+    L = orc.lib()
+    e0, e1 = np.array([0, 0, 0], np.float32), np.array([1, 0, 0], np.float32)
+    o0, o1 = np.array([0.5, 1, 0], np.float32), np.array([0.5, -1, 0], np.float32)
+    assert L.orc_dihedral(F(e0), F(e1), F(o0), F(o1)) == 0.0
+    # lifting o0 by h gives theta ~ +h (o0 is at distance 1 from the hinge)
+    o0h = np.array([0.5, 1, 0.01], np.float32)
+    assert L.orc_dihedral(F(e0), F(e1), F(o0h), F(o1)) == pytest.approx(math.atan2(0.01, 1.0), rel=1e-4)
+
+
+def test_bending_projection_invariants(orc):
+    L = orc.lib()
+    rng = np.random.default_rng(9)
+    ids = np.arange(4, dtype=np.int32)
+    ip = ids.ctypes.data_as(C.POINTER(C.c_int))
+    w = np.ones(4, np.float32)
+    for _ in range(20):
+        q = np.array([[0, 0, 0], [1, 0, 0], [0.5, 1, 0.3], [0.5, -1, 0.1]], np.float32)
+        q += (0.1 * rng.standard_normal(q.shape)).astype(np.float32)
+        th = L.orc_dihedral(*[F(np.ascontiguousarray(q[i])) for i in range(4)])
+        # zero correction at rest
+        q0 = q.copy()
+        L.orc_project_bending(ip, th, 1.0, F(w), F(q0))
+        assert np.array_equal(q0, q)
+        # toward the rest angle, momentum conserved, angle error reduced
+        q1 = q.copy()
+        L.orc_project_bending(ip, th + 0.2, 1.0, F(w), F(q1))
+        dp = (q1 - q).astype(np.float64)
+        assert np.allclose(dp.sum(0), 0.0, atol=1e-6)
+        th1 = L.orc_dihedral(*[F(np.ascontiguousarray(q1[i])) for i in range(4)])
+        assert abs(th1 - (th + 0.2)) < 0.2 * 0.25
+        # rigid motion leaves the angle unchanged
+        Rm = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+        if np.linalg.det(Rm) < 0:
+            Rm[:, 0] *= -1
+        qr = (q.astype(np.float64) @ Rm.T + [0.3, -2, 1]).astype(np.float32)
+        thr = L.orc_dihedral(*[F(np.ascontiguousarray(qr[i])) for i in range(4)])
+        assert abs(thr - th) < 1e-5
