@@ -710,6 +710,7 @@ int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* p) {
             set_device(ctx);
             fill_ktabs(ctx);
             fill_cloth_dev(ctx);
+            replan_if_needed(ctx);
         }
     });
 }
@@ -751,6 +752,7 @@ int pbd_step(pbd_ctx* ctx, int substeps) {
         PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
         PBD_CHECK(substeps >= 0, PBD_ERR_ARG, "substeps must be >= 0");
         set_device(ctx);
+        replan_if_needed(ctx);
         float dt = ctx->sp.dt;
         for (int s = 0; s < substeps; s++) {
             if (ctx->sp.path == PBD_PATH_GLOBAL) {

@@ -413,6 +413,7 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, TilePlan& 
         Cloth& c = cloths[ci];
         per[ci].cloth = (int)ci;
         c.whole = c.nv <= max_load;
+        c.max_loaded = 0;
         if (c.whole) {
             TileOut o;
             o.load.resize(c.nv);
