@@ -153,6 +153,24 @@ PBD_API int pbd_cloth_residual(pbd_ctx* ctx, int cloth, double* max_rel, double*
 /* Sweeps the last projection of this cloth ran. */
 PBD_API int pbd_cloth_last_sweeps(pbd_ctx* ctx, int cloth, int* sweeps);
 
+/* ---- measurement (no counterpart in the reference, whose only timing is the
+ * host Timer around calls, timer.cpp:10-20) --------------------------------- */
+typedef struct {
+    double tile_ms;          /* summed duration of LDS-tile kernel launches (HIP events on the context stream) */
+    long long tile_launches;
+    double tile_alg_bytes;   /* algorithmic bytes of those launches (68 B/distance prim/sweep,
+                                132 B/bending prim/sweep, 100 B/vertex for fused advect+integrate) */
+    double step_ms;          /* summed device-synchronised wall time of pbd_step calls */
+    long long substeps;
+    long long vertex_sweeps; /* sum over cloths and substeps of V x sweeps */
+    long long sweeps_max;    /* largest per-cloth sweep count seen */
+} pbd_profile;
+PBD_API int pbd_profile_enable(pbd_ctx* ctx, int on);
+PBD_API int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset);
+/* Plain device copy of `bytes` (read once, written once), `reps` times; the
+ * HBM calibration kernel for achievable bandwidth and counter scaling. */
+PBD_API int pbd_bench_copy(pbd_ctx* ctx, long long bytes, int reps, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

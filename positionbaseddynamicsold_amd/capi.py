@@ -46,6 +46,12 @@ class SolverParams(C.Structure):
                 ("block_threads", C.c_int)]
 
 
+class Profile(C.Structure):
+    _fields_ = [("tile_ms", C.c_double), ("tile_launches", C.c_longlong), ("tile_alg_bytes", C.c_double),
+                ("step_ms", C.c_double), ("substeps", C.c_longlong), ("vertex_sweeps", C.c_longlong),
+                ("sweeps_max", C.c_longlong)]
+
+
 class ClothInfo(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("nv", "nt", "ne", "nb", "nprims", "ncolors", "color_rounds", "adj_len",
                                        "ntiles", "whole", "max_loaded")]
@@ -84,6 +90,9 @@ SIGNATURES = {
     "pbd_cloth_set_state": [_P, _I, _I, _fp],
     "pbd_cloth_residual": [_P, _I, _dp, _dp],
     "pbd_cloth_last_sweeps": [_P, _I, _ip],
+    "pbd_profile_enable": [_P, _I],
+    "pbd_profile_read": [_P, C.POINTER(Profile), _I],
+    "pbd_bench_copy": [_P, C.c_longlong, _I, _dp],
 }
 
 _lib = None
@@ -267,6 +276,19 @@ class Context:
         m, r = C.c_double(), C.c_double()
         check(self.lib.pbd_cloth_residual(self.h, cloth, C.byref(m), C.byref(r)))
         return m.value, r.value
+
+    def profile(self, on: bool = True):
+        check(self.lib.pbd_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self, reset: bool = True) -> Profile:
+        p = Profile()
+        check(self.lib.pbd_profile_read(self.h, C.byref(p), 1 if reset else 0))
+        return p
+
+    def bench_copy(self, nbytes: int, reps: int = 20) -> float:
+        ms = C.c_double()
+        check(self.lib.pbd_bench_copy(self.h, nbytes, reps, C.byref(ms)))
+        return ms.value
 
     def last_sweeps(self, cloth: int) -> int:
         s = C.c_int()

@@ -1,9 +1,9 @@
 // extern "C" boundary of libpbd.so: context, model build, init pipeline and
 // solver orchestration.  Exceptions never cross this file's functions.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
-#include <mutex>
 
 #include "pbd_internal.hpp"
 
@@ -46,6 +46,14 @@ struct pbd_ctx {
     DBuf<double> rsum;
     DBuf<int> sweeps;
     int nslots = 0;
+    // profiling
+    bool prof_on = false;
+    pbd_profile prof{};
+    std::vector<hipEvent_t> ev_pool;
+    int ev_used = 0;
+    ~pbd_ctx() {
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
@@ -359,6 +367,30 @@ void do_init(pbd_ctx* c, int color_rounds) {
     c->inited = true;
 }
 
+// ---- profiling -------------------------------------------------------------
+hipEvent_t next_event(pbd_ctx* c) {
+    if (c->ev_used == (int)c->ev_pool.size()) {
+        hipEvent_t e;
+        PBD_HIP(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// Sums the durations of the recorded (start, stop) pairs; call after a sync.
+void collect_events(pbd_ctx* c) {
+    for (int i = 0; i + 1 < c->ev_used; i += 2) {
+        float ms = 0.0f;
+        PBD_HIP(hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
+        c->prof.tile_ms += ms;
+        c->prof.tile_launches++;
+    }
+    c->ev_used = 0;
+}
+
+// Algorithmic bytes per prim per sweep and per vertex (fused advect + integrate).
+constexpr double kDistBytes = 68.0, kBendBytes = 132.0, kVertBytes = 100.0;
+
 // ---- solver orchestration ---------------------------------------------------
 void advect_all(pbd_ctx* c, float dt) {
     // the kernel constants hold g*dt for sp.dt; recompute when dt differs
@@ -418,6 +450,14 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
     size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
     int threads = c->sp.block_threads > 0 ? c->sp.block_threads : 1024;
     int cur = c->pcur;
+    auto tiles = [&](int nblocks, const int* order, const float4* pin, float4* pout, const SolveArgs& sa) {
+        if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
+        launch_tiles(s, nblocks, order, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr, c->tbend.ptr,
+                     c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr,
+                     c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, sa, threads,
+                     lds);
+        if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
+    };
     SolveArgs a{};
     a.dt = dt;
     a.check_every = ce;
@@ -430,10 +470,7 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
         a.chunk = 0;
         // whole tiles write pbuf[cur ^ 1]; copied below if the partial chain
         // ends in the other buffer
-        launch_tiles(s, (int)c->order_whole.n, c->order_whole.ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr,
-                     c->tdist_rest.ptr, c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr,
-                     c->cl.ptr, c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->pbuf[cur].ptr,
-                     c->pbuf[cur ^ 1].ptr, c->res_slots.ptr, ncl, c->sweeps.ptr, a, threads, lds);
+        tiles((int)c->order_whole.n, c->order_whole.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
     }
     if (c->any_partial) {
         int sw = 0;
@@ -442,11 +479,7 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
             a.nsweeps = std::min(ts, iterations - sw);
             a.fused = 0;
             a.chunk = ch;
-            launch_tiles(s, (int)c->order_partial.n, c->order_partial.ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr,
-                         c->tdist_rest.ptr, c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr,
-                         c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr,
-                         c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, c->res_slots.ptr, ncl, c->sweeps.ptr, a, threads,
-                         lds);
+            tiles((int)c->order_partial.n, c->order_partial.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
             cur ^= 1;
             sw += a.nsweeps;
             if (tol >= 0.0 && ch + 1 < nchunks) {
@@ -481,6 +514,17 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
     c->sweeps.download(sv, s);
     PBD_HIP(hipStreamSynchronize(s));
     for (int ci = 0; ci < ncl; ci++) c->cloths[ci].last_sweeps = sv[ci];
+    if (c->prof_on) {
+        collect_events(c);
+        double bytes = 0.0;
+        for (auto& k : c->cloths) {
+            bytes += (double)k.last_sweeps * (kDistBytes * k.ne + kBendBytes * k.nb);
+            if (k.whole && fused_whole) bytes += kVertBytes * k.nv;
+            c->prof.vertex_sweeps += (long long)k.nv * k.last_sweeps;
+            c->prof.sweeps_max = std::max<long long>(c->prof.sweeps_max, k.last_sweeps);
+        }
+        c->prof.tile_alg_bytes += bytes;
+    }
 }
 
 void integrate_all(pbd_ctx* c, float dt, bool skip_whole) {
@@ -754,6 +798,7 @@ int pbd_step(pbd_ctx* ctx, int substeps) {
         set_device(ctx);
         replan_if_needed(ctx);
         float dt = ctx->sp.dt;
+        auto t0 = std::chrono::steady_clock::now();
         for (int s = 0; s < substeps; s++) {
             if (ctx->sp.path == PBD_PATH_GLOBAL) {
                 advect_all(ctx, dt);
@@ -768,6 +813,50 @@ int pbd_step(pbd_ctx* ctx, int substeps) {
                 integrate_all(ctx, dt, true);
             }
         }
+        if (ctx->prof_on) {
+            PBD_HIP(hipStreamSynchronize(ctx->stream));
+            ctx->prof.step_ms +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            ctx->prof.substeps += substeps;
+        }
+    });
+}
+
+int pbd_profile_enable(pbd_ctx* ctx, int on) {
+    return guard([&] {
+        PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
+        ctx->prof_on = on != 0;
+    });
+}
+
+int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset) {
+    return guard([&] {
+        PBD_CHECK(ctx && out, PBD_ERR_ARG, "null argument");
+        *out = ctx->prof;
+        if (reset) ctx->prof = pbd_profile{};
+    });
+}
+
+int pbd_bench_copy(pbd_ctx* ctx, long long bytes, int reps, double* ms) {
+    return guard([&] {
+        PBD_CHECK(ctx && ms, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(bytes >= 16 && reps >= 1, PBD_ERR_ARG, "bad copy size");
+        set_device(ctx);
+        size_t n = (size_t)bytes / 16;
+        DBuf<float4> a, b;
+        a.alloc(n);
+        b.alloc(n);
+        PBD_HIP(hipMemsetAsync(a.ptr, 0, n * 16, ctx->stream));
+        launch_copy(ctx->stream, a.ptr, b.ptr, n);  // warm-up
+        hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
+        ctx->ev_used -= 2;
+        PBD_HIP(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; r++) launch_copy(ctx->stream, a.ptr, b.ptr, n);
+        PBD_HIP(hipEventRecord(e1, ctx->stream));
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+        float t = 0.0f;
+        PBD_HIP(hipEventElapsedTime(&t, e0, e1));
+        *ms = (double)t / reps;
     });
 }
 

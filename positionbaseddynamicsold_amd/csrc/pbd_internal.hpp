@@ -206,5 +206,6 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   int nslots_cloth, int* sweeps_out, const SolveArgs& a, int threads,
                   size_t lds_bytes);
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
+void launch_copy(hipStream_t s, const float4* a, float4* b, size_t n);
 
 }  // namespace pbd

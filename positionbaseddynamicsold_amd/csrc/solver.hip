@@ -191,6 +191,20 @@ void launch_residual(hipStream_t s, const int4* dist, int n, const float4* p, un
     PBD_HIP(hipGetLastError());
 }
 
+// Plain streaming copy (HBM calibration): each float4 read once, written once.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) b[i] = a[i];
+}
+
+void launch_copy(hipStream_t s, const float4* a, float4* b, size_t n) {
+    if (!n) return;
+    size_t blocks = std::min<size_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n);
+    PBD_HIP(hipGetLastError());
+}
+
 __global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
