@@ -153,6 +153,57 @@ PBD_API int pbd_cloth_residual(pbd_ctx* ctx, int cloth, double* max_rel, double*
 /* Sweeps the last projection of this cloth ran. */
 PBD_API int pbd_cloth_last_sweeps(pbd_ctx* ctx, int cloth, int* sweeps);
 
+/* ---- broadphase + CCD ---------------------------------------------------- */
+/* SpatialHashSystem(pos, tris, ht, gridCenter, gridSize, cellSize) grid
+ * (SpatialHashSystem.h:15, SpatialHashSystem.cpp:49-66,88-98).  mode 0 is the
+ * reference's dense grid (key (z*gy+y)*gx+x, -1 outside); mode 1 hashes cell
+ * coordinates into a 2^table_bits table (every cell has a slot). */
+typedef struct {
+    int mode;
+    float centre[3];
+    int size[3];
+    float cell[3];
+    int table_bits;
+} pbd_hash_desc;
+
+/* SpatialHashSystem::InitSH + UpdateSH on caller positions
+ * (SpatialHashSystem.cpp:88-121): keys, stable sort, cell ranges. */
+PBD_API int pbd_sh_build(pbd_ctx* ctx, const pbd_hash_desc* desc, const float* pos, int nv, const int* tris,
+                         int nt);
+PBD_API int pbd_sh_num_cells(pbd_ctx* ctx, int* ncells);
+/* m_hTriangleHash/m_hTriangleId after sortTrianglesCPU, the unsorted keys of
+ * calcHashCPU, and m_hCellStart/m_hCellEnd (SpatialHashSystem.cpp:300-389). */
+PBD_API int pbd_sh_get(pbd_ctx* ctx, int* sorted_keys, int* sorted_ids, int* raw_keys, int* cell_start,
+                       int* cell_end);
+/* FindNeighbors over every triangle as CCD_SH_Extended consumes it
+ * (SpatialHashSystem.cpp:134-187, CCD.cpp:1541-1586): the set of candidate
+ * (vertex, face) pairs, v not in f, as sorted keys (v << 32 | f). */
+PBD_API int pbd_sh_pairs(pbd_ctx* ctx, long long cap, int64_t* keys, long long* count);
+/* CCD_SH_Extended(contactData, shs, meshTopol, prdPBuffer, thickness)
+ * (CCD.cpp:1522-1587): x = start-of-step positions, p = predicted. */
+PBD_API int pbd_ccd_detect(pbd_ctx* ctx, const pbd_hash_desc* desc, const float* x, const float* p, int nv,
+                           const int* tris, int nt, float thickness, long long* ncontacts);
+/* ContactData (CCD_Basic.h:12-31) of the last detection, ascending (v, f). */
+PBD_API int pbd_ccd_get_contacts(pbd_ctx* ctx, long long cap, int* vf, float* t, float* n, double* w);
+/* CollisionResolveNew(meshTopol, prdPBuffer, contactData, iterations, thickness)
+ * (CCD.cpp:1874-1994) on the last detection, Jacobi form; p is updated in
+ * place, w[v] == 0 marks static vertices. */
+PBD_API int pbd_ccd_resolve(pbd_ctx* ctx, float* p, const float* w, float thickness, int passes);
+
+/* In-solver collision (SolverPBD::ProjectConstraintWithColli, PBD_cuda.cu:1130-1210):
+ * detection every `detect_every` sweeps and at the last sweep, contact
+ * response after every sweep, over all cloths of the context. */
+typedef struct {
+    int enabled;
+    pbd_hash_desc grid;      /* mode 1: centre ignored, cell from cell_scale when > 0 */
+    float cell_scale;        /* cell = cell_scale x mean rest length (0 = use grid.cell) */
+    float thickness;
+    int resolve_passes;
+    int detect_every;
+} pbd_collision_params;
+PBD_API int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* params);
+PBD_API int pbd_last_contacts(pbd_ctx* ctx, long long* ncontacts, long long* npairs);
+
 /* ---- measurement (no counterpart in the reference, whose only timing is the
  * host Timer around calls, timer.cpp:10-20) --------------------------------- */
 typedef struct {

@@ -303,3 +303,95 @@ class OracleCloth:
         arr = self.p if p is None else np.ascontiguousarray(p, np.float32)
         self.L.orc_residual(C.byref(self.c), F(arr), C.byref(m), C.byref(s), C.byref(n))
         return m.value, (s.value / n.value) ** 0.5 if n.value else 0.0
+
+
+# ---------------------------------------------------------------------------
+# Broadphase / CCD
+# ---------------------------------------------------------------------------
+class OrcContact(C.Structure):
+    _fields_ = [("v", C.c_int), ("f", C.c_int), ("t", C.c_float), ("n", C.c_float * 3), ("w", C.c_double * 4)]
+
+
+def _sig_collide(L):
+    L.orc_ccd_detect.restype = C.c_longlong
+    L.orc_ccd_detect.argtypes = [C.POINTER(OrcGrid), _fp, _fp, _ip, C.c_int, C.c_int, C.c_float, C.c_longlong,
+                                 C.POINTER(OrcContact)]
+    L.orc_ccd_resolve.restype = None
+    L.orc_ccd_resolve.argtypes = [C.POINTER(OrcContact), C.c_longlong, _ip, _fp, _fp, _fp, C.c_float, C.c_int]
+
+
+def grid(mode=0, centre=(0.0, 0.0, 0.0), size=(1, 1, 1), cell=(1.0, 1.0, 1.0), table_bits=16) -> OrcGrid:
+    L = lib()
+    g = OrcGrid()
+    c = np.asarray(centre, np.float32)
+    s = np.asarray(size, np.int32)
+    cl = np.asarray(cell, np.float32)
+    L.orc_grid_init(C.byref(g), F(c), I(s), F(cl))
+    g.mode = mode
+    g.table_bits = table_bits
+    return g
+
+
+def tri_keys(g, pos, tris):
+    L = lib()
+    pos = np.ascontiguousarray(pos, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    keys = np.zeros(len(tris), np.int32)
+    ids = np.zeros(len(tris), np.int32)
+    L.orc_tri_keys(C.byref(g), F(pos), I(tris), len(tris), I(keys), I(ids))
+    return keys, ids
+
+
+def sorted_keys(keys, ids):
+    L = lib()
+    k, i = keys.copy(), ids.copy()
+    L.orc_stable_sort_pairs(I(k), I(i), len(k))
+    return k, i
+
+
+def cell_ranges(g, sk):
+    L = lib()
+    nc = L.orc_num_cells(C.byref(g))
+    cs, ce = np.zeros(nc, np.int32), np.zeros(nc, np.int32)
+    L.orc_cell_ranges(I(np.ascontiguousarray(sk, np.int32)), len(sk), nc, I(cs), I(ce))
+    return cs, ce
+
+
+def candidate_pairs(g, pos, tris, nv=None):
+    L = lib()
+    pos = np.ascontiguousarray(pos, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    nv = len(pos) if nv is None else nv
+    n = L.orc_candidate_pairs(C.byref(g), F(pos), I(tris), len(tris), nv, 0, None)
+    out = np.zeros(max(n, 1), np.int64)
+    L.orc_candidate_pairs(C.byref(g), F(pos), I(tris), len(tris), nv, n, out.ctypes.data_as(_i64p))
+    return out[:n]
+
+
+def ccd_detect(g, x, p, tris, thickness):
+    L = lib()
+    _sig_collide(L)
+    x = np.ascontiguousarray(x, np.float32)
+    p = np.ascontiguousarray(p, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    n = L.orc_ccd_detect(C.byref(g), F(x), F(p), I(tris), len(tris), len(x), thickness, 0, None)
+    arr = (OrcContact * max(n, 1))()
+    L.orc_ccd_detect(C.byref(g), F(x), F(p), I(tris), len(tris), len(x), thickness, n, arr)
+    return arr, n
+
+
+def contacts_arrays(arr, n):
+    vf = np.array([[arr[i].v, arr[i].f] for i in range(n)], np.int32).reshape(-1, 2)
+    t = np.array([arr[i].t for i in range(n)], np.float32)
+    nn = np.array([list(arr[i].n) for i in range(n)], np.float32).reshape(-1, 3)
+    w = np.array([list(arr[i].w) for i in range(n)], np.float64).reshape(-1, 4)
+    return vf, t, nn, w
+
+
+def ccd_resolve(arr, n, tris, x, p, w, thickness, passes):
+    L = lib()
+    _sig_collide(L)
+    p = np.ascontiguousarray(p, np.float32).copy()
+    L.orc_ccd_resolve(arr, n, I(np.ascontiguousarray(tris, np.int32)), F(np.ascontiguousarray(x, np.float32)),
+                      F(p), F(np.ascontiguousarray(w, np.float32)), thickness, passes)
+    return p

@@ -834,13 +834,15 @@ double orc_newtons_method(double a, double b, double c, double d, double x0, int
 }
 
 /* The reference leaves xc[0] unset when solve_quadratic returns 0 roots from
- * its d >= 0 branch (PBD/CCD.cpp:51-54,99-113); that case is defined here as
- * "no interior root" (t = 0 and t = 1 are still examined by the caller). */
+ * its d >= 0 branch (PBD/CCD.cpp:51-54,99-113), which happens when a = b = 0
+ * (the cubic is linear, e.g. a vertex translating against a static face).
+ * Newton's method on a linear function converges from any finite start, so
+ * the unset start is defined as 0. */
 int orc_solve_cubic(double a, double b, double c, double d, double x[3]) {
     double xc[2] = {NAN, NAN};
     int ncrit = orc_solve_quadratic(3 * a, 2 * b, c, xc);
     if (ncrit == 0) {
-        if (xc[0] != xc[0]) return 0;
+        if (xc[0] != xc[0]) xc[0] = 0.0;
         x[0] = orc_newtons_method(a, b, c, d, xc[0], 0);
         return 1;
     } else if (ncrit == 1) {
@@ -931,7 +933,9 @@ int orc_vf_ccd(const float* x0, const float* x1, const float* x2, const float* x
         v3sub(c0, c1, rel);
         double dist = v3dot(rel, cn);
         orc_barycentric(c0, c1, c2, c3, wgt);
-        float ed1 = orc_distance_powf(c1, c2), ed2 = orc_distance_powf(c1, c3), ed3 = orc_distance_powf(c2, c3);
+        /* IO::Distance (powf form) in the reference; sqrtf form here, the
+         * same value up to a rare last-bit difference (pinned by test) */
+        float ed1 = orc_distance(c1, c2), ed2 = orc_distance(c1, c3), ed3 = orc_distance(c2, c3);
         float eps = thickness / ((ed1 + ed2 + ed3) / 3.0);
         int in1 = (wgt[0] <= (1 + eps)) && (wgt[0] >= (-eps));
         int in2 = (wgt[1] <= (1 + eps)) && (wgt[1] >= (-eps));
@@ -988,6 +992,97 @@ void orc_vf_resolve(const float* x0, const float* x1, const float* x2, const flo
         p2[k] = p2[k] + (ffix * wgt[1]) / sw;
         p3[k] = p3[k] + (ffix * wgt[2]) / sw;
     }
+}
+
+/* Narrowphase over the candidate set (ExtendedVFTest per candidate, as
+ * CCD_SH_Extended does, PBD/CCD.cpp:1541-1586).  The reference appends
+ * contacts in discovery order and de-duplicates by linear scan (IsDuplicated,
+ * PBD/CCD.cpp:132-147); here the candidate set is already unique and sorted,
+ * so the contact list is in ascending (v, f) order.  x = start-of-substep
+ * positions (meshTopol.posBuffer), p = current predicted positions. */
+long long orc_ccd_detect(const orc_grid* g, const float* x, const float* p, const int* tris, int nt,
+                         int nv, float thickness, long long cap, orc_contact* out) {
+    long long nk = orc_candidate_pairs(g, p, tris, nt, nv, 0, NULL);
+    int64_t* keys = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nk + 1));
+    orc_candidate_pairs(g, p, tris, nt, nv, nk, keys);
+    long long nc = 0;
+    for (long long i = 0; i < nk; i++) {
+        int v = (int)(keys[i] >> 32), f = (int)(keys[i] & 0xffffffff);
+        const int* fv = tris + 3 * f;
+        float t, n[3];
+        double w[4];
+        if (orc_vf_ccd(x + 3 * v, x + 3 * fv[0], x + 3 * fv[1], x + 3 * fv[2], p + 3 * v, p + 3 * fv[0],
+                       p + 3 * fv[1], p + 3 * fv[2], thickness, &t, n, w)) {
+            if (out && nc < cap) {
+                orc_contact* c = out + nc;
+                c->v = v; c->f = f; c->t = t;
+                c->n[0] = n[0]; c->n[1] = n[1]; c->n[2] = n[2];
+                for (int k = 0; k < 4; k++) c->w[k] = w[k];
+            }
+            nc++;
+        }
+    }
+    free(keys);
+    return nc;
+}
+
+/* Fixed-point accumulation unit: 2^32 per metre.  Integer sums do not depend
+ * on the order of accumulation, which keeps a parallel accumulation
+ * deterministic and equal to this sequential one. */
+static long long to_fixed(float f) { return (long long)rint((double)f * 4294967296.0); }
+
+/* Contact response, Jacobi form of CollisionResolveNew (PBD/CCD.cpp:1874-1994):
+ * each pass evaluates VFResolveTest/VFResolveNew (PBD/CCD.cpp:1996-2090) for
+ * every contact against the positions at the start of the pass, sums the
+ * per-vertex fixes (fixed point) and moves every vertex by the mean of its
+ * fixes.  The reference runs the same per-contact rule Gauss-Seidel style in
+ * discovery order.  Static parts (w == 0): a pinned vertex or face vertex is
+ * not moved, and the other side takes the full separation (twice the
+ * reference's half depth). */
+void orc_ccd_resolve(const orc_contact* c, long long nc, const int* tris, const float* x, float* p,
+                     const float* w, float thickness, int passes) {
+    (void)w;
+    int nv = 0;
+    for (long long i = 0; i < nc; i++) {
+        const int* fv = tris + 3 * c[i].f;
+        int m = c[i].v;
+        for (int k = 0; k < 3; k++) if (fv[k] > m) m = fv[k];
+        if (m + 1 > nv) nv = m + 1;
+    }
+    long long* acc = (long long*)malloc(sizeof(long long) * 3 * (size_t)(nv + 1));
+    int* cnt = (int*)malloc(sizeof(int) * (size_t)(nv + 1));
+    float* q = (float*)malloc(sizeof(float) * 12);
+    for (int pass = 0; pass < passes; pass++) {
+        memset(acc, 0, sizeof(long long) * 3 * (size_t)nv);
+        memset(cnt, 0, sizeof(int) * (size_t)nv);
+        for (long long i = 0; i < nc; i++) {
+            int id[4] = {c[i].v, tris[3 * c[i].f], tris[3 * c[i].f + 1], tris[3 * c[i].f + 2]};
+            for (int k = 0; k < 4; k++) {
+                q[3 * k] = p[3 * id[k]]; q[3 * k + 1] = p[3 * id[k] + 1]; q[3 * k + 2] = p[3 * id[k] + 2];
+            }
+            const float *x0 = x + 3 * id[0], *x1 = x + 3 * id[1], *x2 = x + 3 * id[2], *x3 = x + 3 * id[3];
+            if (!orc_vf_resolve_test(x0, x1, x2, x3, q, q + 3, q + 6, q + 9, thickness)) continue;
+            float before[12];
+            memcpy(before, q, sizeof(before));
+            orc_vf_resolve(x0, x1, x2, x3, q, q + 3, q + 6, q + 9, thickness);
+            float wv = w[id[0]];
+            int face_static = (w[id[1]] == 0.0f) && (w[id[2]] == 0.0f) && (w[id[3]] == 0.0f);
+            for (int k = 0; k < 4; k++) {
+                float sc = 1.0f;
+                if (w[id[k]] == 0.0f) continue;
+                if (k == 0 && face_static) sc = 2.0f;
+                if (k > 0 && wv == 0.0f) sc = 2.0f;
+                for (int a = 0; a < 3; a++) acc[3 * id[k] + a] += to_fixed((q[3 * k + a] - before[3 * k + a]) * sc);
+                cnt[id[k]]++;
+            }
+        }
+        for (int v = 0; v < nv; v++) {
+            if (!cnt[v]) continue;
+            for (int a = 0; a < 3; a++)
+                p[3 * v + a] = p[3 * v + a] + (float)((double)acc[3 * v + a] / 4294967296.0 / (double)cnt[v]);
+        }
+    }
+    free(acc); free(cnt); free(q);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -131,6 +131,20 @@ int orc_vf_resolve_test(const float* x0, const float* x1, const float* x2, const
 void orc_vf_resolve(const float* x0, const float* x1, const float* x2, const float* x3,
                     float* p0, float* p1, float* p2, float* p3, float thickness);
 
+/* Contact list: candidates that pass the narrowphase, in ascending (v, f)
+ * key order.  f indexes the triangle array. */
+typedef struct {
+    int v, f;
+    float t;
+    float n[3];
+    double w[4];
+} orc_contact;
+
+long long orc_ccd_detect(const orc_grid* g, const float* x, const float* p, const int* tris, int nt,
+                         int nv, float thickness, long long cap, orc_contact* out);
+void orc_ccd_resolve(const orc_contact* c, long long nc, const int* tris, const float* x, float* p,
+                     const float* w, float thickness, int passes);
+
 /* ---------------- warm-start MLP, FP64 (new) ------------------------------ */
 #define ORC_MLP_IN 22
 #define ORC_MLP_H 64

@@ -46,6 +46,20 @@ class SolverParams(C.Structure):
                 ("block_threads", C.c_int)]
 
 
+class HashDesc(C.Structure):
+    _fields_ = [("mode", C.c_int), ("centre", C.c_float * 3), ("size", C.c_int * 3), ("cell", C.c_float * 3),
+                ("table_bits", C.c_int)]
+
+
+def hash_desc(mode=0, centre=(0.0, 0.0, 0.0), size=(1, 1, 1), cell=(1.0, 1.0, 1.0), table_bits=16) -> HashDesc:
+    return HashDesc(mode, (C.c_float * 3)(*centre), (C.c_int * 3)(*size), (C.c_float * 3)(*cell), table_bits)
+
+
+class CollisionParams(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("grid", HashDesc), ("cell_scale", C.c_float), ("thickness", C.c_float),
+                ("resolve_passes", C.c_int), ("detect_every", C.c_int)]
+
+
 class Profile(C.Structure):
     _fields_ = [("tile_ms", C.c_double), ("tile_launches", C.c_longlong), ("tile_alg_bytes", C.c_double),
                 ("step_ms", C.c_double), ("substeps", C.c_longlong), ("vertex_sweeps", C.c_longlong),
@@ -90,6 +104,15 @@ SIGNATURES = {
     "pbd_cloth_set_state": [_P, _I, _I, _fp],
     "pbd_cloth_residual": [_P, _I, _dp, _dp],
     "pbd_cloth_last_sweeps": [_P, _I, _ip],
+    "pbd_sh_build": [_P, C.POINTER(HashDesc), _fp, _I, _ip, _I],
+    "pbd_sh_num_cells": [_P, _ip],
+    "pbd_sh_get": [_P, _ip, _ip, _ip, _ip, _ip],
+    "pbd_sh_pairs": [_P, C.c_longlong, C.POINTER(C.c_int64), C.POINTER(C.c_longlong)],
+    "pbd_ccd_detect": [_P, C.POINTER(HashDesc), _fp, _fp, _I, _ip, _I, _F, C.POINTER(C.c_longlong)],
+    "pbd_ccd_get_contacts": [_P, C.c_longlong, _ip, _fp, _fp, _dp],
+    "pbd_ccd_resolve": [_P, _fp, _fp, _F, _I],
+    "pbd_set_collision": [_P, C.POINTER(CollisionParams)],
+    "pbd_last_contacts": [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
     "pbd_profile_enable": [_P, _I],
     "pbd_profile_read": [_P, C.POINTER(Profile), _I],
     "pbd_bench_copy": [_P, C.c_longlong, _I, _dp],
@@ -276,6 +299,61 @@ class Context:
         m, r = C.c_double(), C.c_double()
         check(self.lib.pbd_cloth_residual(self.h, cloth, C.byref(m), C.byref(r)))
         return m.value, r.value
+
+    # ---- broadphase / CCD ---------------------------------------------
+    def sh_build(self, desc: HashDesc, pos: np.ndarray, tris: np.ndarray):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        check(self.lib.pbd_sh_build(self.h, C.byref(desc), fptr(pos), len(pos), iptr(tris), len(tris)))
+        self._sh_nt = len(tris)
+
+    def sh_get(self):
+        nc = C.c_int()
+        check(self.lib.pbd_sh_num_cells(self.h, C.byref(nc)))
+        nt = self._sh_nt
+        keys, ids, raw = (np.zeros(max(nt, 1), np.int32) for _ in range(3))
+        cs, ce = np.zeros(nc.value, np.int32), np.zeros(nc.value, np.int32)
+        check(self.lib.pbd_sh_get(self.h, iptr(keys), iptr(ids), iptr(raw), iptr(cs), iptr(ce)))
+        return keys[:nt], ids[:nt], raw[:nt], cs, ce
+
+    def sh_pairs(self) -> np.ndarray:
+        n = C.c_longlong()
+        check(self.lib.pbd_sh_pairs(self.h, 0, None, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.int64)
+        check(self.lib.pbd_sh_pairs(self.h, n.value, out.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(n)))
+        return out[: n.value]
+
+    def ccd_detect(self, desc: HashDesc, x, p, tris, thickness: float):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+        tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        n = C.c_longlong()
+        check(self.lib.pbd_ccd_detect(self.h, C.byref(desc), fptr(x), fptr(p), len(x), iptr(tris), len(tris),
+                                      thickness, C.byref(n)))
+        m = n.value
+        vf = np.zeros((max(m, 1), 2), np.int32)
+        t = np.zeros(max(m, 1), np.float32)
+        nn = np.zeros((max(m, 1), 3), np.float32)
+        w = np.zeros((max(m, 1), 4), np.float64)
+        check(self.lib.pbd_ccd_get_contacts(self.h, m, iptr(vf), fptr(t), fptr(nn), w.ctypes.data_as(_dp)))
+        return vf[:m], t[:m], nn[:m], w[:m]
+
+    def ccd_resolve(self, p, w, thickness: float, passes: int) -> np.ndarray:
+        p = np.ascontiguousarray(p, np.float32).reshape(-1, 3).copy()
+        w = np.ascontiguousarray(w, np.float32)
+        check(self.lib.pbd_ccd_resolve(self.h, fptr(p), fptr(w), thickness, passes))
+        return p
+
+    def set_collision(self, enabled=1, grid: HashDesc | None = None, cell_scale=1.5, thickness=0.005,
+                      resolve_passes=2, detect_every=10):
+        cp = CollisionParams(enabled, grid if grid is not None else hash_desc(1, table_bits=20), cell_scale,
+                             thickness, resolve_passes, detect_every)
+        check(self.lib.pbd_set_collision(self.h, C.byref(cp)))
+
+    def last_contacts(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        check(self.lib.pbd_last_contacts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def profile(self, on: bool = True):
         check(self.lib.pbd_profile_enable(self.h, 1 if on else 0))

@@ -46,6 +46,18 @@ struct pbd_ctx {
     DBuf<double> rsum;
     DBuf<int> sweeps;
     int nslots = 0;
+    // standalone broadphase / CCD (parity-test entry points)
+    Broadphase sbp;
+    DBuf<float4> sx, sp4;
+    DBuf<int> stris;
+    int snv = 0, snt = 0;
+    bool sh_built = false, ccd_built = false;
+    // in-solver collision
+    pbd_collision_params coll{};
+    HashCfg ccfg{};
+    Broadphase cbp;
+    DBuf<int> gtris;
+    int gnt = 0;
     // profiling
     bool prof_on = false;
     pbd_profile prof{};
@@ -139,6 +151,7 @@ void fill_ktabs(pbd_ctx* c) {
 }
 
 int effective_tile_sweeps(const pbd_ctx* c) {
+    if (c->coll.enabled) return 1;  // contacts are resolved after every sweep
     int s = c->sp.tile_sweeps > 0 ? c->sp.tile_sweeps : c->sp.check_every;
     return std::max(1, s);
 }
@@ -391,6 +404,57 @@ void collect_events(pbd_ctx* c) {
 // Algorithmic bytes per prim per sweep and per vertex (fused advect + integrate).
 constexpr double kDistBytes = 68.0, kBendBytes = 132.0, kVertBytes = 100.0;
 
+// ---- broadphase configuration ----------------------------------------------
+// Grid start as SpatialHashSystem::InitSH computes it (SpatialHashSystem.cpp:88-98):
+// shift = (gridSize * cellSize) / 2 in float, start = centre - shift.
+HashCfg make_hash_cfg(const pbd_hash_desc& d) {
+    PBD_CHECK(d.mode == 0 || d.mode == 1, PBD_ERR_ARG, "hash mode must be 0 (dense) or 1 (hashed)");
+    HashCfg g{};
+    g.mode = d.mode;
+    for (int k = 0; k < 3; k++) {
+        PBD_CHECK(d.cell[k] > 0.0f, PBD_ERR_ARG, "cell size must be positive");
+        PBD_CHECK(d.mode == 1 || d.size[k] > 0, PBD_ERR_ARG, "grid size must be positive");
+        float shift = (float)((unsigned)d.size[k]) * d.cell[k] / 2;
+        g.start[k] = d.centre[k] - shift;
+        g.cell[k] = d.cell[k];
+        g.size[k] = d.size[k];
+    }
+    g.table_bits = d.table_bits;
+    if (d.mode == 0) {
+        long long nc = (long long)d.size[0] * d.size[1] * d.size[2];
+        PBD_CHECK(nc < (1ll << 30), PBD_ERR_LIMIT, "dense grid larger than 2^30 cells");
+        g.ncells = (int)nc;
+    } else {
+        PBD_CHECK(d.table_bits >= 1 && d.table_bits <= 28, PBD_ERR_ARG, "table_bits must be in [1, 28]");
+        g.ncells = 1 << d.table_bits;
+    }
+    return g;
+}
+
+void upload_scene(pbd_ctx* c, const float* x, const float* p, const float* w, int nv, const int* tris, int nt) {
+    PBD_CHECK(nv >= 0 && nt >= 0 && (x || !nv) && (p || !nv) && (tris || !nt), PBD_ERR_ARG, "bad scene arrays");
+    for (long long i = 0; i < 3ll * nt; i++) PBD_CHECK(tris[i] >= 0 && tris[i] < nv, PBD_ERR_ARG, "triangle index out of range");
+    std::vector<float4> X(nv), P(nv);
+    for (int i = 0; i < nv; i++) {
+        float wi = w ? w[i] : 1.0f;
+        X[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], wi);
+        P[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], wi);
+    }
+    c->sx.upload(X, c->stream);
+    c->sp4.upload(P, c->stream);
+    c->stris.upload(std::vector<int>(tris, tris + 3ll * nt), c->stream);
+    c->snv = nv;
+    c->snt = nt;
+}
+
+// Contact detection for the whole context (all cloths, global vertex ids).
+void detect_contacts(pbd_ctx* c, const float4* p) {
+    hipStream_t s = c->stream;
+    c->cbp.build(s, c->ccfg, p, c->gtris.ptr, c->gnt);
+    c->cbp.pairs(s, p, c->gtris.ptr, c->nv_total);
+    c->cbp.narrow(s, c->x.ptr, p, c->gtris.ptr, c->coll.thickness);
+}
+
 // ---- solver orchestration ---------------------------------------------------
 void advect_all(pbd_ctx* c, float dt) {
     // the kernel constants hold g*dt for sp.dt; recompute when dt differs
@@ -530,6 +594,61 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
 void integrate_all(pbd_ctx* c, float dt, bool skip_whole) {
     launch_integrate(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, 0, c->nv_total,
                      dt, skip_whole ? 1 : 0);
+}
+
+// Substep with collisions, in the order of projectConstraintWithColliCPU
+// (PBD_cuda.cu:1130-1210): per sweep ii, [detect if ii % every == 0 or last],
+// one Gauss-Seidel sweep, contact response, analytic colliders.
+void step_ccd(pbd_ctx* c, float dt) {
+    hipStream_t s = c->stream;
+    advect_all(c, dt);
+    replan_if_needed(c);
+    int I = c->sp.iterations, ncl = (int)c->cloths.size();
+    int every = std::max(1, c->coll.detect_every);
+    ensure_slots(c, 1);
+    launch_fill_u64(s, c->res_slots.ptr, (size_t)c->nslots * ncl, 0ull);
+    PBD_HIP(hipMemsetAsync(c->sweeps.ptr, 0, sizeof(int) * ncl, s));
+    size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
+    int threads = c->sp.block_threads > 0 ? c->sp.block_threads : 1024;
+    bool coll = false;
+    for (auto& k : c->cloths) coll |= (k.prm.sphere_on || k.prm.box_on);
+    SolveArgs a{};
+    a.dt = dt;
+    a.check_every = 1;
+    a.tol = -1.0;
+    a.kmax = c->kmax;
+    a.nsweeps = 1;
+    a.fused = 0;
+    a.chunk = 0;
+    a.no_coll = 1;
+    for (int ii = 0; ii < I; ii++) {
+        int cur = c->pcur;
+        if (ii % every == 0 || ii == I - 1) detect_contacts(c, c->pbuf[cur].ptr);
+        a.sweep_base = ii;
+        for (const DBuf<int>* ord : {&c->order_whole, &c->order_partial}) {
+            if (!ord->n) continue;
+            if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
+            launch_tiles(s, (int)ord->n, ord->ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr,
+                         c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr,
+                         c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr,
+                         c->res_slots.ptr, ncl, c->sweeps.ptr, a, threads, lds);
+            if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
+        }
+        c->pcur = cur ^ 1;
+        c->cbp.resolve(s, c->x.ptr, c->pbuf[c->pcur].ptr, c->gtris.ptr, c->nv_total, c->coll.thickness,
+                       c->coll.resolve_passes);
+        if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, c->pbuf[c->pcur].ptr, 0, c->nv_total);
+    }
+    integrate_all(c, dt, false);
+    PBD_HIP(hipStreamSynchronize(s));
+    for (auto& k : c->cloths) k.last_sweeps = I;
+    if (c->prof_on) {
+        collect_events(c);
+        for (auto& k : c->cloths) {
+            c->prof.tile_alg_bytes += (double)I * (kDistBytes * k.ne + kBendBytes * k.nb);
+            c->prof.vertex_sweeps += (long long)k.nv * I;
+        }
+    }
 }
 
 }  // namespace
@@ -800,7 +919,9 @@ int pbd_step(pbd_ctx* ctx, int substeps) {
         float dt = ctx->sp.dt;
         auto t0 = std::chrono::steady_clock::now();
         for (int s = 0; s < substeps; s++) {
-            if (ctx->sp.path == PBD_PATH_GLOBAL) {
+            if (ctx->coll.enabled) {
+                step_ccd(ctx, dt);
+            } else if (ctx->sp.path == PBD_PATH_GLOBAL) {
                 advect_all(ctx, dt);
                 project(ctx, ctx->sp.iterations, false, dt);
                 integrate_all(ctx, dt, false);
@@ -819,6 +940,186 @@ int pbd_step(pbd_ctx* ctx, int substeps) {
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             ctx->prof.substeps += substeps;
         }
+    });
+}
+
+int pbd_sh_build(pbd_ctx* ctx, const pbd_hash_desc* d, const float* pos, int nv, const int* tris, int nt) {
+    return guard([&] {
+        PBD_CHECK(ctx && d, PBD_ERR_ARG, "null argument");
+        set_device(ctx);
+        HashCfg g = make_hash_cfg(*d);
+        upload_scene(ctx, pos, pos, nullptr, nv, tris, nt);
+        ctx->sbp.build(ctx->stream, g, ctx->sp4.ptr, ctx->stris.ptr, nt);
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->sh_built = true;
+        ctx->ccd_built = false;
+    });
+}
+
+int pbd_sh_num_cells(pbd_ctx* ctx, int* ncells) {
+    return guard([&] {
+        PBD_CHECK(ctx && ncells, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(ctx->sh_built, PBD_ERR_STATE, "call pbd_sh_build first");
+        *ncells = ctx->sbp.cfg.ncells;
+    });
+}
+
+int pbd_sh_get(pbd_ctx* ctx, int* keys, int* ids, int* raw, int* cstart, int* cend) {
+    return guard([&] {
+        PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
+        PBD_CHECK(ctx->sh_built, PBD_ERR_STATE, "call pbd_sh_build first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        Broadphase& b = ctx->sbp;
+        int nt = b.nt;
+        if (nt > 0) {
+            std::vector<uint32_t> k1;
+            if (keys) {
+                b.keys.download(k1, s);
+                PBD_HIP(hipStreamSynchronize(s));
+                for (int i = 0; i < nt; i++) keys[i] = (int)k1[i] - 1;
+            }
+            if (ids) PBD_HIP(hipMemcpyAsync(ids, b.ids.ptr, sizeof(int) * nt, hipMemcpyDeviceToHost, s));
+            if (raw) PBD_HIP(hipMemcpyAsync(raw, b.raw.ptr, sizeof(int) * nt, hipMemcpyDeviceToHost, s));
+        }
+        if (cstart) PBD_HIP(hipMemcpyAsync(cstart, b.cstart.ptr, sizeof(int) * b.cfg.ncells, hipMemcpyDeviceToHost, s));
+        if (cend) PBD_HIP(hipMemcpyAsync(cend, b.cend.ptr, sizeof(int) * b.cfg.ncells, hipMemcpyDeviceToHost, s));
+        PBD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int pbd_sh_pairs(pbd_ctx* ctx, long long cap, int64_t* out, long long* count) {
+    return guard([&] {
+        PBD_CHECK(ctx && count, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(ctx->sh_built, PBD_ERR_STATE, "call pbd_sh_build first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        long long n = ctx->sbp.pairs(s, ctx->sp4.ptr, ctx->stris.ptr, ctx->snv);
+        *count = n;
+        if (out && n > 0) {
+            long long m = std::min(n, cap);
+            std::vector<uint32_t> v(m), f(m);
+            PBD_HIP(hipMemcpyAsync(v.data(), ctx->sbp.uv.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+            PBD_HIP(hipMemcpyAsync(f.data(), ctx->sbp.uf.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+            PBD_HIP(hipStreamSynchronize(s));
+            for (long long i = 0; i < m; i++) out[i] = ((int64_t)v[i] << 32) | (int64_t)f[i];
+        }
+    });
+}
+
+int pbd_ccd_detect(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, const float* p, int nv, const int* tris,
+                   int nt, float th, long long* nc) {
+    return guard([&] {
+        PBD_CHECK(ctx && d && nc, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(th >= 0.0f, PBD_ERR_ARG, "thickness must be >= 0");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        HashCfg g = make_hash_cfg(*d);
+        upload_scene(ctx, x, p, nullptr, nv, tris, nt);
+        ctx->sbp.build(s, g, ctx->sp4.ptr, ctx->stris.ptr, nt);
+        ctx->sbp.pairs(s, ctx->sp4.ptr, ctx->stris.ptr, nv);
+        *nc = ctx->sbp.narrow(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, th);
+        ctx->sh_built = true;
+        ctx->ccd_built = true;
+    });
+}
+
+int pbd_ccd_get_contacts(pbd_ctx* ctx, long long cap, int* vf, float* t, float* n, double* w) {
+    return guard([&] {
+        PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
+        PBD_CHECK(ctx->ccd_built, PBD_ERR_STATE, "call pbd_ccd_detect first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        Broadphase& b = ctx->sbp;
+        long long m = std::min(cap, b.ncontacts);
+        if (m <= 0) return;
+        if (vf) {
+            std::vector<int> v(m), f(m);
+            PBD_HIP(hipMemcpyAsync(v.data(), b.cv.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+            PBD_HIP(hipMemcpyAsync(f.data(), b.cf.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+            PBD_HIP(hipStreamSynchronize(s));
+            for (long long i = 0; i < m; i++) {
+                vf[2 * i] = v[i];
+                vf[2 * i + 1] = f[i];
+            }
+        }
+        if (t) PBD_HIP(hipMemcpyAsync(t, b.ct.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+        if (n) PBD_HIP(hipMemcpyAsync(n, b.cn.ptr, 12 * m, hipMemcpyDeviceToHost, s));
+        if (w) PBD_HIP(hipMemcpyAsync(w, b.cw.ptr, 32 * m, hipMemcpyDeviceToHost, s));
+        PBD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int pbd_ccd_resolve(pbd_ctx* ctx, float* p, const float* w, float th, int passes) {
+    return guard([&] {
+        PBD_CHECK(ctx && p, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(ctx->ccd_built, PBD_ERR_STATE, "call pbd_ccd_detect first");
+        PBD_CHECK(passes >= 0, PBD_ERR_ARG, "passes must be >= 0");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        int nv = ctx->snv;
+        std::vector<float4> P(nv), X;
+        ctx->sx.download(X, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < nv; i++) {
+            float wi = w ? w[i] : 1.0f;
+            P[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], wi);
+            X[i].w = wi;
+        }
+        ctx->sp4.upload(P, s);
+        ctx->sx.upload(X, s);
+        ctx->sbp.resolve(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, nv, th, passes);
+        ctx->sp4.download(P, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < nv; i++) {
+            p[3 * i] = P[i].x;
+            p[3 * i + 1] = P[i].y;
+            p[3 * i + 2] = P[i].z;
+        }
+    });
+}
+
+int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* cp) {
+    return guard([&] {
+        PBD_CHECK(ctx && cp, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        set_device(ctx);
+        if (!cp->enabled) {
+            ctx->coll = *cp;
+            return;
+        }
+        PBD_CHECK(cp->thickness >= 0.0f && cp->resolve_passes >= 0, PBD_ERR_ARG, "bad collision parameters");
+        pbd_hash_desc d = cp->grid;
+        if (cp->cell_scale > 0.0f) {
+            double sum = 0.0;
+            long long n = 0;
+            for (auto& k : ctx->cloths)
+                for (int p = 0; p < k.np(); p++)
+                    if (k.prim_list[2 * p + 1] == 2) {
+                        sum += k.rest[p];
+                        n++;
+                    }
+            PBD_CHECK(n > 0, PBD_ERR_STATE, "cell_scale needs distance prims for the mean edge length");
+            float cell = (float)(cp->cell_scale * (sum / n));
+            d.cell[0] = d.cell[1] = d.cell[2] = cell;
+        }
+        ctx->ccfg = make_hash_cfg(d);
+        std::vector<int> gt;
+        for (auto& k : ctx->cloths)
+            for (int i : k.tris) gt.push_back(i + k.vbase);
+        ctx->gtris.upload(gt, ctx->stream);
+        ctx->gnt = (int)gt.size() / 3;
+        ctx->coll = *cp;
+        replan_if_needed(ctx);
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_last_contacts(pbd_ctx* ctx, long long* nc, long long* np) {
+    return guard([&] {
+        PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
+        if (nc) *nc = ctx->cbp.ncontacts;
+        if (np) *np = ctx->cbp.npairs;
     });
 }
 

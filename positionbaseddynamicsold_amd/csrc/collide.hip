@@ -1,0 +1,574 @@
+// Spatial-hash broadphase and vertex-face CCD for gfx950.
+//
+// Broadphase (SpatialHashSystem, SpatialHashSystem.cpp:88-187, 276-413):
+//   triangle centroid -> cell coordinates -> key (dense grid index or -1
+//   outside, or a hashed table slot) -> stable radix sort of (key, id) ->
+//   cell ranges [start, end] -> candidate (vertex, face) pairs from the 27
+//   cells around each triangle's cell, de-duplicated by a 64-bit radix sort.
+// Narrowphase: ExtendedVFTest (CCD.cpp:1733-1808) per candidate, FP64 only in
+//   the coplanarity cubic / time-of-impact solve (CCD.cpp:48-113).
+// Response: VFResolveTest/VFResolveNew (CCD.cpp:1996-2090) evaluated for all
+//   contacts against the same positions, fixes summed per vertex in 2^-32 m
+//   fixed point (integer sums are order independent, so the result does not
+//   depend on atomic ordering) and applied as the mean fix per vertex.
+// Paths are relative to PositionBasedDynamicsCUDA/PositionBasedDynamicsCUDA/.
+#include "pbd_internal.hpp"
+#include "pbd_math.hpp"
+
+namespace pbd {
+
+// ---------------------------------------------------------------------------
+// device scan (exclusive, int64 totals kept in int)
+// ---------------------------------------------------------------------------
+constexpr int SB = 1024;  // elements per scan block (256 threads x 4)
+
+__global__ __launch_bounds__(256) void k_scan_blocks(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                     int* __restrict__ sums) {
+    __shared__ int s[256];
+    int base = blockIdx.x * SB, t = threadIdx.x;
+    int v[4], tot = 0;
+    for (int k = 0; k < 4; k++) {
+        int i = base + t * 4 + k;
+        v[k] = i < n ? in[i] : 0;
+        tot += v[k];
+    }
+    s[t] = tot;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int a = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += a;
+        __syncthreads();
+    }
+    int run = s[t] - tot;
+    for (int k = 0; k < 4; k++) {
+        int i = base + t * 4 + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (t == 255) sums[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int* __restrict__ out, int n, const int* __restrict__ sums) {
+    int i = blockIdx.x * SB + threadIdx.x;
+    int add = sums[blockIdx.x];
+    for (int k = 0; k < 4; k++, i += 256)
+        if (i < n) out[i] += add;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_single(int* __restrict__ a, int n) {
+    __shared__ int part[1024];
+    int t = threadIdx.x;
+    int per = (n + 1023) / 1024;
+    int b = t * per, e = min(n, b + per);
+    int s = 0;
+    for (int i = b; i < e; i++) s += a[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int i = b; i < e; i++) {
+        int v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+void device_scan(hipStream_t s, const int* in, int* out, int n, int* tmp) {
+    if (n <= 0) return;
+    int nb = (n + SB - 1) / SB;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, in, out, n, tmp);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, n, tmp);
+    PBD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// broadphase
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int cell_key(const HashCfg& g, int x, int y, int z) {
+    if (g.mode == 0) {
+        if (x < 0 || y < 0 || z < 0 || x >= g.size[0] || y >= g.size[1] || z >= g.size[2]) return -1;
+        return (int)(((unsigned)z * (unsigned)g.size[1]) * (unsigned)g.size[0] + (unsigned)y * (unsigned)g.size[0] +
+                     (unsigned)x);
+    }
+    unsigned h = ((unsigned)x * 73856093u) ^ ((unsigned)y * 19349663u) ^ ((unsigned)z * 83492791u);
+    return (int)(h & ((1u << g.table_bits) - 1u));
+}
+
+// centroid (a+b+c)/3 per component (evalTriCentroids) and its cell
+// (calcGridPosCPU: floor((c - start)/cell)).
+__device__ __forceinline__ void tri_cell(const HashCfg& g, const float4* __restrict__ p, const int* __restrict__ tris,
+                                         int t, int* gp) {
+    float4 a = p[tris[3 * t]], b = p[tris[3 * t + 1]], c = p[tris[3 * t + 2]];
+    float cx = (a.x + b.x + c.x) / 3, cy = (a.y + b.y + c.y) / 3, cz = (a.z + b.z + c.z) / 3;
+    gp[0] = (int)floorf((cx - g.start[0]) / g.cell[0]);
+    gp[1] = (int)floorf((cy - g.start[1]) / g.cell[1]);
+    gp[2] = (int)floorf((cz - g.start[2]) / g.cell[2]);
+}
+
+__global__ __launch_bounds__(256) void k_tri_keys(HashCfg g, const float4* __restrict__ p,
+                                                  const int* __restrict__ tris, int nt,
+                                                  uint32_t* __restrict__ keys1, uint32_t* __restrict__ ids,
+                                                  int* __restrict__ raw) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int gp[3];
+    tri_cell(g, p, tris, t, gp);
+    int k = cell_key(g, gp[0], gp[1], gp[2]);
+    keys1[t] = (uint32_t)(k + 1);  // -1 sorts first, as the reference's signed sort
+    ids[t] = (uint32_t)t;
+    raw[t] = k;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ a, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Correct ranges of each key in the sorted array (the reference's
+// findCellStartCPU misses some starts, SpatialHashSystem.cpp:357-387).
+__global__ __launch_bounds__(256) void k_cell_ranges(const uint32_t* __restrict__ keys1, int n,
+                                                     int* __restrict__ cstart, int* __restrict__ cend) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = (int)keys1[i] - 1;
+    if (k < 0) return;
+    if (i == 0 || keys1[i - 1] != keys1[i]) cstart[k] = i;
+    if (i == n - 1 || keys1[i + 1] != keys1[i]) cend[k] = i;
+}
+
+// Candidate (v, f): v a vertex of triangle t whose cell is valid, f != t in
+// the 27 cells around t's cell (FindNeighbors, SpatialHashSystem.cpp:134-187),
+// v not a vertex of f.  mode 0 counts, mode 1 writes at off[t].
+template <int EMIT>
+__global__ __launch_bounds__(256) void k_pairs(HashCfg g, const float4* __restrict__ p,
+                                               const int* __restrict__ tris, int nt,
+                                               const int* __restrict__ raw, const uint32_t* __restrict__ sids,
+                                               const int* __restrict__ cstart, const int* __restrict__ cend,
+                                               int* __restrict__ cnt_or_off, uint32_t* __restrict__ ov,
+                                               uint32_t* __restrict__ of) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int n = 0, off = EMIT ? cnt_or_off[t] : 0;
+    if (raw[t] >= 0) {
+        int gp[3];
+        tri_cell(g, p, tris, t, gp);
+        int tv[3] = {tris[3 * t], tris[3 * t + 1], tris[3 * t + 2]};
+        for (int z = -1; z <= 1; z++)
+            for (int y = -1; y <= 1; y++)
+                for (int x = -1; x <= 1; x++) {
+                    int h = cell_key(g, gp[0] + x, gp[1] + y, gp[2] + z);
+                    if (h < 0) continue;
+                    int s = cstart[h];
+                    if (s < 0) continue;
+                    int e = cend[h];
+                    for (int i = s; i <= e; i++) {
+                        int f = (int)sids[i];
+                        if (f == t) continue;
+                        int f0 = tris[3 * f], f1 = tris[3 * f + 1], f2 = tris[3 * f + 2];
+                        for (int k = 0; k < 3; k++) {
+                            int v = tv[k];
+                            if (v == f0 || v == f1 || v == f2) continue;
+                            if (EMIT) {
+                                ov[off + n] = (uint32_t)v;
+                                of[off + n] = (uint32_t)f;
+                            }
+                            n++;
+                        }
+                    }
+                }
+    }
+    if (!EMIT) cnt_or_off[t] = n;
+}
+
+__global__ __launch_bounds__(256) void k_unique_flags(const uint32_t* __restrict__ hi, const uint32_t* __restrict__ lo,
+                                                      int n, int* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || hi[i] != hi[i - 1] || lo[i] != lo[i - 1]) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_compact_pairs(const uint32_t* __restrict__ hi, const uint32_t* __restrict__ lo,
+                                                       int n, const int* __restrict__ flag,
+                                                       const int* __restrict__ pos, uint32_t* __restrict__ ohi,
+                                                       uint32_t* __restrict__ olo) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    ohi[pos[i]] = hi[i];
+    olo[pos[i]] = lo[i];
+}
+
+// ---------------------------------------------------------------------------
+// narrowphase: FP64 cubic (solve_cubic / newtons_method / solve_quadratic,
+// CCD.cpp:48-113), ExtendedVFTest (CCD.cpp:1733-1808)
+// ---------------------------------------------------------------------------
+__device__ int d_solve_quadratic(double a, double b, double c, double* x) {
+    double d = b * b - 4 * a * c;
+    if (d < 0) {
+        x[0] = -b / (2 * a);
+        return 0;
+    }
+    double q = -(b + (b < 0 ? -1.0 : 1.0) * sqrt(d)) / 2;
+    int i = 0;
+    if (fabs(a) > 1e-12 * fabs(q)) x[i++] = q / a;
+    if (fabs(q) > 1e-12 * fabs(c)) x[i++] = c / q;
+    if (i == 2 && x[0] > x[1]) {
+        double t = x[0];
+        x[0] = x[1];
+        x[1] = t;
+    }
+    return i;
+}
+
+__device__ double d_newton(double a, double b, double c, double d, double x0, int init_dir) {
+    if (init_dir != 0) {
+        double y0 = d + x0 * (c + x0 * (b + x0 * a));
+        double ddy0 = 2 * b + x0 * (6 * a);
+        x0 += init_dir * sqrt(fabs(2 * y0 / ddy0));
+    }
+    for (int iter = 0; iter < 100; iter++) {
+        double y = d + x0 * (c + x0 * (b + x0 * a));
+        double dy = c + x0 * (2 * b + x0 * 3 * a);
+        if (dy == 0) return x0;
+        double x1 = x0 - y / dy;
+        if (fabs(x0 - x1) < 1e-6) return x0;
+        x0 = x1;
+    }
+    return x0;
+}
+
+// When the critical-point quadratic leaves its root unset (a = b = 0: the
+// cubic is linear; the reference reads an unset value there), Newton starts
+// from 0, which converges to the linear root.
+__device__ int d_solve_cubic(double a, double b, double c, double d, double* x) {
+    double xc[2];
+    xc[0] = __longlong_as_double(0x7ff8000000000000ll);
+    xc[1] = xc[0];
+    int ncrit = d_solve_quadratic(3 * a, 2 * b, c, xc);
+    if (ncrit == 0) {
+        if (xc[0] != xc[0]) xc[0] = 0.0;
+        x[0] = d_newton(a, b, c, d, xc[0], 0);
+        return 1;
+    } else if (ncrit == 1) {
+        return d_solve_quadratic(b, c, d, x);
+    }
+    double yc0 = d + xc[0] * (c + xc[0] * (b + xc[0] * a)), yc1 = d + xc[1] * (c + xc[1] * (b + xc[1] * a));
+    int i = 0;
+    if (yc0 * a >= 0) x[i++] = d_newton(a, b, c, d, xc[0], -1);
+    if (yc0 * yc1 <= 0) {
+        int closer = fabs(yc0) < fabs(yc1) ? 0 : 1;
+        x[i++] = d_newton(a, b, c, d, xc[closer], closer == 0 ? 1 : -1);
+    }
+    if (yc1 * a <= 0) x[i++] = d_newton(a, b, c, d, xc[1], 1);
+    return i;
+}
+
+__device__ __forceinline__ V3 v3(float4 a) { return mk(a.x, a.y, a.z); }
+__device__ __forceinline__ V3 vnormalize(V3 a) {
+    float inv = 1.0f / sqrtf(dot(a, a));
+    return V3{a.x * inv, a.y * inv, a.z * inv};
+}
+__device__ __forceinline__ float stp(V3 u, V3 v, V3 w) { return dot(u, cross(v, w)); }
+__device__ __forceinline__ V3 lerp(V3 x, V3 p, double t) {
+    float tf = (float)t;
+    return V3{x.x + tf * (p.x - x.x), x.y + tf * (p.y - x.y), x.z + tf * (p.z - x.z)};
+}
+// BarycentricCoord (CCD.cpp:115-130)
+__device__ __forceinline__ V3 bary(V3 pp, V3 a, V3 b, V3 c) {
+    V3 AB = sub(a, c), AC = sub(b, c), AP = sub(pp, c);
+    float d00 = dot(AC, AC), d01 = dot(AC, AB), d02 = dot(AC, AP), d11 = dot(AB, AB), d12 = dot(AB, AP);
+    float inv = 1 / (d00 * d11 - d01 * d01);
+    float v = (d11 * d02 - d01 * d12) * inv;
+    float u = (d00 * d12 - d01 * d02) * inv;
+    float w = 1 - u - v;
+    return V3{u, v, w};
+}
+__device__ __forceinline__ float dist3(V3 a, V3 b) {
+    V3 d = sub(a, b);
+    return sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+}
+
+__device__ bool vf_ccd(V3 x0, V3 x1, V3 x2, V3 x3, V3 p0, V3 p1, V3 p2, V3 p3, float th, float* t_out, V3* n_out,
+                       V3* w_out) {
+    V3 v0 = sub(p0, x0);
+    V3 X1 = sub(x1, x0), X2 = sub(x2, x0), X3 = sub(x3, x0);
+    V3 V1 = sub(sub(p1, x1), v0), V2 = sub(sub(p2, x2), v0), V3_ = sub(sub(p3, x3), v0);
+    double a0 = stp(X1, X2, X3);
+    double a1 = (double)(stp(V1, X2, X3) + stp(X1, V2, X3) + stp(X1, X2, V3_));
+    double a2 = (double)(stp(X1, V2, V3_) + stp(V1, X2, V3_) + stp(V1, V2, X3));
+    double a3 = stp(V1, V2, V3_);
+    double t[5];
+    int nsol = d_solve_cubic(a3, a2, a1, a0, t);
+    int m = 0;
+    for (int i = 0; i < nsol; i++)
+        if (isfinite(t[i])) t[m++] = t[i];
+    t[m] = 1;
+    t[m + 1] = 0;
+    int nt = m + 2;
+    for (int i = 1; i < nt; i++) {
+        double v = t[i];
+        int j = i - 1;
+        while (j >= 0 && t[j] > v) {
+            t[j + 1] = t[j];
+            j--;
+        }
+        t[j + 1] = v;
+    }
+    for (int i = 0; i < nt; i++) {
+        if (t[i] < 0 || t[i] > 1) continue;
+        V3 c0 = lerp(x0, p0, t[i]), c1 = lerp(x1, p1, t[i]), c2 = lerp(x2, p2, t[i]), c3 = lerp(x3, p3, t[i]);
+        V3 nrm = vnormalize(cross(sub(p2, p1), sub(p3, p1)));
+        V3 cn = vnormalize(cross(sub(c2, c1), sub(c3, c1)));
+        double dist = dot(sub(c0, c1), cn);
+        V3 wg = bary(c0, c1, c2, c3);
+        float ed1 = dist3(c1, c2), ed2 = dist3(c1, c3), ed3 = dist3(c2, c3);
+        float eps = th / ((ed1 + ed2 + ed3) / 3.0);
+        bool in1 = (wg.x <= (1 + eps)) && (wg.x >= (-eps));
+        bool in2 = (wg.y <= (1 + eps)) && (wg.y >= (-eps));
+        bool in3 = (wg.z <= (1 + eps)) && (wg.z >= (-eps));
+        if ((fabs(dist) - 2.0f * th < -1e-6) && in1 && in2 && in3) {
+            *t_out = (float)t[i];
+            *n_out = nrm;
+            *w_out = wg;
+            return true;
+        }
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(256) void k_narrow(const uint32_t* __restrict__ cv, const uint32_t* __restrict__ cf,
+                                                int n, const int* __restrict__ tris, const float4* __restrict__ x,
+                                                const float4* __restrict__ p, float th, int* __restrict__ flag,
+                                                float* __restrict__ ct, float* __restrict__ cn,
+                                                double* __restrict__ cw) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = (int)cv[i], f = (int)cf[i];
+    int a = tris[3 * f], b = tris[3 * f + 1], c = tris[3 * f + 2];
+    float t;
+    V3 nr, wg;
+    bool hit = vf_ccd(v3(x[v]), v3(x[a]), v3(x[b]), v3(x[c]), v3(p[v]), v3(p[a]), v3(p[b]), v3(p[c]), th, &t, &nr, &wg);
+    flag[i] = hit ? 1 : 0;
+    if (hit) {
+        ct[i] = t;
+        cn[3 * i] = nr.x;
+        cn[3 * i + 1] = nr.y;
+        cn[3 * i + 2] = nr.z;
+        cw[4 * i] = 1.0;
+        cw[4 * i + 1] = wg.x;
+        cw[4 * i + 2] = wg.y;
+        cw[4 * i + 3] = wg.z;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact_contacts(int n, const int* __restrict__ flag,
+                                                          const int* __restrict__ pos,
+                                                          const uint32_t* __restrict__ cv,
+                                                          const uint32_t* __restrict__ cf, const float* __restrict__ ct,
+                                                          const float* __restrict__ cn, const double* __restrict__ cw,
+                                                          int* __restrict__ ov, int* __restrict__ of,
+                                                          float* __restrict__ ot, float* __restrict__ on,
+                                                          double* __restrict__ ow) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    int o = pos[i];
+    ov[o] = (int)cv[i];
+    of[o] = (int)cf[i];
+    ot[o] = ct[i];
+    for (int k = 0; k < 3; k++) on[3 * o + k] = cn[3 * i + k];
+    for (int k = 0; k < 4; k++) ow[4 * o + k] = cw[4 * i + k];
+}
+
+// ---------------------------------------------------------------------------
+// response
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long long to_fixed(float f) { return (long long)rint((double)f * 4294967296.0); }
+
+__global__ __launch_bounds__(256) void k_resolve_acc(const int* __restrict__ cvv, const int* __restrict__ cff, int nc,
+                                                     const int* __restrict__ tris, const float4* __restrict__ x,
+                                                     const float4* __restrict__ p, float th,
+                                                     unsigned long long* __restrict__ acc, int* __restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    int f = cff[i];
+    int id[4] = {cvv[i], tris[3 * f], tris[3 * f + 1], tris[3 * f + 2]};
+    float4 P[4];
+    V3 X[4], q[4];
+    for (int k = 0; k < 4; k++) {
+        P[k] = p[id[k]];
+        X[k] = v3(x[id[k]]);
+        q[k] = v3(P[k]);
+    }
+    // VFResolveTest (CCD.cpp:1996-2032)
+    V3 n0 = vnormalize(cross(sub(X[2], X[1]), sub(X[3], X[1])));
+    bool dir = dot(sub(X[0], X[1]), n0) >= 0;
+    V3 cr = cross(sub(q[2], q[1]), sub(q[3], q[1]));
+    if (dot(cr, cr) <= 1e-6) return;
+    V3 pn = vnormalize(cr);
+    float s = dir ? 1.0f : -1.0f;
+    V3 sn = V3{pn.x * s, pn.y * s, pn.z * s};
+    float depth0 = dot(sub(q[0], q[1]), sn) - 2.0 * th;
+    if (!(depth0 < -1e-6)) return;
+    // VFResolveNew (CCD.cpp:2034-2090)
+    V3 proj = V3{q[1].x + ((pn.x * s) * 2.0f) * th, q[1].y + ((pn.y * s) * 2.0f) * th, q[1].z + ((pn.z * s) * 2.0f) * th};
+    float depth = dot(sub(proj, q[0]), sn) * 0.5;
+    V3 wg = bary(q[0], q[1], q[2], q[3]);
+    float sw = wg.x * wg.x + wg.y * wg.y + wg.z * wg.z;
+    float wf[3] = {wg.x, wg.y, wg.z};
+    float snv[3] = {sn.x, sn.y, sn.z}, pnv[3] = {pn.x, pn.y, pn.z};
+    float qa[4][3] = {{q[0].x, q[0].y, q[0].z}, {q[1].x, q[1].y, q[1].z}, {q[2].x, q[2].y, q[2].z}, {q[3].x, q[3].y, q[3].z}};
+    float qb[4][3];
+    for (int a = 0; a < 3; a++) {
+        float vfix = depth * snv[a];
+        float ffix = depth * (pnv[a] * -s);
+        qb[0][a] = qa[0][a] + vfix;
+        qb[1][a] = qa[1][a] + (ffix * wf[0]) / sw;
+        qb[2][a] = qa[2][a] + (ffix * wf[1]) / sw;
+        qb[3][a] = qa[3][a] + (ffix * wf[2]) / sw;
+    }
+    bool face_static = (P[1].w == 0.0f) && (P[2].w == 0.0f) && (P[3].w == 0.0f);
+    for (int k = 0; k < 4; k++) {
+        if (P[k].w == 0.0f) continue;
+        float sc = 1.0f;
+        if (k == 0 && face_static) sc = 2.0f;
+        if (k > 0 && P[0].w == 0.0f) sc = 2.0f;
+        for (int a = 0; a < 3; a++)
+            atomicAdd(&acc[3 * (size_t)id[k] + a], (unsigned long long)to_fixed((qb[k][a] - qa[k][a]) * sc));
+        atomicAdd(&cnt[id[k]], 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_resolve_apply(const unsigned long long* __restrict__ acc,
+                                                       const int* __restrict__ cnt, float4* __restrict__ p, int nv) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv || !cnt[v]) return;
+    float4 q = p[v];
+    double c = (double)cnt[v];
+    q.x = q.x + (float)((double)(long long)acc[3 * (size_t)v] / 4294967296.0 / c);
+    q.y = q.y + (float)((double)(long long)acc[3 * (size_t)v + 1] / 4294967296.0 / c);
+    q.z = q.z + (float)((double)(long long)acc[3 * (size_t)v + 2] / 4294967296.0 / c);
+    p[v] = q;
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+static inline int blocks(long long n) { return (int)((n + 255) / 256); }
+
+void Broadphase::build(hipStream_t s, const HashCfg& g, const float4* p, const int* tris, int nt) {
+    cfg = g;
+    this->nt = nt;
+    keys.alloc(std::max(1, nt));
+    ids.alloc(std::max(1, nt));
+    raw.alloc(std::max(1, nt));
+    k2.alloc(std::max(1, nt));
+    v2.alloc(std::max(1, nt));
+    hist.alloc((size_t)256 * ((std::max(1, nt) + 4095) / 4096));
+    cstart.alloc(std::max(1, g.ncells));
+    cend.alloc(std::max(1, g.ncells));
+    if (nt > 0) hipLaunchKernelGGL(k_tri_keys, dim3(blocks(nt)), dim3(256), 0, s, g, p, tris, nt, keys.ptr, ids.ptr, raw.ptr);
+    int bits = 1;
+    while (bits < 32 && ((1ll << bits) < (long long)g.ncells + 1)) bits++;
+    radix_sort_pairs(s, keys.ptr, ids.ptr, k2.ptr, v2.ptr, nt, bits, hist.ptr);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(g.ncells)), dim3(256), 0, s, cstart.ptr, g.ncells, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(g.ncells)), dim3(256), 0, s, cend.ptr, g.ncells, -1);
+    if (nt > 0) hipLaunchKernelGGL(k_cell_ranges, dim3(blocks(nt)), dim3(256), 0, s, keys.ptr, nt, cstart.ptr, cend.ptr);
+    PBD_HIP(hipGetLastError());
+}
+
+long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int nv) {
+    npairs = 0;
+    if (nt <= 0) return 0;
+    cnt.alloc(nt + 1);
+    off.alloc(nt + 1);
+    scan_tmp.alloc((nt + SB) / SB + 2);
+    hipLaunchKernelGGL(k_pairs<0>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr, cstart.ptr,
+                       cend.ptr, cnt.ptr, (uint32_t*)nullptr, (uint32_t*)nullptr);
+    PBD_HIP(hipMemsetAsync(cnt.ptr + nt, 0, sizeof(int), s));
+    device_scan(s, cnt.ptr, off.ptr, nt + 1, scan_tmp.ptr);
+    int total = 0;
+    PBD_HIP(hipMemcpyAsync(&total, off.ptr + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipStreamSynchronize(s));
+    PBD_CHECK(total >= 0, PBD_ERR_LIMIT, "candidate pair count overflows int32");
+    if (total == 0) return 0;
+    pv.alloc(total);
+    pf.alloc(total);
+    t1.alloc(total);
+    t2.alloc(total);
+    flag.alloc(total + 1);
+    pos.alloc(total + 1);
+    hipLaunchKernelGGL(k_pairs<1>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr, cstart.ptr,
+                       cend.ptr, off.ptr, pv.ptr, pf.ptr);
+    // sort by (v, f): by f first, then stably by v
+    int fb = 1, vb = 1;
+    while (fb < 31 && (1ll << fb) < nt) fb++;
+    while (vb < 31 && (1ll << vb) < nv) vb++;
+    hist2.alloc((size_t)256 * ((total + 4095) / 4096));
+    radix_sort_pairs(s, pf.ptr, pv.ptr, t1.ptr, t2.ptr, total, fb, hist2.ptr);
+    radix_sort_pairs(s, pv.ptr, pf.ptr, t1.ptr, t2.ptr, total, vb, hist2.ptr);
+    hipLaunchKernelGGL(k_unique_flags, dim3(blocks(total)), dim3(256), 0, s, pv.ptr, pf.ptr, total, flag.ptr);
+    PBD_HIP(hipMemsetAsync(flag.ptr + total, 0, sizeof(int), s));
+    scan_tmp.alloc((total + 1 + SB) / SB + 2);
+    device_scan(s, flag.ptr, pos.ptr, total + 1, scan_tmp.ptr);
+    int uniq = 0;
+    PBD_HIP(hipMemcpyAsync(&uniq, pos.ptr + total, sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipStreamSynchronize(s));
+    uv.alloc(std::max(1, uniq));
+    uf.alloc(std::max(1, uniq));
+    hipLaunchKernelGGL(k_compact_pairs, dim3(blocks(total)), dim3(256), 0, s, pv.ptr, pf.ptr, total, flag.ptr, pos.ptr,
+                       uv.ptr, uf.ptr);
+    PBD_HIP(hipGetLastError());
+    npairs = uniq;
+    return uniq;
+}
+
+long long Broadphase::narrow(hipStream_t s, const float4* x, const float4* p, const int* tris, float th) {
+    ncontacts = 0;
+    if (npairs <= 0) return 0;
+    long long n = npairs;
+    flag.alloc(n + 1);
+    pos.alloc(n + 1);
+    ctmp_t.alloc(n);
+    ctmp_n.alloc(3 * n);
+    ctmp_w.alloc(4 * n);
+    hipLaunchKernelGGL(k_narrow, dim3(blocks(n)), dim3(256), 0, s, uv.ptr, uf.ptr, (int)n, tris, x, p, th, flag.ptr,
+                       ctmp_t.ptr, ctmp_n.ptr, ctmp_w.ptr);
+    PBD_HIP(hipMemsetAsync(flag.ptr + n, 0, sizeof(int), s));
+    scan_tmp.alloc((n + 1 + SB) / SB + 2);
+    device_scan(s, flag.ptr, pos.ptr, (int)n + 1, scan_tmp.ptr);
+    int nc = 0;
+    PBD_HIP(hipMemcpyAsync(&nc, pos.ptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipStreamSynchronize(s));
+    cv.alloc(std::max(1, nc));
+    cf.alloc(std::max(1, nc));
+    ct.alloc(std::max(1, nc));
+    cn.alloc(3 * std::max(1, nc));
+    cw.alloc(4 * std::max(1, nc));
+    hipLaunchKernelGGL(k_compact_contacts, dim3(blocks(n)), dim3(256), 0, s, (int)n, flag.ptr, pos.ptr, uv.ptr, uf.ptr,
+                       ctmp_t.ptr, ctmp_n.ptr, ctmp_w.ptr, cv.ptr, cf.ptr, ct.ptr, cn.ptr, cw.ptr);
+    PBD_HIP(hipGetLastError());
+    ncontacts = nc;
+    return nc;
+}
+
+void Broadphase::resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes) {
+    if (ncontacts <= 0) return;
+    acc.alloc((size_t)3 * nv);
+    vcnt.alloc(nv);
+    for (int ps = 0; ps < passes; ps++) {
+        PBD_HIP(hipMemsetAsync(acc.ptr, 0, sizeof(unsigned long long) * 3 * (size_t)nv, s));
+        PBD_HIP(hipMemsetAsync(vcnt.ptr, 0, sizeof(int) * (size_t)nv, s));
+        hipLaunchKernelGGL(k_resolve_acc, dim3(blocks(ncontacts)), dim3(256), 0, s, cv.ptr, cf.ptr, (int)ncontacts, tris, x,
+                           (const float4*)p, th, acc.ptr, vcnt.ptr);
+        hipLaunchKernelGGL(k_resolve_apply, dim3(blocks(nv)), dim3(256), 0, s, acc.ptr, vcnt.ptr, p, nv);
+    }
+    PBD_HIP(hipGetLastError());
+}
+
+}  // namespace pbd

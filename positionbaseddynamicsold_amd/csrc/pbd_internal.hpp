@@ -110,6 +110,7 @@ struct SolveArgs {
     int kmax;            // length of each stiffness table
     int fused;           // whole tiles: 1 = advect on load + integrate on store
     int chunk;           // chunk index (selects the residual slot)
+    int no_coll;         // 1 = skip the per-sweep analytic colliders (applied by the caller)
 };
 
 // ---------------------------------------------------------------------------
@@ -162,6 +163,33 @@ void build_prims(Cloth& c);
 void build_adjacency(Cloth& c);
 void read_txt_cloth(const char* topol, const char* constr, std::vector<float>& pos,
                     std::vector<int>& tris, std::vector<int>& pairs, bool& has_pairs);
+
+// Broadphase grid: dense (reference) or hashed table.
+struct HashCfg {
+    int mode;            // 0 dense, 1 hashed
+    float start[3];      // centre - size*cell/2 (dense) / origin of cell coordinates (hashed)
+    float cell[3];
+    int size[3];
+    int table_bits;
+    int ncells;          // dense: size product; hashed: 1 << table_bits
+};
+
+// Broadphase + CCD working set (device buffers reused between calls).
+struct Broadphase {
+    HashCfg cfg{};
+    int nt = 0;
+    long long npairs = 0, ncontacts = 0;
+    DBuf<uint32_t> keys, ids, k2, v2, pv, pf, t1, t2, uv, uf;
+    DBuf<int> raw, hist, hist2, cstart, cend, cnt, off, scan_tmp, flag, pos, cv, cf, vcnt;
+    DBuf<float> ctmp_t, ctmp_n, ct, cn;
+    DBuf<double> ctmp_w, cw;
+    DBuf<unsigned long long> acc;
+    void build(hipStream_t s, const HashCfg& g, const float4* p, const int* tris, int nt);
+    long long pairs(hipStream_t s, const float4* p, const int* tris, int nv);
+    long long narrow(hipStream_t s, const float4* x, const float4* p, const int* tris, float th);
+    void resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes);
+};
+void device_scan(hipStream_t s, const int* in, int* out, int n, int* tmp);
 
 // Tile plan for the whole batch.
 struct TilePlan {

@@ -290,7 +290,7 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     const float* DR = dist_rest + td.dist_off;
     const uint32_t* B = bend + 2 * (size_t)td.bend_off;
     const float* BR = bend_rest + td.bend_off;
-    const bool coll = cd.sphere_on || cd.box_on;
+    const bool coll = (cd.sphere_on || cd.box_on) && !a.no_coll;
     int sw = 0;
     double resid = -1.0;
     for (; sw < a.nsweeps;) {

@@ -1,0 +1,168 @@
+"""Pins the oracle's broadphase and CCD against closed forms, brute force and
+numpy root finding.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+
+def tri_at(centroids):
+    """One triangle per centroid: vertices centroid + offsets summing to 0."""
+    off = np.array([[0.01, 0.0, 0.0], [-0.005, 0.01, 0.0], [-0.005, -0.01, 0.0]], np.float32)
+    pos = (np.repeat(np.asarray(centroids, np.float32), 3, 0) + np.tile(off, (len(centroids), 1))).astype(np.float32)
+    tris = np.arange(3 * len(centroids), dtype=np.int32).reshape(-1, 3)
+    return pos, tris
+
+
+def test_dense_keys_known_cells(orc):
+    # grid 4x3x2 cells of size 1 centred at (2, 1.5, 1): start = (0, 0, 0)
+    g = orc.grid(0, centre=(2.0, 1.5, 1.0), size=(4, 3, 2), cell=(1.0, 1.0, 1.0))
+    assert list(g.start) == [0.0, 0.0, 0.0]
+    cents = [(0.5, 0.5, 0.5), (3.5, 2.5, 1.5), (2.25, 0.75, 1.25), (-0.5, 0.5, 0.5), (0.5, 3.5, 0.5), (4.5, 0.5, 0.5)]
+    pos, tris = tri_at(cents)
+    keys, ids = orc.tri_keys(g, pos, tris)
+    # key = (z*gy + y)*gx + x, -1 outside (calcGridHashCPU)
+    assert keys.tolist() == [0, (1 * 3 + 2) * 4 + 3, (1 * 3 + 0) * 4 + 2, -1, -1, -1]
+    assert ids.tolist() == list(range(6))
+
+
+def test_stable_sort_and_ranges_brute_force(orc):
+    rng = np.random.default_rng(0)
+    keys = rng.integers(-1, 20, 500).astype(np.int32)
+    ids = np.arange(500, dtype=np.int32)
+    sk, si = orc.sorted_keys(keys, ids)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(si, order) and np.array_equal(sk, keys[order])
+    g = orc.grid(0, size=(20, 1, 1))
+    cs, ce = orc.cell_ranges(g, sk)
+    for k in range(20):
+        where = np.nonzero(sk == k)[0]
+        if len(where):
+            assert (cs[k], ce[k]) == (where.min(), where.max())
+        else:
+            assert (cs[k], ce[k]) == (-1, -1)
+
+
+def brute_pairs(keyfun_cell, neighbour_slots, pos, tris):
+    cen = pos[tris].astype(np.float32).sum(1, dtype=np.float32) / np.float32(3)
+    out = set()
+    cells = [keyfun_cell(c) for c in cen]
+    for t in range(len(tris)):
+        gp, key = cells[t]
+        if key < 0:
+            continue
+        slots = neighbour_slots(gp)
+        for f in range(len(tris)):
+            if f == t or cells[f][1] not in slots:
+                continue
+            for v in tris[t]:
+                if v not in tris[f]:
+                    out.add((int(v) << 32) | f)
+    return np.array(sorted(out), np.int64)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_candidate_pairs_brute_force(orc, mode):
+    rng = np.random.default_rng(mode)
+    nv, nt = 90, 70
+    pos = rng.uniform(-1, 1, (nv, 3)).astype(np.float32)
+    tris = np.array([rng.choice(nv, 3, replace=False) for _ in range(nt)], np.int32)
+    g = orc.grid(mode, centre=(0, 0, 0), size=(5, 5, 5), cell=(0.35, 0.35, 0.35), table_bits=5)
+    start = np.array(g.start, np.float32)
+    cell = np.array(g.cell, np.float32)
+
+    def key_of(gp):
+        if mode == 0:
+            if any(c < 0 or c >= 5 for c in gp):
+                return -1
+            return (gp[2] * 5 + gp[1]) * 5 + gp[0]
+        h = (np.uint32(gp[0] & 0xffffffff) * np.uint32(73856093)) ^ (np.uint32(gp[1] & 0xffffffff) * np.uint32(19349663)) ^ \
+            (np.uint32(gp[2] & 0xffffffff) * np.uint32(83492791))
+        return int(h & np.uint32(31))
+
+    def cell_of(c):
+        gp = tuple(int(np.floor(np.float32((c[k] - start[k])) / cell[k])) for k in range(3))
+        return gp, key_of(gp)
+
+    def slots(gp):
+        return {key_of(tuple(gp[k] + d[k] for k in range(3))) for d in itertools.product((-1, 0, 1), repeat=3)} - {-1}
+
+    with np.errstate(over="ignore"):
+        expect = brute_pairs(cell_of, slots, pos, tris)
+    got = orc.candidate_pairs(g, pos, tris)
+    assert np.array_equal(got, expect)
+    assert len(got) > 0
+
+
+def test_cubic_roots_against_numpy(orc):
+    import ctypes as C
+    L = orc.lib()
+    rng = np.random.default_rng(2)
+    x = np.zeros(3)
+    for _ in range(300):
+        r = np.sort(rng.uniform(-2, 2, 3))
+        a = rng.uniform(0.5, 2.0) * rng.choice([-1, 1])
+        c = a * np.poly(r)  # a x^3 + b x^2 + c x + d with roots r
+        n = L.orc_solve_cubic(c[0], c[1], c[2], c[3], x.ctypes.data_as(C.POINTER(C.c_double)))
+        got = np.sort(x[:n])
+        if np.min(np.diff(r)) > 1e-2:
+            assert n == 3
+            assert np.allclose(got, r, atol=1e-5)
+    # one real root: the other two complex
+    c = np.poly([0.3, 1 + 2j, 1 - 2j]).real
+    n = L.orc_solve_cubic(c[0], c[1], c[2], c[3], x.ctypes.data_as(C.POINTER(C.c_double)))
+    assert n == 1 and abs(x[0] - 0.3) < 1e-6
+
+
+def moving_vertex_scene(xy, z0, z1):
+    # static face 0 = (1,2,3) in the z = 0 plane; vertex 0 belongs to a small
+    # vertical triangle 1 = (0,4,5) that translates with it (candidates come
+    # from triangle vertices only, CCD.cpp:1541-1556)
+    tris = np.array([[1, 2, 3], [0, 4, 5]], np.int32)
+    x = np.array([[xy[0], xy[1], z0], [0, 0, 0], [1, 0, 0], [0, 1, 0],
+                  [xy[0] + 0.05, xy[1], z0 + 0.1], [xy[0], xy[1] + 0.05, z0 + 0.1]], np.float32)
+    p = x.copy()
+    p[[0, 4, 5], 2] += np.float32(z1 - z0)
+    return tris, x, p
+
+
+def test_vf_ccd_constructed_trajectory(orc):
+    # vertex from z = +1 to z = -1 through an interior point: coplanar at t = 0.5
+    tris, x, p = moving_vertex_scene((0.2, 0.2), 1.0, -1.0)
+    g = orc.grid(1, cell=(2.0, 2.0, 2.0), table_bits=8)
+    arr, n = orc.ccd_detect(g, x, p, tris, 0.01)
+    vf, t, nn, w = orc.contacts_arrays(arr, n)
+    hit = [i for i in range(n) if vf[i].tolist() == [0, 0]]
+    assert len(hit) == 1
+    i = hit[0]
+    assert abs(t[i] - 0.5) < 1e-6
+    assert np.allclose(np.abs(nn[i]), [0, 0, 1])
+    assert np.allclose(w[i, 1:].sum(), 1.0, atol=1e-6)
+    # outside the face: no vertex-0 / face-0 contact
+    tris, x2, p2 = moving_vertex_scene((0.8, 0.8), 1.0, -1.0)
+    arr, n = orc.ccd_detect(g, x2, p2, tris, 0.01)
+    vf, *_ = orc.contacts_arrays(arr, n)
+    assert [0, 0] not in vf.tolist()
+
+
+def test_resolve_separates_to_thickness(orc):
+    # vertex ended 0.001 below a face it started above; after resolve the
+    # vertex is back on the collision-free side at >= 2h - tol
+    h = 0.01
+    tris, x, p = moving_vertex_scene((0.25, 0.25), 0.05, -0.001)
+    g = orc.grid(1, cell=(2.0, 2.0, 2.0), table_bits=8)
+    arr, n = orc.ccd_detect(g, x, p, tris, h)
+    vf, *_ = orc.contacts_arrays(arr, n)
+    assert [0, 0] in vf.tolist()
+    # keep only the vertex-0 / face-0 contact
+    keep = [i for i in range(n) if vf[i].tolist() == [0, 0]]
+    arr = (type(arr[0]) * 1)(arr[keep[0]])
+    n = 1
+    for w, tol in ((np.ones(6, np.float32), 1e-5), (np.array([1, 0, 0, 0, 1, 1], np.float32), 1e-5)):
+        q = orc.ccd_resolve(arr, n, tris, x, p, w, h, 8)
+        nrm = np.cross(q[2] - q[1], q[3] - q[1])
+        nrm /= np.linalg.norm(nrm)
+        d = float(np.dot(q[0] - q[1], nrm))
+        assert d >= 2 * h - tol
+        if w[1] == 0:
+            assert np.array_equal(q[1:4], p[1:4])  # static face untouched
