@@ -204,6 +204,28 @@ typedef struct {
 PBD_API int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* params);
 PBD_API int pbd_last_contacts(pbd_ctx* ctx, long long* ncontacts, long long* npairs);
 
+/* ---- learned warm start (no counterpart in the reference) -------------------
+ * A per-vertex MLP (22 -> 64 -> 64 -> 3, GELU, BF16 weights on MFMA, FP32
+ * accumulation) predicts the Gauss-Seidel correction of each vertex from its
+ * pre-projection state; the prediction is kept for a cloth only when it does
+ * not raise that cloth's residual above the unwarmed start. */
+PBD_API int pbd_ws_num_params(int* n);
+PBD_API int pbd_ws_init(pbd_ctx* ctx, uint64_t seed);
+PBD_API int pbd_ws_get_params(pbd_ctx* ctx, float* params);
+PBD_API int pbd_ws_set_params(pbd_ctx* ctx, const float* params);
+/* Records (features, correction of `sweeps` sweeps) for every vertex over
+ * `substeps` plain substeps of the current scene. */
+PBD_API int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps);
+PBD_API int pbd_ws_num_samples(pbd_ctx* ctx, long long* n);
+PBD_API int pbd_ws_get_samples(pbd_ctx* ctx, long long first, long long n, float* X, float* Y);
+PBD_API int pbd_ws_set_samples(pbd_ctx* ctx, const float* X, const float* Y, long long n);
+PBD_API int pbd_ws_forward(pbd_ctx* ctx, const float* X, int n, float* Y);
+PBD_API int pbd_ws_gradient(pbd_ctx* ctx, long long first, int n, float* grad, double* loss);
+PBD_API int pbd_ws_adam_step(pbd_ctx* ctx, const float* grad, float lr);
+PBD_API int pbd_ws_train(pbd_ctx* ctx, int epochs, int batch, float lr, uint64_t seed, double* last_loss);
+PBD_API int pbd_ws_enable(pbd_ctx* ctx, int on);
+PBD_API int pbd_ws_last_used(pbd_ctx* ctx, int cloth, int* used);
+
 /* ---- measurement (no counterpart in the reference, whose only timing is the
  * host Timer around calls, timer.cpp:10-20) --------------------------------- */
 typedef struct {

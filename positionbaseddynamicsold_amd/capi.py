@@ -113,6 +113,20 @@ SIGNATURES = {
     "pbd_ccd_resolve": [_P, _fp, _fp, _F, _I],
     "pbd_set_collision": [_P, C.POINTER(CollisionParams)],
     "pbd_last_contacts": [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+    "pbd_ws_num_params": [_ip],
+    "pbd_ws_init": [_P, C.c_uint64],
+    "pbd_ws_get_params": [_P, _fp],
+    "pbd_ws_set_params": [_P, _fp],
+    "pbd_ws_collect": [_P, _I, _I],
+    "pbd_ws_num_samples": [_P, C.POINTER(C.c_longlong)],
+    "pbd_ws_get_samples": [_P, C.c_longlong, C.c_longlong, _fp, _fp],
+    "pbd_ws_set_samples": [_P, _fp, _fp, C.c_longlong],
+    "pbd_ws_forward": [_P, _fp, _I, _fp],
+    "pbd_ws_gradient": [_P, C.c_longlong, _I, _fp, _dp],
+    "pbd_ws_adam_step": [_P, _fp, _F],
+    "pbd_ws_train": [_P, _I, _I, _F, C.c_uint64, _dp],
+    "pbd_ws_enable": [_P, _I],
+    "pbd_ws_last_used": [_P, _I, _ip],
     "pbd_profile_enable": [_P, _I],
     "pbd_profile_read": [_P, C.POINTER(Profile), _I],
     "pbd_bench_copy": [_P, C.c_longlong, _I, _dp],
@@ -354,6 +368,74 @@ class Context:
         a, b = C.c_longlong(), C.c_longlong()
         check(self.lib.pbd_last_contacts(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    # ---- learned warm start --------------------------------------------
+    @staticmethod
+    def ws_num_params() -> int:
+        n = C.c_int()
+        check(load().pbd_ws_num_params(C.byref(n)))
+        return n.value
+
+    def ws_init(self, seed: int = 0):
+        check(self.lib.pbd_ws_init(self.h, seed))
+
+    def ws_params(self) -> np.ndarray:
+        out = np.zeros(self.ws_num_params(), np.float32)
+        check(self.lib.pbd_ws_get_params(self.h, fptr(out)))
+        return out
+
+    def ws_set_params(self, prm: np.ndarray):
+        prm = np.ascontiguousarray(prm, np.float32)
+        check(self.lib.pbd_ws_set_params(self.h, fptr(prm)))
+
+    def ws_collect(self, sweeps: int, substeps: int):
+        check(self.lib.pbd_ws_collect(self.h, sweeps, substeps))
+
+    def ws_num_samples(self) -> int:
+        n = C.c_longlong()
+        check(self.lib.pbd_ws_num_samples(self.h, C.byref(n)))
+        return n.value
+
+    def ws_samples(self, first: int = 0, n: int | None = None):
+        n = self.ws_num_samples() - first if n is None else n
+        X = np.zeros((n, 22), np.float32)
+        Y = np.zeros((n, 3), np.float32)
+        check(self.lib.pbd_ws_get_samples(self.h, first, n, fptr(X), fptr(Y)))
+        return X, Y
+
+    def ws_set_samples(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        check(self.lib.pbd_ws_set_samples(self.h, fptr(X), fptr(Y), len(X)))
+
+    def ws_forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 22)
+        Y = np.zeros((len(X), 3), np.float32)
+        check(self.lib.pbd_ws_forward(self.h, fptr(X), len(X), fptr(Y)))
+        return Y
+
+    def ws_gradient(self, first: int, n: int):
+        g = np.zeros(self.ws_num_params(), np.float32)
+        loss = C.c_double()
+        check(self.lib.pbd_ws_gradient(self.h, first, n, fptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def ws_adam_step(self, grad, lr: float):
+        grad = np.ascontiguousarray(grad, np.float32)
+        check(self.lib.pbd_ws_adam_step(self.h, fptr(grad), lr))
+
+    def ws_train(self, epochs: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> float:
+        loss = C.c_double()
+        check(self.lib.pbd_ws_train(self.h, epochs, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def ws_enable(self, on: bool = True):
+        check(self.lib.pbd_ws_enable(self.h, 1 if on else 0))
+
+    def ws_last_used(self, cloth: int) -> int:
+        u = C.c_int()
+        check(self.lib.pbd_ws_last_used(self.h, cloth, C.byref(u)))
+        return u.value
 
     def profile(self, on: bool = True):
         check(self.lib.pbd_profile_enable(self.h, 1 if on else 0))

@@ -58,6 +58,19 @@ struct pbd_ctx {
     Broadphase cbp;
     DBuf<int> gtris;
     int gnt = 0;
+    // learned warm start
+    struct WarmStart {
+        bool params_ready = false, enabled = false, nbr_built = false;
+        DBuf<float> prm, m, v, grad, partial, X, Y;
+        DBuf<unsigned char> W;
+        int adam_step = 0;
+        DBuf<int> nbr, perm;
+        DBuf<float> nbr_rest;
+        long long ns = 0;
+        DBuf<float4> delta, psave;
+        DBuf<unsigned long long> r0, r1;
+        std::vector<int> used;
+    } ws;
     // profiling
     bool prof_on = false;
     pbd_profile prof{};
@@ -127,6 +140,14 @@ void fill_cloth_dev(pbd_ctx* c) {
         d.nv = k.nv;
         d.ktab = (int)i * c->kmax;
         d.whole = k.whole ? 1 : 0;
+        double sum = 0.0;
+        int ne = 0;
+        for (int p = 0; p < k.np(); p++)
+            if (k.prim_list[2 * p + 1] == 2) {
+                sum += k.rest[p];
+                ne++;
+            }
+        d.hscale = ne ? (float)(sum / ne) : 1.0f;
     }
     c->cl.upload(c->cl_h, c->stream);
 }
@@ -453,6 +474,62 @@ void detect_contacts(pbd_ctx* c, const float4* p) {
     c->cbp.build(s, c->ccfg, p, c->gtris.ptr, c->gnt);
     c->cbp.pairs(s, p, c->gtris.ptr, c->nv_total);
     c->cbp.narrow(s, c->x.ptr, p, c->gtris.ptr, c->coll.thickness);
+}
+
+// ---- learned warm start ------------------------------------------------------
+// First 6 distance prims of each vertex in prim order (the feature one-ring).
+void ws_build_nbr(pbd_ctx* c) {
+    if (c->ws.nbr_built) return;
+    std::vector<int> nbr((size_t)6 * c->nv_total, -1);
+    std::vector<float> rest((size_t)6 * c->nv_total, 0.0f);
+    std::vector<unsigned char> cnt(c->nv_total, 0);
+    for (auto& k : c->cloths)
+        for (int p = 0; p < k.np(); p++) {
+            if (k.prim_list[2 * p + 1] != 2) continue;
+            int a = k.indices[k.prim_list[2 * p]] + k.vbase, b = k.indices[k.prim_list[2 * p] + 1] + k.vbase;
+            if (cnt[a] < 6) { nbr[6 * (size_t)a + cnt[a]] = b; rest[6 * (size_t)a + cnt[a]] = k.rest[p]; cnt[a]++; }
+            if (cnt[b] < 6) { nbr[6 * (size_t)b + cnt[b]] = a; rest[6 * (size_t)b + cnt[b]] = k.rest[p]; cnt[b]++; }
+        }
+    c->ws.nbr.upload(nbr, c->stream);
+    c->ws.nbr_rest.upload(rest, c->stream);
+    c->ws.delta.alloc(c->nv_total);
+    c->ws.psave.alloc(c->nv_total);
+    c->ws.r0.alloc(c->cloths.size());
+    c->ws.r1.alloc(c->cloths.size());
+    c->ws.nbr_built = true;
+}
+
+void ws_require_params(pbd_ctx* c) {
+    PBD_CHECK(c->ws.params_ready, PBD_ERR_STATE, "warm-start weights not initialised (pbd_ws_init or pbd_ws_set_params)");
+}
+
+// Applies the prediction to the current p of every cloth, keeping it only
+// for cloths whose residual does not rise above the unwarmed start.
+void ws_apply_prediction(pbd_ctx* c) {
+    hipStream_t s = c->stream;
+    int ncl = (int)c->cloths.size();
+    ws_build_nbr(c);
+    float4* p = c->pbuf[c->pcur].ptr;
+    ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, c->ws.nbr.ptr, c->ws.nbr_rest.ptr, c->nv_total,
+             c->ws.delta.ptr, nullptr);
+    launch_fill_u64(s, c->ws.r0.ptr, ncl, 0ull);
+    launch_fill_u64(s, c->ws.r1.ptr, ncl, 0ull);
+    ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr);
+    ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr, c->nv_total);
+    {
+        // per-cloth record of whether the prediction was kept
+        std::vector<unsigned long long> a, b;
+        c->ws.r0.download(a, s);
+        c->ws.r1.download(b, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        c->ws.used.assign(ncl, 0);
+        for (int i = 0; i < ncl; i++) {
+            double x, y;
+            memcpy(&x, &a[i], 8);
+            memcpy(&y, &b[i], 8);
+            c->ws.used[i] = y <= x ? 1 : 0;
+        }
+    }
 }
 
 // ---- solver orchestration ---------------------------------------------------
@@ -921,6 +998,11 @@ int pbd_step(pbd_ctx* ctx, int substeps) {
         for (int s = 0; s < substeps; s++) {
             if (ctx->coll.enabled) {
                 step_ccd(ctx, dt);
+            } else if (ctx->ws.enabled) {
+                advect_all(ctx, dt);
+                ws_apply_prediction(ctx);
+                project(ctx, ctx->sp.iterations, false, dt);
+                integrate_all(ctx, dt, false);
             } else if (ctx->sp.path == PBD_PATH_GLOBAL) {
                 advect_all(ctx, dt);
                 project(ctx, ctx->sp.iterations, false, dt);
@@ -1120,6 +1202,262 @@ int pbd_last_contacts(pbd_ctx* ctx, long long* nc, long long* np) {
         PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
         if (nc) *nc = ctx->cbp.ncontacts;
         if (np) *np = ctx->cbp.npairs;
+    });
+}
+
+int pbd_ws_init(pbd_ctx* ctx, uint64_t seed) {
+    return guard([&] {
+        PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
+        set_device(ctx);
+        int n = mlp_nparam();
+        std::vector<float> w(n, 0.0f);
+        uint64_t st = seed;
+        auto fill = [&](int off, int rows, int cols, float scale) {
+            float a = scale * std::sqrt(6.0f / (float)(rows + cols));
+            for (int i = 0; i < rows * cols; i++) w[off + i] = a * uniform_pm1(&st);
+        };
+        // Glorot-uniform weights, zero biases; a small output layer so the
+        // untrained prediction starts near zero
+        fill(0, 64, 22, 1.0f);
+        fill(64 * 22 + 64, 64, 64, 1.0f);
+        fill(64 * 22 + 64 + 4096 + 64, 3, 64, 0.1f);
+        auto& W = ctx->ws;
+        W.prm.upload(w, ctx->stream);
+        W.m.upload(std::vector<float>(n, 0.0f), ctx->stream);
+        W.v.upload(std::vector<float>(n, 0.0f), ctx->stream);
+        W.grad.alloc(n + 1);
+        W.W.alloc(mlp_wbytes());
+        mlp_pack(ctx->stream, W.prm.ptr, W.W.ptr);
+        W.adam_step = 0;
+        W.params_ready = true;
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_ws_num_params(int* n) {
+    return guard([&] {
+        PBD_CHECK(n, PBD_ERR_ARG, "null output");
+        *n = mlp_nparam();
+    });
+}
+
+int pbd_ws_get_params(pbd_ctx* ctx, float* out) {
+    return guard([&] {
+        PBD_CHECK(ctx && out, PBD_ERR_ARG, "null argument");
+        ws_require_params(ctx);
+        set_device(ctx);
+        PBD_HIP(hipMemcpyAsync(out, ctx->ws.prm.ptr, sizeof(float) * mlp_nparam(), hipMemcpyDeviceToHost, ctx->stream));
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_ws_set_params(pbd_ctx* ctx, const float* in) {
+    return guard([&] {
+        PBD_CHECK(ctx && in, PBD_ERR_ARG, "null argument");
+        set_device(ctx);
+        int n = mlp_nparam();
+        auto& W = ctx->ws;
+        W.prm.upload(std::vector<float>(in, in + n), ctx->stream);
+        if (!W.params_ready) {
+            W.m.upload(std::vector<float>(n, 0.0f), ctx->stream);
+            W.v.upload(std::vector<float>(n, 0.0f), ctx->stream);
+            W.grad.alloc(n + 1);
+            W.W.alloc(mlp_wbytes());
+        }
+        mlp_pack(ctx->stream, W.prm.ptr, W.W.ptr);
+        W.params_ready = true;
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_ws_forward(pbd_ctx* ctx, const float* X, int n, float* Y) {
+    return guard([&] {
+        PBD_CHECK(ctx && (X || !n) && (Y || !n) && n >= 0, PBD_ERR_ARG, "bad argument");
+        ws_require_params(ctx);
+        set_device(ctx);
+        DBuf<float> dx, dy;
+        dx.upload(std::vector<float>(X, X + 22ll * n), ctx->stream);
+        dy.alloc(3ll * std::max(n, 1));
+        mlp_forward(ctx->stream, ctx->ws.W.ptr, dx.ptr, n, dy.ptr);
+        if (n) PBD_HIP(hipMemcpyAsync(Y, dy.ptr, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_ws_set_samples(pbd_ctx* ctx, const float* X, const float* Y, long long n) {
+    return guard([&] {
+        PBD_CHECK(ctx && (X || !n) && (Y || !n) && n >= 0 && n < (1ll << 31), PBD_ERR_ARG, "bad argument");
+        set_device(ctx);
+        ctx->ws.X.upload(std::vector<float>(X, X + 22 * n), ctx->stream);
+        ctx->ws.Y.upload(std::vector<float>(Y, Y + 3 * n), ctx->stream);
+        ctx->ws.ns = n;
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_ws_num_samples(pbd_ctx* ctx, long long* n) {
+    return guard([&] {
+        PBD_CHECK(ctx && n, PBD_ERR_ARG, "null argument");
+        *n = ctx->ws.ns;
+    });
+}
+
+int pbd_ws_get_samples(pbd_ctx* ctx, long long first, long long n, float* X, float* Y) {
+    return guard([&] {
+        PBD_CHECK(ctx && first >= 0 && n >= 0 && first + n <= ctx->ws.ns, PBD_ERR_ARG, "sample range out of bounds");
+        set_device(ctx);
+        if (n && X) PBD_HIP(hipMemcpyAsync(X, ctx->ws.X.ptr + 22 * first, sizeof(float) * 22 * n, hipMemcpyDeviceToHost, ctx->stream));
+        if (n && Y) PBD_HIP(hipMemcpyAsync(Y, ctx->ws.Y.ptr + 3 * first, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+// Runs `substeps` plain substeps of every cloth with `sweeps` Gauss-Seidel
+// sweeps each and records, per vertex, the features at the pre-projection
+// state and the correction those sweeps produced.
+int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(sweeps >= 1 && substeps >= 0, PBD_ERR_ARG, "bad sweep / substep count");
+        PBD_CHECK(!ctx->coll.enabled, PBD_ERR_STATE, "sample collection runs without CCD");
+        ws_require_params(ctx);
+        set_device(ctx);
+        ws_build_nbr(ctx);
+        hipStream_t s = ctx->stream;
+        auto& W = ctx->ws;
+        long long need = W.ns + (long long)substeps * ctx->nv_total;
+        PBD_CHECK(need < (1ll << 31), PBD_ERR_LIMIT, "too many samples");
+        if ((long long)W.X.n < 22 * need) {
+            DBuf<float> nx, ny;
+            nx.alloc(22 * need);
+            ny.alloc(3 * need);
+            if (W.ns) {
+                PBD_HIP(hipMemcpyAsync(nx.ptr, W.X.ptr, sizeof(float) * 22 * W.ns, hipMemcpyDeviceToDevice, s));
+                PBD_HIP(hipMemcpyAsync(ny.ptr, W.Y.ptr, sizeof(float) * 3 * W.ns, hipMemcpyDeviceToDevice, s));
+            }
+            std::swap(W.X.ptr, nx.ptr);
+            std::swap(W.X.n, nx.n);
+            std::swap(W.Y.ptr, ny.ptr);
+            std::swap(W.Y.n, ny.n);
+        }
+        if (sweeps > ctx->kmax) {
+            ctx->sp.iterations = sweeps;
+            fill_ktabs(ctx);
+            fill_cloth_dev(ctx);
+        }
+        float dt = ctx->sp.dt;
+        for (int k = 0; k < substeps; k++) {
+            advect_all(ctx, dt);
+            float4* p = ctx->pbuf[ctx->pcur].ptr;
+            ws_infer(s, W.W.ptr, ctx->cl.ptr, ctx->vcloth.ptr, ctx->x.ptr, p, W.nbr.ptr, W.nbr_rest.ptr, ctx->nv_total,
+                     W.delta.ptr, W.X.ptr + 22 * W.ns);
+            PBD_HIP(hipMemcpyAsync(W.psave.ptr, p, sizeof(float4) * ctx->nv_total, hipMemcpyDeviceToDevice, s));
+            double tol = ctx->sp.tol;
+            ctx->sp.tol = -1.0;
+            project(ctx, sweeps, false, dt);
+            ctx->sp.tol = tol;
+            ws_target(s, ctx->cl.ptr, ctx->vcloth.ptr, ctx->pbuf[ctx->pcur].ptr, W.psave.ptr, ctx->nv_total,
+                      W.Y.ptr + 3 * W.ns);
+            integrate_all(ctx, dt, false);
+            W.ns += ctx->nv_total;
+        }
+        PBD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// Gradient of the mean loss over samples [first, first+n) (tests).
+int pbd_ws_gradient(pbd_ctx* ctx, long long first, int n, float* grad, double* loss) {
+    return guard([&] {
+        PBD_CHECK(ctx && grad && n > 0 && first >= 0 && first + n <= ctx->ws.ns, PBD_ERR_ARG, "bad sample range");
+        ws_require_params(ctx);
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        std::vector<int> idx(n);
+        for (int i = 0; i < n; i++) idx[i] = (int)(first + i);
+        auto& W = ctx->ws;
+        W.perm.upload(idx, s);
+        W.partial.alloc((size_t)(((n + 15) / 16 + 2) * (mlp_nparam() + 1)));
+        mlp_grad(s, W.W.ptr, W.X.ptr, W.Y.ptr, W.perm.ptr, n, 1.0f / n, W.partial.ptr, W.grad.ptr);
+        std::vector<float> g;
+        W.grad.download(g, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        std::copy(g.begin(), g.begin() + mlp_nparam(), grad);
+        if (loss) *loss = g[mlp_nparam()];
+    });
+}
+
+int pbd_ws_adam_step(pbd_ctx* ctx, const float* grad, float lr) {
+    return guard([&] {
+        PBD_CHECK(ctx && grad, PBD_ERR_ARG, "null argument");
+        ws_require_params(ctx);
+        set_device(ctx);
+        auto& W = ctx->ws;
+        std::vector<float> g(grad, grad + mlp_nparam());
+        g.push_back(0.0f);
+        W.grad.upload(g, ctx->stream);
+        mlp_adam(ctx->stream, W.prm.ptr, W.grad.ptr, W.m.ptr, W.v.ptr, lr, 0.9f, 0.999f, 1e-8f, ++W.adam_step);
+        mlp_pack(ctx->stream, W.prm.ptr, W.W.ptr);
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+// Minibatch Adam over the collected samples, reshuffled every epoch.
+int pbd_ws_train(pbd_ctx* ctx, int epochs, int batch, float lr, uint64_t seed, double* last_loss) {
+    return guard([&] {
+        PBD_CHECK(ctx && epochs >= 0 && batch >= 16, PBD_ERR_ARG, "bad training arguments");
+        ws_require_params(ctx);
+        PBD_CHECK(ctx->ws.ns > 0, PBD_ERR_STATE, "no samples");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        auto& W = ctx->ws;
+        long long n = W.ns;
+        std::vector<int> perm(n);
+        for (long long i = 0; i < n; i++) perm[i] = (int)i;
+        uint64_t st = seed;
+        W.partial.alloc((size_t)(((batch + 15) / 16 + 2) * (mlp_nparam() + 1)));
+        double loss_sum = 0.0;
+        long long loss_n = 0;
+        std::vector<float> g(1);
+        for (int e = 0; e < epochs; e++) {
+            for (long long i = n - 1; i > 0; i--) {
+                long long j = (long long)(splitmix64(&st) % (uint64_t)(i + 1));
+                std::swap(perm[i], perm[j]);
+            }
+            W.perm.upload(perm, s);
+            loss_sum = 0.0;
+            loss_n = 0;
+            for (long long b0 = 0; b0 + batch <= n || (b0 == 0 && n < batch); b0 += batch) {
+                int m = (int)std::min<long long>(batch, n - b0);
+                mlp_grad(s, W.W.ptr, W.X.ptr, W.Y.ptr, W.perm.ptr + b0, m, 1.0f / m, W.partial.ptr, W.grad.ptr);
+                mlp_adam(s, W.prm.ptr, W.grad.ptr, W.m.ptr, W.v.ptr, lr, 0.9f, 0.999f, 1e-8f, ++W.adam_step);
+                mlp_pack(s, W.prm.ptr, W.W.ptr);
+                if (e == epochs - 1) {
+                    float lv = 0.0f;
+                    PBD_HIP(hipMemcpyAsync(&lv, W.grad.ptr + mlp_nparam(), sizeof(float), hipMemcpyDeviceToHost, s));
+                    PBD_HIP(hipStreamSynchronize(s));
+                    loss_sum += lv;
+                    loss_n++;
+                }
+            }
+        }
+        PBD_HIP(hipStreamSynchronize(s));
+        if (last_loss) *last_loss = loss_n ? loss_sum / loss_n : 0.0;
+    });
+}
+
+int pbd_ws_enable(pbd_ctx* ctx, int on) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        if (on) ws_require_params(ctx);
+        ctx->ws.enabled = on != 0;
+    });
+}
+
+int pbd_ws_last_used(pbd_ctx* ctx, int cloth, int* used) {
+    return guard([&] {
+        cloth_of(ctx, cloth);
+        PBD_CHECK(used, PBD_ERR_ARG, "null output");
+        *used = cloth < (int)ctx->ws.used.size() ? ctx->ws.used[cloth] : 0;
     });
 }
 

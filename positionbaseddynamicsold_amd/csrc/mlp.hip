@@ -1,0 +1,559 @@
+// Learned warm start: a per-vertex MLP 22 -> 64 -> 64 -> 3 (GELU, tanh form)
+// with BF16 weights and activations on MFMA (v_mfma_f32_16x16x16_bf16) and
+// FP32 accumulation; forward, backward and Adam kernels.  No counterpart in
+// the reference.
+//
+// Feature vector of vertex i (cloth mean rest length h, predicted p, previous x):
+//   [0:3]   (p_i - x_i) / h
+//   [3:21]  for its first 6 distance prims (prim order), neighbour j:
+//           ((|e| - L) / L) * e / |e|,  e = p_j - p_i   (zeros when absent)
+//   [21]    1 if the vertex is pinned (w = 0), else 0
+// Output y (3): the predicted Gauss-Seidel correction divided by h.
+//
+// Parameter layout (FP32 master copy, row-major W[out][in]):
+//   W1[64][22] b1[64] W2[64][64] b2[64] W3[3][64] b3[3]
+#include <cmath>
+
+#include "pbd_internal.hpp"
+
+namespace pbd {
+
+// Makes LDS writes of this wave visible to its other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// bf16 round-to-nearest-even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ void gelu(float x, float* y, float* dy) {
+    const float c = 0.7978845608f, k = 0.044715f;
+    float u = c * (x + k * x * x * x);
+    float th = tanhf(u);
+    *y = 0.5f * x * (1.0f + th);
+    if (dy) *dy = 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * c * (1.0f + 3.0f * k * x * x);
+}
+
+// One wave: C[16 x 16*nt] = A[16 x K] * B, with A row-major (lda) and B given
+// as Bt = B^T row-major [16*nt x K] (ldb), all BF16 in LDS.  acc[t][i] is
+// C[4*(lane/16) + i][16*t + lane%16].
+__device__ __forceinline__ void mm16(const uint16_t* A, int lda, const uint16_t* Bt, int ldb, int K, int nt,
+                                     f32x4* acc) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    for (int t = 0; t < nt; t++) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 a = *(const bf16x4*)(A + r * lda + k0 + 4 * q);
+        for (int t = 0; t < nt; t++) {
+            bf16x4 b = *(const bf16x4*)(Bt + (t * 16 + r) * ldb + k0 + 4 * q);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+    }
+}
+
+// Weights as the kernels read them (BF16, K padded to multiples of 16).
+struct alignas(16) MlpW {
+    uint16_t w1[64 * 32];    // W1[out][in], in padded 22 -> 32
+    uint16_t w2[64 * 64];    // W2[out][in]
+    uint16_t w3[16 * 64];    // W3[out][in], out padded 3 -> 16
+    uint16_t w2t[64 * 64];   // W2^T [in][out]
+    float b1[64], b2[64], b3[16];
+};
+
+constexpr int P_W1 = 0, P_B1 = P_W1 + 64 * 22, P_W2 = P_B1 + 64, P_B2 = P_W2 + 64 * 64, P_W3 = P_B2 + 64,
+              P_B3 = P_W3 + 3 * 64, NPARAM = P_B3 + 3;
+static_assert(NPARAM == 5827, "parameter count");
+
+__global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ prm, MlpW* __restrict__ W) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 64 * 32) {
+        int o = i / 32, k = i % 32;
+        W->w1[i] = k < 22 ? f2bf(prm[P_W1 + o * 22 + k]) : 0;
+    }
+    if (i < 64 * 64) {
+        int o = i / 64, k = i % 64;
+        uint16_t v = f2bf(prm[P_W2 + o * 64 + k]);
+        W->w2[i] = v;
+        W->w2t[k * 64 + o] = v;
+    }
+    if (i < 16 * 64) {
+        int o = i / 64, k = i % 64;
+        W->w3[i] = o < 3 ? f2bf(prm[P_W3 + o * 64 + k]) : 0;
+    }
+    if (i < 64) {
+        W->b1[i] = prm[P_B1 + i];
+        W->b2[i] = prm[P_B2 + i];
+    }
+    if (i < 16) W->b3[i] = i < 3 ? prm[P_B3 + i] : 0.0f;
+}
+
+// Per-wave scratch of the forward pass.
+struct alignas(16) FwdLds {
+    uint16_t xs[16 * 32];
+    uint16_t a1[16 * 64];
+    uint16_t a2[16 * 64];
+    float h1[16 * 64];
+    float h2[16 * 64];
+    float y[16 * 16];
+};
+
+__device__ __forceinline__ void load_weights(MlpW* sW, const MlpW* __restrict__ gW) {
+    const uint4* s = (const uint4*)gW;
+    uint4* d = (uint4*)sW;
+    for (int i = threadIdx.x; i < (int)(sizeof(MlpW) / 16); i += blockDim.x) d[i] = s[i];
+    __syncthreads();
+}
+
+// Forward of 16 rows whose BF16 inputs are already in F.xs.  Fills h1/a1/h2/a2
+// and y (16 x 16, columns 0..2 meaningful).
+__device__ void forward16(const MlpW& W, FwdLds& F) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    f32x4 acc[4];
+    mm16(F.xs, 32, W.w1, 32, 32, 4, acc);
+    for (int t = 0; t < 4; t++)
+        for (int i = 0; i < 4; i++) {
+            int row = 4 * q + i, col = 16 * t + r;
+            float h = acc[t][i] + W.b1[col], a;
+            gelu(h, &a, nullptr);
+            F.h1[row * 64 + col] = h;
+            F.a1[row * 64 + col] = f2bf(a);
+        }
+    wave_sync();
+    mm16(F.a1, 64, W.w2, 64, 64, 4, acc);
+    for (int t = 0; t < 4; t++)
+        for (int i = 0; i < 4; i++) {
+            int row = 4 * q + i, col = 16 * t + r;
+            float h = acc[t][i] + W.b2[col], a;
+            gelu(h, &a, nullptr);
+            F.h2[row * 64 + col] = h;
+            F.a2[row * 64 + col] = f2bf(a);
+        }
+    wave_sync();
+    mm16(F.a2, 64, W.w3, 64, 64, 1, acc);
+    for (int i = 0; i < 4; i++) {
+        int row = 4 * q + i, col = r;
+        F.y[row * 16 + col] = acc[0][i] + W.b3[col];
+    }
+    wave_sync();
+}
+
+// ---------------------------------------------------------------------------
+// batch forward on stored samples (evaluation / tests)
+// ---------------------------------------------------------------------------
+constexpr int FWD_WAVES = 4;
+
+__global__ __launch_bounds__(256) void k_mlp_forward(const MlpW* __restrict__ gW, const float* __restrict__ X,
+                                                     int n, float* __restrict__ Y) {
+    __shared__ MlpW W;
+    __shared__ FwdLds F[FWD_WAVES];
+    load_weights(&W, gW);
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    FwdLds& f = F[wv];
+    int base = (blockIdx.x * FWD_WAVES + wv) * 16;
+    if (base >= n) return;
+    for (int e = l; e < 16 * 32; e += 64) {
+        int row = e / 32, k = e % 32, s = base + row;
+        f.xs[e] = (k < 22 && s < n) ? f2bf(X[(size_t)s * 22 + k]) : 0;
+    }
+    wave_sync();
+    forward16(W, f);
+    if (l < 48) {
+        int row = l / 3, o = l % 3, s = base + row;
+        if (s < n) Y[(size_t)s * 3 + o] = f.y[row * 16 + o];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// training: forward + backward over 16 samples per wave; per-wave parameter
+// gradients written to partial[wave][NPARAM+1] (last slot: loss)
+// ---------------------------------------------------------------------------
+struct alignas(16) BwdLds {
+    uint16_t xT[32 * 16];     // X^T [in][sample]
+    uint16_t a1T[64 * 16];    // A1^T
+    uint16_t a2T[64 * 16];    // A2^T
+    uint16_t dyT[16 * 16];    // dY^T [out16][sample]
+    uint16_t dh2[16 * 64];    // dH2 [sample][o]
+    uint16_t dh2T[64 * 16];   // dH2^T
+    uint16_t dh1T[64 * 16];   // dH1^T
+    float dh2f[16 * 64];
+    float dh1f[16 * 64];
+    float dy[16 * 4];
+};
+
+constexpr int TRAIN_WAVES = 2;
+
+__global__ __launch_bounds__(128) void k_mlp_train(const MlpW* __restrict__ gW, const float* __restrict__ X,
+                                                   const float* __restrict__ T, const int* __restrict__ idx,
+                                                   int nbatch, float inv_total, float* __restrict__ partial) {
+    __shared__ MlpW W;
+    __shared__ FwdLds F[TRAIN_WAVES];
+    __shared__ BwdLds B[TRAIN_WAVES];
+    load_weights(&W, gW);
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    FwdLds& f = F[wv];
+    BwdLds& b = B[wv];
+    const int gw = blockIdx.x * TRAIN_WAVES + wv;
+    const int base = gw * 16;
+    float* out = partial + (size_t)gw * (NPARAM + 1);
+    if (base >= nbatch) return;
+    // gather the 16 samples
+    for (int e = l; e < 16 * 32; e += 64) {
+        int row = e / 32, k = e % 32, s = base + row;
+        float v = (k < 22 && s < nbatch) ? X[(size_t)idx[s] * 22 + k] : 0.0f;
+        uint16_t hv = f2bf(v);
+        f.xs[e] = hv;
+        b.xT[k * 16 + row] = hv;
+    }
+    wave_sync();
+    forward16(W, f);
+    // dY = (y - t) / N (loss = sum 0.5 |y - t|^2 / N)
+    float lossp = 0.0f;
+    if (l < 48) {
+        int row = l / 3, o = l % 3, s = base + row;
+        float dy = 0.0f;
+        if (s < nbatch) {
+            float e = f.y[row * 16 + o] - T[(size_t)idx[s] * 3 + o];
+            lossp = 0.5f * e * e * inv_total;
+            dy = e * inv_total;
+        }
+        b.dy[row * 4 + o] = dy;
+    }
+    wave_sync();
+    for (int e = l; e < 16 * 16; e += 64) {
+        int o = e / 16, s = e % 16;
+        b.dyT[e] = o < 3 ? f2bf(b.dy[s * 4 + o]) : 0;
+    }
+    // transposed activations for the weight-gradient products
+    for (int e = l; e < 16 * 64; e += 64) {
+        int row = e / 64, c = e % 64;
+        b.a1T[c * 16 + row] = f.a1[e];
+        b.a2T[c * 16 + row] = f.a2[e];
+    }
+    wave_sync();
+    // dA2 = dY W3 (VALU, 3 terms), dH2 = dA2 * gelu'(H2)
+    for (int e = l; e < 16 * 64; e += 64) {
+        int row = e / 64, c = e % 64;
+        int s = base + row;
+        float da = 0.0f;
+        for (int o = 0; o < 3; o++) da += b.dy[row * 4 + o] * bf2f(W.w3[o * 64 + c]);
+        float g, gd;
+        gelu(f.h2[e], &g, &gd);
+        float d = s < nbatch ? da * gd : 0.0f;
+        b.dh2f[e] = d;
+        uint16_t hv = f2bf(d);
+        b.dh2[e] = hv;
+        b.dh2T[c * 16 + row] = hv;
+    }
+    wave_sync();
+    // dA1 = dH2 W2 (MFMA with B^T = W2^T stored as w2t), dH1 = dA1 * gelu'(H1)
+    {
+        f32x4 acc[4];
+        mm16(b.dh2, 64, W.w2t, 64, 64, 4, acc);
+        for (int t = 0; t < 4; t++)
+            for (int i = 0; i < 4; i++) {
+                int row = 4 * q + i, col = 16 * t + r;
+                float g, gd;
+                gelu(f.h1[row * 64 + col], &g, &gd);
+                float d = (base + row < nbatch) ? acc[t][i] * gd : 0.0f;
+                b.dh1f[row * 64 + col] = d;
+                b.dh1T[col * 16 + row] = f2bf(d);
+            }
+    }
+    wave_sync();
+    // weight gradients: dW = dH^T A (MFMA, K = the 16 samples)
+    {
+        f32x4 acc[4];
+        // dW3 [3 x 64]
+        mm16(b.dyT, 16, b.a2T, 16, 16, 4, acc);
+        for (int t = 0; t < 4; t++)
+            for (int i = 0; i < 4; i++) {
+                int o = 4 * q + i, c = 16 * t + r;
+                if (o < 3) out[P_W3 + o * 64 + c] = acc[t][i];
+            }
+        // dW2 [64 x 64]: four row tiles of o
+        for (int ot = 0; ot < 4; ot++) {
+            mm16(b.dh2T + ot * 16 * 16, 16, b.a1T, 16, 16, 4, acc);
+            for (int t = 0; t < 4; t++)
+                for (int i = 0; i < 4; i++) {
+                    int o = ot * 16 + 4 * q + i, c = 16 * t + r;
+                    out[P_W2 + o * 64 + c] = acc[t][i];
+                }
+        }
+        // dW1 [64 x 22]: K padded 32
+        for (int ot = 0; ot < 4; ot++) {
+            mm16(b.dh1T + ot * 16 * 16, 16, b.xT, 16, 16, 2, acc);
+            for (int t = 0; t < 2; t++)
+                for (int i = 0; i < 4; i++) {
+                    int o = ot * 16 + 4 * q + i, c = 16 * t + r;
+                    if (c < 22) out[P_W1 + o * 22 + c] = acc[t][i];
+                }
+        }
+    }
+    // bias gradients (FP32 sums over the 16 samples) and the loss
+    {
+        float s1 = 0.0f, s2 = 0.0f;
+        for (int row = 0; row < 16; row++) {
+            s1 += b.dh1f[row * 64 + l];
+            s2 += b.dh2f[row * 64 + l];
+        }
+        out[P_B1 + l] = s1;
+        out[P_B2 + l] = s2;
+        if (l < 3) {
+            float s3 = 0.0f;
+            for (int row = 0; row < 16; row++) s3 += b.dy[row * 4 + l];
+            out[P_B3 + l] = s3;
+        }
+        for (int off = 32; off > 0; off >>= 1) lossp += __shfl_xor(lossp, off);
+        if (l == 0) out[NPARAM] = lossp;
+    }
+}
+
+// Sum of the per-wave partials in wave order (deterministic).
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partial, int nw,
+                                                    float* __restrict__ grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > NPARAM) return;
+    float s = 0.0f;
+    for (int w = 0; w < nw; w++) s += partial[(size_t)w * (NPARAM + 1) + i];
+    grad[i] = s;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters; c1 = 1 - b1^t, c2 = 1 - b2^t.
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, int n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float mh = mi / c1, vh = vi / c2;
+    w[i] = w[i] - lr * mh / (sqrtf(vh) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// warm start inside the substep
+// ---------------------------------------------------------------------------
+// Features of 16 vertices into F.xs (4 lanes per vertex), then forward, then
+// delta = h * y (zero for pinned vertices).
+__global__ __launch_bounds__(256) void k_ws_infer(const MlpW* __restrict__ gW, const ClothDev* __restrict__ cl,
+                                                  const int* __restrict__ vcloth, const float4* __restrict__ x,
+                                                  const float4* __restrict__ p, const int* __restrict__ nbr,
+                                                  const float* __restrict__ nbr_rest, int nv,
+                                                  float4* __restrict__ delta, float* __restrict__ feat_out) {
+    __shared__ MlpW W;
+    __shared__ FwdLds F[FWD_WAVES];
+    __shared__ float fe[FWD_WAVES][16][24];
+    load_weights(&W, gW);
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    FwdLds& f = F[wv];
+    const int base = (blockIdx.x * FWD_WAVES + wv) * 16;
+    if (base >= nv) return;
+    {
+        const int row = l & 15, part = l >> 4, v = base + row;
+        float* fr = fe[wv][row];
+        if (v < nv) {
+            float h = cl[vcloth[v]].hscale;
+            float4 P = p[v];
+            if (part == 0) {
+                float4 X = x[v];
+                fr[0] = (P.x - X.x) / h;
+                fr[1] = (P.y - X.y) / h;
+                fr[2] = (P.z - X.z) / h;
+                fr[21] = P.w == 0.0f ? 1.0f : 0.0f;
+            }
+            for (int k = part; k < 6; k += 4) {
+                int j = nbr[(size_t)v * 6 + k];
+                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+                if (j >= 0) {
+                    float4 Q = p[j];
+                    float ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
+                    float d = sqrtf(ex * ex + ey * ey + ez * ez);
+                    float L = nbr_rest[(size_t)v * 6 + k];
+                    if (d > 0.0f) {
+                        float s = ((d - L) / L) / d;
+                        sx = s * ex;
+                        sy = s * ey;
+                        sz = s * ez;
+                    }
+                }
+                fr[3 + 3 * k] = sx;
+                fr[4 + 3 * k] = sy;
+                fr[5 + 3 * k] = sz;
+            }
+        } else if (part == 0) {
+            for (int k = 0; k < 22; k++) fr[k] = 0.0f;
+        }
+    }
+    wave_sync();
+    for (int e = l; e < 16 * 32; e += 64) {
+        int row = e / 32, k = e % 32;
+        f.xs[e] = k < 22 ? f2bf(fe[wv][row][k]) : 0;
+        if (feat_out && k < 22 && base + row < nv) feat_out[(size_t)(base + row) * 22 + k] = fe[wv][row][k];
+    }
+    wave_sync();
+    forward16(W, f);
+    if (l < 16) {
+        int v = base + l;
+        if (v < nv) {
+            float h = cl[vcloth[v]].hscale;
+            bool pinned = p[v].w == 0.0f;
+            float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (!pinned) d = make_float4(h * f.y[l * 16 + 0], h * f.y[l * 16 + 1], h * f.y[l * 16 + 2], 0.0f);
+            delta[v] = d;
+        }
+    }
+}
+
+__device__ __forceinline__ double rel_res(float4 a, float4 b, float L) {
+    double dx = (double)a.x - (double)b.x, dy = (double)a.y - (double)b.y, dz = (double)a.z - (double)b.z;
+    double d = sqrt(dx * dx + dy * dy + dz * dz);
+    return fabs(d - (double)L) / (double)L;
+}
+
+// Residual of every distance prim with and without the predicted delta,
+// max-reduced per cloth (FP64 bit patterns, r >= 0).
+__global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist, int n, const float4* __restrict__ p,
+                                                   const float4* __restrict__ delta,
+                                                   unsigned long long* __restrict__ r0,
+                                                   unsigned long long* __restrict__ r1) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int cl = -1;
+    double a = 0.0, b = 0.0;
+    if (j < n) {
+        int4 q = dist[j];
+        cl = q.w;
+        float4 P0 = p[q.x], P1 = p[q.y], D0 = delta[q.x], D1 = delta[q.y];
+        float L = __int_as_float(q.z);
+        a = rel_res(P0, P1, L);
+        float4 Q0 = make_float4(P0.x + D0.x, P0.y + D0.y, P0.z + D0.z, 0.0f);
+        float4 Q1 = make_float4(P1.x + D1.x, P1.y + D1.y, P1.z + D1.z, 0.0f);
+        b = rel_res(Q0, Q1, L);
+    }
+    int c0 = __shfl(cl, 0);
+    unsigned long long same = __ballot(cl == c0 || cl < 0);
+    if (same == ~0ull) {
+        for (int off = 32; off > 0; off >>= 1) {
+            a = fmax(a, __shfl_xor(a, off));
+            b = fmax(b, __shfl_xor(b, off));
+        }
+        if ((threadIdx.x & 63) == 0 && c0 >= 0) {
+            atomicMax(&r0[c0], (unsigned long long)__double_as_longlong(a));
+            atomicMax(&r1[c0], (unsigned long long)__double_as_longlong(b));
+        }
+    } else if (cl >= 0) {
+        atomicMax(&r0[cl], (unsigned long long)__double_as_longlong(a));
+        atomicMax(&r1[cl], (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// Fallback rule: keep the prediction only where it does not raise the cloth's
+// residual above the unwarmed start.
+__global__ __launch_bounds__(256) void k_ws_apply(const int* __restrict__ vcloth, float4* __restrict__ p,
+                                                  const float4* __restrict__ delta,
+                                                  const unsigned long long* __restrict__ r0,
+                                                  const unsigned long long* __restrict__ r1, int nv) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    int c = vcloth[v];
+    double a = __longlong_as_double((long long)r0[c]), b = __longlong_as_double((long long)r1[c]);
+    if (!(b <= a)) return;
+    float4 P = p[v], D = delta[v];
+    p[v] = make_float4(P.x + D.x, P.y + D.y, P.z + D.z, P.w);
+}
+
+// Gauss-Seidel correction target: y = (p_after - p_before) / h.
+__global__ __launch_bounds__(256) void k_ws_target(const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
+                                                   const float4* __restrict__ pa, const float4* __restrict__ pb,
+                                                   int nv, float* __restrict__ y) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    float h = cl[vcloth[v]].hscale;
+    float4 A = pa[v], B = pb[v];
+    y[(size_t)v * 3 + 0] = (A.x - B.x) / h;
+    y[(size_t)v * 3 + 1] = (A.y - B.y) / h;
+    y[(size_t)v * 3 + 2] = (A.z - B.z) / h;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static inline int nb256(long long n) { return (int)((n + 255) / 256); }
+
+int mlp_nparam() { return NPARAM; }
+size_t mlp_wbytes() { return sizeof(MlpW); }
+
+void mlp_pack(hipStream_t s, const float* prm, void* W) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(16), dim3(256), 0, s, prm, (MlpW*)W);
+    PBD_HIP(hipGetLastError());
+}
+
+void mlp_forward(hipStream_t s, const void* W, const float* X, int n, float* Y) {
+    if (n <= 0) return;
+    int blocks = (n + 16 * FWD_WAVES - 1) / (16 * FWD_WAVES);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(64 * FWD_WAVES), 0, s, (const MlpW*)W, X, n, Y);
+    PBD_HIP(hipGetLastError());
+}
+
+// Gradient of the mean loss over samples idx[0..n) into grad[NPARAM]
+// (grad[NPARAM] = loss).  partial must hold ceil(n/16) * (NPARAM+1) floats.
+void mlp_grad(hipStream_t s, const void* W, const float* X, const float* T, const int* idx, int n, float inv_total,
+              float* partial, float* grad) {
+    int waves = (n + 15) / 16;
+    int blocks = (waves + TRAIN_WAVES - 1) / TRAIN_WAVES;
+    PBD_HIP(hipMemsetAsync(partial, 0, sizeof(float) * (size_t)blocks * TRAIN_WAVES * (NPARAM + 1), s));
+    hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(64 * TRAIN_WAVES), 0, s, (const MlpW*)W, X, T, idx, n,
+                       inv_total, partial);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(nb256(NPARAM + 1)), dim3(256), 0, s, partial, waves, grad);
+    PBD_HIP(hipGetLastError());
+}
+
+void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+              int step) {
+    float c1 = (float)(1.0 - std::pow((double)b1, step)), c2 = (float)(1.0 - std::pow((double)b2, step));
+    hipLaunchKernelGGL(k_adam, dim3(nb256(NPARAM)), dim3(256), 0, s, w, g, m, v, NPARAM, lr, b1, b2, eps, c1, c2);
+    PBD_HIP(hipGetLastError());
+}
+
+void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
+              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out) {
+    if (nv <= 0) return;
+    int blocks = (nv + 16 * FWD_WAVES - 1) / (16 * FWD_WAVES);
+    hipLaunchKernelGGL(k_ws_infer, dim3(blocks), dim3(64 * FWD_WAVES), 0, s, (const MlpW*)W, cl, vcloth, x, p, nbr,
+                       nbr_rest, nv, delta, feat_out);
+    PBD_HIP(hipGetLastError());
+}
+
+void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* r0,
+               unsigned long long* r1) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_ws_resid2, dim3(nb256(n)), dim3(256), 0, s, dist, n, p, delta, r0, r1);
+    PBD_HIP(hipGetLastError());
+}
+
+void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const unsigned long long* r0,
+              const unsigned long long* r1, int nv) {
+    if (nv <= 0) return;
+    hipLaunchKernelGGL(k_ws_apply, dim3(nb256(nv)), dim3(256), 0, s, vcloth, p, delta, r0, r1, nv);
+    PBD_HIP(hipGetLastError());
+}
+
+void ws_target(hipStream_t s, const ClothDev* cl, const int* vcloth, const float4* pa, const float4* pb, int nv,
+               float* y) {
+    if (nv <= 0) return;
+    hipLaunchKernelGGL(k_ws_target, dim3(nb256(nv)), dim3(256), 0, s, cl, vcloth, pa, pb, nv, y);
+    PBD_HIP(hipGetLastError());
+}
+
+}  // namespace pbd

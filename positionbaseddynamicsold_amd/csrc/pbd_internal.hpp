@@ -81,6 +81,7 @@ struct ClothDev {
     int vbase, nv;       // vertex range in the packed arrays
     int ktab;            // offset of this cloth's per-sweep stiffness tables
     int whole;           // 1 if the cloth is a single LDS tile
+    float hscale;        // mean rest length (warm-start feature scale)
 };
 
 // One LDS tile: a set of loaded vertices (interior first) and the prims whose
@@ -190,6 +191,24 @@ struct Broadphase {
     void resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes);
 };
 void device_scan(hipStream_t s, const int* in, int* out, int n, int* tmp);
+
+// Warm-start MLP (mlp.hip).
+int mlp_nparam();
+size_t mlp_wbytes();
+void mlp_pack(hipStream_t s, const float* prm, void* W);
+void mlp_forward(hipStream_t s, const void* W, const float* X, int n, float* Y);
+void mlp_grad(hipStream_t s, const void* W, const float* X, const float* T, const int* idx, int n, float inv_total,
+              float* partial, float* grad);
+void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+              int step);
+void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
+              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out);
+void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* r0,
+               unsigned long long* r1);
+void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const unsigned long long* r0,
+              const unsigned long long* r1, int nv);
+void ws_target(hipStream_t s, const ClothDev* cl, const int* vcloth, const float4* pa, const float4* pb, int nv,
+               float* y);
 
 // Tile plan for the whole batch.
 struct TilePlan {
