@@ -150,6 +150,8 @@ PBD_API int pbd_cloth_get_state(pbd_ctx* ctx, int cloth, int which, float* xyz);
 PBD_API int pbd_cloth_set_state(pbd_ctx* ctx, int cloth, int which, const float* xyz);
 /* Convergence residual over distance prims: max and RMS of |d - L| / L (FP64). */
 PBD_API int pbd_cloth_residual(pbd_ctx* ctx, int cloth, double* max_rel, double* rms);
+/* Residual tolerance of one cloth, overriding pbd_solver_params.tol (< 0 restores it). */
+PBD_API int pbd_cloth_set_tolerance(pbd_ctx* ctx, int cloth, double tol);
 /* Sweeps the last projection of this cloth ran. */
 PBD_API int pbd_cloth_last_sweeps(pbd_ctx* ctx, int cloth, int* sweeps);
 
@@ -223,6 +225,10 @@ PBD_API int pbd_ws_forward(pbd_ctx* ctx, const float* X, int n, float* Y);
 PBD_API int pbd_ws_gradient(pbd_ctx* ctx, long long first, int n, float* grad, double* loss);
 PBD_API int pbd_ws_adam_step(pbd_ctx* ctx, const float* grad, float lr);
 PBD_API int pbd_ws_train(pbd_ctx* ctx, int epochs, int batch, float lr, uint64_t seed, double* last_loss);
+/* Feature mean[22], feature std[22], target std[3]; set by the first
+ * pbd_ws_train from the collected samples. */
+PBD_API int pbd_ws_get_normalization(pbd_ctx* ctx, float* norm47);
+PBD_API int pbd_ws_set_normalization(pbd_ctx* ctx, const float* norm47);
 PBD_API int pbd_ws_enable(pbd_ctx* ctx, int on);
 PBD_API int pbd_ws_last_used(pbd_ctx* ctx, int cloth, int* used);
 

@@ -126,6 +126,9 @@ SIGNATURES = {
     "pbd_ws_adam_step": [_P, _fp, _F],
     "pbd_ws_train": [_P, _I, _I, _F, C.c_uint64, _dp],
     "pbd_ws_enable": [_P, _I],
+    "pbd_ws_get_normalization": [_P, _fp],
+    "pbd_ws_set_normalization": [_P, _fp],
+    "pbd_cloth_set_tolerance": [_P, _I, C.c_double],
     "pbd_ws_last_used": [_P, _I, _ip],
     "pbd_profile_enable": [_P, _I],
     "pbd_profile_read": [_P, C.POINTER(Profile), _I],
@@ -428,6 +431,18 @@ class Context:
         loss = C.c_double()
         check(self.lib.pbd_ws_train(self.h, epochs, batch, lr, seed, C.byref(loss)))
         return loss.value
+
+    def ws_norm(self) -> np.ndarray:
+        out = np.zeros(47, np.float32)
+        check(self.lib.pbd_ws_get_normalization(self.h, fptr(out)))
+        return out
+
+    def ws_set_norm(self, norm):
+        norm = np.ascontiguousarray(norm, np.float32)
+        check(self.lib.pbd_ws_set_normalization(self.h, fptr(norm)))
+
+    def set_tolerance(self, cloth: int, tol: float):
+        check(self.lib.pbd_cloth_set_tolerance(self.h, cloth, tol))
 
     def ws_enable(self, on: bool = True):
         check(self.lib.pbd_ws_enable(self.h, 1 if on else 0))

@@ -61,7 +61,9 @@ struct pbd_ctx {
     // learned warm start
     struct WarmStart {
         bool params_ready = false, enabled = false, nbr_built = false;
-        DBuf<float> prm, m, v, grad, partial, X, Y;
+        DBuf<float> prm, m, v, grad, partial, X, Y, norm;
+        std::vector<float> norm_h;
+        bool samples_normalized = false;
         DBuf<unsigned char> W;
         int adam_step = 0;
         DBuf<int> nbr, perm;
@@ -148,6 +150,7 @@ void fill_cloth_dev(pbd_ctx* c) {
                 ne++;
             }
         d.hscale = ne ? (float)(sum / ne) : 1.0f;
+        d.tol = k.tol;
     }
     c->cl.upload(c->cl_h, c->stream);
 }
@@ -497,6 +500,11 @@ void ws_build_nbr(pbd_ctx* c) {
     c->ws.r0.alloc(c->cloths.size());
     c->ws.r1.alloc(c->cloths.size());
     c->ws.nbr_built = true;
+    if (c->ws.norm_h.empty()) {
+        c->ws.norm_h.assign(47, 1.0f);
+        for (int k = 0; k < 22; k++) c->ws.norm_h[k] = 0.0f;
+        c->ws.norm.upload(c->ws.norm_h, c->stream);
+    }
 }
 
 void ws_require_params(pbd_ctx* c) {
@@ -511,7 +519,7 @@ void ws_apply_prediction(pbd_ctx* c) {
     ws_build_nbr(c);
     float4* p = c->pbuf[c->pcur].ptr;
     ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, c->ws.nbr.ptr, c->ws.nbr_rest.ptr, c->nv_total,
-             c->ws.delta.ptr, nullptr);
+             c->ws.delta.ptr, nullptr, c->ws.norm.ptr);
     launch_fill_u64(s, c->ws.r0.ptr, ncl, 0ull);
     launch_fill_u64(s, c->ws.r1.ptr, ncl, 0ull);
     ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr);
@@ -623,7 +631,9 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
             tiles((int)c->order_partial.n, c->order_partial.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
             cur ^= 1;
             sw += a.nsweeps;
-            if (tol >= 0.0 && ch + 1 < nchunks) {
+            bool any_tol = tol >= 0.0;
+            for (auto& k : c->cloths) any_tol |= k.tol >= 0.0;
+            if (any_tol && ch + 1 < nchunks) {
                 // stop once every multi-tile cloth met the tolerance
                 std::vector<unsigned long long> r(ncl);
                 PBD_HIP(hipMemcpyAsync(r.data(), c->res_slots.ptr + (size_t)ch * ncl, sizeof(unsigned long long) * ncl,
@@ -634,7 +644,8 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
                     if (c->cloths[ci].whole) continue;
                     double dv;
                     memcpy(&dv, &r[ci], 8);
-                    if (dv > tol) all = false;
+                    double tc = c->cloths[ci].tol >= 0.0 ? c->cloths[ci].tol : tol;
+                    if (tc < 0.0 || dv > tc) all = false;
                 }
                 if (all) break;
             }
@@ -1321,6 +1332,7 @@ int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
         PBD_CHECK(sweeps >= 1 && substeps >= 0, PBD_ERR_ARG, "bad sweep / substep count");
         PBD_CHECK(!ctx->coll.enabled, PBD_ERR_STATE, "sample collection runs without CCD");
         ws_require_params(ctx);
+        PBD_CHECK(!ctx->ws.samples_normalized, PBD_ERR_STATE, "samples were standardised by training; collect before pbd_ws_train");
         set_device(ctx);
         ws_build_nbr(ctx);
         hipStream_t s = ctx->stream;
@@ -1350,7 +1362,7 @@ int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
             advect_all(ctx, dt);
             float4* p = ctx->pbuf[ctx->pcur].ptr;
             ws_infer(s, W.W.ptr, ctx->cl.ptr, ctx->vcloth.ptr, ctx->x.ptr, p, W.nbr.ptr, W.nbr_rest.ptr, ctx->nv_total,
-                     W.delta.ptr, W.X.ptr + 22 * W.ns);
+                     W.delta.ptr, W.X.ptr + 22 * W.ns, W.norm.ptr);
             PBD_HIP(hipMemcpyAsync(W.psave.ptr, p, sizeof(float4) * ctx->nv_total, hipMemcpyDeviceToDevice, s));
             double tol = ctx->sp.tol;
             ctx->sp.tol = -1.0;
@@ -1411,6 +1423,31 @@ int pbd_ws_train(pbd_ctx* ctx, int epochs, int batch, float lr, uint64_t seed, d
         hipStream_t s = ctx->stream;
         auto& W = ctx->ws;
         long long n = W.ns;
+        if (!W.samples_normalized) {
+            // grouped scales: one RMS for the displacement (3), one shared
+            // by the 18 strain components (keeps the scaling rotation
+            // equivariant), 1 for the pin flag, one RMS for the 3 targets;
+            // no centring
+            std::vector<float> X, Y;
+            W.X.download(X, s);
+            W.Y.download(Y, s);
+            PBD_HIP(hipStreamSynchronize(s));
+            double sd = 0.0, ss = 0.0, sy = 0.0;
+            for (long long i = 0; i < n; i++) {
+                for (int k = 0; k < 3; k++) sd += (double)X[22 * i + k] * X[22 * i + k];
+                for (int k = 3; k < 21; k++) ss += (double)X[22 * i + k] * X[22 * i + k];
+                for (int k = 0; k < 3; k++) sy += (double)Y[3 * i + k] * Y[3 * i + k];
+            }
+            auto rms = [](double v, double cnt) { double r = std::sqrt(v / cnt); return r > 1e-20 ? (float)r : 1.0f; };
+            W.norm_h.assign(47, 0.0f);
+            for (int k = 0; k < 3; k++) W.norm_h[22 + k] = rms(sd, 3.0 * n);
+            for (int k = 3; k < 21; k++) W.norm_h[22 + k] = rms(ss, 18.0 * n);
+            W.norm_h[22 + 21] = 1.0f;
+            for (int k = 0; k < 3; k++) W.norm_h[44 + k] = rms(sy, 3.0 * n);
+            W.norm.upload(W.norm_h, s);
+            ws_normalize(s, W.X.ptr, W.Y.ptr, n, W.norm.ptr);
+            W.samples_normalized = true;
+        }
         std::vector<int> perm(n);
         for (long long i = 0; i < n; i++) perm[i] = (int)i;
         uint64_t st = seed;
@@ -1442,6 +1479,36 @@ int pbd_ws_train(pbd_ctx* ctx, int epochs, int batch, float lr, uint64_t seed, d
         }
         PBD_HIP(hipStreamSynchronize(s));
         if (last_loss) *last_loss = loss_n ? loss_sum / loss_n : 0.0;
+    });
+}
+
+int pbd_ws_get_normalization(pbd_ctx* ctx, float* norm47) {
+    return guard([&] {
+        PBD_CHECK(ctx && norm47, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(ctx->ws.norm_h.size() == 47, PBD_ERR_STATE, "no normalisation yet");
+        std::copy(ctx->ws.norm_h.begin(), ctx->ws.norm_h.end(), norm47);
+    });
+}
+
+int pbd_ws_set_normalization(pbd_ctx* ctx, const float* norm47) {
+    return guard([&] {
+        PBD_CHECK(ctx && norm47, PBD_ERR_ARG, "null argument");
+        for (int k = 22; k < 47; k++) PBD_CHECK(norm47[k] > 0.0f, PBD_ERR_ARG, "scales must be positive");
+        set_device(ctx);
+        ctx->ws.norm_h.assign(norm47, norm47 + 47);
+        ctx->ws.norm.upload(ctx->ws.norm_h, ctx->stream);
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pbd_cloth_set_tolerance(pbd_ctx* ctx, int cloth, double tol) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, cloth);
+        c.tol = tol;
+        if (ctx->inited) {
+            set_device(ctx);
+            fill_cloth_dev(ctx);
+        }
     });
 }
 

@@ -347,11 +347,14 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float
 // ---------------------------------------------------------------------------
 // Features of 16 vertices into F.xs (4 lanes per vertex), then forward, then
 // delta = h * y (zero for pinned vertices).
+// norm: mu[22], sd[22], sdy[3] (standardisation of features, clamped to
+// +-8, and of targets).
 __global__ __launch_bounds__(256) void k_ws_infer(const MlpW* __restrict__ gW, const ClothDev* __restrict__ cl,
                                                   const int* __restrict__ vcloth, const float4* __restrict__ x,
                                                   const float4* __restrict__ p, const int* __restrict__ nbr,
                                                   const float* __restrict__ nbr_rest, int nv,
-                                                  float4* __restrict__ delta, float* __restrict__ feat_out) {
+                                                  float4* __restrict__ delta, float* __restrict__ feat_out,
+                                                  const float* __restrict__ norm) {
     __shared__ MlpW W;
     __shared__ FwdLds F[FWD_WAVES];
     __shared__ float fe[FWD_WAVES][16][24];
@@ -399,7 +402,7 @@ __global__ __launch_bounds__(256) void k_ws_infer(const MlpW* __restrict__ gW, c
     wave_sync();
     for (int e = l; e < 16 * 32; e += 64) {
         int row = e / 32, k = e % 32;
-        f.xs[e] = k < 22 ? f2bf(fe[wv][row][k]) : 0;
+        f.xs[e] = k < 22 ? f2bf(fminf(fmaxf((fe[wv][row][k] - norm[k]) / norm[22 + k], -8.0f), 8.0f)) : 0;
         if (feat_out && k < 22 && base + row < nv) feat_out[(size_t)(base + row) * 22 + k] = fe[wv][row][k];
     }
     wave_sync();
@@ -410,7 +413,9 @@ __global__ __launch_bounds__(256) void k_ws_infer(const MlpW* __restrict__ gW, c
             float h = cl[vcloth[v]].hscale;
             bool pinned = p[v].w == 0.0f;
             float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (!pinned) d = make_float4(h * f.y[l * 16 + 0], h * f.y[l * 16 + 1], h * f.y[l * 16 + 2], 0.0f);
+            if (!pinned)
+                d = make_float4(h * (f.y[l * 16 + 0] * norm[44]), h * (f.y[l * 16 + 1] * norm[45]),
+                                h * (f.y[l * 16 + 2] * norm[46]), 0.0f);
             delta[v] = d;
         }
     }
@@ -473,6 +478,15 @@ __global__ __launch_bounds__(256) void k_ws_apply(const int* __restrict__ vcloth
     p[v] = make_float4(P.x + D.x, P.y + D.y, P.z + D.z, P.w);
 }
 
+// In-place standardisation of stored samples: X = clamp((X - mu)/sd, +-8), Y = Y/sdy.
+__global__ __launch_bounds__(256) void k_ws_normalize(float* __restrict__ X, float* __restrict__ Y, long long n,
+                                                      const float* __restrict__ norm) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 22; k++) X[i * 22 + k] = fminf(fmaxf((X[i * 22 + k] - norm[k]) / norm[22 + k], -8.0f), 8.0f);
+    for (int k = 0; k < 3; k++) Y[i * 3 + k] = Y[i * 3 + k] / norm[44 + k];
+}
+
 // Gauss-Seidel correction target: y = (p_after - p_before) / h.
 __global__ __launch_bounds__(256) void k_ws_target(const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
                                                    const float4* __restrict__ pa, const float4* __restrict__ pb,
@@ -527,11 +541,17 @@ void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float
 }
 
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
-              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out) {
+              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm) {
     if (nv <= 0) return;
     int blocks = (nv + 16 * FWD_WAVES - 1) / (16 * FWD_WAVES);
     hipLaunchKernelGGL(k_ws_infer, dim3(blocks), dim3(64 * FWD_WAVES), 0, s, (const MlpW*)W, cl, vcloth, x, p, nbr,
-                       nbr_rest, nv, delta, feat_out);
+                       nbr_rest, nv, delta, feat_out, norm);
+    PBD_HIP(hipGetLastError());
+}
+
+void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_ws_normalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, Y, n, norm);
     PBD_HIP(hipGetLastError());
 }
 

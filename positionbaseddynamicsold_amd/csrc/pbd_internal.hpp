@@ -82,6 +82,7 @@ struct ClothDev {
     int ktab;            // offset of this cloth's per-sweep stiffness tables
     int whole;           // 1 if the cloth is a single LDS tile
     float hscale;        // mean rest length (warm-start feature scale)
+    double tol;          // residual tolerance of this cloth (< 0: the solver's)
 };
 
 // One LDS tile: a set of loaded vertices (interior first) and the prims whose
@@ -147,6 +148,7 @@ struct Cloth {
     bool whole = false;
     int max_loaded = 0;
     int last_sweeps = 0;
+    double tol = -1.0;                    // per-cloth residual tolerance (< 0: the solver's)
     int np() const { return ne + nb; }
 };
 
@@ -202,7 +204,8 @@ void mlp_grad(hipStream_t s, const void* W, const float* X, const float* T, cons
 void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
               int step);
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
-              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out);
+              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm);
+void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm);
 void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* r0,
                unsigned long long* r1);
 void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const unsigned long long* r0,

@@ -265,9 +265,10 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     const int T = blockDim.x, tid = threadIdx.x;
     const int* vl = verts + td.load_off;
 
-    if (!td.whole && a.chunk > 0 && a.tol >= 0.0) {
+    const double tol = cd.tol >= 0.0 ? cd.tol : a.tol;
+    if (!td.whole && a.chunk > 0 && tol >= 0.0) {
         double prev = __longlong_as_double((long long)res_slots[(size_t)(a.chunk - 1) * ncl + td.cloth]);
-        if (prev <= a.tol) {
+        if (prev <= tol) {
             for (int i = tid; i < td.n_int; i += T) p_out[vl[i]] = p_in[vl[i]];
             return;
         }
@@ -328,14 +329,14 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
             __syncthreads();
         }
         sw++;
-        if (td.whole && a.tol >= 0.0 && sw % a.check_every == 0) {
+        if (td.whole && tol >= 0.0 && sw % a.check_every == 0) {
             double m = 0.0;
             for (int j = tid; j < td.n_own; j += T) {
                 uint32_t pr = own[td.own_off + j];
                 m = fmax(m, rel_residual(P[pr & 0xffffu], P[pr >> 16], own_rest[td.own_off + j]));
             }
             resid = block_max(m, red);
-            if (resid <= a.tol) break;
+            if (resid <= tol) break;
         }
     }
     if (!td.whole || resid < 0.0 || (sw % a.check_every) != 0) {

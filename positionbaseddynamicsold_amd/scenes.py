@@ -112,3 +112,21 @@ def icosphere(subdiv: int = 4, radius: float = 1.0, centre=(0.0, 0.0, 0.0)):
         f = nf
     pos = (np.array(verts) * radius + np.asarray(centre)).astype(np.float32)
     return pos, np.array(f, np.int32)
+
+
+def config4(seed: int = 0) -> ClothSpec:
+    """Config-1 cloth with a random wind (speed U(0,5) m/s, horizontal
+    direction uniform, drag 1/s), 0-4 random boundary pins and a sphere
+    (r 0.3 m) at a random position under the cloth."""
+    rng = np.random.default_rng(10_000 + seed)
+    s = config1(seed)
+    npins = int(rng.integers(0, 5))
+    s.pins = sorted(rng.choice(boundary_vertices(32, 32), npins, replace=False).tolist()) if npins else []
+    ang = float(rng.uniform(0, 2 * np.pi))
+    spd = float(rng.uniform(0, 5))
+    s.wind = (float(np.float32(spd * np.cos(ang))), 0.0, float(np.float32(spd * np.sin(ang))))
+    s.wind_drag = 1.0
+    c = (float(np.float32(rng.uniform(0.2, 0.8))), float(np.float32(rng.uniform(-0.6, -0.2))),
+         float(np.float32(rng.uniform(0.2, 0.8))))
+    s.sphere = (c, 0.3)
+    return s
