@@ -1,34 +1,49 @@
 #!/usr/bin/env python3
-"""Benchmark: cloth substeps/s of the batched solver on one MI355X.
+"""Benchmark: cloth substeps/s at a fixed residual, warm-start + batched vs
+plain, on one MI355X.
 
-Workload (default): the 512-cloth batch -- side U{16..256}, aspect U(0.5,2),
-1 m wide, 0-4 random boundary pins, one sphere collider each -- packed in one
-context, 10 substeps per 1/60 s frame (dt = 1/600 s), distance constraints,
-20 colour-ordered Gauss-Seidel sweeps per substep (the reference's fixed
-iteration budget, SolverPBD::ProjectConstraint(st, iterations)).  One "step"
-is one substep of the whole batch.  Inputs are generated on the host from a
-seeded RNG and are resident in HBM before timing starts.
+Workload: the 512-cloth batch -- grid side U{16..256} vertices, aspect
+U(0.5, 2), 1 m wide, 0-4 random boundary pins, its own sphere collider
+(r U(0.2, 0.4) m) each -- packed into one context; dt = 1/600 s (10 substeps
+per 1/60 s frame), distance constraints, colour-ordered Gauss-Seidel in LDS
+tiles.  One "step" is one substep of the whole batch.  Inputs are generated
+on the host from a seeded RNG and are resident in HBM before timing starts.
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the LDS-tile
-Gauss-Seidel kernel): algorithmic bytes (68 B per distance prim per sweep +
-100 B per vertex for the fused advect/integrate of single-tile cloths) over
-its summed launch time, measured with HIP events on the library's own stream.
-`cpu_baseline` times the CPU oracle (single thread) on a bounded sample of the
-same workload and scales it to the full batch by vertex-sweeps.
+Protocol (both solvers start every run from the same device snapshot):
+  1. the warm-start network is trained on solver-generated samples of
+     config-4 scenes (32x32 cloth, wind U(0,5) m/s, random pins and sphere);
+  2. plain reference: 20 sweeps per substep (the reference's fixed budget,
+     SolverPBD::ProjectConstraint(st, iterations)); its mean end-of-substep
+     residual (max |d-L|/L per cloth, averaged over cloths and substeps) is
+     the fixed residual R;
+  3. the warm-started solver runs the smallest sweep budget whose mean
+     residual over the same substeps is <= R (searched by bisection);
+  4. both are timed over the same number of substeps.
+`value` is the warm-started batched solver's substeps/s; the plain solver's
+rate at the same residual is in config.plain_substeps_per_s.
+
+`roofline` is for the dominant kernel (the LDS-tile Gauss-Seidel kernel) in
+the timed warm-started run: algorithmic bytes (68 B per distance prim per
+sweep) over its summed launch time, from HIP events on the library's own
+stream.  `cpu_baseline` times the CPU oracle (one thread, plain solver,
+20 sweeps) on a bounded sample of the same batch, scaled to the whole batch by
+vertex-sweeps.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+PLAIN_SWEEPS = 20
 
 
 def log(msg):
@@ -38,22 +53,44 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--sweeps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20, help="timed substeps per solver")
+    ap.add_argument("--warmup", type=int, default=20, help="plain substeps before the snapshot")
+    ap.add_argument("--match-substeps", type=int, default=10, help="substeps of the residual comparison")
     ap.add_argument("--tile-sweeps", type=int, default=2)
     ap.add_argument("--count", type=int, default=512, help="cloths in the batch")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--train-scenes", type=int, default=24)
+    ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
 
+def train_warm_start(capi, scenes, n_scenes, epochs):
+    """~200k samples: n_scenes config-4 cloths x 1024 vertices x 8 collected substeps."""
+    t = time.perf_counter()
+    ctx = capi.Context(0)
+    for s in range(n_scenes):
+        scenes.add_to_context(ctx, scenes.config4(s))
+    ctx.init()
+    ctx.set_solver(dt=scenes.DT, iterations=PLAIN_SWEEPS)
+    ctx.ws_init(0)
+    for _ in range(8):
+        ctx.step(30)
+        ctx.ws_collect(PLAIN_SWEEPS, 1)
+    n = ctx.ws_num_samples()
+    loss = ctx.ws_train(epochs=epochs, batch=2048, lr=1e-3, seed=1)
+    prm, norm = ctx.ws_params(), ctx.ws_norm()
+    ctx.close()
+    info = {"samples": n, "epochs": epochs, "standardised_loss": loss, "zero_predictor_loss": 1.5,
+            "seconds": time.perf_counter() - t}
+    log(f"[bench] warm start trained on {n} samples in {info['seconds']:.1f} s, loss {loss:.3f} (zero predictor 1.5)")
+    return prm, norm, info
+
+
 def cpu_baseline(specs, sweeps, budget_s):
     """Times the oracle (1 thread) on the first cloths of the batch until the
     budget is spent; returns vertex-sweeps/s and a description."""
-    import numpy as np
-
     from oracle import oracle as orc
 
     sample, nv = [], 0
@@ -65,7 +102,6 @@ def cpu_baseline(specs, sweeps, budget_s):
         nv += s.nv
         if nv > 60000:
             break
-    # warm-up one substep each, then whole substeps until the budget is spent
     for oc, prm in sample:
         oc.substep(prm, cap=sweeps)
     done_vs, n_sub, t0 = 0, 0, time.perf_counter()
@@ -75,18 +111,20 @@ def cpu_baseline(specs, sweeps, budget_s):
             done_vs += oc.nv * sweeps
         n_sub += 1
     el = time.perf_counter() - t0
-    desc = (f"CPU oracle, 1 thread, first {len(sample)} cloths of the batch ({nv} vertices), "
-            f"{n_sub} substeps x {sweeps} sweeps each in {el:.1f} s; scaled to the full batch by vertex-sweeps")
+    desc = (f"CPU oracle (plain solver, {sweeps} sweeps), 1 thread, first {len(sample)} cloths of the batch "
+            f"({nv} vertices), {n_sub} substeps each in {el:.1f} s; scaled to the full batch by vertex-sweeps")
     return done_vs / el, desc
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding "
-              f"of this path is a design note, not a measured configuration.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); sharding this path across GPUs "
+              f"is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
     from positionbaseddynamicsold_amd import capi, scenes
+
+    prm, norm, train_info = train_warm_start(capi, scenes, a.train_scenes, a.train_epochs)
 
     specs = scenes.config5(seed=a.seed, count=a.count)
     nv_total = sum(s.nv for s in specs)
@@ -97,32 +135,81 @@ def main():
     t = time.perf_counter()
     ctx.init()
     log(f"[bench] init (constraints, GPU colouring, tile plan) {time.perf_counter() - t:.1f} s")
-    ctx.set_solver(dt=scenes.DT, iterations=a.sweeps, check_every=a.tile_sweeps, tile_sweeps=a.tile_sweeps)
+    ctx.ws_set_params(prm)
+    ctx.ws_set_norm(norm)
+
+    def solver(sweeps, ws):
+        ctx.ws_enable(ws)
+        ctx.set_solver(dt=scenes.DT, iterations=sweeps, check_every=a.tile_sweeps, tile_sweeps=a.tile_sweeps)
+
+    solver(PLAIN_SWEEPS, False)
     ctx.step(a.warmup)
     ctx.sync()
-    log(f"[bench] warm-up {a.warmup} substeps done")
-    ctx.profile(True)
-    ctx.profile_read(reset=True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.step(a.steps)
-    ctx.sync()
-    el = time.perf_counter() - t0
-    prof = ctx.profile_read(reset=True)
-    ctx.profile(False)
-    ms_per_step = el * 1e3 / a.steps
-    value = a.steps / el
-    res = [ctx.residual(i) for i in ids[:64]]
-    rmax = max(r[0] for r in res)
-    rrms = (sum(r[1] ** 2 for r in res) / len(res)) ** 0.5
+    ctx.snapshot()
+    log(f"[bench] warm-up {a.warmup} plain substeps done, snapshot taken")
+
+    def mean_residual(sweeps, ws):
+        ctx.snapshot(restore=True)
+        solver(sweeps, ws)
+        acc, used = 0.0, 0.0
+        for _ in range(a.match_substeps):
+            ctx.step(1)
+            acc += float(ctx.residual_all()[0].mean())
+            if ws:
+                used += sum(ctx.ws_last_used(i) for i in ids) / len(ids)
+        return acc / a.match_substeps, used / a.match_substeps
+
+    r_plain, _ = mean_residual(PLAIN_SWEEPS, False)
+    # smallest warm budget with residual <= the plain solver's (bisection)
+    lo, hi = 1, PLAIN_SWEEPS
+    r_hi, u_hi = mean_residual(hi, True)
+    tried = {hi: r_hi}
+    if r_hi <= r_plain:
+        while hi - lo > 0:
+            mid = (lo + hi) // 2
+            r_mid, _ = mean_residual(mid, True)
+            tried[mid] = r_mid
+            if r_mid <= r_plain:
+                hi = mid
+            else:
+                lo = mid + 1
+        warm_sweeps = hi
+    else:
+        warm_sweeps = PLAIN_SWEEPS
+    r_warm, used_frac = mean_residual(warm_sweeps, True)
+    log(f"[bench] fixed residual R = {r_plain:.5e} (plain, {PLAIN_SWEEPS} sweeps); warm start reaches "
+        f"{r_warm:.5e} with {warm_sweeps} sweeps (tried {dict(sorted(tried.items()))}); prediction kept "
+        f"for {100 * used_frac:.0f}% of cloth-substeps")
+
+    def timed(sweeps, ws):
+        ctx.snapshot(restore=True)
+        solver(sweeps, ws)
+        ctx.step(1)
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(a.steps)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        prof = ctx.profile_read(reset=True)
+        ctx.profile(False)
+        return a.steps / el, el * 1e3 / a.steps, prof
+
+    plain_rate, plain_ms, plain_prof = timed(PLAIN_SWEEPS, False)
+    warm_rate, warm_ms, prof = timed(warm_sweeps, True)
+    log(f"[bench] plain {plain_rate:.2f} substeps/s ({plain_ms:.2f} ms); warm start + batched "
+        f"{warm_rate:.2f} substeps/s ({warm_ms:.2f} ms): x{warm_rate / plain_rate:.2f}")
     achieved = prof.tile_alg_bytes / (prof.tile_ms * 1e-3) / 1e9
+    plain_achieved = plain_prof.tile_alg_bytes / (plain_prof.tile_ms * 1e-3) / 1e9
     copy_ms = ctx.bench_copy(1 << 30, 10)
     copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     infos = [ctx.info(i) for i in ids]
-    log(f"[bench] {value:.2f} substeps/s, {ms_per_step:.3f} ms/substep; tile kernel {prof.tile_launches} launches, "
-        f"{prof.tile_ms / max(1, prof.tile_launches):.3f} ms avg, {achieved:.0f} GB/s algorithmic; copy {copy_gbs:.0f} GB/s")
+    log(f"[bench] k_tiles {prof.tile_launches} launches, {prof.tile_ms / max(1, prof.tile_launches):.3f} ms avg, "
+        f"{achieved:.0f} GB/s algorithmic (plain run {plain_achieved:.0f}); copy kernel {copy_gbs:.0f} GB/s")
     traffic = None
-    tf = ROOT / "profiles" / "tile_kernel_batch512_traffic.json"
+    tf = ROOT / "profiles" / "k_tiles_batch512_traffic.json"
     if tf.exists():
         try:
             traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
@@ -130,32 +217,41 @@ def main():
             traffic = None
     cpu = None
     if not a.no_cpu:
-        vs_rate, desc = cpu_baseline(specs, a.sweeps, a.cpu_seconds)
-        cpu = {"value": vs_rate / (nv_total * a.sweeps), "unit": "substeps/s", "cores": 1, "kind": "port",
+        vs_rate, desc = cpu_baseline(specs, PLAIN_SWEEPS, a.cpu_seconds)
+        cpu = {"value": vs_rate / (nv_total * PLAIN_SWEEPS), "unit": "substeps/s", "cores": 1, "kind": "port",
                "sample": desc, "vertex_sweeps_per_s": vs_rate}
         log(f"[bench] cpu baseline {cpu['value']:.5f} substeps/s ({vs_rate / 1e6:.1f} M vertex-sweeps/s)")
     out = {
         "metric": "cloth substeps/s at fixed residual: warm-start+batched vs plain, 1 MI355X",
-        "value": value,
+        "value": warm_rate,
         "unit": "substeps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": warm_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"batched plain solver: {len(specs)} grid cloths (side U{{16..256}}, aspect U(0.5,2), "
-                        f"0-4 random boundary pins, own sphere r U(0.2,0.4)), {nv_total} vertices, "
-                        f"{ne_total} distance prims, dt 1/600 s, {a.sweeps} colour-ordered GS sweeps/substep",
-            "solver": "plain (no warm start)",
-            "sweeps_per_substep": a.sweeps,
-            "tile_sweeps_per_launch": a.tile_sweeps,
-            "residual_reached": {"max_rel": rmax, "rms_rel": rrms, "over": "first 64 cloths"},
+            "workload": f"{len(specs)} grid cloths in one batch (side U{{16..256}}, aspect U(0.5,2), 0-4 random "
+                        f"boundary pins, own sphere r U(0.2,0.4)): {nv_total} vertices, {ne_total} distance prims, "
+                        f"dt 1/600 s; colour-ordered Gauss-Seidel in LDS tiles",
+            "fixed_residual": {"definition": "mean over cloths and substeps of max |d-L|/L at the end of the "
+                                             "substep, plain solver with 20 sweeps, same start state",
+                               "R": r_plain, "warm_start_residual": r_warm,
+                               "substeps_compared": a.match_substeps},
+            "plain_substeps_per_s": plain_rate,
+            "plain_ms_per_step": plain_ms,
+            "speedup_vs_plain": warm_rate / plain_rate,
+            "plain_sweeps": PLAIN_SWEEPS,
+            "warm_start_sweeps": warm_sweeps,
+            "warm_start_kept_fraction": used_frac,
+            "warm_start_training": train_info,
             "vertex_iterations_per_s": prof.vertex_sweeps / (prof.step_ms * 1e-3) if prof.step_ms else None,
+            "plain_vertex_iterations_per_s": plain_prof.vertex_sweeps / (plain_prof.step_ms * 1e-3),
+            "tile_sweeps_per_launch": a.tile_sweeps,
             "single_tile_cloths": sum(1 for i in infos if i.whole),
             "tiles": sum(i.ntiles for i in infos),
             "copy_kernel_gbs": copy_gbs,
@@ -170,6 +266,7 @@ def main():
             "kernel": "k_tiles",
             "alg_bytes_per_launch": prof.tile_alg_bytes / max(1, prof.tile_launches),
             "avg_launch_ms": prof.tile_ms / max(1, prof.tile_launches),
+            "plain_run_achieved": plain_achieved,
         },
         "cpu_baseline": cpu,
     }

@@ -150,6 +150,16 @@ PBD_API int pbd_cloth_get_state(pbd_ctx* ctx, int cloth, int which, float* xyz);
 PBD_API int pbd_cloth_set_state(pbd_ctx* ctx, int cloth, int which, const float* xyz);
 /* Convergence residual over distance prims: max and RMS of |d - L| / L (FP64). */
 PBD_API int pbd_cloth_residual(pbd_ctx* ctx, int cloth, double* max_rel, double* rms);
+/* Residual of every cloth at once (arrays of the cloth count). */
+PBD_API int pbd_residual_all(pbd_ctx* ctx, double* max_rel, double* rms);
+/* Exact binary checkpoint of the whole context (x and v of every vertex in
+ * packed order, 3 floats each); replaces the lossy text caches of
+ * PBDObject::SaveMeshTopol / ContinueSimInit (PBD_cuda.cu:660-667,907-925). */
+PBD_API int pbd_num_vertices(pbd_ctx* ctx, int* nv);
+PBD_API int pbd_state_save(pbd_ctx* ctx, float* x, float* v);
+PBD_API int pbd_state_load(pbd_ctx* ctx, const float* x, const float* v);
+/* Device-side copy of x and v (restore = 0) and its restore (restore = 1). */
+PBD_API int pbd_snapshot(pbd_ctx* ctx, int restore);
 /* Residual tolerance of one cloth, overriding pbd_solver_params.tol (< 0 restores it). */
 PBD_API int pbd_cloth_set_tolerance(pbd_ctx* ctx, int cloth, double tol);
 /* Sweeps the last projection of this cloth ran. */

@@ -129,6 +129,11 @@ SIGNATURES = {
     "pbd_ws_get_normalization": [_P, _fp],
     "pbd_ws_set_normalization": [_P, _fp],
     "pbd_cloth_set_tolerance": [_P, _I, C.c_double],
+    "pbd_residual_all": [_P, _dp, _dp],
+    "pbd_num_vertices": [_P, _ip],
+    "pbd_state_save": [_P, _fp, _fp],
+    "pbd_state_load": [_P, _fp, _fp],
+    "pbd_snapshot": [_P, _I],
     "pbd_ws_last_used": [_P, _I, _ip],
     "pbd_profile_enable": [_P, _I],
     "pbd_profile_read": [_P, C.POINTER(Profile), _I],
@@ -440,6 +445,31 @@ class Context:
     def ws_set_norm(self, norm):
         norm = np.ascontiguousarray(norm, np.float32)
         check(self.lib.pbd_ws_set_normalization(self.h, fptr(norm)))
+
+    def residual_all(self):
+        n = len(self.cloths)
+        m, r = np.zeros(n), np.zeros(n)
+        check(self.lib.pbd_residual_all(self.h, m.ctypes.data_as(_dp), r.ctypes.data_as(_dp)))
+        return m, r
+
+    def num_vertices(self) -> int:
+        n = C.c_int()
+        check(self.lib.pbd_num_vertices(self.h, C.byref(n)))
+        return n.value
+
+    def state_save(self):
+        n = self.num_vertices()
+        x, v = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        check(self.lib.pbd_state_save(self.h, fptr(x), fptr(v)))
+        return x, v
+
+    def state_load(self, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        check(self.lib.pbd_state_load(self.h, fptr(x), fptr(v)))
+
+    def snapshot(self, restore: bool = False):
+        check(self.lib.pbd_snapshot(self.h, 1 if restore else 0))
 
     def set_tolerance(self, cloth: int, tol: float):
         check(self.lib.pbd_cloth_set_tolerance(self.h, cloth, tol))

@@ -73,6 +73,9 @@ struct pbd_ctx {
         DBuf<unsigned long long> r0, r1;
         std::vector<int> used;
     } ws;
+    // device-side state snapshot
+    DBuf<float4> snap_x, snap_v;
+    bool snap_valid = false;
     // profiling
     bool prof_on = false;
     pbd_profile prof{};
@@ -1622,6 +1625,98 @@ int pbd_cloth_residual(pbd_ctx* ctx, int id, double* max_rel, double* rms) {
         if (max_rel) *max_rel = mv;
         int ne = ctx->cloths[id].ne;
         if (rms) *rms = ne > 0 ? std::sqrt(sum / ne) : 0.0;
+    });
+}
+
+int pbd_residual_all(pbd_ctx* ctx, double* max_rel, double* rms) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        int ncl = (int)ctx->cloths.size();
+        launch_fill_u64(s, ctx->rmax.ptr, ncl, 0ull);
+        PBD_HIP(hipMemsetAsync(ctx->rsum.ptr, 0, sizeof(double) * ncl, s));
+        launch_residual(s, ctx->rdist.ptr, (int)ctx->rdist.n, ctx->pbuf[ctx->pcur].ptr, ctx->rmax.ptr, ctx->rsum.ptr,
+                        nullptr);
+        std::vector<unsigned long long> m;
+        std::vector<double> sum;
+        ctx->rmax.download(m, s);
+        ctx->rsum.download(sum, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < ncl; i++) {
+            double mv;
+            memcpy(&mv, &m[i], 8);
+            if (max_rel) max_rel[i] = mv;
+            int ne = ctx->cloths[i].ne;
+            if (rms) rms[i] = ne > 0 ? std::sqrt(sum[i] / ne) : 0.0;
+        }
+    });
+}
+
+// Whole-context state (every cloth, packed vertex order) to / from host
+// arrays of 3 * total-vertices floats: an exact binary checkpoint of x and v.
+int pbd_state_save(pbd_ctx* ctx, float* x, float* v) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        set_device(ctx);
+        std::vector<float4> X, V;
+        ctx->x.download(X, ctx->stream);
+        ctx->v.download(V, ctx->stream);
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < ctx->nv_total; i++) {
+            if (x) { x[3 * i] = X[i].x; x[3 * i + 1] = X[i].y; x[3 * i + 2] = X[i].z; }
+            if (v) { v[3 * i] = V[i].x; v[3 * i + 1] = V[i].y; v[3 * i + 2] = V[i].z; }
+        }
+    });
+}
+
+int pbd_state_load(pbd_ctx* ctx, const float* x, const float* v) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(x && v, PBD_ERR_ARG, "null state");
+        set_device(ctx);
+        std::vector<float4> X, V(ctx->nv_total);
+        ctx->x.download(X, ctx->stream);
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < ctx->nv_total; i++) {
+            X[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], X[i].w);
+            V[i] = make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0f);
+        }
+        ctx->x.upload(X, ctx->stream);
+        ctx->v.upload(V, ctx->stream);
+        PBD_HIP(hipMemcpyAsync(ctx->pbuf[ctx->pcur].ptr, ctx->x.ptr, sizeof(float4) * ctx->nv_total,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+// Device-side copy of x and v (and restore), for A/B runs from one state.
+int pbd_snapshot(pbd_ctx* ctx, int restore) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        size_t b = sizeof(float4) * ctx->nv_total;
+        if (!restore) {
+            ctx->snap_x.alloc(ctx->nv_total);
+            ctx->snap_v.alloc(ctx->nv_total);
+            PBD_HIP(hipMemcpyAsync(ctx->snap_x.ptr, ctx->x.ptr, b, hipMemcpyDeviceToDevice, s));
+            PBD_HIP(hipMemcpyAsync(ctx->snap_v.ptr, ctx->v.ptr, b, hipMemcpyDeviceToDevice, s));
+            ctx->snap_valid = true;
+        } else {
+            PBD_CHECK(ctx->snap_valid, PBD_ERR_STATE, "no snapshot taken");
+            PBD_HIP(hipMemcpyAsync(ctx->x.ptr, ctx->snap_x.ptr, b, hipMemcpyDeviceToDevice, s));
+            PBD_HIP(hipMemcpyAsync(ctx->v.ptr, ctx->snap_v.ptr, b, hipMemcpyDeviceToDevice, s));
+            PBD_HIP(hipMemcpyAsync(ctx->pbuf[ctx->pcur].ptr, ctx->snap_x.ptr, b, hipMemcpyDeviceToDevice, s));
+        }
+        PBD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int pbd_num_vertices(pbd_ctx* ctx, int* nv) {
+    return guard([&] {
+        PBD_CHECK(ctx && nv && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        *nv = ctx->nv_total;
     });
 }
 

@@ -12,6 +12,7 @@
 //
 // Parameter layout (FP32 master copy, row-major W[out][in]):
 //   W1[64][22] b1[64] W2[64][64] b2[64] W3[3][64] b3[3]
+#include <algorithm>
 #include <cmath>
 
 #include "pbd_internal.hpp"
@@ -36,10 +37,16 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// tanh(u) = 1 - 2 / (exp(2u) + 1) on the hardware exp; saturates cleanly.
+__device__ __forceinline__ float fast_tanh(float u) {
+    u = fminf(fmaxf(u, -15.0f), 15.0f);
+    return 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
+}
+
 __device__ __forceinline__ void gelu(float x, float* y, float* dy) {
     const float c = 0.7978845608f, k = 0.044715f;
     float u = c * (x + k * x * x * x);
-    float th = tanhf(u);
+    float th = fast_tanh(u);
     *y = 0.5f * x * (1.0f + th);
     if (dy) *dy = 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * c * (1.0f + 3.0f * k * x * x);
 }
@@ -143,6 +150,41 @@ __device__ void forward16(const MlpW& W, FwdLds& F) {
         int row = 4 * q + i, col = r;
         F.y[row * 16 + col] = acc[0][i] + W.b3[col];
     }
+    wave_sync();
+}
+
+// Inference-only scratch (no FP32 pre-activations kept).
+struct alignas(16) InfLds {
+    uint16_t xs[16 * 32];
+    uint16_t a1[16 * 64];
+    uint16_t a2[16 * 64];
+    float y[16 * 16];
+    float fe[16 * 24];
+};
+
+__device__ void forward16_infer(const MlpW& W, InfLds& F) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    f32x4 acc[4];
+    mm16(F.xs, 32, W.w1, 32, 32, 4, acc);
+    for (int t = 0; t < 4; t++)
+        for (int i = 0; i < 4; i++) {
+            int row = 4 * q + i, col = 16 * t + r;
+            float a;
+            gelu(acc[t][i] + W.b1[col], &a, nullptr);
+            F.a1[row * 64 + col] = f2bf(a);
+        }
+    wave_sync();
+    mm16(F.a1, 64, W.w2, 64, 64, 4, acc);
+    for (int t = 0; t < 4; t++)
+        for (int i = 0; i < 4; i++) {
+            int row = 4 * q + i, col = 16 * t + r;
+            float a;
+            gelu(acc[t][i] + W.b2[col], &a, nullptr);
+            F.a2[row * 64 + col] = f2bf(a);
+        }
+    wave_sync();
+    mm16(F.a2, 64, W.w3, 64, 64, 1, acc);
+    for (int i = 0; i < 4; i++) F.y[(4 * q + i) * 16 + r] = acc[0][i] + W.b3[r];
     wave_sync();
 }
 
@@ -348,83 +390,91 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float
 // Features of 16 vertices into F.xs (4 lanes per vertex), then forward, then
 // delta = h * y (zero for pinned vertices).
 // norm: mu[22], sd[22], sdy[3] (standardisation of features, clamped to
-// +-8, and of targets).
-__global__ __launch_bounds__(256) void k_ws_infer(const MlpW* __restrict__ gW, const ClothDev* __restrict__ cl,
-                                                  const int* __restrict__ vcloth, const float4* __restrict__ x,
-                                                  const float4* __restrict__ p, const int* __restrict__ nbr,
-                                                  const float* __restrict__ nbr_rest, int nv,
-                                                  float4* __restrict__ delta, float* __restrict__ feat_out,
-                                                  const float* __restrict__ norm) {
+// +-8, and of targets).  Persistent: each block loads the weights once and
+// loops over groups of 16 vertices, one group per wave at a time.
+constexpr int INF_WAVES = 16;
+
+__global__ __launch_bounds__(1024) void k_ws_infer(const MlpW* __restrict__ gW, const ClothDev* __restrict__ cl,
+                                                   const int* __restrict__ vcloth, const float4* __restrict__ x,
+                                                   const float4* __restrict__ p, const int* __restrict__ nbr,
+                                                   const float* __restrict__ nbr_rest, int nv,
+                                                   float4* __restrict__ delta, float* __restrict__ feat_out,
+                                                   const float* __restrict__ norm) {
     __shared__ MlpW W;
-    __shared__ FwdLds F[FWD_WAVES];
-    __shared__ float fe[FWD_WAVES][16][24];
+    __shared__ InfLds F[INF_WAVES];
     load_weights(&W, gW);
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    FwdLds& f = F[wv];
-    const int base = (blockIdx.x * FWD_WAVES + wv) * 16;
-    if (base >= nv) return;
-    {
-        const int row = l & 15, part = l >> 4, v = base + row;
-        float* fr = fe[wv][row];
-        if (v < nv) {
-            float h = cl[vcloth[v]].hscale;
-            float4 P = p[v];
-            if (part == 0) {
-                float4 X = x[v];
-                fr[0] = (P.x - X.x) / h;
-                fr[1] = (P.y - X.y) / h;
-                fr[2] = (P.z - X.z) / h;
-                fr[21] = P.w == 0.0f ? 1.0f : 0.0f;
-            }
-            for (int k = part; k < 6; k += 4) {
-                int j = nbr[(size_t)v * 6 + k];
-                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-                if (j >= 0) {
-                    float4 Q = p[j];
-                    float ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
-                    float d = sqrtf(ex * ex + ey * ey + ez * ez);
-                    float L = nbr_rest[(size_t)v * 6 + k];
-                    if (d > 0.0f) {
-                        float s = ((d - L) / L) / d;
-                        sx = s * ex;
-                        sy = s * ey;
-                        sz = s * ez;
-                    }
+    InfLds& f = F[wv];
+    const int ngroups = (nv + 15) / 16;
+    for (int g = blockIdx.x * INF_WAVES + wv; g < ngroups; g += gridDim.x * INF_WAVES) {
+        const int base = g * 16;
+        {
+            const int row = l & 15, part = l >> 4, v = base + row;
+            float* fr = f.fe + row * 24;
+            if (v < nv) {
+                float h = cl[vcloth[v]].hscale;
+                float4 P = p[v];
+                if (part == 0) {
+                    float4 X = x[v];
+                    fr[0] = (P.x - X.x) / h;
+                    fr[1] = (P.y - X.y) / h;
+                    fr[2] = (P.z - X.z) / h;
+                    fr[21] = P.w == 0.0f ? 1.0f : 0.0f;
                 }
-                fr[3 + 3 * k] = sx;
-                fr[4 + 3 * k] = sy;
-                fr[5 + 3 * k] = sz;
+                for (int k = part; k < 6; k += 4) {
+                    int j = nbr[(size_t)v * 6 + k];
+                    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+                    if (j >= 0) {
+                        float4 Q = p[j];
+                        float ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
+                        float d = sqrtf(ex * ex + ey * ey + ez * ez);
+                        float L = nbr_rest[(size_t)v * 6 + k];
+                        if (d > 0.0f) {
+                            float sc = ((d - L) / L) / d;
+                            sx = sc * ex;
+                            sy = sc * ey;
+                            sz = sc * ez;
+                        }
+                    }
+                    fr[3 + 3 * k] = sx;
+                    fr[4 + 3 * k] = sy;
+                    fr[5 + 3 * k] = sz;
+                }
+            } else if (part == 0) {
+                for (int k = 0; k < 22; k++) fr[k] = 0.0f;
             }
-        } else if (part == 0) {
-            for (int k = 0; k < 22; k++) fr[k] = 0.0f;
         }
-    }
-    wave_sync();
-    for (int e = l; e < 16 * 32; e += 64) {
-        int row = e / 32, k = e % 32;
-        f.xs[e] = k < 22 ? f2bf(fminf(fmaxf((fe[wv][row][k] - norm[k]) / norm[22 + k], -8.0f), 8.0f)) : 0;
-        if (feat_out && k < 22 && base + row < nv) feat_out[(size_t)(base + row) * 22 + k] = fe[wv][row][k];
-    }
-    wave_sync();
-    forward16(W, f);
-    if (l < 16) {
-        int v = base + l;
-        if (v < nv) {
-            float h = cl[vcloth[v]].hscale;
-            bool pinned = p[v].w == 0.0f;
-            float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (!pinned)
-                d = make_float4(h * (f.y[l * 16 + 0] * norm[44]), h * (f.y[l * 16 + 1] * norm[45]),
-                                h * (f.y[l * 16 + 2] * norm[46]), 0.0f);
-            delta[v] = d;
+        wave_sync();
+        for (int e = l; e < 16 * 32; e += 64) {
+            int row = e / 32, k = e % 32;
+            float fv = k < 22 ? f.fe[row * 24 + k] : 0.0f;
+            f.xs[e] = k < 22 ? f2bf(fminf(fmaxf((fv - norm[k]) / norm[22 + k], -8.0f), 8.0f)) : 0;
+            if (feat_out && k < 22 && base + row < nv) feat_out[(size_t)(base + row) * 22 + k] = fv;
         }
+        wave_sync();
+        forward16_infer(W, f);
+        if (l < 16) {
+            int v = base + l;
+            if (v < nv) {
+                float h = cl[vcloth[v]].hscale;
+                bool pinned = p[v].w == 0.0f;
+                float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (!pinned)
+                    d = make_float4(h * (f.y[l * 16 + 0] * norm[44]), h * (f.y[l * 16 + 1] * norm[45]),
+                                    h * (f.y[l * 16 + 2] * norm[46]), 0.0f);
+                delta[v] = d;
+            }
+        }
+        wave_sync();
     }
 }
 
+// Per-prim residual in FP32 (this comparison only decides whether the
+// prediction is kept); the reduction below is an exact max in FP64.
 __device__ __forceinline__ double rel_res(float4 a, float4 b, float L) {
-    double dx = (double)a.x - (double)b.x, dy = (double)a.y - (double)b.y, dz = (double)a.z - (double)b.z;
-    double d = sqrt(dx * dx + dy * dy + dz * dz);
-    return fabs(d - (double)L) / (double)L;
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    return (double)(fabsf(d - L) / L);
 }
 
 // Residual of every distance prim with and without the predicted delta,
@@ -543,8 +593,16 @@ void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
               const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm) {
     if (nv <= 0) return;
-    int blocks = (nv + 16 * FWD_WAVES - 1) / (16 * FWD_WAVES);
-    hipLaunchKernelGGL(k_ws_infer, dim3(blocks), dim3(64 * FWD_WAVES), 0, s, (const MlpW*)W, cl, vcloth, x, p, nbr,
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        PBD_HIP(hipGetDevice(&dev));
+        PBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (ncu <= 0) ncu = 256;
+    }
+    int groups = (nv + 15) / 16;
+    int blocks = std::min((groups + INF_WAVES - 1) / INF_WAVES, ncu);
+    hipLaunchKernelGGL(k_ws_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, s, (const MlpW*)W, cl, vcloth, x, p, nbr,
                        nbr_rest, nv, delta, feat_out, norm);
     PBD_HIP(hipGetLastError());
 }
