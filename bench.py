@@ -9,25 +9,31 @@ per 1/60 s frame), distance constraints, colour-ordered Gauss-Seidel in LDS
 tiles.  One "step" is one substep of the whole batch.  Inputs are generated
 on the host from a seeded RNG and are resident in HBM before timing starts.
 
-Protocol (both solvers start every run from the same device snapshot):
+Protocol (every timed run starts from the same device snapshot):
   1. the warm-start network is trained on solver-generated samples of
      config-4 scenes (32x32 cloth, wind U(0,5) m/s, random pins and sphere);
-  2. plain reference: 20 sweeps per substep (the reference's fixed budget,
+  2. the autotuner picks block size, sweeps per launch and tile size for the
+     batch (results do not depend on the choice);
+  3. plain solver: 20 sweeps per substep (the reference's fixed budget,
      SolverPBD::ProjectConstraint(st, iterations)); its mean end-of-substep
      residual (max |d-L|/L per cloth, averaged over cloths and substeps) is
-     the fixed residual R;
-  3. the warm-started solver runs the smallest sweep budget whose mean
+     the fixed residual R.  Batched and unbatched plain runs give the same
+     positions bit for bit, hence the same R;
+  4. the warm-started solver runs the smallest sweep budget whose mean
      residual over the same substeps is <= R (searched by bisection);
-  4. both are timed over the same number of substeps.
-`value` is the warm-started batched solver's substeps/s; the plain solver's
-rate at the same residual is in config.plain_substeps_per_s.
+  5. timed: the plain solver unbatched (each cloth stepped alone, as the
+     reference's per-object SolverPBD), the plain solver batched, and the
+     warm-started batched solver (substeps replayed from a captured graph).
+`value` is the warm-started batched solver's substeps/s;
+config.speedup_vs_plain divides it by the unbatched plain solver's rate and
+config.speedup_vs_plain_batched by the batched plain solver's.
 
-`roofline` is for the dominant kernel (the LDS-tile Gauss-Seidel kernel) in
-the timed warm-started run: algorithmic bytes (68 B per distance prim per
-sweep) over its summed launch time, from HIP events on the library's own
-stream.  `cpu_baseline` times the CPU oracle (one thread, plain solver,
-20 sweeps) on a bounded sample of the same batch, scaled to the whole batch by
-vertex-sweeps.
+`roofline` is for the dominant kernel (the LDS-tile Gauss-Seidel kernel),
+from a second, eager pass over the same warm-started substeps with HIP events
+around every launch on the library's own stream: algorithmic bytes (68 B per
+distance prim per sweep) over the summed launch time.  `cpu_baseline` times
+the CPU oracle (one thread, plain solver, 20 sweeps) on a bounded sample of
+the same batch, scaled to the whole batch by vertex-sweeps.
 """
 from __future__ import annotations
 
@@ -63,6 +69,8 @@ def parse():
     ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--unbatched-steps", type=int, default=3, help="timed substeps of the unbatched plain solver")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use --tile-sweeps, 1024 threads)")
     return ap.parse_args()
 
 
@@ -137,10 +145,23 @@ def main():
     log(f"[bench] init (constraints, GPU colouring, tile plan) {time.perf_counter() - t:.1f} s")
     ctx.ws_set_params(prm)
     ctx.ws_set_norm(norm)
+    tuned = {"tile_sweeps": a.tile_sweeps, "block_threads": 1024, "tile_max_verts": 0}
+    ctx.set_solver(dt=scenes.DT, iterations=PLAIN_SWEEPS, check_every=a.tile_sweeps, tile_sweeps=a.tile_sweeps,
+                   graph=1)
+    ctx.step(2)
+    if not a.no_tune:
+        t = time.perf_counter()
+        best, _, best_ms, trials = ctx.autotune(2)
+        tuned = {"tile_sweeps": best.tile_sweeps, "block_threads": best.block_threads,
+                 "tile_max_verts": best.tile_max_verts, "trials": trials, "best_ms_per_substep": best_ms,
+                 "seconds": time.perf_counter() - t}
+        log(f"[bench] autotune: {tuned}")
 
-    def solver(sweeps, ws):
+    def solver(sweeps, ws, graph=1):
         ctx.ws_enable(ws)
-        ctx.set_solver(dt=scenes.DT, iterations=sweeps, check_every=a.tile_sweeps, tile_sweeps=a.tile_sweeps)
+        ctx.set_solver(dt=scenes.DT, iterations=sweeps, check_every=tuned["tile_sweeps"],
+                       tile_sweeps=tuned["tile_sweeps"], tile_max_verts=tuned["tile_max_verts"],
+                       block_threads=tuned["block_threads"], graph=graph)
 
     solver(PLAIN_SWEEPS, False)
     ctx.step(a.warmup)
@@ -181,26 +202,55 @@ def main():
         f"{r_warm:.5e} with {warm_sweeps} sweeps (tried {dict(sorted(tried.items()))}); prediction kept "
         f"for {100 * used_frac:.0f}% of cloth-substeps")
 
-    def timed(sweeps, ws):
+    def timed(sweeps, ws, steps):
         ctx.snapshot(restore=True)
         solver(sweeps, ws)
         ctx.step(1)
         ctx.sync()
-        ctx.profile(True)
         ctx.profile_read(reset=True)
-        ctx.sync()
         t0 = time.perf_counter()
-        ctx.step(a.steps)
+        ctx.step(steps)
         ctx.sync()
         el = time.perf_counter() - t0
+        replays = ctx.profile_read(reset=True).graph_replays
+        return steps / el, el * 1e3 / steps, replays
+
+    def timed_unbatched(sweeps, steps):
+        ctx.snapshot(restore=True)
+        solver(sweeps, False, graph=0)
+        for i in ids:
+            ctx.step_cloth(i, 1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for i in ids:
+                ctx.step_cloth(i, 1)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        return steps / el, el * 1e3 / steps
+
+    def profiled(sweeps, ws, steps):
+        ctx.snapshot(restore=True)
+        solver(sweeps, ws, graph=0)
+        ctx.step(1)
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        ctx.step(steps)
+        ctx.sync()
         prof = ctx.profile_read(reset=True)
         ctx.profile(False)
-        return a.steps / el, el * 1e3 / a.steps, prof
+        return prof
 
-    plain_rate, plain_ms, plain_prof = timed(PLAIN_SWEEPS, False)
-    warm_rate, warm_ms, prof = timed(warm_sweeps, True)
-    log(f"[bench] plain {plain_rate:.2f} substeps/s ({plain_ms:.2f} ms); warm start + batched "
-        f"{warm_rate:.2f} substeps/s ({warm_ms:.2f} ms): x{warm_rate / plain_rate:.2f}")
+    unb_rate, unb_ms = timed_unbatched(PLAIN_SWEEPS, a.unbatched_steps)
+    log(f"[bench] plain unbatched (one cloth at a time) {unb_rate:.2f} substeps/s ({unb_ms:.2f} ms)")
+    plain_rate, plain_ms, plain_replays = timed(PLAIN_SWEEPS, False, a.steps)
+    warm_rate, warm_ms, warm_replays = timed(warm_sweeps, True, a.steps)
+    log(f"[bench] plain batched {plain_rate:.2f} substeps/s ({plain_ms:.2f} ms); warm start + batched "
+        f"{warm_rate:.2f} substeps/s ({warm_ms:.2f} ms): x{warm_rate / unb_rate:.2f} vs plain unbatched, "
+        f"x{warm_rate / plain_rate:.2f} vs plain batched (graph replays {plain_replays}, {warm_replays})")
+    prof = profiled(warm_sweeps, True, a.steps)
+    plain_prof = profiled(PLAIN_SWEEPS, False, a.steps)
     achieved = prof.tile_alg_bytes / (prof.tile_ms * 1e-3) / 1e9
     plain_achieved = plain_prof.tile_alg_bytes / (plain_prof.tile_ms * 1e-3) / 1e9
     copy_ms = ctx.bench_copy(1 << 30, 10)
@@ -242,16 +292,22 @@ def main():
                                              "substep, plain solver with 20 sweeps, same start state",
                                "R": r_plain, "warm_start_residual": r_warm,
                                "substeps_compared": a.match_substeps},
-            "plain_substeps_per_s": plain_rate,
-            "plain_ms_per_step": plain_ms,
-            "speedup_vs_plain": warm_rate / plain_rate,
+            "plain_substeps_per_s": unb_rate,
+            "plain_ms_per_step": unb_ms,
+            "plain_definition": "plain solver, unbatched: each cloth stepped alone (its own launches, LDS "
+                                "tiles), 20 sweeps, no warm start",
+            "speedup_vs_plain": warm_rate / unb_rate,
+            "plain_batched_substeps_per_s": plain_rate,
+            "plain_batched_ms_per_step": plain_ms,
+            "speedup_vs_plain_batched": warm_rate / plain_rate,
+            "graph_replays_timed": warm_replays,
+            "autotuned": tuned,
             "plain_sweeps": PLAIN_SWEEPS,
             "warm_start_sweeps": warm_sweeps,
             "warm_start_kept_fraction": used_frac,
             "warm_start_training": train_info,
-            "vertex_iterations_per_s": prof.vertex_sweeps / (prof.step_ms * 1e-3) if prof.step_ms else None,
-            "plain_vertex_iterations_per_s": plain_prof.vertex_sweeps / (plain_prof.step_ms * 1e-3),
-            "tile_sweeps_per_launch": a.tile_sweeps,
+            "vertex_iterations_per_s": nv_total * warm_sweeps * warm_rate,
+            "plain_batched_vertex_iterations_per_s": nv_total * PLAIN_SWEEPS * plain_rate,
             "single_tile_cloths": sum(1 for i in infos if i.whole),
             "tiles": sum(i.ntiles for i in infos),
             "copy_kernel_gbs": copy_gbs,

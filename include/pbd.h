@@ -86,6 +86,8 @@ typedef struct {
     int tile_sweeps;        /* sweeps per launch for multi-tile cloths (0 = check_every) */
     int tile_max_verts;     /* LDS vertices per tile (0 = from the device's LDS size) */
     int block_threads;      /* threads per workgroup of the tile kernel (0 = 1024) */
+    int graph;              /* 1: pbd_step replays each substep from a captured hipGraph whenever the
+                               substep makes no host decision (fixed sweeps, no collisions, profiling off) */
 } pbd_solver_params;
 
 typedef struct {
@@ -146,6 +148,12 @@ PBD_API int pbd_integrate(pbd_ctx* ctx, float dt);
 /* The driver loop body Advect -> ProjectConstraint -> Integration
  * (PositionBasedDynamics.cpp:263-290), `substeps` times, fused. */
 PBD_API int pbd_step(pbd_ctx* ctx, int substeps);
+/* SolverPBD::SetTarget(obj) and the same loop body on that one object
+ * (PBD_Basic.cuh:224-233, PositionBasedDynamics.cpp:263-290): substeps of a
+ * single cloth whose launches cover only its own vertices and prims -- the
+ * unbatched solver.  Collisions and the warm start run on the whole batch
+ * only (PBD_ERR_STATE while enabled). */
+PBD_API int pbd_step_cloth(pbd_ctx* ctx, int cloth, int substeps);
 PBD_API int pbd_cloth_get_state(pbd_ctx* ctx, int cloth, int which, float* xyz);
 PBD_API int pbd_cloth_set_state(pbd_ctx* ctx, int cloth, int which, const float* xyz);
 /* Convergence residual over distance prims: max and RMS of |d - L| / L (FP64). */
@@ -253,9 +261,23 @@ typedef struct {
     long long substeps;
     long long vertex_sweeps; /* sum over cloths and substeps of V x sweeps */
     long long sweeps_max;    /* largest per-cloth sweep count seen */
+    long long graph_replays; /* substeps replayed from a captured graph */
 } pbd_profile;
 PBD_API int pbd_profile_enable(pbd_ctx* ctx, int on);
 PBD_API int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset);
+/* Autotuner (no counterpart in the reference).  From the current state, times
+ * `trial_substeps` pbd_step substeps (after one untimed substep) for every
+ * candidate of block_threads {256, 512, 1024} x tile_sweeps {1, 2, 4, 5, 10,
+ * 20 that are <= iterations} x tile_max_verts {device maximum, 4096, 2048}
+ * and, with collisions enabled, the hash table size (table_bits - 2 ..
+ * table_bits + 2 of the hashed grid), restoring x and v after every trial.
+ * The fastest candidate becomes the solver configuration and is returned in
+ * `best` (and `best_table_bits`, -1 without collisions).  No candidate
+ * changes results: tile plans keep the colour order, hashed candidate sets
+ * do not depend on the table size, and with a residual tolerance only
+ * tile_sweeps == check_every is tried. */
+PBD_API int pbd_autotune(pbd_ctx* ctx, int trial_substeps, pbd_solver_params* best, int* best_table_bits,
+                         double* best_ms_per_substep, int* trials);
 /* Plain device copy of `bytes` (read once, written once), `reps` times; the
  * HBM calibration kernel for achievable bandwidth and counter scaling. */
 PBD_API int pbd_bench_copy(pbd_ctx* ctx, long long bytes, int reps, double* ms_per_copy);

@@ -745,7 +745,11 @@ static int cmp_i64(const void* a, const void* b) {
  * as consumed by CCD_SH_Extended (PBD/CCD.cpp:1541-1586): for each triangle t
  * whose centroid cell is valid, each vertex v of t, each triangle f != t
  * stored in one of the 27 cells around t's cell (z, y, x order, skipping
- * cells outside a dense grid) -> candidate (v, f).  Returned as the sorted
+ * cells outside a dense grid) -> candidate (v, f).  In hashed mode a face
+ * counts only in the slot of its own centroid cell (cells that merely share
+ * the slot are skipped), which makes the set independent of the table size;
+ * in dense mode every slot is one cell and the check never fails.  Returned
+ * as the sorted
  * set of keys (v << 32 | f) with pairs where v is a vertex of f removed (the
  * narrowphase rejects them first, PBD/CCD.cpp:1737-1738).  Returns the set
  * size (writes at most `cap` keys). */
@@ -766,6 +770,13 @@ long long orc_candidate_pairs(const orc_grid* g, const float* pos, const int* tr
     int64_t* buf = (int64_t*)malloc(sizeof(int64_t) * bufcap);
     float c[3];
     int gp[3], q[3];
+    int* tc = (int*)malloc(sizeof(int) * 3 * ((size_t)nt + 1));
+    for (int t = 0; t < nt; t++) {
+        const int* tv = tris + 3 * t;
+        for (int k = 0; k < 3; k++)
+            c[k] = (pos[3 * tv[0] + k] + pos[3 * tv[1] + k] + pos[3 * tv[2] + k]) / 3;
+        orc_cell_coords(g, c, tc + 3 * t);
+    }
     for (int t = 0; t < nt; t++) {
         if (keys[t] < 0) continue;
         const int* tv = tris + 3 * t;
@@ -781,6 +792,7 @@ long long orc_candidate_pairs(const orc_grid* g, const float* pos, const int* tr
                     for (int i = cs[h]; i <= ce[h]; i++) {
                         int f = ids[i];
                         if (f == t) continue;
+                        if (tc[3 * f] != q[0] || tc[3 * f + 1] != q[1] || tc[3 * f + 2] != q[2]) continue;
                         const int* fv = tris + 3 * f;
                         for (int k = 0; k < 3; k++) {
                             int v = tv[k];
@@ -795,7 +807,7 @@ long long orc_candidate_pairs(const orc_grid* g, const float* pos, const int* tr
     size_t u = 0;
     for (size_t i = 0; i < n; i++) if (i == 0 || buf[i] != buf[i - 1]) buf[u++] = buf[i];
     if (out) for (size_t i = 0; i < u && (long long)i < cap; i++) out[i] = buf[i];
-    free(buf); free(keys); free(ids); free(sk); free(cs); free(ce);
+    free(buf); free(keys); free(ids); free(sk); free(cs); free(ce); free(tc);
     return (long long)u;
 }
 

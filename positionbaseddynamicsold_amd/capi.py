@@ -43,7 +43,7 @@ class ClothParams(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("iterations", C.c_int), ("tol", C.c_double), ("check_every", C.c_int),
                 ("path", C.c_int), ("tile_sweeps", C.c_int), ("tile_max_verts", C.c_int),
-                ("block_threads", C.c_int)]
+                ("block_threads", C.c_int), ("graph", C.c_int)]
 
 
 class HashDesc(C.Structure):
@@ -63,7 +63,7 @@ class CollisionParams(C.Structure):
 class Profile(C.Structure):
     _fields_ = [("tile_ms", C.c_double), ("tile_launches", C.c_longlong), ("tile_alg_bytes", C.c_double),
                 ("step_ms", C.c_double), ("substeps", C.c_longlong), ("vertex_sweeps", C.c_longlong),
-                ("sweeps_max", C.c_longlong)]
+                ("sweeps_max", C.c_longlong), ("graph_replays", C.c_longlong)]
 
 
 class ClothInfo(C.Structure):
@@ -100,6 +100,8 @@ SIGNATURES = {
     "pbd_project": [_P, _I],
     "pbd_integrate": [_P, _F],
     "pbd_step": [_P, _I],
+    "pbd_step_cloth": [_P, _I, _I],
+    "pbd_autotune": [_P, _I, C.POINTER(SolverParams), _ip, _dp, _ip],
     "pbd_cloth_get_state": [_P, _I, _I, _fp],
     "pbd_cloth_set_state": [_P, _I, _I, _fp],
     "pbd_cloth_residual": [_P, _I, _dp, _dp],
@@ -288,9 +290,20 @@ class Context:
 
     # ---- solver --------------------------------------------------------
     def set_solver(self, dt=1.0 / 600.0, iterations=20, tol=-1.0, check_every=1, path=PATH_AUTO,
-                   tile_sweeps=0, tile_max_verts=0, block_threads=0):
-        p = SolverParams(dt, iterations, tol, check_every, path, tile_sweeps, tile_max_verts, block_threads)
+                   tile_sweeps=0, tile_max_verts=0, block_threads=0, graph=0):
+        p = SolverParams(dt, iterations, tol, check_every, path, tile_sweeps, tile_max_verts, block_threads, graph)
         check(self.lib.pbd_set_solver_params(self.h, C.byref(p)))
+
+    def set_solver_params(self, p: SolverParams):
+        check(self.lib.pbd_set_solver_params(self.h, C.byref(p)))
+
+    def autotune(self, trial_substeps: int = 3):
+        """Picks block size, sweeps per launch, tile size (and hash table size
+        with collisions) by timing; returns (params, table_bits, ms, trials)."""
+        p = SolverParams()
+        bits, ms, n = C.c_int(), C.c_double(), C.c_int()
+        check(self.lib.pbd_autotune(self.h, trial_substeps, C.byref(p), C.byref(bits), C.byref(ms), C.byref(n)))
+        return p, bits.value, ms.value, n.value
 
     def advect(self, dt: float):
         check(self.lib.pbd_advect(self.h, dt))
@@ -303,6 +316,10 @@ class Context:
 
     def step(self, substeps: int = 1):
         check(self.lib.pbd_step(self.h, substeps))
+
+    def step_cloth(self, cloth: int, substeps: int = 1):
+        """Substeps of one cloth alone (the unbatched solver)."""
+        check(self.lib.pbd_step_cloth(self.h, cloth, substeps))
 
     def sync(self):
         check(self.lib.pbd_synchronize(self.h))

@@ -37,7 +37,7 @@ struct pbd_ctx {
     TilePlan plan;
     int plan_sweeps = 0, plan_max_load = 0;
     DBuf<TileDev> tiles;
-    DBuf<int> tverts, tcols, order_whole, order_partial;
+    DBuf<int> tverts, tcols, order_whole, order_partial, tile_ids;
     DBuf<uint32_t> tdist, tbend, town;
     DBuf<float> tdist_rest, tbend_rest, town_rest;
     bool any_whole = false, any_partial = false;
@@ -72,7 +72,12 @@ struct pbd_ctx {
         DBuf<float4> delta, psave;
         DBuf<unsigned long long> r0, r1;
         std::vector<int> used;
+        bool used_pending = false;  // r0/r1 hold a decision not yet read back
     } ws;
+    // substep captured as a graph, one per starting p buffer; the p buffer
+    // the replay ends in is graph_end
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_end[2] = {0, 0};
     // device-side state snapshot
     DBuf<float4> snap_x, snap_v;
     bool snap_valid = false;
@@ -83,6 +88,8 @@ struct pbd_ctx {
     int ev_used = 0;
     ~pbd_ctx() {
         for (auto e : ev_pool) (void)hipEventDestroy(e);
+        for (auto& g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
     }
 };
 
@@ -113,6 +120,15 @@ Cloth& cloth_of(pbd_ctx* c, int id) {
 
 void set_device(pbd_ctx* c) { PBD_HIP(hipSetDevice(c->device)); }
 
+// Captured substeps bake in buffer pointers and by-value kernel arguments;
+// every change of those drops them.
+void drop_graphs(pbd_ctx* c) {
+    for (auto& g : c->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
 pbd_cloth_params default_params() {
     pbd_cloth_params p{};
     p.damping = 1.0f;
@@ -122,6 +138,7 @@ pbd_cloth_params default_params() {
 
 // Per-cloth kernel constants for substep length dt.
 void fill_cloth_dev(pbd_ctx* c) {
+    drop_graphs(c);
     float dt = c->sp.dt;
     c->cl_h.resize(c->cloths.size());
     for (size_t i = 0; i < c->cloths.size(); i++) {
@@ -202,6 +219,10 @@ void upload_plan(pbd_ctx* c) {
     c->town_rest.upload(p.own_rest, c->stream);
     c->order_whole.upload(p.whole_order, c->stream);
     c->order_partial.upload(p.partial_order, c->stream);
+    std::vector<int> ids(p.tiles.size());
+    for (size_t t = 0; t < ids.size(); t++) ids[t] = (int)t;
+    c->tile_ids.upload(ids, c->stream);
+    drop_graphs(c);
     c->any_whole = false;
     c->any_partial = false;
     for (auto& t : p.tiles) (t.whole ? c->any_whole : c->any_partial) = true;
@@ -210,6 +231,9 @@ void upload_plan(pbd_ctx* c) {
 void replan_if_needed(pbd_ctx* c) {
     int s = effective_tile_sweeps(c), ml = effective_max_load(c);
     if (s == c->plan_sweeps && ml == c->plan_max_load) return;
+    // a failed plan leaves no valid one behind: the next call plans again
+    c->plan_sweeps = 0;
+    c->plan_max_load = 0;
     plan_tiles(c->cloths, ml, s, c->plan);
     c->plan_sweeps = s;
     c->plan_max_load = ml;
@@ -220,6 +244,7 @@ void replan_if_needed(pbd_ctx* c) {
 void ensure_slots(pbd_ctx* c, int nchunks) {
     int need = std::max(2, nchunks + 1);
     if (need > c->nslots) {
+        drop_graphs(c);
         c->res_slots.alloc((size_t)need * c->cloths.size());
         c->nslots = need;
     }
@@ -337,12 +362,18 @@ void do_init(pbd_ctx* c, int color_rounds) {
     std::vector<float2> gbr;
     c->gd_off.assign(c->ncolors_max + 1, 0);
     c->gb_off.assign(c->ncolors_max + 1, 0);
+    for (auto& k : c->cloths) {
+        k.gd_cut.assign(2 * (size_t)k.ncolors, 0);
+        k.gb_cut.assign(2 * (size_t)k.ncolors, 0);
+    }
     for (int col = 0; col < c->ncolors_max; col++) {
         c->gd_off[col] = (int)gd.size();
         c->gb_off[col] = (int)gb.size();
         for (size_t ci = 0; ci < c->cloths.size(); ci++) {
             Cloth& k = c->cloths[ci];
             if (col >= k.ncolors) continue;
+            k.gd_cut[2 * col] = (int)gd.size();
+            k.gb_cut[2 * col] = (int)gb.size();
             int st = k.worksets[2 * col], n = k.worksets[2 * col + 1];
             for (int j = st; j < st + n; j++) {
                 int p = k.sorted[j];
@@ -360,12 +391,15 @@ void do_init(pbd_ctx* c, int color_rounds) {
                     gbr.push_back(make_float2(k.rest[p], cb));
                 }
             }
+            k.gd_cut[2 * col + 1] = (int)gd.size();
+            k.gb_cut[2 * col + 1] = (int)gb.size();
         }
     }
     c->gd_off[c->ncolors_max] = (int)gd.size();
     c->gb_off[c->ncolors_max] = (int)gb.size();
     for (size_t ci = 0; ci < c->cloths.size(); ci++) {
         Cloth& k = c->cloths[ci];
+        k.rd_lo = (int)rd.size();
         for (int p = 0; p < k.np(); p++) {
             if (k.prim_list[2 * p + 1] != 2) continue;
             const int* ix = &k.indices[k.prim_list[2 * p]];
@@ -374,6 +408,7 @@ void do_init(pbd_ctx* c, int color_rounds) {
             memcpy(&rb, &r, 4);
             rd.push_back(make_int4(ix[0] + k.vbase, ix[1] + k.vbase, rb, (int)ci));
         }
+        k.rd_hi = (int)rd.size();
     }
     c->gdist.upload(gd, s);
     c->gbidx.upload(gb, s);
@@ -419,6 +454,7 @@ hipEvent_t next_event(pbd_ctx* c) {
 
 // Sums the durations of the recorded (start, stop) pairs; call after a sync.
 void collect_events(pbd_ctx* c) {
+    if (c->ev_used > 0) PBD_HIP(hipEventSynchronize(c->ev_pool[c->ev_used - 1]));
     for (int i = 0; i + 1 < c->ev_used; i += 2) {
         float ms = 0.0f;
         PBD_HIP(hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
@@ -503,6 +539,7 @@ void ws_build_nbr(pbd_ctx* c) {
     c->ws.r0.alloc(c->cloths.size());
     c->ws.r1.alloc(c->cloths.size());
     c->ws.nbr_built = true;
+    drop_graphs(c);
     if (c->ws.norm_h.empty()) {
         c->ws.norm_h.assign(47, 1.0f);
         for (int k = 0; k < 22; k++) c->ws.norm_h[k] = 0.0f;
@@ -527,23 +564,39 @@ void ws_apply_prediction(pbd_ctx* c) {
     launch_fill_u64(s, c->ws.r1.ptr, ncl, 0ull);
     ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr);
     ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr, c->nv_total);
-    {
-        // per-cloth record of whether the prediction was kept
-        std::vector<unsigned long long> a, b;
-        c->ws.r0.download(a, s);
-        c->ws.r1.download(b, s);
-        PBD_HIP(hipStreamSynchronize(s));
-        c->ws.used.assign(ncl, 0);
-        for (int i = 0; i < ncl; i++) {
-            double x, y;
-            memcpy(&x, &a[i], 8);
-            memcpy(&y, &b[i], 8);
-            c->ws.used[i] = y <= x ? 1 : 0;
-        }
+    c->ws.used_pending = true;
+}
+
+// Per-cloth record of whether the last prediction was kept, read back on
+// demand so that the substep itself never waits for the host.
+void ws_read_used(pbd_ctx* c) {
+    if (!c->ws.used_pending) return;
+    hipStream_t s = c->stream;
+    int ncl = (int)c->cloths.size();
+    std::vector<unsigned long long> a, b;
+    c->ws.r0.download(a, s);
+    c->ws.r1.download(b, s);
+    PBD_HIP(hipStreamSynchronize(s));
+    c->ws.used.assign(ncl, 0);
+    for (int i = 0; i < ncl; i++) {
+        double x, y;
+        memcpy(&x, &a[i], 8);
+        memcpy(&y, &b[i], 8);
+        c->ws.used[i] = y <= x ? 1 : 0;
     }
+    c->ws.used_pending = false;
 }
 
 // ---- solver orchestration ---------------------------------------------------
+// True when some cloth stops on a residual tolerance (the host then reads
+// residuals back between launches).
+bool has_tolerance(const pbd_ctx* c) {
+    if (c->sp.tol >= 0.0) return true;
+    for (auto& k : c->cloths)
+        if (k.tol >= 0.0) return true;
+    return false;
+}
+
 void advect_all(pbd_ctx* c, float dt) {
     // the kernel constants hold g*dt for sp.dt; recompute when dt differs
     if (dt != c->sp.dt) {
@@ -560,7 +613,8 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
     int ncl = (int)c->cloths.size();
     double tol = c->sp.tol;
     int ce = std::max(1, c->sp.check_every);
-    PBD_HIP(hipMemsetAsync(c->sweeps.ptr, 0, sizeof(int) * ncl, s));
+    const bool any_tol = has_tolerance(c);
+    launch_fill_int(s, c->sweeps.ptr, ncl, 0);
     if (c->sp.path == PBD_PATH_GLOBAL) {
         PBD_CHECK(!fused_whole, PBD_ERR_STATE, "internal: global path is never fused");
         float4* p = c->pbuf[c->pcur].ptr;
@@ -590,8 +644,6 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
             }
         }
         for (auto& k : c->cloths) k.last_sweeps = sw;
-        std::vector<int> sv(ncl, sw);
-        PBD_HIP(hipMemcpyAsync(c->sweeps.ptr, sv.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, s));
         return;
     }
     replan_if_needed(c);
@@ -634,8 +686,6 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
             tiles((int)c->order_partial.n, c->order_partial.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
             cur ^= 1;
             sw += a.nsweeps;
-            bool any_tol = tol >= 0.0;
-            for (auto& k : c->cloths) any_tol |= k.tol >= 0.0;
             if (any_tol && ch + 1 < nchunks) {
                 // stop once every multi-tile cloth met the tolerance
                 std::vector<unsigned long long> r(ncl);
@@ -660,15 +710,18 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
     if (c->any_whole && c->any_partial && cur != (c->pcur ^ 1)) {
         // copy whole-cloth vertices into the final buffer
         for (auto& k : c->cloths)
-            if (k.whole)
-                PBD_HIP(hipMemcpyAsync(c->pbuf[cur].ptr + k.vbase, c->pbuf[c->pcur ^ 1].ptr + k.vbase,
-                                       sizeof(float4) * k.nv, hipMemcpyDeviceToDevice, s));
+            if (k.whole) launch_copy(s, c->pbuf[c->pcur ^ 1].ptr + k.vbase, c->pbuf[cur].ptr + k.vbase, k.nv);
     }
     c->pcur = cur;
-    std::vector<int> sv;
-    c->sweeps.download(sv, s);
-    PBD_HIP(hipStreamSynchronize(s));
-    for (int ci = 0; ci < ncl; ci++) c->cloths[ci].last_sweeps = sv[ci];
+    if (any_tol) {
+        std::vector<int> sv;
+        c->sweeps.download(sv, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        for (int ci = 0; ci < ncl; ci++) c->cloths[ci].last_sweeps = sv[ci];
+    } else {
+        // fixed sweeps: every cloth ran all of them
+        for (auto& k : c->cloths) k.last_sweeps = iterations;
+    }
     if (c->prof_on) {
         collect_events(c);
         double bytes = 0.0;
@@ -739,6 +792,191 @@ void step_ccd(pbd_ctx* c, float dt) {
             c->prof.tile_alg_bytes += (double)I * (kDistBytes * k.ne + kBendBytes * k.nb);
             c->prof.vertex_sweeps += (long long)k.nv * I;
         }
+    }
+}
+
+// One substep of the whole batch.
+void substep(pbd_ctx* c, float dt) {
+    if (c->coll.enabled) {
+        step_ccd(c, dt);
+    } else if (c->ws.enabled) {
+        advect_all(c, dt);
+        ws_apply_prediction(c);
+        project(c, c->sp.iterations, false, dt);
+        integrate_all(c, dt, false);
+    } else if (c->sp.path == PBD_PATH_GLOBAL) {
+        advect_all(c, dt);
+        project(c, c->sp.iterations, false, dt);
+        integrate_all(c, dt, false);
+    } else {
+        // multi-tile cloths advect separately; whole cloths fuse it
+        if (c->any_partial)
+            launch_advect(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, 0,
+                          c->nv_total, dt, 1);
+        project(c, c->sp.iterations, true, dt);
+        integrate_all(c, dt, true);
+    }
+}
+
+// A substep can be replayed from a graph when it makes no decision on the
+// host: fixed sweeps, no collision pipeline (its pair counts size the
+// buffers), no per-launch profiling events.
+bool graph_eligible(const pbd_ctx* c) {
+    return c->sp.graph && !c->coll.enabled && !c->prof_on && !has_tolerance(c);
+}
+
+// Captures the substep that starts from p buffer pcur on first use, then
+// replays it.  The host-side effects of a substep (which p buffer is current,
+// sweep counts, the pending fallback record) are applied after every replay.
+void graph_substep(pbd_ctx* c, float dt) {
+    hipStream_t s = c->stream;
+    const int start = c->pcur;
+    if (!c->graph[start]) {
+        hipGraph_t g = nullptr;
+        PBD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        try {
+            substep(c, dt);
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(s, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            c->pcur = start;
+            throw;
+        }
+        PBD_HIP(hipStreamEndCapture(s, &g));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PBD_HIP(e);
+        c->graph[start] = ex;
+        c->graph_end[start] = c->pcur;
+        c->pcur = start;
+    }
+    PBD_HIP(hipGraphLaunch(c->graph[start], s));
+    c->pcur = c->graph_end[start];
+    for (auto& k : c->cloths) k.last_sweeps = c->sp.iterations;
+    if (c->ws.enabled) c->ws.used_pending = true;
+    c->prof.graph_replays++;
+}
+
+// One substep of a single cloth, every launch restricted to its vertex and
+// prim ranges: the reference's solver driven one PBDObject at a time
+// (SolverPBD::SetTarget, PBD_Basic.cuh:224-233; loop body
+// PositionBasedDynamics.cpp:263-290).  Same arithmetic and order as the batch.
+void step_cloth(pbd_ctx* c, int ci, float dt) {
+    hipStream_t s = c->stream;
+    Cloth& k = c->cloths[ci];
+    const int ncl = (int)c->cloths.size();
+    const int I = c->sp.iterations, ce = std::max(1, c->sp.check_every);
+    const double tol = k.tol >= 0.0 ? k.tol : c->sp.tol;
+    const int vb = k.vbase, ve = k.vbase + k.nv;
+    if (dt != c->sp.dt) {
+        c->sp.dt = dt;
+        fill_cloth_dev(c);
+    }
+    if (c->sp.path == PBD_PATH_GLOBAL) {
+        float4* p = c->pbuf[c->pcur].ptr;
+        launch_advect(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, p, vb, ve, dt, 0);
+        const bool coll = k.prm.sphere_on || k.prm.box_on;
+        int sw = 0;
+        while (sw < I) {
+            for (int col = 0; col < k.ncolors; col++)
+                launch_project_colour(s, c->gdist.ptr, k.gd_cut[2 * col], k.gd_cut[2 * col + 1], c->gbidx.ptr,
+                                      c->gbrc.ptr, k.gb_cut[2 * col], k.gb_cut[2 * col + 1], c->ktab_d.ptr,
+                                      c->ktab_b.ptr, c->kmax, sw, p);
+            if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, p, vb, ve);
+            sw++;
+            if (tol >= 0.0 && sw % ce == 0) {
+                launch_fill_u64(s, c->rmax.ptr + ci, 1, 0ull);
+                launch_residual(s, c->rdist.ptr + k.rd_lo, k.rd_hi - k.rd_lo, p, c->rmax.ptr, c->rsum.ptr, nullptr);
+                unsigned long long r = 0;
+                PBD_HIP(hipMemcpyAsync(&r, c->rmax.ptr + ci, 8, hipMemcpyDeviceToHost, s));
+                PBD_HIP(hipStreamSynchronize(s));
+                double m;
+                memcpy(&m, &r, 8);
+                if (m <= tol) break;
+            }
+        }
+        k.last_sweeps = sw;
+        launch_integrate(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, p, vb, ve, dt, 0);
+        return;
+    }
+    // LDS tiles of this cloth only
+    const int nblocks = k.tile_end - k.tile_begin;
+    const int* order = c->tile_ids.ptr + k.tile_begin;
+    int ts = effective_tile_sweeps(c);
+    int nchunks = (I + ts - 1) / ts;
+    ensure_slots(c, nchunks);
+    launch_fill_u64(s, c->res_slots.ptr, (size_t)c->nslots * ncl, 0ull);
+    launch_fill_int(s, c->sweeps.ptr + ci, 1, 0);
+    size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
+    int threads = c->sp.block_threads > 0 ? c->sp.block_threads : 1024;
+    SolveArgs a{};
+    a.dt = dt;
+    a.check_every = ce;
+    a.tol = c->sp.tol;
+    a.kmax = c->kmax;
+    auto tiles = [&](const float4* pin, float4* pout) {
+        launch_tiles(s, nblocks, order, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr, c->tbend.ptr,
+                     c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr,
+                     c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, a, threads,
+                     lds);
+    };
+    int cur = c->pcur;
+    if (k.whole) {
+        // advect on load, all sweeps, integrate on store: one launch
+        a.nsweeps = I;
+        a.fused = 1;
+        tiles(c->pbuf[cur ^ 1].ptr, c->pbuf[cur].ptr);  // p_in is not read when fused
+    } else {
+        launch_advect(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[cur].ptr, vb, ve, dt, 0);
+        int sw = 0;
+        for (int ch = 0; ch < nchunks; ch++) {
+            a.sweep_base = sw;
+            a.nsweeps = std::min(ts, I - sw);
+            a.chunk = ch;
+            tiles(c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr);
+            cur ^= 1;
+            sw += a.nsweeps;
+            if (tol >= 0.0 && ch + 1 < nchunks) {
+                unsigned long long r = 0;
+                PBD_HIP(hipMemcpyAsync(&r, c->res_slots.ptr + (size_t)ch * ncl + ci, 8, hipMemcpyDeviceToHost, s));
+                PBD_HIP(hipStreamSynchronize(s));
+                double m;
+                memcpy(&m, &r, 8);
+                if (m <= tol) break;
+            }
+        }
+        // the batch's current p buffer must hold this cloth's result
+        if (cur != c->pcur) launch_copy(s, c->pbuf[cur].ptr + vb, c->pbuf[c->pcur].ptr + vb, k.nv);
+        launch_integrate(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, vb, ve, dt, 0);
+    }
+    if (tol >= 0.0) {
+        int sv = 0;
+        PBD_HIP(hipMemcpyAsync(&sv, c->sweeps.ptr + ci, sizeof(int), hipMemcpyDeviceToHost, s));
+        PBD_HIP(hipStreamSynchronize(s));
+        k.last_sweeps = sv;
+    } else {
+        k.last_sweeps = I;
+    }
+}
+
+// pbd_step's loop: graph replays where eligible, eager substeps otherwise.
+void run_substeps(pbd_ctx* c, int n) {
+    replan_if_needed(c);
+    const float dt = c->sp.dt;
+    const bool graph = graph_eligible(c);
+    if (graph) {
+        // everything a capture must not do (allocations, uploads) happens first
+        int ts = effective_tile_sweeps(c);
+        ensure_slots(c, (c->sp.iterations + ts - 1) / ts);
+        if (c->ws.enabled) ws_build_nbr(c);
+    }
+    for (int s = 0; s < n; s++) {
+        if (graph)
+            graph_substep(c, dt);
+        else
+            substep(c, dt);
     }
 }
 
@@ -1006,36 +1244,27 @@ int pbd_step(pbd_ctx* ctx, int substeps) {
         PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
         PBD_CHECK(substeps >= 0, PBD_ERR_ARG, "substeps must be >= 0");
         set_device(ctx);
-        replan_if_needed(ctx);
-        float dt = ctx->sp.dt;
         auto t0 = std::chrono::steady_clock::now();
-        for (int s = 0; s < substeps; s++) {
-            if (ctx->coll.enabled) {
-                step_ccd(ctx, dt);
-            } else if (ctx->ws.enabled) {
-                advect_all(ctx, dt);
-                ws_apply_prediction(ctx);
-                project(ctx, ctx->sp.iterations, false, dt);
-                integrate_all(ctx, dt, false);
-            } else if (ctx->sp.path == PBD_PATH_GLOBAL) {
-                advect_all(ctx, dt);
-                project(ctx, ctx->sp.iterations, false, dt);
-                integrate_all(ctx, dt, false);
-            } else {
-                // multi-tile cloths advect separately; whole cloths fuse it
-                if (ctx->any_partial)
-                    launch_advect(ctx->stream, ctx->cl.ptr, ctx->vcloth.ptr, ctx->x.ptr, ctx->v.ptr,
-                                  ctx->pbuf[ctx->pcur].ptr, 0, ctx->nv_total, dt, 1);
-                project(ctx, ctx->sp.iterations, true, dt);
-                integrate_all(ctx, dt, true);
-            }
-        }
+        run_substeps(ctx, substeps);
         if (ctx->prof_on) {
             PBD_HIP(hipStreamSynchronize(ctx->stream));
             ctx->prof.step_ms +=
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             ctx->prof.substeps += substeps;
         }
+    });
+}
+
+int pbd_step_cloth(pbd_ctx* ctx, int cloth, int substeps) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        cloth_of(ctx, cloth);
+        PBD_CHECK(substeps >= 0, PBD_ERR_ARG, "substeps must be >= 0");
+        PBD_CHECK(!ctx->coll.enabled, PBD_ERR_STATE, "collisions are solved for the whole batch (use pbd_step)");
+        PBD_CHECK(!ctx->ws.enabled, PBD_ERR_STATE, "the warm start runs on the whole batch (use pbd_step)");
+        set_device(ctx);
+        replan_if_needed(ctx);
+        for (int s = 0; s < substeps; s++) step_cloth(ctx, cloth, ctx->sp.dt);
     });
 }
 
@@ -1180,6 +1409,7 @@ int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* cp) {
         PBD_CHECK(ctx && cp, PBD_ERR_ARG, "null argument");
         PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
         set_device(ctx);
+        drop_graphs(ctx);
         if (!cp->enabled) {
             ctx->coll = *cp;
             return;
@@ -1520,6 +1750,7 @@ int pbd_ws_enable(pbd_ctx* ctx, int on) {
         PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
         if (on) ws_require_params(ctx);
         ctx->ws.enabled = on != 0;
+        drop_graphs(ctx);
     });
 }
 
@@ -1527,6 +1758,8 @@ int pbd_ws_last_used(pbd_ctx* ctx, int cloth, int* used) {
     return guard([&] {
         cloth_of(ctx, cloth);
         PBD_CHECK(used, PBD_ERR_ARG, "null output");
+        set_device(ctx);
+        ws_read_used(ctx);
         *used = cloth < (int)ctx->ws.used.size() ? ctx->ws.used[cloth] : 0;
     });
 }
@@ -1710,6 +1943,109 @@ int pbd_snapshot(pbd_ctx* ctx, int restore) {
             PBD_HIP(hipMemcpyAsync(ctx->pbuf[ctx->pcur].ptr, ctx->snap_x.ptr, b, hipMemcpyDeviceToDevice, s));
         }
         PBD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int pbd_autotune(pbd_ctx* ctx, int trial_substeps, pbd_solver_params* best, int* best_table_bits,
+                 double* best_ms_per_substep, int* trials) {
+    return guard([&] {
+        PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(trial_substeps >= 1, PBD_ERR_ARG, "trial_substeps must be >= 1");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        const size_t nv = (size_t)ctx->nv_total;
+        DBuf<float4> sx, sv;
+        sx.alloc(nv);
+        sv.alloc(nv);
+        launch_copy(s, ctx->x.ptr, sx.ptr, nv);
+        launch_copy(s, ctx->v.ptr, sv.ptr, nv);
+        auto restore = [&] {
+            launch_copy(s, sx.ptr, ctx->x.ptr, nv);
+            launch_copy(s, sv.ptr, ctx->v.ptr, nv);
+            launch_copy(s, sx.ptr, ctx->pbuf[ctx->pcur].ptr, nv);
+        };
+        const pbd_solver_params base = ctx->sp;
+        const HashCfg hbase = ctx->ccfg;
+        std::vector<int> threads = {256, 512, 1024}, sweeps, maxv = {0, 4096, 2048}, bits = {-1};
+        if (ctx->coll.enabled) {
+            sweeps = {base.tile_sweeps};  // contacts force one sweep per launch
+            if (hbase.mode == 1) {
+                bits.clear();
+                for (int d = -2; d <= 2; d++)
+                    if (hbase.table_bits + d >= 8 && hbase.table_bits + d <= 26) bits.push_back(hbase.table_bits + d);
+            }
+        } else if (has_tolerance(ctx)) {
+            sweeps = {std::max(1, base.check_every)};  // chunk = check interval keeps the stopping points
+        } else {
+            for (int t : {1, 2, 4, 5, 10, 20})
+                if (t <= base.iterations) sweeps.push_back(t);
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        PBD_HIP(hipEventCreate(&e0));
+        PBD_HIP(hipEventCreate(&e1));
+        double best_t = 1e300;
+        pbd_solver_params bp = base;
+        int bb = -1, n = 0;
+        try {
+            for (int hb : bits) {
+                if (hb >= 0) {
+                    ctx->ccfg.table_bits = hb;
+                    ctx->ccfg.ncells = 1 << hb;
+                }
+                for (int ts : sweeps)
+                    for (int mv : maxv)
+                        for (int th : threads) {
+                            ctx->sp = base;
+                            ctx->sp.tile_sweeps = ts;
+                            ctx->sp.tile_max_verts = mv;
+                            ctx->sp.block_threads = th;
+                            drop_graphs(ctx);
+                            try {
+                                replan_if_needed(ctx);
+                            } catch (const Error& e) {
+                                if (e.code != PBD_ERR_LIMIT) throw;
+                                continue;  // no tile plan fits this candidate
+                            }
+                            restore();
+                            run_substeps(ctx, 1);  // untimed: capture, first-touch
+                            restore();
+                            PBD_HIP(hipEventRecord(e0, s));
+                            run_substeps(ctx, trial_substeps);
+                            PBD_HIP(hipEventRecord(e1, s));
+                            PBD_HIP(hipEventSynchronize(e1));
+                            float ms = 0.0f;
+                            PBD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                            n++;
+                            if (ms < best_t) {
+                                best_t = ms;
+                                bp = ctx->sp;
+                                bb = hb;
+                            }
+                        }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            ctx->sp = base;
+            ctx->ccfg = hbase;
+            drop_graphs(ctx);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->sp = bp;
+        if (bb >= 0) {
+            ctx->ccfg.table_bits = bb;
+            ctx->ccfg.ncells = 1 << bb;
+        }
+        drop_graphs(ctx);
+        replan_if_needed(ctx);
+        restore();
+        PBD_HIP(hipStreamSynchronize(s));
+        if (best) *best = bp;
+        if (best_table_bits) *best_table_bits = bb;
+        if (best_ms_per_substep) *best_ms_per_substep = best_t / trial_substeps;
+        if (trials) *trials = n;
     });
 }
 

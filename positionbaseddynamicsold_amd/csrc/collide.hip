@@ -115,7 +115,7 @@ __device__ __forceinline__ void tri_cell(const HashCfg& g, const float4* __restr
 __global__ __launch_bounds__(256) void k_tri_keys(HashCfg g, const float4* __restrict__ p,
                                                   const int* __restrict__ tris, int nt,
                                                   uint32_t* __restrict__ keys1, uint32_t* __restrict__ ids,
-                                                  int* __restrict__ raw) {
+                                                  int* __restrict__ raw, int4* __restrict__ tcell) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     int gp[3];
@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) void k_tri_keys(HashCfg g, const float4* __res
     keys1[t] = (uint32_t)(k + 1);  // -1 sorts first, as the reference's signed sort
     ids[t] = (uint32_t)t;
     raw[t] = k;
+    tcell[t] = make_int4(gp[0], gp[1], gp[2], 0);
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ a, int n, int v) {
@@ -145,14 +146,17 @@ __global__ __launch_bounds__(256) void k_cell_ranges(const uint32_t* __restrict_
 
 // Candidate (v, f): v a vertex of triangle t whose cell is valid, f != t in
 // the 27 cells around t's cell (FindNeighbors, SpatialHashSystem.cpp:134-187),
-// v not a vertex of f.  mode 0 counts, mode 1 writes at off[t].
+// v not a vertex of f.  A face is taken only from the slot of its own cell,
+// so hashed tables give the same set at every size (other cells that share
+// the slot are skipped); dense keys are one slot per cell already.
+// mode 0 counts, mode 1 writes at off[t].
 template <int EMIT>
 __global__ __launch_bounds__(256) void k_pairs(HashCfg g, const float4* __restrict__ p,
                                                const int* __restrict__ tris, int nt,
                                                const int* __restrict__ raw, const uint32_t* __restrict__ sids,
                                                const int* __restrict__ cstart, const int* __restrict__ cend,
-                                               int* __restrict__ cnt_or_off, uint32_t* __restrict__ ov,
-                                               uint32_t* __restrict__ of) {
+                                               const int4* __restrict__ tcell, int* __restrict__ cnt_or_off,
+                                               uint32_t* __restrict__ ov, uint32_t* __restrict__ of) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     int n = 0, off = EMIT ? cnt_or_off[t] : 0;
@@ -163,7 +167,8 @@ __global__ __launch_bounds__(256) void k_pairs(HashCfg g, const float4* __restri
         for (int z = -1; z <= 1; z++)
             for (int y = -1; y <= 1; y++)
                 for (int x = -1; x <= 1; x++) {
-                    int h = cell_key(g, gp[0] + x, gp[1] + y, gp[2] + z);
+                    const int qx = gp[0] + x, qy = gp[1] + y, qz = gp[2] + z;
+                    int h = cell_key(g, qx, qy, qz);
                     if (h < 0) continue;
                     int s = cstart[h];
                     if (s < 0) continue;
@@ -171,6 +176,8 @@ __global__ __launch_bounds__(256) void k_pairs(HashCfg g, const float4* __restri
                     for (int i = s; i <= e; i++) {
                         int f = (int)sids[i];
                         if (f == t) continue;
+                        int4 fc = tcell[f];
+                        if (fc.x != qx || fc.y != qy || fc.z != qz) continue;
                         int f0 = tris[3 * f], f1 = tris[3 * f + 1], f2 = tris[3 * f + 2];
                         for (int k = 0; k < 3; k++) {
                             int v = tv[k];
@@ -467,12 +474,14 @@ void Broadphase::build(hipStream_t s, const HashCfg& g, const float4* p, const i
     keys.alloc(std::max(1, nt));
     ids.alloc(std::max(1, nt));
     raw.alloc(std::max(1, nt));
+    tcell.alloc(std::max(1, nt));
     k2.alloc(std::max(1, nt));
     v2.alloc(std::max(1, nt));
     hist.alloc((size_t)256 * ((std::max(1, nt) + 4095) / 4096));
     cstart.alloc(std::max(1, g.ncells));
     cend.alloc(std::max(1, g.ncells));
-    if (nt > 0) hipLaunchKernelGGL(k_tri_keys, dim3(blocks(nt)), dim3(256), 0, s, g, p, tris, nt, keys.ptr, ids.ptr, raw.ptr);
+    if (nt > 0) hipLaunchKernelGGL(k_tri_keys, dim3(blocks(nt)), dim3(256), 0, s, g, p, tris, nt, keys.ptr, ids.ptr, raw.ptr,
+                                  tcell.ptr);
     int bits = 1;
     while (bits < 32 && ((1ll << bits) < (long long)g.ncells + 1)) bits++;
     radix_sort_pairs(s, keys.ptr, ids.ptr, k2.ptr, v2.ptr, nt, bits, hist.ptr);
@@ -489,7 +498,7 @@ long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int
     off.alloc(nt + 1);
     scan_tmp.alloc((nt + SB) / SB + 2);
     hipLaunchKernelGGL(k_pairs<0>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr, cstart.ptr,
-                       cend.ptr, cnt.ptr, (uint32_t*)nullptr, (uint32_t*)nullptr);
+                       cend.ptr, tcell.ptr, cnt.ptr, (uint32_t*)nullptr, (uint32_t*)nullptr);
     PBD_HIP(hipMemsetAsync(cnt.ptr + nt, 0, sizeof(int), s));
     device_scan(s, cnt.ptr, off.ptr, nt + 1, scan_tmp.ptr);
     int total = 0;
@@ -504,7 +513,7 @@ long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int
     flag.alloc(total + 1);
     pos.alloc(total + 1);
     hipLaunchKernelGGL(k_pairs<1>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr, cstart.ptr,
-                       cend.ptr, off.ptr, pv.ptr, pf.ptr);
+                       cend.ptr, tcell.ptr, off.ptr, pv.ptr, pf.ptr);
     // sort by (v, f): by f first, then stably by v
     int fb = 1, vb = 1;
     while (fb < 31 && (1ll << fb) < nt) fb++;

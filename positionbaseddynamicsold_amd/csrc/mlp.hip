@@ -478,23 +478,38 @@ __device__ __forceinline__ double rel_res(float4 a, float4 b, float L) {
 }
 
 // Residual of every distance prim with and without the predicted delta,
-// max-reduced per cloth (FP64 bit patterns, r >= 0).
+// max-reduced per cloth (FP64 bit patterns, r >= 0).  The prim list is in
+// cloth order, so each thread keeps a running max over RESID_PER_THREAD prims
+// (stride 256) and flushes it with an atomic only when its cloth changes or
+// at the end; a wave whose lanes all end on one cloth reduces first.
+constexpr int RESID_PER_THREAD = 8;
+
 __global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist, int n, const float4* __restrict__ p,
                                                    const float4* __restrict__ delta,
                                                    unsigned long long* __restrict__ r0,
                                                    unsigned long long* __restrict__ r1) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long base = (long long)blockIdx.x * (256 * RESID_PER_THREAD) + threadIdx.x;
     int cl = -1;
     double a = 0.0, b = 0.0;
-    if (j < n) {
+    for (int k = 0; k < RESID_PER_THREAD; k++) {
+        long long j = base + 256ll * k;
+        if (j >= n) break;
         int4 q = dist[j];
-        cl = q.w;
+        if (q.w != cl) {
+            if (cl >= 0) {
+                atomicMax(&r0[cl], (unsigned long long)__double_as_longlong(a));
+                atomicMax(&r1[cl], (unsigned long long)__double_as_longlong(b));
+            }
+            cl = q.w;
+            a = 0.0;
+            b = 0.0;
+        }
         float4 P0 = p[q.x], P1 = p[q.y], D0 = delta[q.x], D1 = delta[q.y];
         float L = __int_as_float(q.z);
-        a = rel_res(P0, P1, L);
+        a = fmax(a, rel_res(P0, P1, L));
         float4 Q0 = make_float4(P0.x + D0.x, P0.y + D0.y, P0.z + D0.z, 0.0f);
         float4 Q1 = make_float4(P1.x + D1.x, P1.y + D1.y, P1.z + D1.z, 0.0f);
-        b = rel_res(Q0, Q1, L);
+        b = fmax(b, rel_res(Q0, Q1, L));
     }
     int c0 = __shfl(cl, 0);
     unsigned long long same = __ballot(cl == c0 || cl < 0);
@@ -616,7 +631,9 @@ void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* n
 void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* r0,
                unsigned long long* r1) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_ws_resid2, dim3(nb256(n)), dim3(256), 0, s, dist, n, p, delta, r0, r1);
+    if (n <= 0) return;
+    int blocks = (int)(((long long)n + 256 * RESID_PER_THREAD - 1) / (256 * RESID_PER_THREAD));
+    hipLaunchKernelGGL(k_ws_resid2, dim3(blocks), dim3(256), 0, s, dist, n, p, delta, r0, r1);
     PBD_HIP(hipGetLastError());
 }
 

@@ -145,6 +145,8 @@ struct Cloth {
     int vbase = 0;                        // offset in packed vertex arrays
     int pbase = 0;                        // offset in packed prim arrays
     int tile_begin = 0, tile_end = 0;     // range in the tile table
+    std::vector<int> gd_cut, gb_cut;      // per colour: [begin, end) in the batch colour-major prim arrays
+    int rd_lo = 0, rd_hi = 0;             // range in the cloth-ordered residual list
     bool whole = false;
     int max_loaded = 0;
     int last_sweeps = 0;
@@ -187,6 +189,7 @@ struct Broadphase {
     DBuf<float> ctmp_t, ctmp_n, ct, cn;
     DBuf<double> ctmp_w, cw;
     DBuf<unsigned long long> acc;
+    DBuf<int4> tcell;    // centroid cell coordinates per triangle
     void build(hipStream_t s, const HashCfg& g, const float4* p, const int* tris, int nt);
     long long pairs(hipStream_t s, const float4* p, const int* tris, int nv);
     long long narrow(hipStream_t s, const float4* x, const float4* p, const int* tris, float th);
@@ -256,6 +259,7 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   int nslots_cloth, int* sweeps_out, const SolveArgs& a, int threads,
                   size_t lds_bytes);
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
+void launch_fill_int(hipStream_t s, int* p, size_t n, int v);
 void launch_copy(hipStream_t s, const float4* a, float4* b, size_t n);
 
 }  // namespace pbd

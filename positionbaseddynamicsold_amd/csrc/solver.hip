@@ -216,6 +216,19 @@ void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned lo
     PBD_HIP(hipGetLastError());
 }
 
+__global__ void k_fill_int(int* p, size_t n, int v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Fill kernel instead of hipMemsetAsync so that the substep can be captured
+// into a graph.
+void launch_fill_int(hipStream_t s, int* p, size_t n, int v) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    PBD_HIP(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------
 // LDS tile kernel.  One workgroup per tile: the tile's loaded vertices live
 // in LDS as float4 for the whole launch; each colour phase processes the
@@ -371,12 +384,12 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   float4* p_out, unsigned long long* res_slots, int ncl, int* sweeps_out,
                   const SolveArgs& a, int threads, size_t lds_bytes) {
     if (nblocks <= 0) return;
-    static bool attr_set = false;
-    if (!attr_set) {
+    static size_t attr_bytes = 0;
+    if (lds_bytes > attr_bytes) {
         // not every runtime needs or supports the opt-in; the launch reports real failures
         (void)hipFuncSetAttribute((const void*)k_tiles, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
         (void)hipGetLastError();
-        attr_set = true;
+        attr_bytes = lds_bytes;
     }
     hipLaunchKernelGGL(k_tiles, dim3(nblocks), dim3(threads), lds_bytes, s, order, tiles, verts, dist, dist_rest,
                        bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v, p_in, p_out, res_slots,

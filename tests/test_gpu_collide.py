@@ -38,8 +38,22 @@ def test_broadphase_bit_exact(orc, mode, bits):
     assert np.array_equal(cs, ocs) and np.array_equal(ce, oce)
     op = orc.candidate_pairs(g, pos, tris)
     assert len(op) > 0 and np.array_equal(pairs, op)
-    if mode == 0:
-        assert (rk == -1).any() or True
+
+
+def test_hashed_pairs_independent_of_table_size(orc):
+    # a 4-bit table folds many cells into one slot; the candidate set must
+    # equal the 16-bit table's and the dense grid's (every triangle inside it)
+    pos, tris = deformed_cloth(orc, 30)
+    size, cell, centre = (12, 4, 12), (0.1, 0.1, 0.1), (0.5, 0.0, 0.5)
+    sets = []
+    with capi.Context() as ctx:
+        for mode, bits in ((0, 0), (1, 4), (1, 7), (1, 16)):
+            ctx.sh_build(capi.hash_desc(mode, centre, size, cell, bits), pos, tris)
+            assert (ctx.sh_get()[2] >= 0).all()
+            sets.append(ctx.sh_pairs())
+    assert len(sets[0]) > 0
+    for s in sets[1:]:
+        assert np.array_equal(s, sets[0])
 
 
 def test_ccd_detect_and_resolve_bit_exact(orc):
@@ -121,3 +135,40 @@ def test_cloth_on_icosphere_ccd_bit_exact(orc):
         assert contacts_seen > 0, "scene produced no contacts"
         assert np.array_equal(x, oc.x[: len(cpos)])
         assert np.array_equal(ctx.get(c1), spos)
+
+
+def test_autotune_with_collisions_leaves_results_unchanged():
+    # the hash table size is one of the tuned choices; contacts and positions
+    # must not depend on it
+    n, I, th = 28, 10, 0.004
+    spos, stris = scenes.icosphere(2, 0.3)
+
+    def build(ctx):
+        from oracle import oracle as orc
+        cpos, ctris = orc.grid_mesh(n, n, 0.8, 0.8, jitter=1e-4, seed=3)
+        cpos = (cpos + np.array([-0.4, 0.31, -0.4], np.float32)).astype(np.float32)
+        c0 = ctx.add_mesh(cpos, ctris)
+        ctx.set_distance_pairs(c0, orc.grid_pairs(n, n))
+        c1 = ctx.add_mesh(spos, stris)
+        ctx.set_constraint_option(c1, 0)
+        ctx.set_pins(c1, np.arange(len(spos)))
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=I)
+        ctx.set_collision(1, capi.hash_desc(1, cell=(0.06, 0.06, 0.06), table_bits=12), cell_scale=0.0,
+                          thickness=th, resolve_passes=2, detect_every=5)
+        return c0
+
+    with capi.Context() as a, capi.Context() as b:
+        c0 = build(a)
+        build(b)
+        a.step(10)
+        b.step(10)
+        p, bits, ms, trials = a.autotune(1)
+        print(f"autotune with collisions: table_bits {bits}, {ms:.3f} ms/substep, {trials} trials", flush=True)
+        assert bits in (10, 11, 12, 13, 14) and trials == 5 * 3 * 3
+        assert np.array_equal(a.get(c0), b.get(c0))
+        a.step(20)
+        b.step(20)
+        assert a.last_contacts()[0] > 0
+        assert np.array_equal(a.get(c0), b.get(c0))
+        assert np.array_equal(a.get(c0, capi.STATE_V), b.get(c0, capi.STATE_V))

@@ -164,6 +164,29 @@ def test_fallback_discards_a_bad_prediction():
         b.close()
 
 
+def test_zero_prediction_is_kept_and_changes_nothing():
+    # a network whose output layer is zero predicts delta = 0: the residual
+    # with the prediction equals the unwarmed one, the rule keeps it, and the
+    # substep equals the plain solver bit for bit
+    a, ids_a = config4_context(range(4))
+    b, ids_b = config4_context(range(4))
+    try:
+        b.ws_init(0)
+        z = b.ws_params()
+        z[5632:5827] = 0.0  # W3 / b3
+        b.ws_set_params(z)
+        b.ws_enable(True)
+        for _ in range(4):
+            a.step(1)
+            b.step(1)
+            assert [b.ws_last_used(i) for i in ids_b] == [1, 1, 1, 1]
+        for i, j in zip(ids_a, ids_b):
+            assert np.array_equal(a.get(i), b.get(j))
+    finally:
+        a.close()
+        b.close()
+
+
 def test_warm_start_gain_over_plain():
     # train on solver-generated samples of config-4 scenes (wind, pins and
     # sphere position varied by seed), then compare the sweeps both solvers

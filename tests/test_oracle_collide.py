@@ -43,7 +43,9 @@ def test_stable_sort_and_ranges_brute_force(orc):
             assert (cs[k], ce[k]) == (-1, -1)
 
 
-def brute_pairs(keyfun_cell, neighbour_slots, pos, tris):
+def brute_pairs(keyfun_cell, pos, tris):
+    # (v, f): f's centroid cell within one cell of t's on every axis, both
+    # triangles with a valid key (inside the grid in dense mode)
     cen = pos[tris].astype(np.float32).sum(1, dtype=np.float32) / np.float32(3)
     out = set()
     cells = [keyfun_cell(c) for c in cen]
@@ -51,9 +53,9 @@ def brute_pairs(keyfun_cell, neighbour_slots, pos, tris):
         gp, key = cells[t]
         if key < 0:
             continue
-        slots = neighbour_slots(gp)
         for f in range(len(tris)):
-            if f == t or cells[f][1] not in slots:
+            gf, kf = cells[f]
+            if f == t or kf < 0 or any(abs(gf[k] - gp[k]) > 1 for k in range(3)):
                 continue
             for v in tris[t]:
                 if v not in tris[f]:
@@ -84,14 +86,27 @@ def test_candidate_pairs_brute_force(orc, mode):
         gp = tuple(int(np.floor(np.float32((c[k] - start[k])) / cell[k])) for k in range(3))
         return gp, key_of(gp)
 
-    def slots(gp):
-        return {key_of(tuple(gp[k] + d[k] for k in range(3))) for d in itertools.product((-1, 0, 1), repeat=3)} - {-1}
-
     with np.errstate(over="ignore"):
-        expect = brute_pairs(cell_of, slots, pos, tris)
+        expect = brute_pairs(cell_of, pos, tris)
     got = orc.candidate_pairs(g, pos, tris)
     assert np.array_equal(got, expect)
     assert len(got) > 0
+
+
+def test_hashed_pairs_independent_of_table_size(orc):
+    # a 5-bit table collides many cells into one slot; the set must not change
+    # with the table size and must equal the dense grid's when every triangle
+    # lies inside the dense grid
+    rng = np.random.default_rng(11)
+    nv, nt = 160, 140
+    pos = rng.uniform(-0.8, 0.8, (nv, 3)).astype(np.float32)
+    tris = np.array([rng.choice(nv, 3, replace=False) for _ in range(nt)], np.int32)
+    cell, size = (0.3, 0.3, 0.3), (8, 8, 8)  # dense grid spans [-1.2, 1.2]^3
+    sets = [orc.candidate_pairs(orc.grid(1, size=size, cell=cell, table_bits=b), pos, tris) for b in (3, 5, 8, 16)]
+    dense = orc.candidate_pairs(orc.grid(0, size=size, cell=cell), pos, tris)
+    assert len(dense) > 0
+    for s_ in sets:
+        assert np.array_equal(s_, dense)
 
 
 def test_cubic_roots_against_numpy(orc):
