@@ -10,20 +10,27 @@ tiles.  One "step" is one substep of the whole batch.  Inputs are generated
 on the host from a seeded RNG and are resident in HBM before timing starts.
 
 Protocol (every timed run starts from the same device snapshot):
-  1. the warm-start network is trained on solver-generated samples of
-     config-4 scenes (32x32 cloth, wind U(0,5) m/s, random pins and sphere);
-  2. the autotuner picks block size, sweeps per launch and tile size for the
-     batch (results do not depend on the choice);
+  1. learned warm start on the 32x32 scene (wind U(0,5) m/s, random pins and
+     sphere): trained on ~200k solver-generated samples, then evaluated on
+     held-out scenes as the sweeps needed to reach each cloth's plain
+     residual level with and without it (config.warm_start_32x32);
+  2. the batch: the autotuner picks block size, sweeps per launch and tile
+     size (results do not depend on the choice); the warm-start network for
+     the batch is trained on samples the solver generates during the
+     batch's own warm-up substeps, before the snapshot;
   3. plain solver: 20 sweeps per substep (the reference's fixed budget,
      SolverPBD::ProjectConstraint(st, iterations)); its mean end-of-substep
      residual (max |d-L|/L per cloth, averaged over cloths and substeps) is
      the fixed residual R.  Batched and unbatched plain runs give the same
      positions bit for bit, hence the same R;
-  4. the warm-started solver runs the smallest sweep budget whose mean
-     residual over the same substeps is <= R (searched by bisection);
+  4. warm-started solver: a cloth whose prediction is kept runs a short
+     budget k, a cloth whose prediction is discarded runs the plain 20
+     sweeps; k is the smallest budget whose mean residual over the same
+     substeps is <= R (bisection);
   5. timed: the plain solver unbatched (each cloth stepped alone, as the
      reference's per-object SolverPBD), the plain solver batched, and the
-     warm-started batched solver (substeps replayed from a captured graph).
+     warm-started batched solver, both batched runs replayed from captured
+     graphs.
 `value` is the warm-started batched solver's substeps/s;
 config.speedup_vs_plain divides it by the unbatched plain solver's rate and
 config.speedup_vs_plain_batched by the batched plain solver's.
@@ -31,9 +38,10 @@ config.speedup_vs_plain_batched by the batched plain solver's.
 `roofline` is for the dominant kernel (the LDS-tile Gauss-Seidel kernel),
 from a second, eager pass over the same warm-started substeps with HIP events
 around every launch on the library's own stream: algorithmic bytes (68 B per
-distance prim per sweep) over the summed launch time.  `cpu_baseline` times
-the CPU oracle (one thread, plain solver, 20 sweeps) on a bounded sample of
-the same batch, scaled to the whole batch by vertex-sweeps.
+distance prim per sweep actually run) over the summed launch time.
+`cpu_baseline` times the CPU oracle (one thread, plain solver, 20 sweeps) on
+a bounded sample of the same batch, scaled to the whole batch by
+vertex-sweeps.
 """
 from __future__ import annotations
 
@@ -60,13 +68,16 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed substeps per solver")
-    ap.add_argument("--warmup", type=int, default=20, help="plain substeps before the snapshot")
+    ap.add_argument("--warmup", type=int, default=12, help="plain substeps before the training samples")
     ap.add_argument("--match-substeps", type=int, default=10, help="substeps of the residual comparison")
-    ap.add_argument("--tile-sweeps", type=int, default=2)
+    ap.add_argument("--tile-sweeps", type=int, default=4, help="sweeps per launch when not autotuned")
     ap.add_argument("--count", type=int, default=512, help="cloths in the batch")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--train-scenes", type=int, default=24)
-    ap.add_argument("--train-epochs", type=int, default=30)
+    ap.add_argument("--train-scenes", type=int, default=24, help="32x32 scenes for the reference-scene network")
+    ap.add_argument("--eval-scenes", type=int, default=16, help="held-out 32x32 scenes")
+    ap.add_argument("--collect-substeps", type=int, default=2, help="batch substeps recorded as training samples")
+    ap.add_argument("--target-sweeps", type=int, default=60, help="sweeps behind each training target")
+    ap.add_argument("--batch-epochs", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--unbatched-steps", type=int, default=3, help="timed substeps of the unbatched plain solver")
@@ -74,11 +85,13 @@ def parse():
     return ap.parse_args()
 
 
-def train_warm_start(capi, scenes, n_scenes, epochs):
-    """~200k samples: n_scenes config-4 cloths x 1024 vertices x 8 collected substeps."""
+def scene_network(capi, scenes, n_train, n_eval):
+    """The 32x32 reference scene: ~200k samples from n_train seeds, then the
+    mean sweeps to each held-out cloth's plain residual level (its mean
+    20-sweep residual) with and without the warm start."""
     t = time.perf_counter()
     ctx = capi.Context(0)
-    for s in range(n_scenes):
+    for s in range(n_train):
         scenes.add_to_context(ctx, scenes.config4(s))
     ctx.init()
     ctx.set_solver(dt=scenes.DT, iterations=PLAIN_SWEEPS)
@@ -87,13 +100,44 @@ def train_warm_start(capi, scenes, n_scenes, epochs):
         ctx.step(30)
         ctx.ws_collect(PLAIN_SWEEPS, 1)
     n = ctx.ws_num_samples()
-    loss = ctx.ws_train(epochs=epochs, batch=2048, lr=1e-3, seed=1)
+    loss = ctx.ws_train(epochs=30, batch=2048, lr=1e-3, seed=1)
     prm, norm = ctx.ws_params(), ctx.ws_norm()
     ctx.close()
-    info = {"samples": n, "epochs": epochs, "standardised_loss": loss, "zero_predictor_loss": 1.5,
-            "seconds": time.perf_counter() - t}
-    log(f"[bench] warm start trained on {n} samples in {info['seconds']:.1f} s, loss {loss:.3f} (zero predictor 1.5)")
-    return prm, norm, info
+    ev = capi.Context(0)
+    ids = [scenes.add_to_context(ev, scenes.config4(1000 + s)) for s in range(n_eval)]
+    ev.init()
+    ev.set_solver(dt=scenes.DT, iterations=PLAIN_SWEEPS)
+    ev.ws_set_params(prm)
+    ev.ws_set_norm(norm)
+    ev.step(120)
+    st = [(ev.get(i, 0), ev.get(i, 1)) for i in ids]
+    tau = np.zeros(len(ids))
+    for _ in range(10):
+        ev.step(1)
+        tau += [ev.residual(i)[0] for i in ids]
+    tau /= 10
+    mean = {}
+    for ws in (False, True):
+        for i, (x, v), tv in zip(ids, st, tau):
+            ev.set(i, 0, x)
+            ev.set(i, 1, v)
+            ev.set_tolerance(i, float(tv))
+        ev.ws_enable(ws)
+        ev.set_solver(dt=scenes.DT, iterations=80, tol=0.0, check_every=1)
+        tot = 0
+        for _ in range(40):
+            ev.step(1)
+            tot += sum(ev.last_sweeps(i) for i in ids)
+        mean[ws] = tot / (40 * len(ids))
+    ev.close()
+    info = {"samples": n, "epochs": 30, "standardised_loss": loss, "zero_predictor_loss": 1.5,
+            "held_out_scenes": n_eval, "mean_sweeps_plain": mean[False], "mean_sweeps_warm": mean[True],
+            "seconds": time.perf_counter() - t,
+            "definition": "mean sweeps per cloth-substep to reach the cloth's own mean plain 20-sweep residual, "
+                          "residual checked after every sweep, 40 substeps from the same state"}
+    log(f"[bench] 32x32 scene: network from {n} samples (loss {loss:.3f}); sweeps to the plain residual level: "
+        f"plain {mean[False]:.2f}, warm {mean[True]:.2f}")
+    return info
 
 
 def cpu_baseline(specs, sweeps, budget_s):
@@ -132,7 +176,7 @@ def main():
         sys.exit(2)
     from positionbaseddynamicsold_amd import capi, scenes
 
-    prm, norm, train_info = train_warm_start(capi, scenes, a.train_scenes, a.train_epochs)
+    scene_info = scene_network(capi, scenes, a.train_scenes, a.eval_scenes)
 
     specs = scenes.config5(seed=a.seed, count=a.count)
     nv_total = sum(s.nv for s in specs)
@@ -143,8 +187,6 @@ def main():
     t = time.perf_counter()
     ctx.init()
     log(f"[bench] init (constraints, GPU colouring, tile plan) {time.perf_counter() - t:.1f} s")
-    ctx.ws_set_params(prm)
-    ctx.ws_set_norm(norm)
     tuned = {"tile_sweeps": a.tile_sweeps, "block_threads": 1024, "tile_max_verts": 0}
     ctx.set_solver(dt=scenes.DT, iterations=PLAIN_SWEEPS, check_every=a.tile_sweeps, tile_sweeps=a.tile_sweeps,
                    graph=1)
@@ -157,54 +199,75 @@ def main():
                  "seconds": time.perf_counter() - t}
         log(f"[bench] autotune: {tuned}")
 
-    def solver(sweeps, ws, graph=1):
+    def solver(sweeps, ws, graph=1, warm=0):
         ctx.ws_enable(ws)
         ctx.set_solver(dt=scenes.DT, iterations=sweeps, check_every=tuned["tile_sweeps"],
                        tile_sweeps=tuned["tile_sweeps"], tile_max_verts=tuned["tile_max_verts"],
-                       block_threads=tuned["block_threads"], graph=graph)
+                       block_threads=tuned["block_threads"], graph=graph, warm_iterations=warm)
 
+    # warm-up, with the batch's own training samples recorded along the way
     solver(PLAIN_SWEEPS, False)
     ctx.step(a.warmup)
+    t = time.perf_counter()
+    ctx.ws_init(0)
+    for _ in range(a.collect_substeps):
+        ctx.ws_collect(a.target_sweeps, 1)
+        ctx.step(1)
+    n_samples = ctx.ws_num_samples()
+    loss = ctx.ws_train(epochs=a.batch_epochs, batch=4096, lr=1e-3, seed=1)
+    batch_train = {"samples": n_samples, "epochs": a.batch_epochs, "target_sweeps": a.target_sweeps,
+                   "standardised_loss": loss, "zero_predictor_loss": 1.5, "seconds": time.perf_counter() - t,
+                   "source": f"the batch's own substeps after {a.warmup} warm-up substeps (before the snapshot): "
+                             f"features at the predicted state, target = the correction of {a.target_sweeps} "
+                             f"Gauss-Seidel sweeps"}
+    log(f"[bench] batch network: {n_samples} samples, {a.batch_epochs} epochs, loss {loss:.3f} "
+        f"in {batch_train['seconds']:.1f} s")
+    solver(PLAIN_SWEEPS, False)
+    ctx.step(2)
     ctx.sync()
     ctx.snapshot()
-    log(f"[bench] warm-up {a.warmup} plain substeps done, snapshot taken")
+    log("[bench] snapshot taken")
 
-    def mean_residual(sweeps, ws):
+    def residuals(ws, warm=0):
         ctx.snapshot(restore=True)
-        solver(sweeps, ws)
-        acc, used = 0.0, 0.0
+        solver(PLAIN_SWEEPS, ws, warm=warm)
+        acc, used = np.zeros(len(ids)), 0.0
         for _ in range(a.match_substeps):
             ctx.step(1)
-            acc += float(ctx.residual_all()[0].mean())
+            acc += ctx.residual_all()[0]
             if ws:
                 used += sum(ctx.ws_last_used(i) for i in ids) / len(ids)
         return acc / a.match_substeps, used / a.match_substeps
 
-    r_plain, _ = mean_residual(PLAIN_SWEEPS, False)
-    # smallest warm budget with residual <= the plain solver's (bisection)
-    lo, hi = 1, PLAIN_SWEEPS
-    r_hi, u_hi = mean_residual(hi, True)
-    tried = {hi: r_hi}
-    if r_hi <= r_plain:
-        while hi - lo > 0:
-            mid = (lo + hi) // 2
-            r_mid, _ = mean_residual(mid, True)
-            tried[mid] = r_mid
-            if r_mid <= r_plain:
-                hi = mid
-            else:
-                lo = mid + 1
-        warm_sweeps = hi
-    else:
-        warm_sweeps = PLAIN_SWEEPS
-    r_warm, used_frac = mean_residual(warm_sweeps, True)
-    log(f"[bench] fixed residual R = {r_plain:.5e} (plain, {PLAIN_SWEEPS} sweeps); warm start reaches "
-        f"{r_warm:.5e} with {warm_sweeps} sweeps (tried {dict(sorted(tried.items()))}); prediction kept "
-        f"for {100 * used_frac:.0f}% of cloth-substeps")
+    r_plain_c, _ = residuals(False)
+    r_plain = float(r_plain_c.mean())
+    # smallest warm budget k (fallback cloths keep 20) with mean residual <= R
+    tried = {}
 
-    def timed(sweeps, ws, steps):
+    def mean_warm(k):
+        r, u = residuals(True, warm=k)
+        tried[k] = float(r.mean())
+        return r, u
+
+    lo, hi = 1, PLAIN_SWEEPS
+    while hi > lo:
+        mid = (lo + hi) // 2
+        if mean_warm(mid)[0].mean() <= r_plain:
+            hi = mid
+        else:
+            lo = mid + 1
+    warm_k = hi
+    r_warm_c, used_frac = mean_warm(warm_k)
+    r_warm = float(r_warm_c.mean())
+    ratio = r_warm_c / np.maximum(r_plain_c, 1e-30)
+    log(f"[bench] fixed residual R = {r_plain:.5e} (plain, {PLAIN_SWEEPS} sweeps); warm start reaches "
+        f"{r_warm:.5e} with budget {warm_k} (kept) / {PLAIN_SWEEPS} (discarded); tried "
+        f"{dict(sorted(tried.items()))}; kept for {100 * used_frac:.0f}% of cloth-substeps; per-cloth residual "
+        f"ratio warm/plain median {np.median(ratio):.3f}, p90 {np.quantile(ratio, 0.9):.3f}")
+
+    def timed(ws, steps, warm=0):
         ctx.snapshot(restore=True)
-        solver(sweeps, ws)
+        solver(PLAIN_SWEEPS, ws, warm=warm)
         ctx.step(1)
         ctx.sync()
         ctx.profile_read(reset=True)
@@ -215,9 +278,9 @@ def main():
         replays = ctx.profile_read(reset=True).graph_replays
         return steps / el, el * 1e3 / steps, replays
 
-    def timed_unbatched(sweeps, steps):
+    def timed_unbatched(steps):
         ctx.snapshot(restore=True)
-        solver(sweeps, False, graph=0)
+        solver(PLAIN_SWEEPS, False, graph=0)
         for i in ids:
             ctx.step_cloth(i, 1)
         ctx.sync()
@@ -229,9 +292,9 @@ def main():
         el = time.perf_counter() - t0
         return steps / el, el * 1e3 / steps
 
-    def profiled(sweeps, ws, steps):
+    def profiled(ws, steps, warm=0):
         ctx.snapshot(restore=True)
-        solver(sweeps, ws, graph=0)
+        solver(PLAIN_SWEEPS, ws, graph=0, warm=warm)
         ctx.step(1)
         ctx.sync()
         ctx.profile(True)
@@ -242,17 +305,18 @@ def main():
         ctx.profile(False)
         return prof
 
-    unb_rate, unb_ms = timed_unbatched(PLAIN_SWEEPS, a.unbatched_steps)
+    unb_rate, unb_ms = timed_unbatched(a.unbatched_steps)
     log(f"[bench] plain unbatched (one cloth at a time) {unb_rate:.2f} substeps/s ({unb_ms:.2f} ms)")
-    plain_rate, plain_ms, plain_replays = timed(PLAIN_SWEEPS, False, a.steps)
-    warm_rate, warm_ms, warm_replays = timed(warm_sweeps, True, a.steps)
+    plain_rate, plain_ms, plain_replays = timed(False, a.steps)
+    warm_rate, warm_ms, warm_replays = timed(True, a.steps, warm=warm_k)
     log(f"[bench] plain batched {plain_rate:.2f} substeps/s ({plain_ms:.2f} ms); warm start + batched "
         f"{warm_rate:.2f} substeps/s ({warm_ms:.2f} ms): x{warm_rate / unb_rate:.2f} vs plain unbatched, "
         f"x{warm_rate / plain_rate:.2f} vs plain batched (graph replays {plain_replays}, {warm_replays})")
-    prof = profiled(warm_sweeps, True, a.steps)
-    plain_prof = profiled(PLAIN_SWEEPS, False, a.steps)
+    prof = profiled(True, a.steps, warm=warm_k)
+    plain_prof = profiled(False, a.steps)
     achieved = prof.tile_alg_bytes / (prof.tile_ms * 1e-3) / 1e9
     plain_achieved = plain_prof.tile_alg_bytes / (plain_prof.tile_ms * 1e-3) / 1e9
+    warm_vertex_sweeps = prof.vertex_sweeps / max(1, a.steps)
     copy_ms = ctx.bench_copy(1 << 30, 10)
     copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     infos = [ctx.info(i) for i in ids]
@@ -291,7 +355,10 @@ def main():
             "fixed_residual": {"definition": "mean over cloths and substeps of max |d-L|/L at the end of the "
                                              "substep, plain solver with 20 sweeps, same start state",
                                "R": r_plain, "warm_start_residual": r_warm,
-                               "substeps_compared": a.match_substeps},
+                               "substeps_compared": a.match_substeps,
+                               "per_cloth_ratio_median": float(np.median(ratio)),
+                               "per_cloth_ratio_p90": float(np.quantile(ratio, 0.9)),
+                               "budgets_tried": {str(k): v for k, v in sorted(tried.items())}},
             "plain_substeps_per_s": unb_rate,
             "plain_ms_per_step": unb_ms,
             "plain_definition": "plain solver, unbatched: each cloth stepped alone (its own launches, LDS "
@@ -300,14 +367,17 @@ def main():
             "plain_batched_substeps_per_s": plain_rate,
             "plain_batched_ms_per_step": plain_ms,
             "speedup_vs_plain_batched": warm_rate / plain_rate,
+            "plain_sweeps": PLAIN_SWEEPS,
+            "warm_start_budget_kept": warm_k,
+            "warm_start_budget_discarded": PLAIN_SWEEPS,
+            "warm_start_kept_fraction": used_frac,
+            "warm_start_mean_sweeps": warm_vertex_sweeps / nv_total,
+            "warm_start_training": batch_train,
+            "warm_start_32x32": scene_info,
+            "vertex_iterations_per_s": warm_vertex_sweeps * warm_rate,
+            "plain_batched_vertex_iterations_per_s": nv_total * PLAIN_SWEEPS * plain_rate,
             "graph_replays_timed": warm_replays,
             "autotuned": tuned,
-            "plain_sweeps": PLAIN_SWEEPS,
-            "warm_start_sweeps": warm_sweeps,
-            "warm_start_kept_fraction": used_frac,
-            "warm_start_training": train_info,
-            "vertex_iterations_per_s": nv_total * warm_sweeps * warm_rate,
-            "plain_batched_vertex_iterations_per_s": nv_total * PLAIN_SWEEPS * plain_rate,
             "single_tile_cloths": sum(1 for i in infos if i.whole),
             "tiles": sum(i.ntiles for i in infos),
             "copy_kernel_gbs": copy_gbs,

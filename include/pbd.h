@@ -88,6 +88,9 @@ typedef struct {
     int block_threads;      /* threads per workgroup of the tile kernel (0 = 1024) */
     int graph;              /* 1: pbd_step replays each substep from a captured hipGraph whenever the
                                substep makes no host decision (fixed sweeps, no collisions, profiling off) */
+    int warm_iterations;    /* with the warm start on: sweep budget of a cloth whose prediction was kept
+                               (0 = iterations); a cloth whose prediction is discarded runs `iterations`,
+                               exactly the plain solver's substep */
 } pbd_solver_params;
 
 typedef struct {
@@ -249,6 +252,10 @@ PBD_API int pbd_ws_get_normalization(pbd_ctx* ctx, float* norm47);
 PBD_API int pbd_ws_set_normalization(pbd_ctx* ctx, const float* norm47);
 PBD_API int pbd_ws_enable(pbd_ctx* ctx, int on);
 PBD_API int pbd_ws_last_used(pbd_ctx* ctx, int cloth, int* used);
+/* The correction the network predicted for this cloth in the last warm-started
+ * substep (3 floats per vertex, zero for pinned vertices), before the
+ * fallback decision. */
+PBD_API int pbd_ws_last_delta(pbd_ctx* ctx, int cloth, float* xyz);
 
 /* ---- measurement (no counterpart in the reference, whose only timing is the
  * host Timer around calls, timer.cpp:10-20) --------------------------------- */

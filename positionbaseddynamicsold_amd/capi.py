@@ -43,7 +43,7 @@ class ClothParams(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("iterations", C.c_int), ("tol", C.c_double), ("check_every", C.c_int),
                 ("path", C.c_int), ("tile_sweeps", C.c_int), ("tile_max_verts", C.c_int),
-                ("block_threads", C.c_int), ("graph", C.c_int)]
+                ("block_threads", C.c_int), ("graph", C.c_int), ("warm_iterations", C.c_int)]
 
 
 class HashDesc(C.Structure):
@@ -137,6 +137,7 @@ SIGNATURES = {
     "pbd_state_load": [_P, _fp, _fp],
     "pbd_snapshot": [_P, _I],
     "pbd_ws_last_used": [_P, _I, _ip],
+    "pbd_ws_last_delta": [_P, _I, _fp],
     "pbd_profile_enable": [_P, _I],
     "pbd_profile_read": [_P, C.POINTER(Profile), _I],
     "pbd_bench_copy": [_P, C.c_longlong, _I, _dp],
@@ -290,8 +291,9 @@ class Context:
 
     # ---- solver --------------------------------------------------------
     def set_solver(self, dt=1.0 / 600.0, iterations=20, tol=-1.0, check_every=1, path=PATH_AUTO,
-                   tile_sweeps=0, tile_max_verts=0, block_threads=0, graph=0):
-        p = SolverParams(dt, iterations, tol, check_every, path, tile_sweeps, tile_max_verts, block_threads, graph)
+                   tile_sweeps=0, tile_max_verts=0, block_threads=0, graph=0, warm_iterations=0):
+        p = SolverParams(dt, iterations, tol, check_every, path, tile_sweeps, tile_max_verts, block_threads, graph,
+                         warm_iterations)
         check(self.lib.pbd_set_solver_params(self.h, C.byref(p)))
 
     def set_solver_params(self, p: SolverParams):
@@ -493,6 +495,11 @@ class Context:
 
     def ws_enable(self, on: bool = True):
         check(self.lib.pbd_ws_enable(self.h, 1 if on else 0))
+
+    def ws_last_delta(self, cloth: int) -> np.ndarray:
+        out = np.zeros((self.info(cloth).nv, 3), np.float32)
+        check(self.lib.pbd_ws_last_delta(self.h, cloth, fptr(out)))
+        return out
 
     def ws_last_used(self, cloth: int) -> int:
         u = C.c_int()

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "pbd_internal.hpp"
 
@@ -73,7 +74,11 @@ struct pbd_ctx {
         DBuf<unsigned long long> r0, r1;
         std::vector<int> used;
         bool used_pending = false;  // r0/r1 hold a decision not yet read back
+        DBuf<int> cap;              // per-cloth sweep budget of the current substep
+        DBuf<uint4> feat;           // BF16 feature rows of the network kernel
+        DBuf<float> hs;             // per-vertex output scale (0 = pinned)
     } ws;
+    bool sweeps_pending = false;    // the device sweep counts are newer than Cloth::last_sweeps
     // substep captured as a graph, one per starting p buffer; the p buffer
     // the replay ends in is graph_end
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -538,6 +543,9 @@ void ws_build_nbr(pbd_ctx* c) {
     c->ws.psave.alloc(c->nv_total);
     c->ws.r0.alloc(c->cloths.size());
     c->ws.r1.alloc(c->cloths.size());
+    c->ws.cap.alloc(c->cloths.size());
+    c->ws.feat.alloc(ws_feature_bytes(c->nv_total) / sizeof(uint4));
+    c->ws.hs.alloc(c->nv_total);
     c->ws.nbr_built = true;
     drop_graphs(c);
     if (c->ws.norm_h.empty()) {
@@ -559,12 +567,25 @@ void ws_apply_prediction(pbd_ctx* c) {
     ws_build_nbr(c);
     float4* p = c->pbuf[c->pcur].ptr;
     ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, c->ws.nbr.ptr, c->ws.nbr_rest.ptr, c->nv_total,
-             c->ws.delta.ptr, nullptr, c->ws.norm.ptr);
+             c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr);
     launch_fill_u64(s, c->ws.r0.ptr, ncl, 0ull);
     launch_fill_u64(s, c->ws.r1.ptr, ncl, 0ull);
     ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr);
     ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr, c->nv_total);
+    int full = c->sp.iterations;
+    int warm = c->sp.warm_iterations > 0 ? std::min(c->sp.warm_iterations, full) : full;
+    ws_decide(s, c->ws.r0.ptr, c->ws.r1.ptr, ncl, warm, full, c->ws.cap.ptr);
     c->ws.used_pending = true;
+}
+
+// Per-cloth sweep counts of the last projection from the device.
+void read_sweeps(pbd_ctx* c) {
+    if (!c->sweeps_pending) return;
+    std::vector<int> sv;
+    c->sweeps.download(sv, c->stream);
+    PBD_HIP(hipStreamSynchronize(c->stream));
+    for (size_t ci = 0; ci < c->cloths.size(); ci++) c->cloths[ci].last_sweeps = sv[ci];
+    c->sweeps_pending = false;
 }
 
 // Per-cloth record of whether the last prediction was kept, read back on
@@ -608,12 +629,14 @@ void advect_all(pbd_ctx* c, float dt) {
 }
 
 // Sweeps on the current p.  Whole-cloth tiles run unfused (p_in -> p_out).
-void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
+// cap: optional per-cloth sweep budgets (device array) below `iterations`.
+void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* cap = nullptr) {
     hipStream_t s = c->stream;
     int ncl = (int)c->cloths.size();
     double tol = c->sp.tol;
     int ce = std::max(1, c->sp.check_every);
     const bool any_tol = has_tolerance(c);
+    c->sweeps_pending = false;
     launch_fill_int(s, c->sweeps.ptr, ncl, 0);
     if (c->sp.path == PBD_PATH_GLOBAL) {
         PBD_CHECK(!fused_whole, PBD_ERR_STATE, "internal: global path is never fused");
@@ -624,8 +647,9 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
         while (sw < iterations) {
             for (int col = 0; col < c->ncolors_max; col++)
                 launch_project_colour(s, c->gdist.ptr, c->gd_off[col], c->gd_off[col + 1], c->gbidx.ptr, c->gbrc.ptr,
-                                      c->gb_off[col], c->gb_off[col + 1], c->ktab_d.ptr, c->ktab_b.ptr, c->kmax, sw, p);
-            if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, p, 0, c->nv_total);
+                                      c->gb_off[col], c->gb_off[col + 1], c->ktab_d.ptr, c->ktab_b.ptr, c->kmax, sw,
+                                      cap, p);
+            if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, p, 0, c->nv_total, cap, sw);
             sw++;
             if (tol >= 0.0 && sw % ce == 0) {
                 launch_fill_u64(s, c->rmax.ptr, ncl, 0ull);
@@ -644,6 +668,10 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
             }
         }
         for (auto& k : c->cloths) k.last_sweeps = sw;
+        if (cap) {
+            launch_min_sweeps(s, c->sweeps.ptr, cap, sw, ncl);
+            c->sweeps_pending = true;
+        }
         return;
     }
     replan_if_needed(c);
@@ -658,8 +686,8 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
         if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
         launch_tiles(s, nblocks, order, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr, c->tbend.ptr,
                      c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr,
-                     c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, sa, threads,
-                     lds);
+                     c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, cap, sa,
+                     threads, lds);
         if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
     };
     SolveArgs a{};
@@ -713,11 +741,10 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt) {
             if (k.whole) launch_copy(s, c->pbuf[c->pcur ^ 1].ptr + k.vbase, c->pbuf[cur].ptr + k.vbase, k.nv);
     }
     c->pcur = cur;
-    if (any_tol) {
-        std::vector<int> sv;
-        c->sweeps.download(sv, s);
-        PBD_HIP(hipStreamSynchronize(s));
-        for (int ci = 0; ci < ncl; ci++) c->cloths[ci].last_sweeps = sv[ci];
+    if (any_tol || cap) {
+        // per-cloth counts live on the device; read them now only if needed
+        c->sweeps_pending = true;
+        if (any_tol || c->prof_on) read_sweeps(c);
     } else {
         // fixed sweeps: every cloth ran all of them
         for (auto& k : c->cloths) k.last_sweeps = iterations;
@@ -775,7 +802,7 @@ void step_ccd(pbd_ctx* c, float dt) {
             launch_tiles(s, (int)ord->n, ord->ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr,
                          c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr,
                          c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr,
-                         c->res_slots.ptr, ncl, c->sweeps.ptr, a, threads, lds);
+                         c->res_slots.ptr, ncl, c->sweeps.ptr, nullptr, a, threads, lds);
             if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
         }
         c->pcur = cur ^ 1;
@@ -802,7 +829,7 @@ void substep(pbd_ctx* c, float dt) {
     } else if (c->ws.enabled) {
         advect_all(c, dt);
         ws_apply_prediction(c);
-        project(c, c->sp.iterations, false, dt);
+        project(c, c->sp.iterations, false, dt, c->ws.cap.ptr);
         integrate_all(c, dt, false);
     } else if (c->sp.path == PBD_PATH_GLOBAL) {
         advect_all(c, dt);
@@ -854,8 +881,12 @@ void graph_substep(pbd_ctx* c, float dt) {
     }
     PBD_HIP(hipGraphLaunch(c->graph[start], s));
     c->pcur = c->graph_end[start];
-    for (auto& k : c->cloths) k.last_sweeps = c->sp.iterations;
-    if (c->ws.enabled) c->ws.used_pending = true;
+    if (c->ws.enabled) {
+        c->ws.used_pending = true;
+        c->sweeps_pending = true;
+    } else {
+        for (auto& k : c->cloths) k.last_sweeps = c->sp.iterations;
+    }
     c->prof.graph_replays++;
 }
 
@@ -883,7 +914,7 @@ void step_cloth(pbd_ctx* c, int ci, float dt) {
             for (int col = 0; col < k.ncolors; col++)
                 launch_project_colour(s, c->gdist.ptr, k.gd_cut[2 * col], k.gd_cut[2 * col + 1], c->gbidx.ptr,
                                       c->gbrc.ptr, k.gb_cut[2 * col], k.gb_cut[2 * col + 1], c->ktab_d.ptr,
-                                      c->ktab_b.ptr, c->kmax, sw, p);
+                                      c->ktab_b.ptr, c->kmax, sw, nullptr, p);
             if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, p, vb, ve);
             sw++;
             if (tol >= 0.0 && sw % ce == 0) {
@@ -919,8 +950,8 @@ void step_cloth(pbd_ctx* c, int ci, float dt) {
     auto tiles = [&](const float4* pin, float4* pout) {
         launch_tiles(s, nblocks, order, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr, c->tbend.ptr,
                      c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr,
-                     c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, a, threads,
-                     lds);
+                     c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, nullptr, a,
+                     threads, lds);
     };
     int cur = c->pcur;
     if (k.whole) {
@@ -1585,17 +1616,28 @@ int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
             std::swap(W.Y.ptr, ny.ptr);
             std::swap(W.Y.n, ny.n);
         }
-        if (sweeps > ctx->kmax) {
-            ctx->sp.iterations = sweeps;
+        // stiffness tables for `sweeps` sweeps while recording; the solver's
+        // own budget is restored afterwards
+        const int budget = ctx->sp.iterations;
+        auto use_budget = [&](int it) {
+            if (ctx->sp.iterations == it) return;
+            ctx->sp.iterations = it;
             fill_ktabs(ctx);
             fill_cloth_dev(ctx);
-        }
+        };
+        use_budget(sweeps);
+        struct Restore {
+            std::function<void()> f;
+            ~Restore() {
+                try { f(); } catch (...) {}
+            }
+        } restore{[&] { use_budget(budget); }};
         float dt = ctx->sp.dt;
         for (int k = 0; k < substeps; k++) {
             advect_all(ctx, dt);
             float4* p = ctx->pbuf[ctx->pcur].ptr;
             ws_infer(s, W.W.ptr, ctx->cl.ptr, ctx->vcloth.ptr, ctx->x.ptr, p, W.nbr.ptr, W.nbr_rest.ptr, ctx->nv_total,
-                     W.delta.ptr, W.X.ptr + 22 * W.ns, W.norm.ptr);
+                     W.delta.ptr, W.X.ptr + 22 * W.ns, W.norm.ptr, W.feat.ptr, W.hs.ptr);
             PBD_HIP(hipMemcpyAsync(W.psave.ptr, p, sizeof(float4) * ctx->nv_total, hipMemcpyDeviceToDevice, s));
             double tol = ctx->sp.tol;
             ctx->sp.tol = -1.0;
@@ -1761,6 +1803,24 @@ int pbd_ws_last_used(pbd_ctx* ctx, int cloth, int* used) {
         set_device(ctx);
         ws_read_used(ctx);
         *used = cloth < (int)ctx->ws.used.size() ? ctx->ws.used[cloth] : 0;
+    });
+}
+
+int pbd_ws_last_delta(pbd_ctx* ctx, int cloth, float* xyz) {
+    return guard([&] {
+        Cloth& k = cloth_of(ctx, cloth);
+        PBD_CHECK(xyz, PBD_ERR_ARG, "null output");
+        PBD_CHECK(ctx->ws.nbr_built && ctx->ws.delta.ptr, PBD_ERR_STATE, "no prediction made yet");
+        set_device(ctx);
+        std::vector<float4> d(k.nv);
+        PBD_HIP(hipMemcpyAsync(d.data(), ctx->ws.delta.ptr + k.vbase, sizeof(float4) * k.nv, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        PBD_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < k.nv; i++) {
+            xyz[3 * i] = d[i].x;
+            xyz[3 * i + 1] = d[i].y;
+            xyz[3 * i + 2] = d[i].z;
+        }
     });
 }
 
@@ -2060,6 +2120,8 @@ int pbd_cloth_last_sweeps(pbd_ctx* ctx, int id, int* sweeps) {
     return guard([&] {
         Cloth& c = cloth_of(ctx, id);
         PBD_CHECK(sweeps, PBD_ERR_ARG, "null output");
+        set_device(ctx);
+        read_sweeps(ctx);
         *sweeps = c.last_sweeps;
     });
 }

@@ -13,6 +13,7 @@
 // Parameter layout (FP32 master copy, row-major W[out][in]):
 //   W1[64][22] b1[64] W2[64][64] b2[64] W3[3][64] b3[3]
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 
 #include "pbd_internal.hpp"
@@ -29,26 +30,35 @@ __device__ __forceinline__ void wave_sync() {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// bf16 round-to-nearest-even (NaN kept quiet)
-__device__ __forceinline__ uint16_t f2bf(float x) {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+// bf16 round-to-nearest-even (v_cvt_pk_bf16_f32; NaN stays NaN)
+__device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+
+typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ bf16x4 pack_bf16(float a, float b, float c, float d) {
+    bf16v4 v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+    return __builtin_bit_cast(bf16x4, v);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// tanh(u) = 1 - 2 / (exp(2u) + 1) on the hardware exp; saturates cleanly.
-__device__ __forceinline__ float fast_tanh(float u) {
-    u = fminf(fmaxf(u, -15.0f), 15.0f);
-    return 1.0f - 2.0f / (__expf(2.0f * u) + 1.0f);
+// GELU in its tanh form, 0.5 x (1 + tanh(u)) with u = sqrt(2/pi) (x + 0.044715 x^3),
+// evaluated as the identical x * sigmoid(2u) on the hardware exp2 and
+// reciprocal; saturates to 0 / x without clamping.  Training and inference
+// share it.  dy/dx = s + x s (1 - s) 2u'(x).
+__device__ __forceinline__ void gelu(float x, float* y, float* dy) {
+    const float c2 = 1.5957691216f;  // 2 sqrt(2/pi)
+    const float k = 0.044715f;
+    const float x2 = x * x;
+    const float z = c2 * x * (1.0f + k * x2);                          // 2u
+    const float e = __builtin_amdgcn_exp2f(-z * 1.4426950409f);        // exp(-2u)
+    const float sg = __builtin_amdgcn_rcpf(1.0f + e);
+    *y = x * sg;
+    if (dy) *dy = sg + x * sg * (1.0f - sg) * c2 * (1.0f + 3.0f * k * x2);
 }
 
-__device__ __forceinline__ void gelu(float x, float* y, float* dy) {
-    const float c = 0.7978845608f, k = 0.044715f;
-    float u = c * (x + k * x * x * x);
-    float th = fast_tanh(u);
-    *y = 0.5f * x * (1.0f + th);
-    if (dy) *dy = 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * c * (1.0f + 3.0f * k * x * x);
+// Standardised feature as the network sees it (training and inference);
+// rsd = 1 / sd rounded to FP32.
+__device__ __forceinline__ float norm_feature(float f, float mu, float rsd) {
+    return fminf(fmaxf((f - mu) * rsd, -8.0f), 8.0f);
 }
 
 // One wave: C[16 x 16*nt] = A[16 x K] * B, with A row-major (lda) and B given
@@ -73,6 +83,12 @@ struct alignas(16) MlpW {
     uint16_t w2[64 * 64];    // W2[out][in]
     uint16_t w3[16 * 64];    // W3[out][in], out padded 3 -> 16
     uint16_t w2t[64 * 64];   // W2^T [in][out]
+    // inference copies: row o of layer n holds W_n[o][16s + 4q + j] at
+    // 8q + 4s + j (layer 1) / 16q + 4s + j (layers 2, 3), so that lane quad q
+    // reads all its K-steps of a row with one 16- or 32-byte load
+    uint16_t i1[64 * 32];
+    uint16_t i2[64 * 64];
+    uint16_t i3[16 * 64];
     float b1[64], b2[64], b3[16];
 };
 
@@ -95,6 +111,15 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ prm,
     if (i < 16 * 64) {
         int o = i / 64, k = i % 64;
         W->w3[i] = o < 3 ? f2bf(prm[P_W3 + o * 64 + k]) : 0;
+    }
+    if (i < 64 * 32) {
+        int o = i / 32, c = i % 32, q = c / 8, st = (c % 8) / 4, j = c % 4, k = 16 * st + 4 * q + j;
+        W->i1[i] = k < 22 ? f2bf(prm[P_W1 + o * 22 + k]) : 0;
+    }
+    if (i < 64 * 64) {
+        int o = i / 64, c = i % 64, q = c / 16, st = (c % 16) / 4, j = c % 4, k = 16 * st + 4 * q + j;
+        W->i2[i] = f2bf(prm[P_W2 + o * 64 + k]);
+        if (o < 16) W->i3[i] = o < 3 ? f2bf(prm[P_W3 + o * 64 + k]) : 0;
     }
     if (i < 64) {
         W->b1[i] = prm[P_B1 + i];
@@ -150,41 +175,6 @@ __device__ void forward16(const MlpW& W, FwdLds& F) {
         int row = 4 * q + i, col = r;
         F.y[row * 16 + col] = acc[0][i] + W.b3[col];
     }
-    wave_sync();
-}
-
-// Inference-only scratch (no FP32 pre-activations kept).
-struct alignas(16) InfLds {
-    uint16_t xs[16 * 32];
-    uint16_t a1[16 * 64];
-    uint16_t a2[16 * 64];
-    float y[16 * 16];
-    float fe[16 * 24];
-};
-
-__device__ void forward16_infer(const MlpW& W, InfLds& F) {
-    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
-    f32x4 acc[4];
-    mm16(F.xs, 32, W.w1, 32, 32, 4, acc);
-    for (int t = 0; t < 4; t++)
-        for (int i = 0; i < 4; i++) {
-            int row = 4 * q + i, col = 16 * t + r;
-            float a;
-            gelu(acc[t][i] + W.b1[col], &a, nullptr);
-            F.a1[row * 64 + col] = f2bf(a);
-        }
-    wave_sync();
-    mm16(F.a1, 64, W.w2, 64, 64, 4, acc);
-    for (int t = 0; t < 4; t++)
-        for (int i = 0; i < 4; i++) {
-            int row = 4 * q + i, col = 16 * t + r;
-            float a;
-            gelu(acc[t][i] + W.b2[col], &a, nullptr);
-            F.a2[row * 64 + col] = f2bf(a);
-        }
-    wave_sync();
-    mm16(F.a2, 64, W.w3, 64, 64, 1, acc);
-    for (int i = 0; i < 4; i++) F.y[(4 * q + i) * 16 + r] = acc[0][i] + W.b3[r];
     wave_sync();
 }
 
@@ -387,85 +377,173 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float
 // ---------------------------------------------------------------------------
 // warm start inside the substep
 // ---------------------------------------------------------------------------
-// Features of 16 vertices into F.xs (4 lanes per vertex), then forward, then
-// delta = h * y (zero for pinned vertices).
-// norm: mu[22], sd[22], sdy[3] (standardisation of features, clamped to
-// +-8, and of targets).  Persistent: each block loads the weights once and
-// loops over groups of 16 vertices, one group per wave at a time.
-constexpr int INF_WAVES = 16;
+// ---------------------------------------------------------------------------
+// Inference in two kernels.
+//
+// k_ws_features: one thread per vertex gathers its 22 features, standardises
+// them and writes them as one 64-byte BF16 row laid out in MFMA operand order:
+// 16-byte chunk q holds features 4q..4q+3 then 16+4q..16+4q+3 (zero past 21),
+// so lane (r, q) of the network kernel loads both K-steps of its A operand
+// for vertex r with a single 16-byte load.  Also writes the output scale
+// h (0 for pinned vertices) and, when asked, the raw FP32 features.
+//
+// k_ws_mlp: persistent; the weights stay in LDS, each wave takes 16 vertices
+// at a time with the next group's features already in flight, and writes
+// delta = h * (y * sdy) straight from the accumulators.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
+                                                     const float4* __restrict__ x, const float4* __restrict__ p,
+                                                     const int* __restrict__ nbr, const float* __restrict__ nbr_rest,
+                                                     int nv, const float* __restrict__ norm, uint4* __restrict__ feat,
+                                                     float* __restrict__ hs, float* __restrict__ feat_out) {
+    __shared__ float s_mu[22], s_rsd[22];
+    if (threadIdx.x < 22) {
+        s_mu[threadIdx.x] = norm[threadIdx.x];
+        s_rsd[threadIdx.x] = 1.0f / norm[22 + threadIdx.x];
+    }
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const float h = cl[vcloth[v]].hscale;
+    const float4 P = p[v], X = x[v];
+    float f[22];
+    f[0] = (P.x - X.x) / h;
+    f[1] = (P.y - X.y) / h;
+    f[2] = (P.z - X.z) / h;
+    for (int k = 0; k < 6; k++) {
+        const int j = nbr[(size_t)v * 6 + k];
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        if (j >= 0) {
+            const float4 Q = p[j];
+            const float ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
+            const float d = sqrtf(ex * ex + ey * ey + ez * ez);
+            const float L = nbr_rest[(size_t)v * 6 + k];
+            if (d > 0.0f) {
+                const float sc = (d - L) / (L * d);
+                sx = sc * ex;
+                sy = sc * ey;
+                sz = sc * ez;
+            }
+        }
+        f[3 + 3 * k] = sx;
+        f[4 + 3 * k] = sy;
+        f[5 + 3 * k] = sz;
+    }
+    const bool pinned = P.w == 0.0f;
+    f[21] = pinned ? 1.0f : 0.0f;
+    if (feat_out)
+        for (int k = 0; k < 22; k++) feat_out[(size_t)v * 22 + k] = f[k];
+    uint32_t b[16];  // 32 BF16, two per word, in MFMA operand order
+    for (int q = 0; q < 4; q++)
+        for (int half = 0; half < 2; half++)
+            for (int j = 0; j < 4; j += 2) {
+                const int k0 = 16 * half + 4 * q + j;
+                const uint32_t lo = k0 < 22 ? f2bf(norm_feature(f[k0], s_mu[k0], s_rsd[k0])) : 0u;
+                const uint32_t hi = k0 + 1 < 22 ? f2bf(norm_feature(f[k0 + 1], s_mu[k0 + 1], s_rsd[k0 + 1])) : 0u;
+                b[4 * q + 2 * half + j / 2] = lo | (hi << 16);
+            }
+    uint4* row = feat + (size_t)v * 4;
+    for (int q = 0; q < 4; q++) row[q] = make_uint4(b[4 * q], b[4 * q + 1], b[4 * q + 2], b[4 * q + 3]);
+    hs[v] = pinned ? 0.0f : h;
+}
 
-__global__ __launch_bounds__(1024) void k_ws_infer(const MlpW* __restrict__ gW, const ClothDev* __restrict__ cl,
-                                                   const int* __restrict__ vcloth, const float4* __restrict__ x,
-                                                   const float4* __restrict__ p, const int* __restrict__ nbr,
-                                                   const float* __restrict__ nbr_rest, int nv,
-                                                   float4* __restrict__ delta, float* __restrict__ feat_out,
-                                                   const float* __restrict__ norm) {
-    __shared__ MlpW W;
-    __shared__ InfLds F[INF_WAVES];
-    load_weights(&W, gW);
-    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    InfLds& f = F[wv];
+// Weights the network kernel keeps in LDS (the inference copies).
+struct alignas(16) InfW {
+    uint16_t i1[64 * 32];
+    uint16_t i2[64 * 64];
+    uint16_t i3[16 * 64];
+    float b1[64], b2[64], b3[16];
+};
+
+static_assert(offsetof(MlpW, b3) - offsetof(MlpW, i1) == offsetof(InfW, b3) &&
+                  sizeof(MlpW) - offsetof(MlpW, i1) == sizeof(InfW),
+              "the inference copies must be one contiguous block laid out as InfW");
+
+constexpr int MLP_WAVES = 8;
+
+__device__ __forceinline__ bf16x4 as_bf16x4(uint32_t lo, uint32_t hi) {
+    return __builtin_bit_cast(bf16x4, make_uint2(lo, hi));
+}
+
+__device__ __forceinline__ bf16x4 gelu4(f32x4 a) {
+    float y0, y1, y2, y3;
+    gelu(a[0], &y0, nullptr);
+    gelu(a[1], &y1, nullptr);
+    gelu(a[2], &y2, nullptr);
+    gelu(a[3], &y3, nullptr);
+    return pack_bf16(y0, y1, y2, y3);
+}
+
+// Every layer is computed transposed, out^T = W * act^T, weights as the MFMA A
+// operand and activations as B.  Lane (r = l % 16, q = l / 16) then holds,
+// for output tile t, outputs 16t + 4q + i of vertex r -- exactly its B
+// operand for K-step t of the next layer -- so activations never leave the
+// registers; biases enter as the accumulator input.
+__global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_mlp(const MlpW* __restrict__ gW, const uint4* __restrict__ feat,
+                                                           const float* __restrict__ hs,
+                                                           const float* __restrict__ norm, int nv,
+                                                           float4* __restrict__ delta) {
+    __shared__ InfW W;
+    {
+        const uint4* src = (const uint4*)gW->i1;  // i1, i2, i3, b1, b2, b3 are contiguous
+        uint4* dst = (uint4*)&W;
+        for (int i = threadIdx.x; i < (int)(sizeof(InfW) / 16); i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    const float sdy0 = norm[44], sdy1 = norm[45], sdy2 = norm[46];
     const int ngroups = (nv + 15) / 16;
-    for (int g = blockIdx.x * INF_WAVES + wv; g < ngroups; g += gridDim.x * INF_WAVES) {
-        const int base = g * 16;
-        {
-            const int row = l & 15, part = l >> 4, v = base + row;
-            float* fr = f.fe + row * 24;
-            if (v < nv) {
-                float h = cl[vcloth[v]].hscale;
-                float4 P = p[v];
-                if (part == 0) {
-                    float4 X = x[v];
-                    fr[0] = (P.x - X.x) / h;
-                    fr[1] = (P.y - X.y) / h;
-                    fr[2] = (P.z - X.z) / h;
-                    fr[21] = P.w == 0.0f ? 1.0f : 0.0f;
-                }
-                for (int k = part; k < 6; k += 4) {
-                    int j = nbr[(size_t)v * 6 + k];
-                    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-                    if (j >= 0) {
-                        float4 Q = p[j];
-                        float ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
-                        float d = sqrtf(ex * ex + ey * ey + ez * ez);
-                        float L = nbr_rest[(size_t)v * 6 + k];
-                        if (d > 0.0f) {
-                            float sc = ((d - L) / L) / d;
-                            sx = sc * ex;
-                            sy = sc * ey;
-                            sz = sc * ez;
-                        }
-                    }
-                    fr[3 + 3 * k] = sx;
-                    fr[4 + 3 * k] = sy;
-                    fr[5 + 3 * k] = sz;
-                }
-            } else if (part == 0) {
-                for (int k = 0; k < 22; k++) fr[k] = 0.0f;
-            }
+    const int stride = gridDim.x * MLP_WAVES;
+    int g = blockIdx.x * MLP_WAVES + wv;
+    auto fetch = [&](int grp) {
+        const int v = grp * 16 + r;
+        return (grp < ngroups && v < nv) ? feat[(size_t)v * 4 + q] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto fetch_h = [&](int grp) {
+        const int v = grp * 16 + r;
+        return (q == 0 && grp < ngroups && v < nv) ? hs[v] : 0.0f;
+    };
+    uint4 next = fetch(g);
+    float h_next = fetch_h(g);
+    for (; g < ngroups; g += stride) {
+        const uint4 cur = next;
+        const float h = h_next;
+        next = fetch(g + stride);
+        h_next = fetch_h(g + stride);
+        const int v = g * 16 + r;
+        const bf16x4 x0 = as_bf16x4(cur.x, cur.y), x1 = as_bf16x4(cur.z, cur.w);
+        f32x4 acc[4];
+        bf16x4 h1[4], h2[4];
+        // layer 1: K = 32 (features); A operand of K-step st at 8q + 4st
+        for (int t = 0; t < 4; t++) acc[t] = *(const f32x4*)(W.b1 + 16 * t + 4 * q);
+        for (int st = 0; st < 2; st++) {
+            const bf16x4 x = st == 0 ? x0 : x1;
+            bf16x4 a[4];
+            for (int t = 0; t < 4; t++) a[t] = *(const bf16x4*)(W.i1 + (16 * t + r) * 32 + 8 * q + 4 * st);
+            for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], x, acc[t], 0, 0, 0);
         }
-        wave_sync();
-        for (int e = l; e < 16 * 32; e += 64) {
-            int row = e / 32, k = e % 32;
-            float fv = k < 22 ? f.fe[row * 24 + k] : 0.0f;
-            f.xs[e] = k < 22 ? f2bf(fminf(fmaxf((fv - norm[k]) / norm[22 + k], -8.0f), 8.0f)) : 0;
-            if (feat_out && k < 22 && base + row < nv) feat_out[(size_t)(base + row) * 22 + k] = fv;
+        for (int t = 0; t < 4; t++) h1[t] = gelu4(acc[t]);
+        // layer 2: K = 64; A operand of K-step st at 16q + 4st
+        for (int t = 0; t < 4; t++) acc[t] = *(const f32x4*)(W.b2 + 16 * t + 4 * q);
+        for (int st = 0; st < 4; st++) {
+            bf16x4 a[4];
+            for (int t = 0; t < 4; t++) a[t] = *(const bf16x4*)(W.i2 + (16 * t + r) * 64 + 16 * q + 4 * st);
+            for (int t = 0; t < 4; t++)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], h1[st], acc[t], 0, 0, 0);
         }
-        wave_sync();
-        forward16_infer(W, f);
-        if (l < 16) {
-            int v = base + l;
-            if (v < nv) {
-                float h = cl[vcloth[v]].hscale;
-                bool pinned = p[v].w == 0.0f;
-                float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (!pinned)
-                    d = make_float4(h * (f.y[l * 16 + 0] * norm[44]), h * (f.y[l * 16 + 1] * norm[45]),
-                                    h * (f.y[l * 16 + 2] * norm[46]), 0.0f);
-                delta[v] = d;
-            }
+        for (int t = 0; t < 4; t++) h2[t] = gelu4(acc[t]);
+        // layer 3: 16 output rows (3 used), K = 64
+        f32x4 o = *(const f32x4*)(W.b3 + 4 * q);
+        for (int st = 0; st < 4; st++) {
+            const bf16x4 a = *(const bf16x4*)(W.i3 + r * 64 + 16 * q + 4 * st);
+            o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[st], o, 0, 0, 0);
         }
-        wave_sync();
+        // lane (r, 0) holds outputs 0..3 of vertex r
+        if (q == 0 && v < nv) {
+            float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (h != 0.0f) d = make_float4(h * (o[0] * sdy0), h * (o[1] * sdy1), h * (o[2] * sdy2), 0.0f);
+            delta[v] = d;
+        }
     }
 }
 
@@ -544,11 +622,22 @@ __global__ __launch_bounds__(256) void k_ws_apply(const int* __restrict__ vcloth
 }
 
 // In-place standardisation of stored samples: X = clamp((X - mu)/sd, +-8), Y = Y/sdy.
+// Per-cloth sweep budget of the substep: the warm budget where the prediction
+// was kept, the plain solver's full budget where it was discarded.
+__global__ __launch_bounds__(256) void k_ws_decide(const unsigned long long* __restrict__ r0,
+                                                   const unsigned long long* __restrict__ r1, int ncl,
+                                                   int warm_sweeps, int full_sweeps, int* __restrict__ cap) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncl) return;
+    double a = __longlong_as_double((long long)r0[c]), b = __longlong_as_double((long long)r1[c]);
+    cap[c] = b <= a ? warm_sweeps : full_sweeps;
+}
+
 __global__ __launch_bounds__(256) void k_ws_normalize(float* __restrict__ X, float* __restrict__ Y, long long n,
                                                       const float* __restrict__ norm) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    for (int k = 0; k < 22; k++) X[i * 22 + k] = fminf(fmaxf((X[i * 22 + k] - norm[k]) / norm[22 + k], -8.0f), 8.0f);
+    for (int k = 0; k < 22; k++) X[i * 22 + k] = norm_feature(X[i * 22 + k], norm[k], 1.0f / norm[22 + k]);
     for (int k = 0; k < 3; k++) Y[i * 3 + k] = Y[i * 3 + k] / norm[44 + k];
 }
 
@@ -605,20 +694,36 @@ void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float
     PBD_HIP(hipGetLastError());
 }
 
+// BF16 feature rows of the network kernel: 64 bytes per vertex.
+size_t ws_feature_bytes(int nv) { return (size_t)std::max(nv, 1) * 64; }
+
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
-              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm) {
+              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm,
+              void* feat_scratch, float* hs_scratch) {
     if (nv <= 0) return;
-    static int ncu = 0;
-    if (!ncu) {
-        int dev = 0;
+    static int max_blocks = 0;
+    if (!max_blocks) {
+        int dev = 0, ncu = 0, per_cu = 0;
         PBD_HIP(hipGetDevice(&dev));
         PBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        if (ncu <= 0) ncu = 256;
+        PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ws_mlp, 64 * MLP_WAVES, 0));
+        max_blocks = std::max(1, ncu) * std::max(1, per_cu);
     }
+    uint4* feat = (uint4*)feat_scratch;
+    hipLaunchKernelGGL(k_ws_features, dim3(nb256(nv)), dim3(256), 0, s, cl, vcloth, x, p, nbr, nbr_rest, nv, norm,
+                       feat, hs_scratch, feat_out);
+    PBD_HIP(hipGetLastError());
     int groups = (nv + 15) / 16;
-    int blocks = std::min((groups + INF_WAVES - 1) / INF_WAVES, ncu);
-    hipLaunchKernelGGL(k_ws_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, s, (const MlpW*)W, cl, vcloth, x, p, nbr,
-                       nbr_rest, nv, delta, feat_out, norm);
+    int blocks = std::min((groups + MLP_WAVES - 1) / MLP_WAVES, max_blocks);
+    hipLaunchKernelGGL(k_ws_mlp, dim3(blocks), dim3(64 * MLP_WAVES), 0, s, (const MlpW*)W, (const uint4*)feat,
+                       (const float*)hs_scratch, norm, nv, delta);
+    PBD_HIP(hipGetLastError());
+}
+
+void ws_decide(hipStream_t s, const unsigned long long* r0, const unsigned long long* r1, int ncl, int warm_sweeps,
+               int full_sweeps, int* cap) {
+    if (ncl <= 0) return;
+    hipLaunchKernelGGL(k_ws_decide, dim3(nb256(ncl)), dim3(256), 0, s, r0, r1, ncl, warm_sweeps, full_sweeps, cap);
     PBD_HIP(hipGetLastError());
 }
 

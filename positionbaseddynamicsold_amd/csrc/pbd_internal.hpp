@@ -207,12 +207,16 @@ void mlp_grad(hipStream_t s, const void* W, const float* X, const float* T, cons
 void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
               int step);
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
-              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm);
+              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm,
+              void* feat_scratch, float* hs_scratch);
+size_t ws_feature_bytes(int nv);
 void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm);
 void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* r0,
                unsigned long long* r1);
 void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const unsigned long long* r0,
               const unsigned long long* r1, int nv);
+void ws_decide(hipStream_t s, const unsigned long long* r0, const unsigned long long* r1, int ncl, int warm_sweeps,
+               int full_sweeps, int* cap);
 void ws_target(hipStream_t s, const ClothDev* cl, const int* vcloth, const float4* pa, const float4* pb, int nv,
                float* y);
 
@@ -245,9 +249,10 @@ void launch_integrate(hipStream_t s, const ClothDev* cloths, const int* vcloth, 
                       const float4* p, int nv_begin, int nv_end, float dt, int skip_whole);
 void launch_project_colour(hipStream_t s, const int4* dist, int d_begin, int d_end,
                            const int4* bend_idx, const float2* bend_rk, int b_begin, int b_end,
-                           const float* ktab_d, const float* ktab_b, int kmax, int sweep, float4* p);
+                           const float* ktab_d, const float* ktab_b, int kmax, int sweep, const int* cap,
+                           float4* p);
 void launch_colliders(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* p,
-                      int nv_begin, int nv_end);
+                      int nv_begin, int nv_end, const int* cap = nullptr, int sweep = 0);
 void launch_residual(hipStream_t s, const int4* dist, int n, const float4* p,
                      unsigned long long* rmax, double* rsum, int* rcnt);
 void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* tiles,
@@ -256,10 +261,11 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   const float* own_rest, const int* cols, const ClothDev* cloths,
                   const float* ktab_d, const float* ktab_b, float4* x, float4* v,
                   const float4* p_in, float4* p_out, unsigned long long* res_slots,
-                  int nslots_cloth, int* sweeps_out, const SolveArgs& a, int threads,
-                  size_t lds_bytes);
+                  int nslots_cloth, int* sweeps_out, const int* sweep_cap, const SolveArgs& a,
+                  int threads, size_t lds_bytes);
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
 void launch_fill_int(hipStream_t s, int* p, size_t n, int v);
+void launch_min_sweeps(hipStream_t s, int* sweeps, const int* cap, int sw, int n);
 void launch_copy(hipStream_t s, const float4* a, float4* b, size_t n);
 
 }  // namespace pbd

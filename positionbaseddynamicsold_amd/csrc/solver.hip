@@ -80,12 +80,14 @@ __global__ __launch_bounds__(256) void k_project_colour(const int4* __restrict__
                                                         const float2* __restrict__ brc, int bb, int be,
                                                         const float* __restrict__ ktab_d,
                                                         const float* __restrict__ ktab_b, int kmax,
-                                                        int sweep, float4* __restrict__ p) {
+                                                        int sweep, const int* __restrict__ cap,
+                                                        float4* __restrict__ p) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int nd = de - db;
     int ks = sweep < kmax ? sweep : kmax - 1;
     if (j < nd) {
         int4 r = dist[db + j];
+        if (cap && sweep >= cap[r.w]) return;  // this cloth's sweep budget is spent
         float k = ktab_d[r.w * kmax + ks];
         float4 a = p[r.x], b = p[r.y];
         project_distance(a, b, __int_as_float(r.z), k);
@@ -97,6 +99,7 @@ __global__ __launch_bounds__(256) void k_project_colour(const int4* __restrict__
     if (j < be - bb) {
         int4 q = bidx[bb + j];
         float2 rc = brc[bb + j];
+        if (cap && sweep >= cap[__float_as_int(rc.y)]) return;
         float k = ktab_b[__float_as_int(rc.y) * kmax + ks];
         float4 q0 = p[q.x], q1 = p[q.y], q2 = p[q.z], q3 = p[q.w];
         project_bending(q0, q1, q2, q3, rc.x, k);
@@ -109,29 +112,33 @@ __global__ __launch_bounds__(256) void k_project_colour(const int4* __restrict__
 
 void launch_project_colour(hipStream_t s, const int4* dist, int db, int de, const int4* bidx,
                            const float2* brc, int bb, int be, const float* ktab_d,
-                           const float* ktab_b, int kmax, int sweep, float4* p) {
+                           const float* ktab_b, int kmax, int sweep, const int* cap, float4* p) {
     int n = (de - db) + (be - bb);
     if (n <= 0) return;
     hipLaunchKernelGGL(k_project_colour, dim3((n + 255) / 256), dim3(256), 0, s, dist, db, de, bidx, brc, bb,
-                       be, ktab_d, ktab_b, kmax, sweep, p);
+                       be, ktab_d, ktab_b, kmax, sweep, cap, p);
     PBD_HIP(hipGetLastError());
 }
 
 __global__ __launch_bounds__(256) void k_colliders(const ClothDev* __restrict__ cl,
                                                    const int* __restrict__ vcloth, float4* __restrict__ p,
-                                                   int b, int e) {
+                                                   int b, int e, const int* __restrict__ cap, int sweep) {
     int i = b + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= e) return;
-    const ClothDev& c = cl[vcloth[i]];
+    const int ci = vcloth[i];
+    if (cap && sweep >= cap[ci]) return;
+    const ClothDev& c = cl[ci];
     if (!c.sphere_on && !c.box_on) return;
     float4 q = p[i];
     collide_point(q, c.sphere_on, c.sphere_c, c.sphere_r, c.box_on, c.box_lo, c.box_hi);
     p[i] = q;
 }
 
-void launch_colliders(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* p, int b, int e) {
+void launch_colliders(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* p, int b, int e,
+                      const int* cap, int sweep) {
     if (e <= b) return;
-    hipLaunchKernelGGL(k_colliders, dim3((e - b + 255) / 256), dim3(256), 0, s, cloths, vcloth, p, b, e);
+    hipLaunchKernelGGL(k_colliders, dim3((e - b + 255) / 256), dim3(256), 0, s, cloths, vcloth, p, b, e, cap,
+                       sweep);
     PBD_HIP(hipGetLastError());
 }
 
@@ -216,6 +223,18 @@ void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned lo
     PBD_HIP(hipGetLastError());
 }
 
+__global__ void k_min_sweeps(int* __restrict__ sweeps, const int* __restrict__ cap, int sw, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sweeps[i] = min(cap[i], sw);
+}
+
+// Per-cloth sweep counts of a per-colour (global path) projection with budgets.
+void launch_min_sweeps(hipStream_t s, int* sweeps, const int* cap, int sw, int n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_min_sweeps, dim3((n + 255) / 256), dim3(256), 0, s, sweeps, cap, sw, n);
+    PBD_HIP(hipGetLastError());
+}
+
 __global__ void k_fill_int(int* p, size_t n, int v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -268,7 +287,8 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
                                                 float4* __restrict__ v, const float4* __restrict__ p_in,
                                                 float4* __restrict__ p_out,
                                                 unsigned long long* __restrict__ res_slots, int ncl,
-                                                int* __restrict__ sweeps_out, SolveArgs a) {
+                                                int* __restrict__ sweeps_out, const int* __restrict__ sweep_cap,
+                                                SolveArgs a) {
     extern __shared__ float4 P[];
     __shared__ double red[16];
     int slot = order[blockIdx.x];
@@ -278,13 +298,19 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     const int T = blockDim.x, tid = threadIdx.x;
     const int* vl = verts + td.load_off;
 
+    // per-cloth sweep budget (warm start: short when the prediction was kept)
+    int nsweeps = a.nsweeps;
+    if (sweep_cap) nsweeps = min(nsweeps, sweep_cap[td.cloth] - a.sweep_base);
     const double tol = cd.tol >= 0.0 ? cd.tol : a.tol;
+    bool skip = !td.whole && nsweeps <= 0;
     if (!td.whole && a.chunk > 0 && tol >= 0.0) {
         double prev = __longlong_as_double((long long)res_slots[(size_t)(a.chunk - 1) * ncl + td.cloth]);
-        if (prev <= tol) {
-            for (int i = tid; i < td.n_int; i += T) p_out[vl[i]] = p_in[vl[i]];
-            return;
-        }
+        if (prev <= tol) skip = true;
+    }
+    if (skip) {
+        // this cloth is done: carry its interior into the other buffer
+        for (int i = tid; i < td.n_int; i += T) p_out[vl[i]] = p_in[vl[i]];
+        return;
     }
     if (td.whole && a.fused) {
         for (int i = tid; i < td.n_load; i += T) {
@@ -307,7 +333,7 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     const bool coll = (cd.sphere_on || cd.box_on) && !a.no_coll;
     int sw = 0;
     double resid = -1.0;
-    for (; sw < a.nsweeps;) {
+    for (; sw < nsweeps;) {
         int gs = a.sweep_base + sw;
         int ks = gs < a.kmax ? gs : a.kmax - 1;
         float kd = ktab_d[cd.ktab + ks];
@@ -381,7 +407,7 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   const uint32_t* dist, const float* dist_rest, const uint32_t* bend, const float* bend_rest,
                   const uint32_t* own, const float* own_rest, const int* cols, const ClothDev* cloths,
                   const float* ktab_d, const float* ktab_b, float4* x, float4* v, const float4* p_in,
-                  float4* p_out, unsigned long long* res_slots, int ncl, int* sweeps_out,
+                  float4* p_out, unsigned long long* res_slots, int ncl, int* sweeps_out, const int* sweep_cap,
                   const SolveArgs& a, int threads, size_t lds_bytes) {
     if (nblocks <= 0) return;
     static size_t attr_bytes = 0;
@@ -393,7 +419,7 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
     }
     hipLaunchKernelGGL(k_tiles, dim3(nblocks), dim3(threads), lds_bytes, s, order, tiles, verts, dist, dist_rest,
                        bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v, p_in, p_out, res_slots,
-                       ncl, sweeps_out, a);
+                       ncl, sweeps_out, sweep_cap, a);
     PBD_HIP(hipGetLastError());
 }
 

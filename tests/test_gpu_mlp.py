@@ -131,6 +131,19 @@ def test_collect_features_and_targets(orc):
     assert np.abs(X[i, 3:21]).max() < 0.2
 
 
+def test_collect_keeps_the_solver_budget():
+    with capi.Context() as ctx:
+        cid = ctx.add_grid(16, 16, 1.0, 1.0, jitter=1e-4, seed=0)
+        ctx.set_pins(cid, [0, 15])
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=20)
+        ctx.ws_init(0)
+        ctx.ws_collect(35, 1)
+        assert ctx.last_sweeps(cid) == 35
+        ctx.step(1)
+        assert ctx.last_sweeps(cid) == 20
+
+
 def config4_context(seeds, iterations=20):
     from positionbaseddynamicsold_amd import scenes
     ctx = capi.Context()
@@ -138,6 +151,46 @@ def config4_context(seeds, iterations=20):
     ctx.init()
     ctx.set_solver(dt=scenes.DT, iterations=iterations)
     return ctx, ids
+
+
+def test_inference_matches_batch_forward():
+    # the correction the solver predicts equals h * (y * sdy) with y the batch
+    # forward of the same features (recorded by a twin context at the same
+    # state), within the bf16 network's rounding
+    from positionbaseddynamicsold_amd import scenes
+    rng = np.random.default_rng(9)
+    prm = rand_params(11, 0.25)
+    norm = np.concatenate([rng.normal(0, 0.1, 22), rng.uniform(0.5, 2.0, 22), rng.uniform(0.5, 2.0, 3)])
+    norm = norm.astype(np.float32)
+    a, ids = config4_context(range(3))
+    b, _ = config4_context(range(3))
+    try:
+        for c in (a, b):
+            c.ws_set_params(prm)
+            c.ws_set_norm(norm)
+            c.step(15)
+        a.ws_enable(True)
+        a.step(1)
+        b.ws_collect(20, 1)
+        X, _ = b.ws_samples()
+        rsd = (np.float32(1.0) / norm[22:44]).astype(np.float32)
+        Xn = np.clip((X - norm[:22]) * rsd, -8, 8).astype(np.float32)
+        Y = b.ws_forward(Xn)
+        off = 0
+        for i in ids:
+            d = a.ws_last_delta(i)
+            n = len(d)
+            _, _, rest = a.prims(i)
+            h = np.float32(rest.astype(np.float64).mean())
+            ref = h * (Y[off:off + n] * norm[44:47])
+            ref[X[off:off + n, 21] == 1] = 0.0
+            assert np.all(d[X[off:off + n, 21] == 1] == 0.0)
+            err = np.abs(d - ref).max() / np.abs(ref).max()
+            assert err < 2e-2, (i, err)
+            off += n
+    finally:
+        a.close()
+        b.close()
 
 
 def test_fallback_discards_a_bad_prediction():
@@ -185,6 +238,37 @@ def test_zero_prediction_is_kept_and_changes_nothing():
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("path", [capi.PATH_AUTO, capi.PATH_GLOBAL])
+def test_sweep_budget_follows_the_fallback_rule(path):
+    # warm budget 5, plain budget 20: a discarded prediction (corrupted
+    # weights) gives the plain 20-sweep substep, a kept zero prediction the
+    # plain 5-sweep substep, both bit for bit, with per-cloth sweep counts
+    from positionbaseddynamicsold_amd import scenes
+    print(f"sweep budget path={path}", flush=True)
+    for scale, expect in ((50.0, 20), (0.0, 5)):
+        a, ids_a = config4_context(range(3))
+        b, ids_b = config4_context(range(3))
+        try:
+            a.set_solver(dt=scenes.DT, iterations=expect, path=path)
+            b.set_solver(dt=scenes.DT, iterations=20, warm_iterations=5, path=path, graph=1)
+            b.ws_init(0)
+            w = b.ws_params()
+            w[5632:5827] = scale  # W3 / b3
+            b.ws_set_params(w)
+            b.ws_enable(True)
+            for _ in range(4):
+                a.step(1)
+                b.step(1)
+                assert [b.last_sweeps(i) for i in ids_b] == [expect] * 3
+                assert [b.ws_last_used(i) for i in ids_b] == [int(expect == 5)] * 3
+            for i, j in zip(ids_a, ids_b):
+                assert np.array_equal(a.get(i), b.get(j))
+                assert np.array_equal(a.get(i, capi.STATE_V), b.get(j, capi.STATE_V))
+        finally:
+            a.close()
+            b.close()
 
 
 def test_warm_start_gain_over_plain():
