@@ -91,6 +91,8 @@ typedef struct {
     int warm_iterations;    /* with the warm start on: sweep budget of a cloth whose prediction was kept
                                (0 = iterations); a cloth whose prediction is discarded runs `iterations`,
                                exactly the plain solver's substep */
+    int warm_rule;          /* fallback rule: the prediction is kept iff its residual does not exceed the
+                               unwarmed start's, measured as max |d-L|/L (0) or as the RMS of |d-L|/L (1) */
 } pbd_solver_params;
 
 typedef struct {

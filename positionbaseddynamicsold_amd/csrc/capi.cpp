@@ -22,6 +22,10 @@ struct pbd_ctx {
     // state
     DBuf<float4> x, v, pbuf[2];
     int pcur = 0;
+    // after a tile projection each cloth's result sits in p buffer pfinal[c]
+    // until integrate (or a consolidation) gathers it into pbuf[pcur]
+    DBuf<int> pfinal;
+    bool pfinal_valid = false;
     DBuf<int> vcloth;
     std::vector<ClothDev> cl_h;
     DBuf<ClothDev> cl;
@@ -71,7 +75,8 @@ struct pbd_ctx {
         DBuf<float> nbr_rest;
         long long ns = 0;
         DBuf<float4> delta, psave;
-        DBuf<unsigned long long> r0, r1;
+        DBuf<unsigned long long> rs;  // per cloth: max residual without/with, squared sums without/with
+        DBuf<int> keep;               // per cloth: 1 if the last prediction was kept
         std::vector<int> used;
         bool used_pending = false;  // r0/r1 hold a decision not yet read back
         DBuf<int> cap;              // per-cloth sweep budget of the current substep
@@ -439,6 +444,8 @@ void do_init(pbd_ctx* c, int color_rounds) {
     c->rsum.alloc(c->cloths.size());
     c->sweeps.alloc(c->cloths.size());
     PBD_HIP(hipMemsetAsync(c->sweeps.ptr, 0, sizeof(int) * c->cloths.size(), s));
+    c->pfinal.alloc(c->cloths.size());
+    c->pfinal_valid = false;
     fill_ktabs(c);
     c->plan_sweeps = 0;
     c->plan_max_load = 0;
@@ -541,8 +548,8 @@ void ws_build_nbr(pbd_ctx* c) {
     c->ws.nbr_rest.upload(rest, c->stream);
     c->ws.delta.alloc(c->nv_total);
     c->ws.psave.alloc(c->nv_total);
-    c->ws.r0.alloc(c->cloths.size());
-    c->ws.r1.alloc(c->cloths.size());
+    c->ws.rs.alloc(4 * c->cloths.size());
+    c->ws.keep.alloc(c->cloths.size());
     c->ws.cap.alloc(c->cloths.size());
     c->ws.feat.alloc(ws_feature_bytes(c->nv_total) / sizeof(uint4));
     c->ws.hs.alloc(c->nv_total);
@@ -568,13 +575,12 @@ void ws_apply_prediction(pbd_ctx* c) {
     float4* p = c->pbuf[c->pcur].ptr;
     ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, c->ws.nbr.ptr, c->ws.nbr_rest.ptr, c->nv_total,
              c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr);
-    launch_fill_u64(s, c->ws.r0.ptr, ncl, 0ull);
-    launch_fill_u64(s, c->ws.r1.ptr, ncl, 0ull);
-    ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr);
-    ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.r0.ptr, c->ws.r1.ptr, c->nv_total);
+    launch_fill_u64(s, c->ws.rs.ptr, 4 * (size_t)ncl, 0ull);
+    ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.rs.ptr, ncl);
     int full = c->sp.iterations;
     int warm = c->sp.warm_iterations > 0 ? std::min(c->sp.warm_iterations, full) : full;
-    ws_decide(s, c->ws.r0.ptr, c->ws.r1.ptr, ncl, warm, full, c->ws.cap.ptr);
+    ws_decide(s, c->ws.rs.ptr, ncl, c->sp.warm_rule, warm, full, c->ws.cap.ptr, c->ws.keep.ptr);
+    ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.keep.ptr, c->nv_total);
     c->ws.used_pending = true;
 }
 
@@ -592,19 +598,8 @@ void read_sweeps(pbd_ctx* c) {
 // demand so that the substep itself never waits for the host.
 void ws_read_used(pbd_ctx* c) {
     if (!c->ws.used_pending) return;
-    hipStream_t s = c->stream;
-    int ncl = (int)c->cloths.size();
-    std::vector<unsigned long long> a, b;
-    c->ws.r0.download(a, s);
-    c->ws.r1.download(b, s);
-    PBD_HIP(hipStreamSynchronize(s));
-    c->ws.used.assign(ncl, 0);
-    for (int i = 0; i < ncl; i++) {
-        double x, y;
-        memcpy(&x, &a[i], 8);
-        memcpy(&y, &b[i], 8);
-        c->ws.used[i] = y <= x ? 1 : 0;
-    }
+    c->ws.keep.download(c->ws.used, c->stream);
+    PBD_HIP(hipStreamSynchronize(c->stream));
     c->ws.used_pending = false;
 }
 
@@ -640,6 +635,7 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     launch_fill_int(s, c->sweeps.ptr, ncl, 0);
     if (c->sp.path == PBD_PATH_GLOBAL) {
         PBD_CHECK(!fused_whole, PBD_ERR_STATE, "internal: global path is never fused");
+        c->pfinal_valid = false;
         float4* p = c->pbuf[c->pcur].ptr;
         bool coll = false;
         for (auto& k : c->cloths) coll |= (k.prm.sphere_on || k.prm.box_on);
@@ -695,18 +691,21 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     a.check_every = ce;
     a.tol = tol;
     a.kmax = c->kmax;
+    a.copy_spent = any_tol ? 1 : 0;
+    const int start = cur;
+    int launched = 0;
     if (c->any_whole) {
         a.sweep_base = 0;
         a.nsweeps = iterations;
         a.fused = fused_whole ? 1 : 0;
         a.chunk = 0;
-        // whole tiles write pbuf[cur ^ 1]; copied below if the partial chain
-        // ends in the other buffer
+        // whole tiles write pbuf[start ^ 1]
         tiles((int)c->order_whole.n, c->order_whole.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
     }
     if (c->any_partial) {
         int sw = 0;
         for (int ch = 0; ch < nchunks; ch++) {
+            launched++;
             a.sweep_base = sw;
             a.nsweeps = std::min(ts, iterations - sw);
             a.fused = 0;
@@ -734,12 +733,9 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     } else {
         cur ^= 1;
     }
-    // whole tiles wrote pbuf[pcur^1]; make the final index hold them too
-    if (c->any_whole && c->any_partial && cur != (c->pcur ^ 1)) {
-        // copy whole-cloth vertices into the final buffer
-        for (auto& k : c->cloths)
-            if (k.whole) launch_copy(s, c->pbuf[c->pcur ^ 1].ptr + k.vbase, c->pbuf[cur].ptr + k.vbase, k.nv);
-    }
+    launch_final_buffers(s, c->cl.ptr, ncl, start, launched, ts, iterations, any_tol ? nullptr : cap,
+                         c->pfinal.ptr);
+    c->pfinal_valid = true;
     c->pcur = cur;
     if (any_tol || cap) {
         // per-cloth counts live on the device; read them now only if needed
@@ -763,8 +759,18 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
 }
 
 void integrate_all(pbd_ctx* c, float dt, bool skip_whole) {
-    launch_integrate(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, 0, c->nv_total,
-                     dt, skip_whole ? 1 : 0);
+    launch_integrate(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[0].ptr, c->pbuf[1].ptr,
+                     c->pfinal_valid ? c->pfinal.ptr : nullptr, c->pcur, 0, c->nv_total, dt, skip_whole ? 1 : 0);
+    c->pfinal_valid = false;
+}
+
+// Gathers every cloth's projected p into pbuf[pcur] (callers that read p
+// between projection and integration).
+void consolidate(pbd_ctx* c) {
+    if (!c->pfinal_valid) return;
+    launch_consolidate(c->stream, c->vcloth.ptr, c->pbuf[0].ptr, c->pbuf[1].ptr, c->pfinal.ptr, c->pcur,
+                       c->nv_total);
+    c->pfinal_valid = false;
 }
 
 // Substep with collisions, in the order of projectConstraintWithColliCPU
@@ -929,7 +935,8 @@ void step_cloth(pbd_ctx* c, int ci, float dt) {
             }
         }
         k.last_sweeps = sw;
-        launch_integrate(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, p, vb, ve, dt, 0);
+        launch_integrate(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[0].ptr, c->pbuf[1].ptr, nullptr,
+                         c->pcur, vb, ve, dt, 0);
         return;
     }
     // LDS tiles of this cloth only
@@ -980,7 +987,8 @@ void step_cloth(pbd_ctx* c, int ci, float dt) {
         }
         // the batch's current p buffer must hold this cloth's result
         if (cur != c->pcur) launch_copy(s, c->pbuf[cur].ptr + vb, c->pbuf[c->pcur].ptr + vb, k.nv);
-        launch_integrate(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, vb, ve, dt, 0);
+        launch_integrate(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[0].ptr, c->pbuf[1].ptr, nullptr,
+                         c->pcur, vb, ve, dt, 0);
     }
     if (tol >= 0.0) {
         int sv = 0;
@@ -1228,6 +1236,8 @@ int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* p) {
         PBD_CHECK(p->tile_sweeps >= 0 && p->tile_max_verts >= 0, PBD_ERR_ARG, "negative tile setting");
         PBD_CHECK(p->block_threads == 0 || (p->block_threads % 64 == 0 && p->block_threads <= 1024), PBD_ERR_ARG,
                   "block_threads must be a multiple of 64 up to 1024");
+        PBD_CHECK(p->warm_iterations >= 0, PBD_ERR_ARG, "warm_iterations must be >= 0");
+        PBD_CHECK(p->warm_rule == 0 || p->warm_rule == 1, PBD_ERR_ARG, "warm_rule must be 0 (max) or 1 (RMS)");
         ctx->sp = *p;
         if (ctx->inited) {
             set_device(ctx);
@@ -1258,6 +1268,7 @@ int pbd_project(pbd_ctx* ctx, int iterations) {
             fill_cloth_dev(ctx);
         }
         project(ctx, iterations, false, ctx->sp.dt);
+        consolidate(ctx);
     });
 }
 
@@ -1642,6 +1653,7 @@ int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
             double tol = ctx->sp.tol;
             ctx->sp.tol = -1.0;
             project(ctx, sweeps, false, dt);
+            consolidate(ctx);
             ctx->sp.tol = tol;
             ws_target(s, ctx->cl.ptr, ctx->vcloth.ptr, ctx->pbuf[ctx->pcur].ptr, W.psave.ptr, ctx->nv_total,
                       W.Y.ptr + 3 * W.ns);

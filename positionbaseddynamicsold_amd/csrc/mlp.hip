@@ -349,14 +349,24 @@ __global__ __launch_bounds__(128) void k_mlp_train(const MlpW* __restrict__ gW, 
     }
 }
 
-// Sum of the per-wave partials in wave order (deterministic).
+// Sum of the per-wave partials: a block takes 16 parameters x 16 wave
+// slices; slice j adds waves j, j + 16, ... in order and the 16 slice sums
+// are added in slice order (deterministic).
 __global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partial, int nw,
                                                     float* __restrict__ grad) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > NPARAM) return;
-    float s = 0.0f;
-    for (int w = 0; w < nw; w++) s += partial[(size_t)w * (NPARAM + 1) + i];
-    grad[i] = s;
+    __shared__ float red[16][17];
+    const int pi = threadIdx.x & 15, sl = threadIdx.x >> 4;
+    const int i = blockIdx.x * 16 + pi;
+    float acc = 0.0f;
+    if (i <= NPARAM)
+        for (int w = sl; w < nw; w += 16) acc += partial[(size_t)w * (NPARAM + 1) + i];
+    red[sl][pi] = acc;
+    __syncthreads();
+    if (sl == 0 && i <= NPARAM) {
+        float t = 0.0f;
+        for (int k = 0; k < 16; k++) t += red[k][pi];
+        grad[i] = t;
+    }
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master parameters; c1 = 1 - b1^t, c2 = 1 - b2^t.
@@ -547,47 +557,64 @@ __global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_mlp(const MlpW* __rest
     }
 }
 
-// Per-prim residual in FP32 (this comparison only decides whether the
-// prediction is kept); the reduction below is an exact max in FP64.
-__device__ __forceinline__ double rel_res(float4 a, float4 b, float L) {
+// Per-prim relative residual |d - L| / L in FP32 (it only decides whether the
+// prediction is kept; the reductions are exact).
+__device__ __forceinline__ float rel_res(float4 a, float4 b, float L) {
     float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
     float d = sqrtf(dx * dx + dy * dy + dz * dz);
-    return (double)(fabsf(d - L) / L);
+    return fabsf(d - L) / L;
 }
 
-// Residual of every distance prim with and without the predicted delta,
-// max-reduced per cloth (FP64 bit patterns, r >= 0).  The prim list is in
-// cloth order, so each thread keeps a running max over RESID_PER_THREAD prims
-// (stride 256) and flushes it with an atomic only when its cloth changes or
-// at the end; a wave whose lanes all end on one cloth reduces first.
+// Residual of every distance prim with and without the predicted delta, per
+// cloth: the max (FP64 bit patterns, r >= 0) and the sum of squares in
+// 2^-40 fixed point (r^2 clamped to 1; integer sums do not depend on the
+// order of the atomics).  rs = [max0 | max1 | sq0 | sq1], ncl entries each.
+// The prim list is in cloth order, so each thread keeps running values over
+// RESID_PER_THREAD prims (stride 256) and flushes them with atomics only when
+// its cloth changes or at the end; a wave whose lanes all end on one cloth
+// reduces first.
 constexpr int RESID_PER_THREAD = 8;
+
+__device__ __forceinline__ unsigned long long sq_fixed(float r) {
+    const float q = fminf(r * r, 1.0f);
+    return (unsigned long long)((double)q * 1099511627776.0);  // 2^40
+}
+
+__device__ __forceinline__ void flush_resid(unsigned long long* rs, int ncl, int c, double a, double b,
+                                            unsigned long long sa, unsigned long long sb) {
+    atomicMax(&rs[c], (unsigned long long)__double_as_longlong(a));
+    atomicMax(&rs[ncl + c], (unsigned long long)__double_as_longlong(b));
+    atomicAdd(&rs[2 * ncl + c], sa);
+    atomicAdd(&rs[3 * ncl + c], sb);
+}
 
 __global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist, int n, const float4* __restrict__ p,
                                                    const float4* __restrict__ delta,
-                                                   unsigned long long* __restrict__ r0,
-                                                   unsigned long long* __restrict__ r1) {
+                                                   unsigned long long* __restrict__ rs, int ncl) {
     const long long base = (long long)blockIdx.x * (256 * RESID_PER_THREAD) + threadIdx.x;
     int cl = -1;
     double a = 0.0, b = 0.0;
+    unsigned long long sa = 0, sb = 0;
     for (int k = 0; k < RESID_PER_THREAD; k++) {
         long long j = base + 256ll * k;
         if (j >= n) break;
         int4 q = dist[j];
         if (q.w != cl) {
-            if (cl >= 0) {
-                atomicMax(&r0[cl], (unsigned long long)__double_as_longlong(a));
-                atomicMax(&r1[cl], (unsigned long long)__double_as_longlong(b));
-            }
+            if (cl >= 0) flush_resid(rs, ncl, cl, a, b, sa, sb);
             cl = q.w;
-            a = 0.0;
-            b = 0.0;
+            a = b = 0.0;
+            sa = sb = 0;
         }
         float4 P0 = p[q.x], P1 = p[q.y], D0 = delta[q.x], D1 = delta[q.y];
         float L = __int_as_float(q.z);
-        a = fmax(a, rel_res(P0, P1, L));
+        const float r0 = rel_res(P0, P1, L);
         float4 Q0 = make_float4(P0.x + D0.x, P0.y + D0.y, P0.z + D0.z, 0.0f);
         float4 Q1 = make_float4(P1.x + D1.x, P1.y + D1.y, P1.z + D1.z, 0.0f);
-        b = fmax(b, rel_res(Q0, Q1, L));
+        const float r1 = rel_res(Q0, Q1, L);
+        a = fmax(a, (double)r0);
+        b = fmax(b, (double)r1);
+        sa += sq_fixed(r0);
+        sb += sq_fixed(r1);
     }
     int c0 = __shfl(cl, 0);
     unsigned long long same = __ballot(cl == c0 || cl < 0);
@@ -595,42 +622,43 @@ __global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist
         for (int off = 32; off > 0; off >>= 1) {
             a = fmax(a, __shfl_xor(a, off));
             b = fmax(b, __shfl_xor(b, off));
+            sa += __shfl_xor(sa, off);
+            sb += __shfl_xor(sb, off);
         }
-        if ((threadIdx.x & 63) == 0 && c0 >= 0) {
-            atomicMax(&r0[c0], (unsigned long long)__double_as_longlong(a));
-            atomicMax(&r1[c0], (unsigned long long)__double_as_longlong(b));
-        }
+        if ((threadIdx.x & 63) == 0 && c0 >= 0) flush_resid(rs, ncl, c0, a, b, sa, sb);
     } else if (cl >= 0) {
-        atomicMax(&r0[cl], (unsigned long long)__double_as_longlong(a));
-        atomicMax(&r1[cl], (unsigned long long)__double_as_longlong(b));
+        flush_resid(rs, ncl, cl, a, b, sa, sb);
     }
 }
 
-// Fallback rule: keep the prediction only where it does not raise the cloth's
-// residual above the unwarmed start.
-__global__ __launch_bounds__(256) void k_ws_apply(const int* __restrict__ vcloth, float4* __restrict__ p,
-                                                  const float4* __restrict__ delta,
-                                                  const unsigned long long* __restrict__ r0,
-                                                  const unsigned long long* __restrict__ r1, int nv) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nv) return;
-    int c = vcloth[v];
-    double a = __longlong_as_double((long long)r0[c]), b = __longlong_as_double((long long)r1[c]);
-    if (!(b <= a)) return;
-    float4 P = p[v], D = delta[v];
-    p[v] = make_float4(P.x + D.x, P.y + D.y, P.z + D.z, P.w);
-}
-
-// In-place standardisation of stored samples: X = clamp((X - mu)/sd, +-8), Y = Y/sdy.
-// Per-cloth sweep budget of the substep: the warm budget where the prediction
-// was kept, the plain solver's full budget where it was discarded.
-__global__ __launch_bounds__(256) void k_ws_decide(const unsigned long long* __restrict__ r0,
-                                                   const unsigned long long* __restrict__ r1, int ncl,
-                                                   int warm_sweeps, int full_sweeps, int* __restrict__ cap) {
+// Fallback rule and per-cloth sweep budget of the substep: keep the
+// prediction iff its residual (rule 0: max, rule 1: RMS) does not exceed the
+// unwarmed start's; a kept cloth runs the warm budget, a discarded one the
+// plain solver's full budget.
+__global__ __launch_bounds__(256) void k_ws_decide(const unsigned long long* __restrict__ rs, int ncl, int rule,
+                                                   int warm_sweeps, int full_sweeps, int* __restrict__ cap,
+                                                   int* __restrict__ keep) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncl) return;
-    double a = __longlong_as_double((long long)r0[c]), b = __longlong_as_double((long long)r1[c]);
-    cap[c] = b <= a ? warm_sweeps : full_sweeps;
+    bool k;
+    if (rule == 1) {
+        k = rs[3 * ncl + c] <= rs[2 * ncl + c];
+    } else {
+        double a = __longlong_as_double((long long)rs[c]), b = __longlong_as_double((long long)rs[ncl + c]);
+        k = b <= a;
+    }
+    keep[c] = k ? 1 : 0;
+    cap[c] = k ? warm_sweeps : full_sweeps;
+}
+
+__global__ __launch_bounds__(256) void k_ws_apply(const int* __restrict__ vcloth, float4* __restrict__ p,
+                                                  const float4* __restrict__ delta, const int* __restrict__ keep,
+                                                  int nv) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    if (!keep[vcloth[v]]) return;
+    float4 P = p[v], D = delta[v];
+    p[v] = make_float4(P.x + D.x, P.y + D.y, P.z + D.z, P.w);
 }
 
 __global__ __launch_bounds__(256) void k_ws_normalize(float* __restrict__ X, float* __restrict__ Y, long long n,
@@ -683,7 +711,7 @@ void mlp_grad(hipStream_t s, const void* W, const float* X, const float* T, cons
     PBD_HIP(hipMemsetAsync(partial, 0, sizeof(float) * (size_t)blocks * TRAIN_WAVES * (NPARAM + 1), s));
     hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(64 * TRAIN_WAVES), 0, s, (const MlpW*)W, X, T, idx, n,
                        inv_total, partial);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3(nb256(NPARAM + 1)), dim3(256), 0, s, partial, waves, grad);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + 1 + 15) / 16), dim3(256), 0, s, partial, waves, grad);
     PBD_HIP(hipGetLastError());
 }
 
@@ -720,10 +748,26 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
     PBD_HIP(hipGetLastError());
 }
 
-void ws_decide(hipStream_t s, const unsigned long long* r0, const unsigned long long* r1, int ncl, int warm_sweeps,
-               int full_sweeps, int* cap) {
+
+void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* rs,
+               int ncl) {
+    if (n <= 0) return;
+    int blocks = (int)(((long long)n + 256 * RESID_PER_THREAD - 1) / (256 * RESID_PER_THREAD));
+    hipLaunchKernelGGL(k_ws_resid2, dim3(blocks), dim3(256), 0, s, dist, n, p, delta, rs, ncl);
+    PBD_HIP(hipGetLastError());
+}
+
+void ws_decide(hipStream_t s, const unsigned long long* rs, int ncl, int rule, int warm_sweeps, int full_sweeps,
+               int* cap, int* keep) {
     if (ncl <= 0) return;
-    hipLaunchKernelGGL(k_ws_decide, dim3(nb256(ncl)), dim3(256), 0, s, r0, r1, ncl, warm_sweeps, full_sweeps, cap);
+    hipLaunchKernelGGL(k_ws_decide, dim3(nb256(ncl)), dim3(256), 0, s, rs, ncl, rule, warm_sweeps, full_sweeps, cap,
+                       keep);
+    PBD_HIP(hipGetLastError());
+}
+
+void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const int* keep, int nv) {
+    if (nv <= 0) return;
+    hipLaunchKernelGGL(k_ws_apply, dim3(nb256(nv)), dim3(256), 0, s, vcloth, p, delta, keep, nv);
     PBD_HIP(hipGetLastError());
 }
 
@@ -733,21 +777,7 @@ void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* n
     PBD_HIP(hipGetLastError());
 }
 
-void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* r0,
-               unsigned long long* r1) {
-    if (n <= 0) return;
-    if (n <= 0) return;
-    int blocks = (int)(((long long)n + 256 * RESID_PER_THREAD - 1) / (256 * RESID_PER_THREAD));
-    hipLaunchKernelGGL(k_ws_resid2, dim3(blocks), dim3(256), 0, s, dist, n, p, delta, r0, r1);
-    PBD_HIP(hipGetLastError());
-}
 
-void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const unsigned long long* r0,
-              const unsigned long long* r1, int nv) {
-    if (nv <= 0) return;
-    hipLaunchKernelGGL(k_ws_apply, dim3(nb256(nv)), dim3(256), 0, s, vcloth, p, delta, r0, r1, nv);
-    PBD_HIP(hipGetLastError());
-}
 
 void ws_target(hipStream_t s, const ClothDev* cl, const int* vcloth, const float4* pa, const float4* pb, int nv,
                float* y) {

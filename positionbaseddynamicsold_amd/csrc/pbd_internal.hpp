@@ -113,6 +113,7 @@ struct SolveArgs {
     int fused;           // whole tiles: 1 = advect on load + integrate on store
     int chunk;           // chunk index (selects the residual slot)
     int no_coll;         // 1 = skip the per-sweep analytic colliders (applied by the caller)
+    int copy_spent;      // 1 = a multi-tile cloth past its sweep budget is copied forward
 };
 
 // ---------------------------------------------------------------------------
@@ -211,12 +212,11 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
               void* feat_scratch, float* hs_scratch);
 size_t ws_feature_bytes(int nv);
 void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm);
-void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* r0,
-               unsigned long long* r1);
-void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const unsigned long long* r0,
-              const unsigned long long* r1, int nv);
-void ws_decide(hipStream_t s, const unsigned long long* r0, const unsigned long long* r1, int ncl, int warm_sweeps,
-               int full_sweeps, int* cap);
+void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* rs,
+               int ncl);
+void ws_decide(hipStream_t s, const unsigned long long* rs, int ncl, int rule, int warm_sweeps, int full_sweeps,
+               int* cap, int* keep);
+void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const int* keep, int nv);
 void ws_target(hipStream_t s, const ClothDev* cl, const int* vcloth, const float4* pa, const float4* pb, int nv,
                float* y);
 
@@ -246,7 +246,12 @@ void radix_sort_pairs(hipStream_t s, uint32_t* k, uint32_t* v, uint32_t* k2, uin
 void launch_advect(hipStream_t s, const ClothDev* cloths, const int* vcloth, const float4* x,
                    float4* v, float4* p, int nv_begin, int nv_end, float dt, int skip_whole);
 void launch_integrate(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* x, float4* v,
-                      const float4* p, int nv_begin, int nv_end, float dt, int skip_whole);
+                      float4* p0, float4* p1, const int* pfinal, int pend, int nv_begin, int nv_end, float dt,
+                      int skip_whole);
+void launch_consolidate(hipStream_t s, const int* vcloth, float4* p0, float4* p1, const int* pfinal, int pend,
+                        int n);
+void launch_final_buffers(hipStream_t s, const ClothDev* cl, int ncl, int start, int launched, int ts,
+                          int iterations, const int* cap, int* pfinal);
 void launch_project_colour(hipStream_t s, const int4* dist, int d_begin, int d_end,
                            const int4* bend_idx, const float2* bend_rk, int b_begin, int b_end,
                            const float* ktab_d, const float* ktab_b, int kmax, int sweep, const int* cap,

@@ -51,23 +51,75 @@ void launch_advect(hipStream_t s, const ClothDev* cloths, const int* vcloth, con
 }
 
 // IntegrationGPUKernel (PBD_cuda.cu:1467-1482): v = (p - x)/dt ; x = p.
+// p of vertex i is read from buffer pfinal[cloth] (pend when pfinal is null)
+// and, when that is not pend, also stored into pend so that the current p
+// buffer holds every cloth's result.  skip_whole: single-tile cloths were
+// integrated by the tile kernel already (only their p is carried over).
 __global__ __launch_bounds__(256) void k_integrate(const ClothDev* __restrict__ cl,
                                                    const int* __restrict__ vcloth, float4* __restrict__ x,
-                                                   float4* __restrict__ v, const float4* __restrict__ p, int b,
-                                                   int e, float dt, int skip_whole) {
+                                                   float4* __restrict__ v, float4* __restrict__ p0,
+                                                   float4* __restrict__ p1, const int* __restrict__ pfinal, int pend,
+                                                   int b, int e, float dt, int skip_whole) {
     int i = b + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= e) return;
-    if (skip_whole && cl[vcloth[i]].whole) return;
-    float4 P = p[i], X = x[i];
-    v[i] = make_float4((P.x - X.x) / dt, (P.y - X.y) / dt, (P.z - X.z) / dt, 0.0f);
-    x[i] = make_float4(P.x, P.y, P.z, X.w);
+    const int c = vcloth[i];
+    const int src = pfinal ? pfinal[c] : pend;
+    const float4 P = (src ? p1 : p0)[i];
+    if (!(skip_whole && cl[c].whole)) {
+        const float4 X = x[i];
+        v[i] = make_float4((P.x - X.x) / dt, (P.y - X.y) / dt, (P.z - X.z) / dt, 0.0f);
+        x[i] = make_float4(P.x, P.y, P.z, X.w);
+    }
+    if (src != pend) (pend ? p1 : p0)[i] = P;
 }
 
-void launch_integrate(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* x, float4* v,
-                      const float4* p, int b, int e, float dt, int skip_whole) {
+void launch_integrate(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* x, float4* v, float4* p0,
+                      float4* p1, const int* pfinal, int pend, int b, int e, float dt, int skip_whole) {
     if (e <= b) return;
-    hipLaunchKernelGGL(k_integrate, dim3((e - b + 255) / 256), dim3(256), 0, s, cloths, vcloth, x, v, p, b, e, dt,
-                       skip_whole);
+    hipLaunchKernelGGL(k_integrate, dim3((e - b + 255) / 256), dim3(256), 0, s, cloths, vcloth, x, v, p0, p1, pfinal,
+                       pend, b, e, dt, skip_whole);
+    PBD_HIP(hipGetLastError());
+}
+
+// Copies each cloth's projected p from buffer pfinal[cloth] into pend.
+__global__ __launch_bounds__(256) void k_consolidate(const int* __restrict__ vcloth, float4* __restrict__ p0,
+                                                     float4* __restrict__ p1, const int* __restrict__ pfinal,
+                                                     int pend, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int src = pfinal[vcloth[i]];
+    if (src != pend) (pend ? p1 : p0)[i] = (src ? p1 : p0)[i];
+}
+
+void launch_consolidate(hipStream_t s, const int* vcloth, float4* p0, float4* p1, const int* pfinal, int pend,
+                        int n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_consolidate, dim3((n + 255) / 256), dim3(256), 0, s, vcloth, p0, p1, pfinal, pend, n);
+    PBD_HIP(hipGetLastError());
+}
+
+// Which p buffer holds each cloth after a tile projection that started in
+// buffer `start`: single-tile cloths wrote start ^ 1; multi-tile cloths
+// flipped once per chunk they ran -- ceil(min(cap, iterations) / ts) chunks
+// when a per-cloth budget ends them early without copying, else every one
+// of the `launched` chunks.
+__global__ void k_final_buffers(const ClothDev* __restrict__ cl, int ncl, int start, int launched, int ts,
+                                int iterations, const int* __restrict__ cap, int* __restrict__ pfinal) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncl) return;
+    int run = launched;
+    if (cap) {
+        const int k = min(cap[c], iterations);
+        run = (k + ts - 1) / ts;
+    }
+    pfinal[c] = cl[c].whole ? (start ^ 1) : (start ^ (run & 1));
+}
+
+void launch_final_buffers(hipStream_t s, const ClothDev* cl, int ncl, int start, int launched, int ts,
+                          int iterations, const int* cap, int* pfinal) {
+    if (ncl <= 0) return;
+    hipLaunchKernelGGL(k_final_buffers, dim3((ncl + 255) / 256), dim3(256), 0, s, cl, ncl, start, launched, ts,
+                       iterations, cap, pfinal);
     PBD_HIP(hipGetLastError());
 }
 
@@ -298,17 +350,22 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     const int T = blockDim.x, tid = threadIdx.x;
     const int* vl = verts + td.load_off;
 
-    // per-cloth sweep budget (warm start: short when the prediction was kept)
+    // per-cloth sweep budget (warm start: short when the prediction was kept);
+    // a cloth whose budget is spent is left where it is (its final buffer is
+    // tracked per cloth), one that met the tolerance is carried forward
     int nsweeps = a.nsweeps;
     if (sweep_cap) nsweeps = min(nsweeps, sweep_cap[td.cloth] - a.sweep_base);
     const double tol = cd.tol >= 0.0 ? cd.tol : a.tol;
-    bool skip = !td.whole && nsweeps <= 0;
+    bool carry = false;
     if (!td.whole && a.chunk > 0 && tol >= 0.0) {
         double prev = __longlong_as_double((long long)res_slots[(size_t)(a.chunk - 1) * ncl + td.cloth]);
-        if (prev <= tol) skip = true;
+        if (prev <= tol) carry = true;
     }
-    if (skip) {
-        // this cloth is done: carry its interior into the other buffer
+    if (!td.whole && nsweeps <= 0) {
+        if (a.copy_spent) carry = true;
+        else return;
+    }
+    if (carry) {
         for (int i = tid; i < td.n_int; i += T) p_out[vl[i]] = p_in[vl[i]];
         return;
     }

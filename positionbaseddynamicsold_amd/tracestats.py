@@ -1,0 +1,119 @@
+"""Summaries of rocprofv3 CSV output of bench.py.
+
+    python -m positionbaseddynamicsold_amd.tracestats TRACE.csv [--steps K]
+        [--fetch FETCH_counter_collection.csv --write WRITE_counter_collection.csv]
+
+TRACE.csv is `rocprofv3 --kernel-trace --output-format csv` output
+(`*_kernel_trace.csv`).  Prints per-kernel statistics for the whole run, then
+for the two eager passes bench.py times with HIP events after its headline
+runs -- K warm-started substeps followed by K plain substeps, each preceded
+by one untimed substep, just before the copy-kernel calibration -- so the
+tile kernel's average launch duration can be compared with the bench's own
+figure.  Substeps are delimited by the batch-wide integrate launches.
+
+With counter files from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
+runs of the same command (kernel filter "k_tiles|k_copy|k_integrate"), it also
+prints the tile kernel's HBM traffic per launch in those passes, scaled by
+what the counters read for the calibration copy (1 GiB read + 1 GiB written
+per launch).
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import statistics
+from collections import defaultdict
+
+COPY_BYTES = 1 << 30  # bench.py calibrates with pbd_bench_copy(1 GiB)
+
+
+def short(name: str) -> str:
+    return name.split("(")[0].replace("pbd::", "")
+
+
+def load_trace(path):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    return [(short(r["Kernel_Name"]), int(r["Start_Timestamp"]), int(r["End_Timestamp"]), int(r["Grid_Size_X"]))
+            for r in rows]
+
+
+def passes(seq, steps, name_of=lambda e: e[0], grid_of=lambda e: e[3]):
+    """Index ranges [a, b) of the warm and plain profiled passes in seq."""
+    copies = [i for i, e in enumerate(seq) if name_of(e) == "k_copy"]
+    big = max(grid_of(seq[i]) for i in copies)
+    c0 = max(i for i in copies if grid_of(seq[i]) == big)
+    while c0 > 0 and name_of(seq[c0 - 1]) == "k_copy" and grid_of(seq[c0 - 1]) == big:
+        c0 -= 1
+    ints = [i for i in range(c0) if name_of(seq[i]) == "k_integrate"]
+    top = max(grid_of(seq[i]) for i in ints)
+    ints = [i for i in ints if grid_of(seq[i]) == top]
+    need = 2 * (steps + 1)
+    if len(ints) < need:
+        raise SystemExit("trace does not hold the two profiled passes")
+    ends = ints[-need:]
+    # pass = substeps 1..steps of each group of steps + 1 (the first is untimed)
+    warm = (ends[0] + 1, ends[steps] + 1)
+    plain = (ends[steps + 1] + 1, ends[2 * steps + 1] + 1)
+    return warm, plain, c0
+
+
+def table(events, title):
+    per = defaultdict(list)
+    for n, s, e, _ in events:
+        per[n].append((e - s) / 1e3)
+    tot = sum(sum(v) for v in per.values())
+    print(f"\n{title}")
+    print(f"{'kernel':28s} {'calls':>7s} {'total ms':>10s} {'avg us':>10s} {'share':>6s}")
+    for n, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        print(f"{n:28s} {len(v):7d} {sum(v) / 1e3:10.3f} {statistics.fmean(v):10.1f} {100 * sum(v) / tot:5.1f}%")
+    return per
+
+
+def counters(path):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    out = []
+    for r in rows:
+        grid = int(r.get("Grid_Size", r.get("Grid_Size_X", 0)) or 0)
+        out.append((short(r["Kernel_Name"]), float(r["Counter_Value"]), grid))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("trace")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    a = ap.parse_args()
+    ev = load_trace(a.trace)
+    table(ev, "whole run (all launches)")
+    (wa, wb), (pa, pb), _ = passes(ev, a.steps)
+    w = table(ev[wa:wb], f"profiled warm-started pass ({a.steps} substeps)")
+    p = table(ev[pa:pb], f"profiled plain pass ({a.steps} substeps)")
+    for label, per in (("warm", w), ("plain", p)):
+        t = per.get("k_tiles", [])
+        if t:
+            print(f"k_tiles {label}: {len(t)} launches, average {statistics.fmean(t) / 1e3:.4f} ms")
+    if a.fetch and a.write:
+        for label, rng in (("warm", 0), ("plain", 1)):
+            vals = []
+            for path in (a.fetch, a.write):
+                seq = counters(path)
+                (wa2, wb2), (pa2, pb2), c0 = passes(seq, a.steps, grid_of=lambda e: e[2])
+                cal = statistics.fmean(v for n, v, g in seq[c0:] if n == "k_copy")
+                a2, b2 = (wa2, wb2) if rng == 0 else (pa2, pb2)
+                tiles = [v for n, v, _ in seq[a2:b2] if n == "k_tiles"]
+                vals.append((statistics.fmean(tiles), cal))
+            (f, fc), (wr, wc) = vals
+            scale_f = COPY_BYTES / (fc * 1024.0)
+            scale_w = COPY_BYTES / (wc * 1024.0)
+            traffic = f * 1024.0 * scale_f + wr * 1024.0 * scale_w
+            print(f"k_tiles {label}: FETCH_SIZE {f:.0f} KiB, WRITE_SIZE {wr:.0f} KiB per launch; calibration copy "
+                  f"reads {fc:.0f} KiB / writes {wc:.0f} KiB for 1 GiB each (scale {scale_f:.4f}, {scale_w:.4f}); "
+                  f"HBM traffic {traffic / 1e9:.4f} GB per launch")
+
+
+if __name__ == "__main__":
+    main()

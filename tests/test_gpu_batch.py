@@ -35,6 +35,7 @@ def assert_same(a, b, ids):
     for i in ids:
         assert np.array_equal(a.get(i, capi.STATE_X), b.get(i, capi.STATE_X)), i
         assert np.array_equal(a.get(i, capi.STATE_V), b.get(i, capi.STATE_V)), i
+        assert np.array_equal(a.get(i, capi.STATE_P), a.get(i, capi.STATE_X)), i
 
 
 @pytest.mark.parametrize("path", [capi.PATH_AUTO, capi.PATH_GLOBAL])

@@ -240,6 +240,54 @@ def test_zero_prediction_is_kept_and_changes_nothing():
         b.close()
 
 
+def host_decision(p, d, pairs, L):
+    """The fallback comparison as the kernel evaluates it: FP32 per-prim
+    |d - L| / L, FP64 max, squares clamped to 1 and summed in 2^-40 fixed point."""
+    def rel(P):
+        e = (P[pairs[:, 0]] - P[pairs[:, 1]]).astype(np.float32)
+        dist = np.sqrt(e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1] + e[:, 2] * e[:, 2])
+        return np.abs(dist - L) / L
+
+    r0, r1 = rel(p), rel((p + d).astype(np.float32))
+    sq = lambda r: int(np.sum(np.floor(np.minimum(r * r, np.float32(1.0)).astype(np.float64) * 2.0 ** 40)))
+    return int(r1.max() <= r0.max()), int(sq(r1) <= sq(r0))
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_fallback_decision_matches_host(orc, rule):
+    # a nearly-zero output layer gives small predictions that help some cloths
+    # and hurt others; the kept set must equal the host evaluation of the rule
+    from positionbaseddynamicsold_amd import scenes
+    seeds = range(12)
+    a, ids = config4_context(seeds)
+    b, _ = config4_context(seeds)
+    try:
+        rng = np.random.default_rng(3)
+        w = rand_params(5, 0.3)
+        w[5632:5827] = (rng.standard_normal(195) * 2e-3).astype(np.float32)
+        for c in (a, b):
+            c.ws_set_params(w)
+            c.set_solver(dt=scenes.DT, iterations=20, warm_rule=rule)
+            c.step(20)
+        b.advect(scenes.DT)
+        a.ws_enable(True)
+        a.step(1)
+        kept = []
+        for i, sd in zip(ids, seeds):
+            spec = scenes.config4(sd)
+            p = b.get(i, capi.STATE_P)
+            d = a.ws_last_delta(i)
+            _, idx, rest = a.prims(i)
+            pairs = idx.reshape(-1, 2)
+            want = host_decision(p, d, pairs, rest)[rule]
+            assert a.ws_last_used(i) == want, (i, rule)
+            kept.append(want)
+        assert 0 < sum(kept) < len(kept), kept  # both outcomes occur
+    finally:
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("path", [capi.PATH_AUTO, capi.PATH_GLOBAL])
 def test_sweep_budget_follows_the_fallback_rule(path):
     # warm budget 5, plain budget 20: a discarded prediction (corrupted
@@ -266,6 +314,8 @@ def test_sweep_budget_follows_the_fallback_rule(path):
             for i, j in zip(ids_a, ids_b):
                 assert np.array_equal(a.get(i), b.get(j))
                 assert np.array_equal(a.get(i, capi.STATE_V), b.get(j, capi.STATE_V))
+                # the current p buffer holds every cloth's result after the substep
+                assert np.array_equal(b.get(j, capi.STATE_P), b.get(j))
         finally:
             a.close()
             b.close()
