@@ -83,6 +83,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--unbatched-steps", type=int, default=3, help="timed substeps of the unbatched plain solver")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (use --tile-sweeps, 1024 threads)")
+    ap.add_argument("--save-network", metavar="NPZ", help="write the batch network and the settings that "
+                    "positionbaseddynamicsold_amd.tilework needs to repeat the profiled passes")
     ap.add_argument("--warm-rule", type=int, default=1, choices=(0, 1),
                     help="fallback rule of the headline run: 0 max |d-L|/L, 1 RMS (the other is reported too)")
     return ap.parse_args()
@@ -282,6 +284,12 @@ def main():
     other = budget(1 - a.warm_rule)
     head = budget(a.warm_rule)
     warm_k, r_warm, used_frac = head["budget_kept"], head["residual"], head["kept_fraction"]
+    if a.save_network:
+        np.savez(a.save_network, params=ctx.ws_params(), norm=ctx.ws_norm(), count=a.count, seed=a.seed,
+                 warmup=a.warmup, collect_substeps=a.collect_substeps, target_sweeps=a.target_sweeps, settle=2,
+                 plain_sweeps=PLAIN_SWEEPS, warm_k=warm_k, warm_rule=a.warm_rule,
+                 tile_sweeps=tuned["tile_sweeps"], tile_max_verts=tuned["tile_max_verts"],
+                 block_threads=tuned["block_threads"])
 
     def timed(ws, steps, warm=0):
         ctx.snapshot(restore=True)

@@ -576,7 +576,7 @@ void ws_apply_prediction(pbd_ctx* c) {
     ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, c->ws.nbr.ptr, c->ws.nbr_rest.ptr, c->nv_total,
              c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr);
     launch_fill_u64(s, c->ws.rs.ptr, 4 * (size_t)ncl, 0ull);
-    ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.rs.ptr, ncl);
+    ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.rs.ptr, ncl, c->sp.warm_rule);
     int full = c->sp.iterations;
     int warm = c->sp.warm_iterations > 0 ? std::min(c->sp.warm_iterations, full) : full;
     ws_decide(s, c->ws.rs.ptr, ncl, c->sp.warm_rule, warm, full, c->ws.cap.ptr, c->ws.keep.ptr);

@@ -580,14 +580,19 @@ __device__ __forceinline__ unsigned long long sq_fixed(float r) {
     return (unsigned long long)((double)q * 1099511627776.0);  // 2^40
 }
 
+template <int RULE>
 __device__ __forceinline__ void flush_resid(unsigned long long* rs, int ncl, int c, double a, double b,
                                             unsigned long long sa, unsigned long long sb) {
-    atomicMax(&rs[c], (unsigned long long)__double_as_longlong(a));
-    atomicMax(&rs[ncl + c], (unsigned long long)__double_as_longlong(b));
-    atomicAdd(&rs[2 * ncl + c], sa);
-    atomicAdd(&rs[3 * ncl + c], sb);
+    if (RULE == 0) {
+        atomicMax(&rs[c], (unsigned long long)__double_as_longlong(a));
+        atomicMax(&rs[ncl + c], (unsigned long long)__double_as_longlong(b));
+    } else {
+        atomicAdd(&rs[2 * ncl + c], sa);
+        atomicAdd(&rs[3 * ncl + c], sb);
+    }
 }
 
+template <int RULE>  // 0: max only, 1: squared sums only (what the fallback rule reads)
 __global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist, int n, const float4* __restrict__ p,
                                                    const float4* __restrict__ delta,
                                                    unsigned long long* __restrict__ rs, int ncl) {
@@ -600,7 +605,7 @@ __global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist
         if (j >= n) break;
         int4 q = dist[j];
         if (q.w != cl) {
-            if (cl >= 0) flush_resid(rs, ncl, cl, a, b, sa, sb);
+            if (cl >= 0) flush_resid<RULE>(rs, ncl, cl, a, b, sa, sb);
             cl = q.w;
             a = b = 0.0;
             sa = sb = 0;
@@ -611,23 +616,29 @@ __global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist
         float4 Q0 = make_float4(P0.x + D0.x, P0.y + D0.y, P0.z + D0.z, 0.0f);
         float4 Q1 = make_float4(P1.x + D1.x, P1.y + D1.y, P1.z + D1.z, 0.0f);
         const float r1 = rel_res(Q0, Q1, L);
-        a = fmax(a, (double)r0);
-        b = fmax(b, (double)r1);
-        sa += sq_fixed(r0);
-        sb += sq_fixed(r1);
+        if (RULE == 0) {
+            a = fmax(a, (double)r0);
+            b = fmax(b, (double)r1);
+        } else {
+            sa += sq_fixed(r0);
+            sb += sq_fixed(r1);
+        }
     }
     int c0 = __shfl(cl, 0);
     unsigned long long same = __ballot(cl == c0 || cl < 0);
     if (same == ~0ull) {
         for (int off = 32; off > 0; off >>= 1) {
-            a = fmax(a, __shfl_xor(a, off));
-            b = fmax(b, __shfl_xor(b, off));
-            sa += __shfl_xor(sa, off);
-            sb += __shfl_xor(sb, off);
+            if (RULE == 0) {
+                a = fmax(a, __shfl_xor(a, off));
+                b = fmax(b, __shfl_xor(b, off));
+            } else {
+                sa += __shfl_xor(sa, off);
+                sb += __shfl_xor(sb, off);
+            }
         }
-        if ((threadIdx.x & 63) == 0 && c0 >= 0) flush_resid(rs, ncl, c0, a, b, sa, sb);
+        if ((threadIdx.x & 63) == 0 && c0 >= 0) flush_resid<RULE>(rs, ncl, c0, a, b, sa, sb);
     } else if (cl >= 0) {
-        flush_resid(rs, ncl, cl, a, b, sa, sb);
+        flush_resid<RULE>(rs, ncl, cl, a, b, sa, sb);
     }
 }
 
@@ -750,10 +761,13 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
 
 
 void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* rs,
-               int ncl) {
+               int ncl, int rule) {
     if (n <= 0) return;
     int blocks = (int)(((long long)n + 256 * RESID_PER_THREAD - 1) / (256 * RESID_PER_THREAD));
-    hipLaunchKernelGGL(k_ws_resid2, dim3(blocks), dim3(256), 0, s, dist, n, p, delta, rs, ncl);
+    if (rule == 1)
+        hipLaunchKernelGGL(k_ws_resid2<1>, dim3(blocks), dim3(256), 0, s, dist, n, p, delta, rs, ncl);
+    else
+        hipLaunchKernelGGL(k_ws_resid2<0>, dim3(blocks), dim3(256), 0, s, dist, n, p, delta, rs, ncl);
     PBD_HIP(hipGetLastError());
 }
 

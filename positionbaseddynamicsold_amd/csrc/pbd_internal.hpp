@@ -213,7 +213,7 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
 size_t ws_feature_bytes(int nv);
 void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm);
 void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* rs,
-               int ncl);
+               int ncl, int rule);
 void ws_decide(hipStream_t s, const unsigned long long* rs, int ncl, int rule, int warm_sweeps, int full_sweeps,
                int* cap, int* keep);
 void ws_apply(hipStream_t s, const int* vcloth, float4* p, const float4* delta, const int* keep, int nv);

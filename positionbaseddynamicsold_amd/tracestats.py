@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 CSV output of bench.py.
 
-    python -m positionbaseddynamicsold_amd.tracestats TRACE.csv [--steps K]
+    python -m positionbaseddynamicsold_amd.tracestats [TRACE.csv] [--steps K]
         [--fetch FETCH_counter_collection.csv --write WRITE_counter_collection.csv]
 
 TRACE.csv is `rocprofv3 --kernel-trace --output-format csv` output
@@ -12,7 +12,8 @@ tile kernel's average launch duration can be compared with the bench's own
 figure.  Substeps are delimited by the batch-wide integrate launches.
 
 With counter files from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
-runs of the same command (kernel filter "k_tiles|k_copy|k_integrate"), it also
+runs (kernel filter "k_tiles|k_copy|k_integrate") of bench.py or of
+positionbaseddynamicsold_amd.tilework, which repeats the same two passes, it
 prints the tile kernel's HBM traffic per launch in those passes, scaled by
 what the counters read for the calibration copy (1 GiB read + 1 GiB written
 per launch).
@@ -82,20 +83,21 @@ def counters(path):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("trace")
+    ap.add_argument("trace", nargs="?")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     a = ap.parse_args()
-    ev = load_trace(a.trace)
-    table(ev, "whole run (all launches)")
-    (wa, wb), (pa, pb), _ = passes(ev, a.steps)
-    w = table(ev[wa:wb], f"profiled warm-started pass ({a.steps} substeps)")
-    p = table(ev[pa:pb], f"profiled plain pass ({a.steps} substeps)")
-    for label, per in (("warm", w), ("plain", p)):
-        t = per.get("k_tiles", [])
-        if t:
-            print(f"k_tiles {label}: {len(t)} launches, average {statistics.fmean(t) / 1e3:.4f} ms")
+    if a.trace:
+        ev = load_trace(a.trace)
+        table(ev, "whole run (all launches)")
+        (wa, wb), (pa, pb), _ = passes(ev, a.steps)
+        w = table(ev[wa:wb], f"profiled warm-started pass ({a.steps} substeps)")
+        p = table(ev[pa:pb], f"profiled plain pass ({a.steps} substeps)")
+        for label, per in (("warm", w), ("plain", p)):
+            t = per.get("k_tiles", [])
+            if t:
+                print(f"k_tiles {label}: {len(t)} launches, average {statistics.fmean(t) / 1e3:.4f} ms")
     if a.fetch and a.write:
         for label, rng in (("warm", 0), ("plain", 1)):
             vals = []
