@@ -55,8 +55,25 @@ PBD_HD void project_distance(float4& a, float4& b, float rest, float k) {
     float inv = 1.0f / d;
     float rx = vx * inv, ry = vy * inv, rz = vz * inv;
     float c = d - rest;
-    float s0 = (-w0 / ws) * c;
-    float s1 = (w1 / ws) * c;
+    // -w0/ws and w1/ws; equal or zero weights give exactly representable
+    // quotients (-1/2, 1/2, -0, 1, -1, 0), which IEEE division returns
+    // exactly, so the shortcut is bit-identical to dividing
+    float q0, q1;
+    if (w0 == w1) {
+        q0 = -0.5f;
+        q1 = 0.5f;
+    } else if (w0 == 0.0f) {
+        q0 = -0.0f;
+        q1 = 1.0f;
+    } else if (w1 == 0.0f) {
+        q0 = -1.0f;
+        q1 = 0.0f;
+    } else {
+        q0 = -w0 / ws;
+        q1 = w1 / ws;
+    }
+    float s0 = q0 * c;
+    float s1 = q1 * c;
     a.x = a.x + (s0 * rx) * k;
     a.y = a.y + (s0 * ry) * k;
     a.z = a.z + (s0 * rz) * k;

@@ -323,6 +323,32 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     return m;
 }
 
+constexpr int PREFETCH = 8;       // distance records a thread loads ahead per colour phase
+constexpr int BEND_PREFETCH = 4;  // bending records a thread loads ahead per colour phase
+constexpr int VERT_PREFETCH = 4;  // vertices a thread gathers ahead when loading or storing a tile
+
+// This thread's max relative residual over the tile's owned distance prims
+// (records fetched PREFETCH at a time).
+__device__ __forceinline__ double own_residual(const float4* P, const uint32_t* __restrict__ own,
+                                               const float* __restrict__ rest, int n) {
+    const int T = blockDim.x;
+    double m = 0.0;
+    for (int j0 = threadIdx.x; j0 < n; j0 += PREFETCH * T) {
+        uint32_t pr[PREFETCH];
+        float rl[PREFETCH];
+#pragma unroll
+        for (int u = 0; u < PREFETCH; u++) {
+            const int j = j0 + u * T;
+            pr[u] = j < n ? own[j] : 0u;
+            rl[u] = j < n ? rest[j] : 1.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < PREFETCH; u++)
+            if (j0 + u * T < n) m = fmax(m, rel_residual(P[pr[u] & 0xffffu], P[pr[u] >> 16], rl[u]));
+    }
+    return m;
+}
+
 __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
                                                 const TileDev* __restrict__ tiles,
                                                 const int* __restrict__ verts,
@@ -365,18 +391,48 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
         if (a.copy_spent) carry = true;
         else return;
     }
+    // Gathers and scatters below go in batches of VERT_PREFETCH per thread:
+    // indices first, then the dependent accesses, so each batch waits for
+    // one memory round trip rather than one per vertex.
     if (carry) {
-        for (int i = tid; i < td.n_int; i += T) p_out[vl[i]] = p_in[vl[i]];
+        for (int i0 = tid; i0 < td.n_int; i0 += VERT_PREFETCH * T) {
+            int g[VERT_PREFETCH];
+            float4 q[VERT_PREFETCH];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < td.n_int ? vl[i0 + u * T] : -1;
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) q[u] = p_in[max(g[u], 0)];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) p_out[g[u]] = q[u];
+        }
         return;
     }
-    if (td.whole && a.fused) {
-        for (int i = tid; i < td.n_load; i += T) {
-            int g = vl[i];
-            float4 V;
-            P[i] = advect_one(cd, x[g], v[g], a.dt, &V);
+    for (int i0 = tid; i0 < td.n_load; i0 += VERT_PREFETCH * T) {
+        int g[VERT_PREFETCH];
+#pragma unroll
+        for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < td.n_load ? vl[i0 + u * T] : -1;
+        if (td.whole && a.fused) {
+            float4 X[VERT_PREFETCH], V[VERT_PREFETCH];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) {
+                X[u] = x[max(g[u], 0)];
+                V[u] = v[max(g[u], 0)];
+            }
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) {
+                    float4 Vn;
+                    P[i0 + u * T] = advect_one(cd, X[u], V[u], a.dt, &Vn);
+                }
+        } else {
+            float4 q[VERT_PREFETCH];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) q[u] = p_in[max(g[u], 0)];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) P[i0 + u * T] = q[u];
         }
-    } else {
-        for (int i = tid; i < td.n_load; i += T) P[i] = p_in[vl[i]];
     }
     __syncthreads();
 
@@ -396,23 +452,51 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
         float kd = ktab_d[cd.ktab + ks];
         float kb = ktab_b[cd.ktab + ks];
         for (int c = 0; c < C; c++) {
-            for (int j = dc[c] + tid; j < dc[c + 1]; j += T) {
-                uint32_t pr = D[j];
-                int i0 = pr & 0xffffu, i1 = pr >> 16;
-                float4 p0 = P[i0], p1 = P[i1];
-                project_distance(p0, p1, DR[j], kd);
-                P[i0] = p0;
-                P[i1] = p1;
+            // the prim records of a phase are fetched from global memory
+            // in batches of PREFETCH per thread before any is processed, so
+            // the phase waits for one round trip instead of one per prim
+            const int je = dc[c + 1];
+            for (int j0 = dc[c] + tid; j0 < je; j0 += PREFETCH * T) {
+                uint32_t pr[PREFETCH];
+                float rl[PREFETCH];
+#pragma unroll
+                for (int u = 0; u < PREFETCH; u++) {
+                    const int j = j0 + u * T;
+                    pr[u] = j < je ? D[j] : 0u;
+                    rl[u] = j < je ? DR[j] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < PREFETCH; u++) {
+                    if (j0 + u * T >= je) break;
+                    const int i0 = pr[u] & 0xffffu, i1 = pr[u] >> 16;
+                    float4 p0 = P[i0], p1 = P[i1];
+                    project_distance(p0, p1, rl[u], kd);
+                    P[i0] = p0;
+                    P[i1] = p1;
+                }
             }
-            for (int j = bc[c] + tid; j < bc[c + 1]; j += T) {
-                uint32_t w0 = B[2 * j], w1 = B[2 * j + 1];
-                int i0 = w0 & 0xffffu, i1 = w0 >> 16, i2 = w1 & 0xffffu, i3 = w1 >> 16;
-                float4 q0 = P[i0], q1 = P[i1], q2 = P[i2], q3 = P[i3];
-                project_bending(q0, q1, q2, q3, BR[j], kb);
-                P[i0] = q0;
-                P[i1] = q1;
-                P[i2] = q2;
-                P[i3] = q3;
+            const int be = bc[c + 1];
+            for (int j0 = bc[c] + tid; j0 < be; j0 += BEND_PREFETCH * T) {
+                uint2 bw[BEND_PREFETCH];
+                float br[BEND_PREFETCH];
+#pragma unroll
+                for (int u = 0; u < BEND_PREFETCH; u++) {
+                    const int j = j0 + u * T;
+                    bw[u] = j < be ? make_uint2(B[2 * j], B[2 * j + 1]) : make_uint2(0u, 0u);
+                    br[u] = j < be ? BR[j] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < BEND_PREFETCH; u++) {
+                    if (j0 + u * T >= be) break;
+                    const int i0 = bw[u].x & 0xffffu, i1 = bw[u].x >> 16, i2 = bw[u].y & 0xffffu,
+                              i3 = bw[u].y >> 16;
+                    float4 q0 = P[i0], q1 = P[i1], q2 = P[i2], q3 = P[i3];
+                    project_bending(q0, q1, q2, q3, br[u], kb);
+                    P[i0] = q0;
+                    P[i1] = q1;
+                    P[i2] = q2;
+                    P[i3] = q3;
+                }
             }
             __syncthreads();
         }
@@ -426,37 +510,37 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
         }
         sw++;
         if (td.whole && tol >= 0.0 && sw % a.check_every == 0) {
-            double m = 0.0;
-            for (int j = tid; j < td.n_own; j += T) {
-                uint32_t pr = own[td.own_off + j];
-                m = fmax(m, rel_residual(P[pr & 0xffffu], P[pr >> 16], own_rest[td.own_off + j]));
-            }
-            resid = block_max(m, red);
+            resid = block_max(own_residual(P, own + td.own_off, own_rest + td.own_off, td.n_own), red);
             if (resid <= tol) break;
         }
     }
-    if (!td.whole || resid < 0.0 || (sw % a.check_every) != 0) {
-        double m = 0.0;
-        for (int j = tid; j < td.n_own; j += T) {
-            uint32_t pr = own[td.own_off + j];
-            m = fmax(m, rel_residual(P[pr & 0xffffu], P[pr >> 16], own_rest[td.own_off + j]));
-        }
-        resid = block_max(m, red);
-    }
+    if (!td.whole || resid < 0.0 || (sw % a.check_every) != 0)
+        resid = block_max(own_residual(P, own + td.own_off, own_rest + td.own_off, td.n_own), red);
     if (tid == 0) {
         atomicMax(&res_slots[(size_t)a.chunk * ncl + td.cloth], (unsigned long long)__double_as_longlong(resid));
         atomicMax(&sweeps_out[td.cloth], a.sweep_base + sw);
     }
-    if (td.whole && a.fused) {
-        for (int i = tid; i < td.n_int; i += T) {
-            int g = vl[i];
-            float4 Q = P[i], X = x[g];
-            v[g] = make_float4((Q.x - X.x) / a.dt, (Q.y - X.y) / a.dt, (Q.z - X.z) / a.dt, 0.0f);
-            x[g] = make_float4(Q.x, Q.y, Q.z, X.w);
-            p_out[g] = Q;
+    for (int i0 = tid; i0 < td.n_int; i0 += VERT_PREFETCH * T) {
+        int g[VERT_PREFETCH];
+#pragma unroll
+        for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < td.n_int ? vl[i0 + u * T] : -1;
+        if (td.whole && a.fused) {
+            float4 X[VERT_PREFETCH];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) X[u] = x[max(g[u], 0)];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) {
+                    const float4 Q = P[i0 + u * T];
+                    v[g[u]] = make_float4((Q.x - X[u].x) / a.dt, (Q.y - X[u].y) / a.dt, (Q.z - X[u].z) / a.dt, 0.0f);
+                    x[g[u]] = make_float4(Q.x, Q.y, Q.z, X[u].w);
+                    p_out[g[u]] = Q;
+                }
+        } else {
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) p_out[g[u]] = P[i0 + u * T];
         }
-    } else {
-        for (int i = tid; i < td.n_int; i += T) p_out[vl[i]] = P[i];
     }
 }
 
