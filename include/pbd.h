@@ -208,6 +208,14 @@ PBD_API int pbd_sh_pairs(pbd_ctx* ctx, long long cap, int64_t* keys, long long* 
  * (CCD.cpp:1522-1587): x = start-of-step positions, p = predicted. */
 PBD_API int pbd_ccd_detect(pbd_ctx* ctx, const pbd_hash_desc* desc, const float* x, const float* p, int nv,
                            const int* tris, int nt, float thickness, long long* ncontacts);
+/* Detection with the swept-box candidate set (broadphase 1 of
+ * pbd_collision_params); contacts equal CCD_N2's (CCD.cpp:1040-1098) over
+ * all (v, f), v a vertex of some triangle and not in f.  No reference
+ * counterpart for the candidate set itself. */
+PBD_API int pbd_ccd_detect_swept(pbd_ctx* ctx, const pbd_hash_desc* desc, const float* x, const float* p, int nv,
+                                 const int* tris, int nt, float thickness, long long* npairs, long long* ncontacts);
+/* Candidate pairs of the last detection, ascending keys (v << 32 | f). */
+PBD_API int pbd_ccd_get_pairs(pbd_ctx* ctx, long long cap, int64_t* keys, long long* count);
 /* ContactData (CCD_Basic.h:12-31) of the last detection, ascending (v, f). */
 PBD_API int pbd_ccd_get_contacts(pbd_ctx* ctx, long long cap, int* vf, float* t, float* n, double* w);
 /* CollisionResolveNew(meshTopol, prdPBuffer, contactData, iterations, thickness)
@@ -225,6 +233,17 @@ typedef struct {
     float thickness;
     int resolve_passes;
     int detect_every;
+    /* The response skips a face whose unnormalised normal has
+     * |(p2 - p1) x (p3 - p1)|^2 <= min_normal2 (m^4).  0 selects the
+     * reference's fixed 1e-6 (VFResolveTest, CCD.cpp:2008), which also skips
+     * every well-shaped face with edges below about 3.4 cm. */
+    float min_normal2;
+    /* Candidate pairs: 0 = the reference's centroid cells (FindNeighbors,
+     * SpatialHashSystem.cpp:134-187), which misses faces that extend more
+     * than about a cell from their centroid; 1 = swept boxes, whose contacts
+     * equal the all-pairs test (CCD_N2, CCD.cpp:1040-1098) at any cell size
+     * (the cell only sets the cost).  No reference counterpart for 1. */
+    int broadphase;
 } pbd_collision_params;
 PBD_API int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* params);
 PBD_API int pbd_last_contacts(pbd_ctx* ctx, long long* ncontacts, long long* npairs);

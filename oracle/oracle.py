@@ -91,7 +91,7 @@ def lib() -> C.CDLL:
             "orc_solve_cubic": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
             "orc_barycentric": (None, [_fp, _fp, _fp, _fp, _fp]),
             "orc_vf_ccd": (C.c_int, [_fp] * 8 + [C.c_float, _fp, _fp, _dp]),
-            "orc_vf_resolve_test": (C.c_int, [_fp] * 8 + [C.c_float]),
+            "orc_vf_resolve_test": (C.c_int, [_fp] * 8 + [C.c_float, C.c_float]),
             "orc_vf_resolve": (None, [_fp] * 8 + [C.c_float]),
             "orc_gelu": (None, [C.c_double, _dp, _dp]),
             "orc_mlp_forward": (None, [_dp] * 7 + [C.c_int] + [_dp] * 5),
@@ -316,8 +316,14 @@ def _sig_collide(L):
     L.orc_ccd_detect.restype = C.c_longlong
     L.orc_ccd_detect.argtypes = [C.POINTER(OrcGrid), _fp, _fp, _ip, C.c_int, C.c_int, C.c_float, C.c_longlong,
                                  C.POINTER(OrcContact)]
+    L.orc_swept_pairs.restype = C.c_longlong
+    L.orc_swept_pairs.argtypes = [_fp, _fp, _ip, C.c_int, C.c_int, C.c_float, C.c_int, C.c_longlong, _i64p]
+    L.orc_ccd_detect_swept.restype = C.c_longlong
+    L.orc_ccd_detect_swept.argtypes = [_fp, _fp, _ip, C.c_int, C.c_int, C.c_float, C.c_int, C.c_longlong,
+                                       C.POINTER(OrcContact)]
     L.orc_ccd_resolve.restype = None
-    L.orc_ccd_resolve.argtypes = [C.POINTER(OrcContact), C.c_longlong, _ip, _fp, _fp, _fp, C.c_float, C.c_int]
+    L.orc_ccd_resolve.argtypes = [C.POINTER(OrcContact), C.c_longlong, _ip, _fp, _fp, _fp, C.c_float, C.c_int,
+                                  C.c_float]
 
 
 def grid(mode=0, centre=(0.0, 0.0, 0.0), size=(1, 1, 1), cell=(1.0, 1.0, 1.0), table_bits=16) -> OrcGrid:
@@ -380,6 +386,32 @@ def ccd_detect(g, x, p, tris, thickness):
     return arr, n
 
 
+def swept_pairs(x, p, tris, thickness, all_pairs=False):
+    """Swept-box candidate keys (v << 32 | f), ascending; all_pairs drops the box test."""
+    L = lib()
+    _sig_collide(L)
+    x = np.ascontiguousarray(x, np.float32)
+    p = np.ascontiguousarray(p, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    n = L.orc_swept_pairs(F(x), F(p), I(tris), len(tris), len(x), thickness, int(all_pairs), 0, None)
+    out = np.zeros(max(n, 1), np.int64)
+    L.orc_swept_pairs(F(x), F(p), I(tris), len(tris), len(x), thickness, int(all_pairs), n, out.ctypes.data_as(_i64p))
+    return out[:n]
+
+
+def ccd_detect_swept(x, p, tris, thickness, all_pairs=False):
+    """Narrowphase over the swept-box set (or over every pair)."""
+    L = lib()
+    _sig_collide(L)
+    x = np.ascontiguousarray(x, np.float32)
+    p = np.ascontiguousarray(p, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    n = L.orc_ccd_detect_swept(F(x), F(p), I(tris), len(tris), len(x), thickness, int(all_pairs), 0, None)
+    arr = (OrcContact * max(n, 1))()
+    L.orc_ccd_detect_swept(F(x), F(p), I(tris), len(tris), len(x), thickness, int(all_pairs), n, arr)
+    return arr, n
+
+
 def contacts_arrays(arr, n):
     vf = np.array([[arr[i].v, arr[i].f] for i in range(n)], np.int32).reshape(-1, 2)
     t = np.array([arr[i].t for i in range(n)], np.float32)
@@ -388,10 +420,11 @@ def contacts_arrays(arr, n):
     return vf, t, nn, w
 
 
-def ccd_resolve(arr, n, tris, x, p, w, thickness, passes):
+def ccd_resolve(arr, n, tris, x, p, w, thickness, passes, min_n2=1e-6):
+    """min_n2: degenerate-face bound of VFResolveTest (1e-6 in the reference)."""
     L = lib()
     _sig_collide(L)
     p = np.ascontiguousarray(p, np.float32).copy()
     L.orc_ccd_resolve(arr, n, I(np.ascontiguousarray(tris, np.int32)), F(np.ascontiguousarray(x, np.float32)),
-                      F(p), F(np.ascontiguousarray(w, np.float32)), thickness, passes)
+                      F(p), F(np.ascontiguousarray(w, np.float32)), thickness, passes, min_n2)
     return p

@@ -962,16 +962,18 @@ int orc_vf_ccd(const float* x0, const float* x1, const float* x2, const float* x
     return 0;
 }
 
-/* VFResolveTest  PBD/CCD.cpp:1996-2032 */
+/* VFResolveTest  PBD/CCD.cpp:1996-2032.  min_n2 is the degenerate-face
+ * bound on |(p2 - p1) x (p3 - p1)|^2, fixed at 1e-6 in the reference
+ * (PBD/CCD.cpp:2008). */
 int orc_vf_resolve_test(const float* x0, const float* x1, const float* x2, const float* x3,
                         const float* p0, const float* p1, const float* p2, const float* p3,
-                        float thickness) {
+                        float thickness, float min_n2) {
     float e1[3], e2[3], cr[3], n0[3], pn[3], rel[3];
     v3sub(x2, x1, e1); v3sub(x3, x1, e2); v3cross(e1, e2, cr); v3normalize(cr, n0);
     v3sub(x0, x1, rel);
     int dir = v3dot(rel, n0) >= 0;
     v3sub(p2, p1, e1); v3sub(p3, p1, e2); v3cross(e1, e2, cr);
-    if (v3dot(cr, cr) <= 1e-6) return 0;
+    if (v3dot(cr, cr) <= min_n2) return 0;
     v3normalize(cr, pn);
     float s = dir ? 1.0f : -1.0f;
     float sn[3] = {pn[0] * s, pn[1] * s, pn[2] * s};
@@ -1038,6 +1040,87 @@ long long orc_ccd_detect(const orc_grid* g, const float* x, const float* p, cons
     return nc;
 }
 
+/* Swept-box candidate set (new, no reference counterpart): (v, f) with v a
+ * vertex of some triangle, v not in f, and AABB(x_v, p_v) meeting the AABB
+ * of f's six start and end vertex positions widened per side by
+ * 4 thickness + 1e-6 |bound|.  Brute force over all (v, f), ascending keys
+ * (v << 32 | f).  mode 2 drops the box test: every (v, f), v not in f, as
+ * CCD_N2 (PBD/CCD.cpp:1040-1098) visits them. */
+static float widen_lo(float v, float m) { return v - (m + 1e-6f * fabsf(v)); }
+static float widen_hi(float v, float m) { return v + (m + 1e-6f * fabsf(v)); }
+
+long long orc_swept_pairs(const float* x, const float* p, const int* tris, int nt, int nv, float thickness,
+                          int all_pairs, long long cap, int64_t* out) {
+    float* fb = (float*)malloc(sizeof(float) * 6 * ((size_t)nt + 1));
+    char* vin = (char*)calloc((size_t)nv + 1, 1);
+    const float m = 4.0f * thickness;
+    for (int f = 0; f < nt; f++) {
+        const int* fv = tris + 3 * f;
+        for (int k = 0; k < 3; k++) {
+            float lo = x[3 * fv[0] + k], hi = lo;
+            for (int i = 0; i < 3; i++) {
+                float a = x[3 * fv[i] + k], b = p[3 * fv[i] + k];
+                lo = fminf(lo, fminf(a, b));
+                hi = fmaxf(hi, fmaxf(a, b));
+            }
+            fb[6 * f + k] = widen_lo(lo, m);
+            fb[6 * f + 3 + k] = widen_hi(hi, m);
+        }
+        vin[fv[0]] = vin[fv[1]] = vin[fv[2]] = 1;
+    }
+    long long n = 0;
+    for (int v = 0; v < nv; v++) {
+        if (!vin[v]) continue;
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(x[3 * v + k], p[3 * v + k]);
+            hi[k] = fmaxf(x[3 * v + k], p[3 * v + k]);
+        }
+        for (int f = 0; f < nt; f++) {
+            const int* fv = tris + 3 * f;
+            if (fv[0] == v || fv[1] == v || fv[2] == v) continue;
+            if (!all_pairs) {
+                const float* b = fb + 6 * f;
+                if (lo[0] > b[3] || b[0] > hi[0] || lo[1] > b[4] || b[1] > hi[1] || lo[2] > b[5] || b[2] > hi[2])
+                    continue;
+            }
+            if (out && n < cap) out[n] = ((int64_t)v << 32) | (int64_t)(uint32_t)f;
+            n++;
+        }
+    }
+    free(fb);
+    free(vin);
+    return n;
+}
+
+/* Narrowphase (as orc_ccd_detect) over the swept-box set, or over every pair
+ * when all_pairs is set. */
+long long orc_ccd_detect_swept(const float* x, const float* p, const int* tris, int nt, int nv, float thickness,
+                               int all_pairs, long long cap, orc_contact* out) {
+    long long nk = orc_swept_pairs(x, p, tris, nt, nv, thickness, all_pairs, 0, NULL);
+    int64_t* keys = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nk + 1));
+    orc_swept_pairs(x, p, tris, nt, nv, thickness, all_pairs, nk, keys);
+    long long nc = 0;
+    for (long long i = 0; i < nk; i++) {
+        int v = (int)(keys[i] >> 32), f = (int)(keys[i] & 0xffffffff);
+        const int* fv = tris + 3 * f;
+        float t, n[3];
+        double w[4];
+        if (orc_vf_ccd(x + 3 * v, x + 3 * fv[0], x + 3 * fv[1], x + 3 * fv[2], p + 3 * v, p + 3 * fv[0],
+                       p + 3 * fv[1], p + 3 * fv[2], thickness, &t, n, w)) {
+            if (out && nc < cap) {
+                orc_contact* c = out + nc;
+                c->v = v; c->f = f; c->t = t;
+                c->n[0] = n[0]; c->n[1] = n[1]; c->n[2] = n[2];
+                for (int k = 0; k < 4; k++) c->w[k] = w[k];
+            }
+            nc++;
+        }
+    }
+    free(keys);
+    return nc;
+}
+
 /* Fixed-point accumulation unit: 2^32 per metre.  Integer sums do not depend
  * on the order of accumulation, which keeps a parallel accumulation
  * deterministic and equal to this sequential one. */
@@ -1052,7 +1135,7 @@ static long long to_fixed(float f) { return (long long)rint((double)f * 42949672
  * not moved, and the other side takes the full separation (twice the
  * reference's half depth). */
 void orc_ccd_resolve(const orc_contact* c, long long nc, const int* tris, const float* x, float* p,
-                     const float* w, float thickness, int passes) {
+                     const float* w, float thickness, int passes, float min_n2) {
     (void)w;
     int nv = 0;
     for (long long i = 0; i < nc; i++) {
@@ -1073,7 +1156,7 @@ void orc_ccd_resolve(const orc_contact* c, long long nc, const int* tris, const 
                 q[3 * k] = p[3 * id[k]]; q[3 * k + 1] = p[3 * id[k] + 1]; q[3 * k + 2] = p[3 * id[k] + 2];
             }
             const float *x0 = x + 3 * id[0], *x1 = x + 3 * id[1], *x2 = x + 3 * id[2], *x3 = x + 3 * id[3];
-            if (!orc_vf_resolve_test(x0, x1, x2, x3, q, q + 3, q + 6, q + 9, thickness)) continue;
+            if (!orc_vf_resolve_test(x0, x1, x2, x3, q, q + 3, q + 6, q + 9, thickness, min_n2)) continue;
             float before[12];
             memcpy(before, q, sizeof(before));
             orc_vf_resolve(x0, x1, x2, x3, q, q + 3, q + 6, q + 9, thickness);

@@ -127,7 +127,7 @@ int orc_vf_ccd(const float* x0, const float* x1, const float* x2, const float* x
                float thickness, float* t_out, float* n_out, double* w_out);
 int orc_vf_resolve_test(const float* x0, const float* x1, const float* x2, const float* x3,
                         const float* p0, const float* p1, const float* p2, const float* p3,
-                        float thickness);
+                        float thickness, float min_n2);
 void orc_vf_resolve(const float* x0, const float* x1, const float* x2, const float* x3,
                     float* p0, float* p1, float* p2, float* p3, float thickness);
 
@@ -140,10 +140,14 @@ typedef struct {
     double w[4];
 } orc_contact;
 
+long long orc_swept_pairs(const float* x, const float* p, const int* tris, int nt, int nv, float thickness,
+                          int all_pairs, long long cap, int64_t* out);
+long long orc_ccd_detect_swept(const float* x, const float* p, const int* tris, int nt, int nv, float thickness,
+                               int all_pairs, long long cap, orc_contact* out);
 long long orc_ccd_detect(const orc_grid* g, const float* x, const float* p, const int* tris, int nt,
                          int nv, float thickness, long long cap, orc_contact* out);
 void orc_ccd_resolve(const orc_contact* c, long long nc, const int* tris, const float* x, float* p,
-                     const float* w, float thickness, int passes);
+                     const float* w, float thickness, int passes, float min_n2);
 
 /* ---------------- warm-start MLP, FP64 (new) ------------------------------ */
 #define ORC_MLP_IN 22

@@ -58,7 +58,8 @@ def hash_desc(mode=0, centre=(0.0, 0.0, 0.0), size=(1, 1, 1), cell=(1.0, 1.0, 1.
 
 class CollisionParams(C.Structure):
     _fields_ = [("enabled", C.c_int), ("grid", HashDesc), ("cell_scale", C.c_float), ("thickness", C.c_float),
-                ("resolve_passes", C.c_int), ("detect_every", C.c_int)]
+                ("resolve_passes", C.c_int), ("detect_every", C.c_int), ("min_normal2", C.c_float),
+                ("broadphase", C.c_int)]
 
 
 class Profile(C.Structure):
@@ -113,6 +114,9 @@ SIGNATURES = {
     "pbd_sh_pairs": [_P, C.c_longlong, C.POINTER(C.c_int64), C.POINTER(C.c_longlong)],
     "pbd_ccd_detect": [_P, C.POINTER(HashDesc), _fp, _fp, _I, _ip, _I, _F, C.POINTER(C.c_longlong)],
     "pbd_ccd_get_contacts": [_P, C.c_longlong, _ip, _fp, _fp, _dp],
+    "pbd_ccd_detect_swept": [_P, C.POINTER(HashDesc), _fp, _fp, _I, _ip, _I, _F, C.POINTER(C.c_longlong),
+                             C.POINTER(C.c_longlong)],
+    "pbd_ccd_get_pairs": [_P, C.c_longlong, C.POINTER(C.c_int64), C.POINTER(C.c_longlong)],
     "pbd_ccd_resolve": [_P, _fp, _fp, _F, _I],
     "pbd_set_collision": [_P, C.POINTER(CollisionParams)],
     "pbd_last_contacts": [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
@@ -365,13 +369,20 @@ class Context:
         check(self.lib.pbd_sh_pairs(self.h, n.value, out.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(n)))
         return out[: n.value]
 
-    def ccd_detect(self, desc: HashDesc, x, p, tris, thickness: float):
+    def ccd_detect(self, desc: HashDesc, x, p, tris, thickness: float, swept: bool = False):
+        """Contacts (vf, t, n, w) of one detection; swept=True uses the
+        swept-box candidate set (its pairs: ccd_pairs())."""
         x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
         p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
         tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
         n = C.c_longlong()
-        check(self.lib.pbd_ccd_detect(self.h, C.byref(desc), fptr(x), fptr(p), len(x), iptr(tris), len(tris),
-                                      thickness, C.byref(n)))
+        if swept:
+            npairs = C.c_longlong()
+            check(self.lib.pbd_ccd_detect_swept(self.h, C.byref(desc), fptr(x), fptr(p), len(x), iptr(tris),
+                                                len(tris), thickness, C.byref(npairs), C.byref(n)))
+        else:
+            check(self.lib.pbd_ccd_detect(self.h, C.byref(desc), fptr(x), fptr(p), len(x), iptr(tris), len(tris),
+                                          thickness, C.byref(n)))
         m = n.value
         vf = np.zeros((max(m, 1), 2), np.int32)
         t = np.zeros(max(m, 1), np.float32)
@@ -380,6 +391,14 @@ class Context:
         check(self.lib.pbd_ccd_get_contacts(self.h, m, iptr(vf), fptr(t), fptr(nn), w.ctypes.data_as(_dp)))
         return vf[:m], t[:m], nn[:m], w[:m]
 
+    def ccd_pairs(self) -> np.ndarray:
+        """Candidate pairs of the last detection as sorted keys v << 32 | f."""
+        n = C.c_longlong()
+        check(self.lib.pbd_ccd_get_pairs(self.h, 0, None, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.int64)
+        check(self.lib.pbd_ccd_get_pairs(self.h, n.value, out.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(n)))
+        return out[: n.value]
+
     def ccd_resolve(self, p, w, thickness: float, passes: int) -> np.ndarray:
         p = np.ascontiguousarray(p, np.float32).reshape(-1, 3).copy()
         w = np.ascontiguousarray(w, np.float32)
@@ -387,9 +406,11 @@ class Context:
         return p
 
     def set_collision(self, enabled=1, grid: HashDesc | None = None, cell_scale=1.5, thickness=0.005,
-                      resolve_passes=2, detect_every=10):
+                      resolve_passes=2, detect_every=10, min_normal2=0.0, broadphase=0):
+        """min_normal2 = 0 keeps the reference's degenerate-face bound (1e-6 m^4);
+        broadphase 0 = the reference's centroid cells, 1 = swept boxes."""
         cp = CollisionParams(enabled, grid if grid is not None else hash_desc(1, table_bits=20), cell_scale,
-                             thickness, resolve_passes, detect_every)
+                             thickness, resolve_passes, detect_every, min_normal2, broadphase)
         check(self.lib.pbd_set_collision(self.h, C.byref(cp)))
 
     def last_contacts(self):

@@ -525,8 +525,12 @@ void upload_scene(pbd_ctx* c, const float* x, const float* p, const float* w, in
 // Contact detection for the whole context (all cloths, global vertex ids).
 void detect_contacts(pbd_ctx* c, const float4* p) {
     hipStream_t s = c->stream;
-    c->cbp.build(s, c->ccfg, p, c->gtris.ptr, c->gnt);
-    c->cbp.pairs(s, p, c->gtris.ptr, c->nv_total);
+    if (c->coll.broadphase == 1) {
+        c->cbp.pairs_swept(s, c->ccfg, c->x.ptr, p, c->gtris.ptr, c->gnt, c->nv_total, c->coll.thickness);
+    } else {
+        c->cbp.build(s, c->ccfg, p, c->gtris.ptr, c->gnt);
+        c->cbp.pairs(s, p, c->gtris.ptr, c->nv_total);
+    }
     c->cbp.narrow(s, c->x.ptr, p, c->gtris.ptr, c->coll.thickness);
 }
 
@@ -813,7 +817,7 @@ void step_ccd(pbd_ctx* c, float dt) {
         }
         c->pcur = cur ^ 1;
         c->cbp.resolve(s, c->x.ptr, c->pbuf[c->pcur].ptr, c->gtris.ptr, c->nv_total, c->coll.thickness,
-                       c->coll.resolve_passes);
+                       c->coll.resolve_passes, c->coll.min_normal2 > 0.0f ? c->coll.min_normal2 : 1e-6f);
         if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, c->pbuf[c->pcur].ptr, 0, c->nv_total);
     }
     integrate_all(c, dt, false);
@@ -1391,6 +1395,42 @@ int pbd_ccd_detect(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, const f
     });
 }
 
+int pbd_ccd_detect_swept(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, const float* p, int nv,
+                         const int* tris, int nt, float th, long long* npairs, long long* nc) {
+    return guard([&] {
+        PBD_CHECK(ctx && d && nc, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(th >= 0.0f, PBD_ERR_ARG, "thickness must be >= 0");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        HashCfg g = make_hash_cfg(*d);
+        upload_scene(ctx, x, p, nullptr, nv, tris, nt);
+        long long np = ctx->sbp.pairs_swept(s, g, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, nt, nv, th);
+        if (npairs) *npairs = np;
+        *nc = ctx->sbp.narrow(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, th);
+        ctx->sh_built = false;  // the cell table holds swept entries, not centroid keys
+        ctx->ccd_built = true;
+    });
+}
+
+int pbd_ccd_get_pairs(pbd_ctx* ctx, long long cap, int64_t* out, long long* count) {
+    return guard([&] {
+        PBD_CHECK(ctx && count, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(ctx->ccd_built, PBD_ERR_STATE, "call pbd_ccd_detect or pbd_ccd_detect_swept first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        const long long n = ctx->sbp.npairs;
+        *count = n;
+        if (out && n > 0) {
+            long long m = std::min(n, cap);
+            std::vector<uint32_t> v(m), f(m);
+            PBD_HIP(hipMemcpyAsync(v.data(), ctx->sbp.uv.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+            PBD_HIP(hipMemcpyAsync(f.data(), ctx->sbp.uf.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+            PBD_HIP(hipStreamSynchronize(s));
+            for (long long i = 0; i < m; i++) out[i] = ((int64_t)v[i] << 32) | (int64_t)f[i];
+        }
+    });
+}
+
 int pbd_ccd_get_contacts(pbd_ctx* ctx, long long cap, int* vf, float* t, float* n, double* w) {
     return guard([&] {
         PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
@@ -1435,7 +1475,7 @@ int pbd_ccd_resolve(pbd_ctx* ctx, float* p, const float* w, float th, int passes
         }
         ctx->sp4.upload(P, s);
         ctx->sx.upload(X, s);
-        ctx->sbp.resolve(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, nv, th, passes);
+        ctx->sbp.resolve(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, nv, th, passes, 1e-6f);
         ctx->sp4.download(P, s);
         PBD_HIP(hipStreamSynchronize(s));
         for (int i = 0; i < nv; i++) {
@@ -1456,7 +1496,9 @@ int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* cp) {
             ctx->coll = *cp;
             return;
         }
-        PBD_CHECK(cp->thickness >= 0.0f && cp->resolve_passes >= 0, PBD_ERR_ARG, "bad collision parameters");
+        PBD_CHECK(cp->thickness >= 0.0f && cp->resolve_passes >= 0 && cp->min_normal2 >= 0.0f &&
+                      (cp->broadphase == 0 || cp->broadphase == 1),
+                  PBD_ERR_ARG, "bad collision parameters");
         pbd_hash_desc d = cp->grid;
         if (cp->cell_scale > 0.0f) {
             double sum = 0.0;

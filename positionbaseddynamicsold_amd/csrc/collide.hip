@@ -212,6 +212,174 @@ __global__ __launch_bounds__(256) void k_compact_pairs(const uint32_t* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Swept-box broadphase (a conservative alternative to the centroid-cell
+// candidate set above, which misses faces larger than a cell).  Candidate
+// (v, f): v is a vertex of some triangle, v not in f, and v's box over the
+// substep, AABB(x_v, p_v), meets f's box over the substep, AABB of the six
+// points x_f*, p_f*, widened per side by 4 thickness plus 1e-6 |bound|.  The
+// narrowphase accepts a vertex within 2 thickness of the face plane and
+// inside the face's barycentric band of width thickness / mean edge, i.e.
+// within sqrt(13) thickness of the face, at an interpolated position inside
+// both boxes, so every pair it accepts is a candidate: the contact set
+// equals that of the all-pairs test (CCD_N2, CCD.cpp:1040-1098).  The cell
+// grid only accelerates the box tests and does not change the set.
+// ---------------------------------------------------------------------------
+constexpr int SWEPT_MAX_CELLS = 4096;  // cells one box may span
+
+__device__ __forceinline__ float widen_lo(float v, float m) { return v - (m + 1e-6f * fabsf(v)); }
+__device__ __forceinline__ float widen_hi(float v, float m) { return v + (m + 1e-6f * fabsf(v)); }
+
+__device__ __forceinline__ int cell_coord(const HashCfg& g, float v, int k) {
+    float c = floorf((v - g.start[k]) / g.cell[k]);
+    c = fminf(fmaxf(c, -536870912.0f), 536870912.0f);  // keeps hi - lo + 1 within int
+    int i = (int)c;
+    if (g.mode == 0) i = min(max(i, 0), g.size[k] - 1);  // clamping keeps overlapping ranges overlapping
+    return i;
+}
+
+struct SweptBox {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ SweptBox face_box(const float4* __restrict__ x, const float4* __restrict__ p,
+                                             const int* __restrict__ tris, int f, float margin) {
+    SweptBox b;
+    const float4 a[6] = {x[tris[3 * f]], x[tris[3 * f + 1]], x[tris[3 * f + 2]],
+                         p[tris[3 * f]], p[tris[3 * f + 1]], p[tris[3 * f + 2]]};
+    float lo[3] = {a[0].x, a[0].y, a[0].z}, hi[3] = {a[0].x, a[0].y, a[0].z};
+    for (int i = 1; i < 6; i++) {
+        const float c[3] = {a[i].x, a[i].y, a[i].z};
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], c[k]);
+            hi[k] = fmaxf(hi[k], c[k]);
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        b.lo[k] = widen_lo(lo[k], margin);
+        b.hi[k] = widen_hi(hi[k], margin);
+    }
+    return b;
+}
+
+__device__ __forceinline__ SweptBox vertex_box(const float4 X, const float4 P) {
+    SweptBox b;
+    b.lo[0] = fminf(X.x, P.x); b.hi[0] = fmaxf(X.x, P.x);
+    b.lo[1] = fminf(X.y, P.y); b.hi[1] = fmaxf(X.y, P.y);
+    b.lo[2] = fminf(X.z, P.z); b.hi[2] = fmaxf(X.z, P.z);
+    return b;
+}
+
+// Face boxes, their cell ranges and cell counts; err[0] = 1 if a face spans
+// more than SWEPT_MAX_CELLS cells (its count is then 0).
+__global__ __launch_bounds__(256) void k_swept_faces(HashCfg g, const float4* __restrict__ x, const float4* __restrict__ p,
+                                                     const int* __restrict__ tris, int nt, float margin,
+                                                     float4* __restrict__ fbox, int4* __restrict__ fcell,
+                                                     int* __restrict__ cnt, int* __restrict__ err) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nt) return;
+    const SweptBox b = face_box(x, p, tris, f, margin);
+    int lo[3], hi[3];
+    long long n = 1;
+    for (int k = 0; k < 3; k++) {
+        lo[k] = cell_coord(g, b.lo[k], k);
+        hi[k] = cell_coord(g, b.hi[k], k);
+        // per-axis bound first: the product of three spans of up to 2^31 overflows
+        n *= (long long)min(hi[k] - lo[k] + 1, SWEPT_MAX_CELLS + 1);
+    }
+    fbox[2 * f] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f);
+    fbox[2 * f + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
+    fcell[2 * f] = make_int4(lo[0], lo[1], lo[2], 0);
+    fcell[2 * f + 1] = make_int4(hi[0], hi[1], hi[2], 0);
+    if (n > SWEPT_MAX_CELLS) {
+        err[0] = 1;
+        n = 0;
+    }
+    cnt[f] = (int)n;
+}
+
+// One (cell key + 1, face) entry per cell of each face box, x fastest.
+__global__ __launch_bounds__(256) void k_swept_entries(HashCfg g, int nt, const int4* __restrict__ fcell,
+                                                       const int* __restrict__ off, uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ ids) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nt) return;
+    int o = off[f];
+    if (off[f + 1] == o) return;
+    const int4 lo = fcell[2 * f], hi = fcell[2 * f + 1];
+    for (int z = lo.z; z <= hi.z; z++)
+        for (int y = lo.y; y <= hi.y; y++)
+            for (int xx = lo.x; xx <= hi.x; xx++) {
+                keys[o] = (uint32_t)(cell_key(g, xx, y, z) + 1);
+                ids[o] = (uint32_t)f;
+                o++;
+            }
+}
+
+__global__ __launch_bounds__(256) void k_mark_vertices(const int* __restrict__ tris, int nt, int* __restrict__ vin) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * nt) vin[tris[i]] = 1;
+}
+
+// Per vertex: faces found in the cells of its box whose boxes meet it.  A
+// pair is taken at the lowest cell of the two cell ranges' intersection
+// only (a face stored twice in one slot is removed by the later unique
+// pass).  err[1] = 1 if a vertex box spans more than SWEPT_MAX_CELLS cells.
+template <int EMIT>
+__global__ __launch_bounds__(256) void k_swept_pairs(HashCfg g, const float4* __restrict__ x,
+                                                     const float4* __restrict__ p, const int* __restrict__ vin, int nv,
+                                                     const int* __restrict__ tris, const float4* __restrict__ fbox,
+                                                     const int4* __restrict__ fcell, const uint32_t* __restrict__ sids,
+                                                     const int* __restrict__ cstart, const int* __restrict__ cend,
+                                                     int* __restrict__ cnt_or_off, uint32_t* __restrict__ ov,
+                                                     uint32_t* __restrict__ of, int* __restrict__ err) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    int n = 0;
+    const int base = EMIT ? cnt_or_off[v] : 0;
+    if (vin[v]) {
+        const SweptBox b = vertex_box(x[v], p[v]);
+        int lo[3], hi[3];
+        long long span = 1;
+        for (int k = 0; k < 3; k++) {
+            lo[k] = cell_coord(g, b.lo[k], k);
+            hi[k] = cell_coord(g, b.hi[k], k);
+            span *= (long long)min(hi[k] - lo[k] + 1, SWEPT_MAX_CELLS + 1);
+        }
+        if (span > SWEPT_MAX_CELLS) {
+            if (!EMIT) err[1] = 1;
+        } else {
+            for (int qz = lo[2]; qz <= hi[2]; qz++)
+                for (int qy = lo[1]; qy <= hi[1]; qy++)
+                    for (int qx = lo[0]; qx <= hi[0]; qx++) {
+                        const int h = cell_key(g, qx, qy, qz);
+                        if (h < 0) continue;
+                        const int s = cstart[h];
+                        if (s < 0) continue;
+                        const int e = cend[h];
+                        for (int i = s; i <= e; i++) {
+                            const int f = (int)sids[i];
+                            const int4 fl = fcell[2 * f], fh = fcell[2 * f + 1];
+                            // the pair's first shared cell, and f really covers it
+                            if (qx != max(lo[0], fl.x) || qy != max(lo[1], fl.y) || qz != max(lo[2], fl.z)) continue;
+                            if (qx > fh.x || qy > fh.y || qz > fh.z) continue;
+                            const float4 flo = fbox[2 * f], fhi = fbox[2 * f + 1];
+                            if (b.lo[0] > fhi.x || flo.x > b.hi[0] || b.lo[1] > fhi.y || flo.y > b.hi[1] ||
+                                b.lo[2] > fhi.z || flo.z > b.hi[2])
+                                continue;
+                            if (tris[3 * f] == v || tris[3 * f + 1] == v || tris[3 * f + 2] == v) continue;
+                            if (EMIT) {
+                                ov[base + n] = (uint32_t)v;
+                                of[base + n] = (uint32_t)f;
+                            }
+                            n++;
+                        }
+                    }
+        }
+    }
+    if (!EMIT) cnt_or_off[v] = n;
+}
+
+// ---------------------------------------------------------------------------
 // narrowphase: FP64 cubic (solve_cubic / newtons_method / solve_quadratic,
 // CCD.cpp:48-113), ExtendedVFTest (CCD.cpp:1733-1808)
 // ---------------------------------------------------------------------------
@@ -399,7 +567,7 @@ __device__ __forceinline__ long long to_fixed(float f) { return (long long)rint(
 
 __global__ __launch_bounds__(256) void k_resolve_acc(const int* __restrict__ cvv, const int* __restrict__ cff, int nc,
                                                      const int* __restrict__ tris, const float4* __restrict__ x,
-                                                     const float4* __restrict__ p, float th,
+                                                     const float4* __restrict__ p, float th, float min_n2,
                                                      unsigned long long* __restrict__ acc, int* __restrict__ cnt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc) return;
@@ -416,7 +584,7 @@ __global__ __launch_bounds__(256) void k_resolve_acc(const int* __restrict__ cvv
     V3 n0 = vnormalize(cross(sub(X[2], X[1]), sub(X[3], X[1])));
     bool dir = dot(sub(X[0], X[1]), n0) >= 0;
     V3 cr = cross(sub(q[2], q[1]), sub(q[3], q[1]));
-    if (dot(cr, cr) <= 1e-6) return;
+    if (dot(cr, cr) <= min_n2) return;
     V3 pn = vnormalize(cr);
     float s = dir ? 1.0f : -1.0f;
     V3 sn = V3{pn.x * s, pn.y * s, pn.z * s};
@@ -514,6 +682,11 @@ long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int
     pos.alloc(total + 1);
     hipLaunchKernelGGL(k_pairs<1>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr, cstart.ptr,
                        cend.ptr, tcell.ptr, off.ptr, pv.ptr, pf.ptr);
+    return unique_pairs(s, total, nt, nv);
+}
+
+// Sorts the `total` raw pairs in pv/pf by (v, f) and keeps one of each in uv/uf.
+long long Broadphase::unique_pairs(hipStream_t s, int total, int nt, int nv) {
     // sort by (v, f): by f first, then stably by v
     int fb = 1, vb = 1;
     while (fb < 31 && (1ll << fb) < nt) fb++;
@@ -535,6 +708,75 @@ long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int
     PBD_HIP(hipGetLastError());
     npairs = uniq;
     return uniq;
+}
+
+template <typename T>
+static void grow(DBuf<T>& b, size_t n) {
+    if (b.n < n || !b.ptr) b.alloc(std::max<size_t>(n, 1));
+}
+
+long long Broadphase::pairs_swept(hipStream_t s, const HashCfg& g, const float4* x, const float4* p, const int* tris,
+                                  int nt_, int nv, float th) {
+    cfg = g;
+    nt = nt_;
+    npairs = 0;
+    if (nt <= 0 || nv <= 0) return 0;
+    grow(sw_box, 2 * (size_t)nt);
+    grow(sw_cell, 2 * (size_t)nt);
+    grow(cnt, (size_t)std::max(nt, nv) + 1);
+    grow(off, (size_t)std::max(nt, nv) + 1);
+    grow(sw_err, 2);
+    grow(sw_vin, nv);
+    scan_tmp.alloc((std::max(nt, nv) + SB) / SB + 2);
+    PBD_HIP(hipMemsetAsync(sw_err.ptr, 0, 2 * sizeof(int), s));
+    hipLaunchKernelGGL(k_swept_faces, dim3(blocks(nt)), dim3(256), 0, s, g, x, p, tris, nt, 4.0f * th, sw_box.ptr,
+                       sw_cell.ptr, cnt.ptr, sw_err.ptr);
+    PBD_HIP(hipMemsetAsync(cnt.ptr + nt, 0, sizeof(int), s));
+    device_scan(s, cnt.ptr, off.ptr, nt + 1, scan_tmp.ptr);
+    int hst[3] = {0, 0, 0};
+    PBD_HIP(hipMemcpyAsync(&hst[0], off.ptr + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipMemcpyAsync(&hst[1], sw_err.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipStreamSynchronize(s));
+    PBD_CHECK(!hst[1], PBD_ERR_LIMIT, "a face's swept box spans more than 4096 cells: use a larger cell");
+    const int ne = hst[0];
+    PBD_CHECK(ne >= 0, PBD_ERR_LIMIT, "swept-box cell entries overflow int32");
+    grow(keys, ne);
+    grow(ids, ne);
+    grow(k2, ne);
+    grow(v2, ne);
+    grow(hist, (size_t)256 * ((std::max(1, ne) + 4095) / 4096));
+    grow(cstart, std::max(1, g.ncells));
+    grow(cend, std::max(1, g.ncells));
+    hipLaunchKernelGGL(k_swept_entries, dim3(blocks(nt)), dim3(256), 0, s, g, nt, sw_cell.ptr, off.ptr, keys.ptr, ids.ptr);
+    int bits = 1;
+    while (bits < 32 && ((1ll << bits) < (long long)g.ncells + 1)) bits++;
+    radix_sort_pairs(s, keys.ptr, ids.ptr, k2.ptr, v2.ptr, ne, bits, hist.ptr);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(g.ncells)), dim3(256), 0, s, cstart.ptr, g.ncells, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(g.ncells)), dim3(256), 0, s, cend.ptr, g.ncells, -1);
+    if (ne > 0) hipLaunchKernelGGL(k_cell_ranges, dim3(blocks(ne)), dim3(256), 0, s, keys.ptr, ne, cstart.ptr, cend.ptr);
+    PBD_HIP(hipMemsetAsync(sw_vin.ptr, 0, sizeof(int) * (size_t)nv, s));
+    hipLaunchKernelGGL(k_mark_vertices, dim3(blocks(3ll * nt)), dim3(256), 0, s, tris, nt, sw_vin.ptr);
+    hipLaunchKernelGGL(k_swept_pairs<0>, dim3(blocks(nv)), dim3(256), 0, s, g, x, p, sw_vin.ptr, nv, tris, sw_box.ptr,
+                       sw_cell.ptr, ids.ptr, cstart.ptr, cend.ptr, cnt.ptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                       sw_err.ptr);
+    PBD_HIP(hipMemsetAsync(cnt.ptr + nv, 0, sizeof(int), s));
+    device_scan(s, cnt.ptr, off.ptr, nv + 1, scan_tmp.ptr);
+    PBD_HIP(hipMemcpyAsync(&hst[0], off.ptr + nv, sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipMemcpyAsync(&hst[1], sw_err.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipStreamSynchronize(s));
+    PBD_CHECK(!hst[2], PBD_ERR_LIMIT, "a vertex's swept box spans more than 4096 cells: use a larger cell");
+    const int total = hst[0];
+    PBD_CHECK(total >= 0, PBD_ERR_LIMIT, "candidate pair count overflows int32");
+    if (total == 0) return 0;
+    pv.alloc(total);
+    pf.alloc(total);
+    t1.alloc(total);
+    t2.alloc(total);
+    flag.alloc(total + 1);
+    pos.alloc(total + 1);
+    hipLaunchKernelGGL(k_swept_pairs<1>, dim3(blocks(nv)), dim3(256), 0, s, g, x, p, sw_vin.ptr, nv, tris, sw_box.ptr,
+                       sw_cell.ptr, ids.ptr, cstart.ptr, cend.ptr, off.ptr, pv.ptr, pf.ptr, sw_err.ptr);
+    return unique_pairs(s, total, nt, nv);
 }
 
 long long Broadphase::narrow(hipStream_t s, const float4* x, const float4* p, const int* tris, float th) {
@@ -566,7 +808,8 @@ long long Broadphase::narrow(hipStream_t s, const float4* x, const float4* p, co
     return nc;
 }
 
-void Broadphase::resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes) {
+void Broadphase::resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes,
+                         float min_n2) {
     if (ncontacts <= 0) return;
     acc.alloc((size_t)3 * nv);
     vcnt.alloc(nv);
@@ -574,7 +817,7 @@ void Broadphase::resolve(hipStream_t s, const float4* x, float4* p, const int* t
         PBD_HIP(hipMemsetAsync(acc.ptr, 0, sizeof(unsigned long long) * 3 * (size_t)nv, s));
         PBD_HIP(hipMemsetAsync(vcnt.ptr, 0, sizeof(int) * (size_t)nv, s));
         hipLaunchKernelGGL(k_resolve_acc, dim3(blocks(ncontacts)), dim3(256), 0, s, cv.ptr, cf.ptr, (int)ncontacts, tris, x,
-                           (const float4*)p, th, acc.ptr, vcnt.ptr);
+                           (const float4*)p, th, min_n2, acc.ptr, vcnt.ptr);
         hipLaunchKernelGGL(k_resolve_apply, dim3(blocks(nv)), dim3(256), 0, s, acc.ptr, vcnt.ptr, p, nv);
     }
     PBD_HIP(hipGetLastError());

@@ -191,10 +191,20 @@ struct Broadphase {
     DBuf<double> ctmp_w, cw;
     DBuf<unsigned long long> acc;
     DBuf<int4> tcell;    // centroid cell coordinates per triangle
+    DBuf<float4> sw_box; // swept-box broadphase: face boxes (lo, hi)
+    DBuf<int4> sw_cell;  // and their cell ranges (lo, hi)
+    DBuf<int> sw_err, sw_vin;
     void build(hipStream_t s, const HashCfg& g, const float4* p, const int* tris, int nt);
     long long pairs(hipStream_t s, const float4* p, const int* tris, int nv);
+    long long unique_pairs(hipStream_t s, int total, int nt, int nv);
+    // Swept-box candidate set (replaces build + pairs); x = start of substep.
+    long long pairs_swept(hipStream_t s, const HashCfg& g, const float4* x, const float4* p, const int* tris, int nt,
+                          int nv, float th);
     long long narrow(hipStream_t s, const float4* x, const float4* p, const int* tris, float th);
-    void resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes);
+    // min_n2: faces with |(p2 - p1) x (p3 - p1)|^2 <= min_n2 are skipped
+    // (VFResolveTest's degenerate-face test, 1e-6 in the reference)
+    void resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes,
+                 float min_n2);
 };
 void device_scan(hipStream_t s, const int* in, int* out, int n, int* tmp);
 

@@ -10,7 +10,7 @@ config 5: 512 cloths of uneven size -- side U{16..256} vertices, aspect
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -29,6 +29,7 @@ class ClothSpec:
     sphere: tuple | None = None          # ((cx, cy, cz), r)
     wind: tuple = (0.0, 0.0, 0.0)
     wind_drag: float = 0.0
+    origin: tuple = (0.0, 0.0, 0.0)      # translation after generation
 
     @property
     def nv(self) -> int:
@@ -70,7 +71,7 @@ def config5(seed: int = 0, count: int = 512) -> list[ClothSpec]:
 def add_to_context(ctx, spec: ClothSpec, bending: bool = False, stiffness=(1.0, 1.0, 1.0)) -> int:
     from . import capi
 
-    cid = ctx.add_grid(spec.rx, spec.ry, spec.sx, spec.sy, jitter=spec.jitter, seed=spec.seed)
+    cid = ctx.add_grid(spec.rx, spec.ry, spec.sx, spec.sy, origin=spec.origin, jitter=spec.jitter, seed=spec.seed)
     ctx.set_constraint_option(cid, capi.DISTANCE | (capi.BENDING if bending else 0), stiffness)
     if spec.pins:
         ctx.set_pins(cid, spec.pins)
@@ -130,3 +131,140 @@ def config4(seed: int = 0) -> ClothSpec:
          float(np.float32(rng.uniform(0.2, 0.8))))
     s.sphere = (c, 0.3)
     return s
+
+
+# ---------------------------------------------------------------------------
+# Scenes with a static triangle mesh and vertex-face CCD
+# ---------------------------------------------------------------------------
+@dataclass
+class CcdScene:
+    """A grid cloth, a static triangle mesh and the CCD settings of a config."""
+    cloth: ClothSpec
+    static_pos: np.ndarray
+    static_tris: np.ndarray
+    bending: bool = False
+    stiffness: tuple = (1.0, 1.0, 1.0)
+    iterations: int = 20
+    table_bits: int = 20
+    cell_scale: float = 1.5       # hash cell = cell_scale x mean distance-prim rest length
+    thickness: float = 0.001
+    resolve_passes: int = 2
+    detect_every: int = 10        # projectConstraintWithColliCPU detects at ii % 10 == 0 and the last sweep
+    fold: tuple | None = None     # (angle, radius) of the initial fold, rest state flat
+    # degenerate-face bound of the response on |(p2-p1) x (p3-p1)|^2 in m^4;
+    # 0 = the reference's 1e-6, which skips every face with edges below ~3.4 cm
+    min_normal2: float = 0.0
+    # candidate pairs: 0 = the reference's centroid cells, 1 = swept boxes
+    broadphase: int = 0
+
+    def robust(self):
+        """The same scene with the swept-box broadphase and a degenerate-face
+        bound far below any face of the scene (1e-20 m^4), so that contacts
+        are found and resolved on faces of every size."""
+        return replace(self, broadphase=1, min_normal2=1e-20)
+
+
+def box_mesh(size: float = 0.4, centre=(0.0, 0.2, 0.0), div: int = 16):
+    """Closed cube of edge `size`, every face a div x div grid of quads split
+    into two triangles, wound outward."""
+    h = size / 2.0
+    c = np.asarray(centre, np.float64)
+    pos, tris = [], []
+    # (normal axis, sign); u, v axes chosen so that u x v = outward normal
+    faces = [(0, 1, 1, 2), (0, -1, 2, 1), (1, 1, 2, 0), (1, -1, 0, 2), (2, 1, 0, 1), (2, -1, 1, 0)]
+    for ax, sg, ua, va in faces:
+        base = len(pos)
+        for j in range(div + 1):
+            for i in range(div + 1):
+                q = np.zeros(3)
+                q[ax] = sg * h
+                q[ua] = -h + size * i / div
+                q[va] = -h + size * j / div
+                pos.append(c + q)
+        for j in range(div):
+            for i in range(div):
+                a = base + j * (div + 1) + i
+                b, d, e = a + 1, a + div + 1, a + div + 2
+                tris += [(a, b, e), (a, e, d)]
+    return np.asarray(pos, np.float32), np.asarray(tris, np.int32)
+
+
+def fold_positions(pos: np.ndarray, angle: float, radius: float) -> np.ndarray:
+    """Folds a flat cloth (y constant) once about the line through its XZ
+    centre with direction (cos a, 0, sin a): points on the positive side roll
+    over a half cylinder of the given radius and lie flat on top, 2 * radius
+    above (isometric, so edge lengths keep their rest values)."""
+    p = pos.astype(np.float64)
+    ctr = 0.5 * (p.min(0) + p.max(0))
+    d = np.array([np.cos(angle), 0.0, np.sin(angle)])
+    nrm = np.array([-np.sin(angle), 0.0, np.cos(angle)])
+    rel = p - ctr
+    t = rel @ d
+    sd = rel @ nrm
+    y = p[:, 1].copy()
+    s2 = sd.copy()
+    bend = (sd > 0) & (sd <= np.pi * radius)
+    th = sd[bend] / radius
+    s2[bend] = radius * np.sin(th)
+    y[bend] += radius * (1.0 - np.cos(th))
+    top = sd > np.pi * radius
+    s2[top] = -(sd[top] - np.pi * radius)
+    y[top] += 2.0 * radius
+    out = ctr + np.outer(t, d) + np.outer(s2, nrm)
+    out[:, 1] = y
+    return out.astype(np.float32)
+
+
+def config2(n: int = 256, subdiv: int = 4, seed: int = 0) -> CcdScene:
+    """n x n cloth (1 m, jitter 1e-4), distance + bending 0.1, dropped from
+    0.5 m above a static icosphere (r 0.3 m at the origin; subdivision 4 =
+    2,562 vertices); hashed grid with cell 1.5 x mean edge, table 2^20."""
+    cloth = ClothSpec(n, n, 1.0, 1.0, pins=[], jitter=1e-4, seed=seed, origin=(-0.5, 0.5, -0.5))
+    sp, st = icosphere(subdiv, 0.3)
+    return CcdScene(cloth, sp, st, bending=True, stiffness=(1.0, 0.1, 1.0), iterations=20, table_bits=20)
+
+
+def config3(n: int = 512, seed: int = 0, box_div: int = 64) -> CcdScene:
+    """n x n cloth (1 m), folded once about a random axis through its centre
+    (fold radius 2.5 grid spacings: 4.9 mm, layers 1 cm apart at n = 512),
+    5 mm above a static 0.4 m box mesh; 0-4 random boundary pins; self and
+    cloth-box contact, table 2^22, 30 sweeps."""
+    rng = np.random.default_rng(30_000 + seed)
+    npins = int(rng.integers(0, 5))
+    pins = sorted(rng.choice(boundary_vertices(n, n), npins, replace=False).tolist()) if npins else []
+    angle = float(rng.uniform(0.0, np.pi))
+    cloth = ClothSpec(n, n, 1.0, 1.0, pins=pins, jitter=1e-4, seed=seed, origin=(-0.5, 0.405, -0.5))
+    bp, bt = box_mesh(0.4, (0.0, 0.2, 0.0), box_div)
+    return CcdScene(cloth, bp, bt, iterations=30, table_bits=22, fold=(angle, 2.5 / (n - 1)))
+
+
+def hash_cell(rest: np.ndarray, scale: float) -> float:
+    """scale x mean rest length, with the library's arithmetic (double sum in
+    prim order, float result)."""
+    s = np.cumsum(rest.astype(np.float64))[-1] if len(rest) else 0.0
+    return float(np.float32(np.float64(np.float32(scale)) * (s / len(rest))))
+
+
+def add_ccd_scene(ctx, sc: CcdScene, graph: int = 0):
+    """Adds the cloth and the static mesh, initialises, applies the fold and
+    the collision settings; returns (cloth id, static id, hash cell)."""
+    from . import capi
+
+    c = sc.cloth
+    cid = ctx.add_grid(c.rx, c.ry, c.sx, c.sy, origin=c.origin, jitter=c.jitter, seed=c.seed)
+    ctx.set_constraint_option(cid, capi.DISTANCE | (capi.BENDING if sc.bending else 0), sc.stiffness)
+    if c.pins:
+        ctx.set_pins(cid, c.pins)
+    sid = ctx.add_mesh(sc.static_pos, sc.static_tris)
+    ctx.set_constraint_option(sid, 0)
+    ctx.set_pins(sid, np.arange(len(sc.static_pos)))
+    ctx.init()
+    _, _, rest = ctx.prims(cid)
+    cell = hash_cell(rest[: ctx.info(cid).ne], sc.cell_scale)
+    if sc.fold is not None:
+        ctx.set(cid, capi.STATE_X, fold_positions(ctx.get(cid), *sc.fold))
+    ctx.set_solver(dt=DT, iterations=sc.iterations, graph=graph)
+    ctx.set_collision(1, capi.hash_desc(1, cell=(cell, cell, cell), table_bits=sc.table_bits), cell_scale=0.0,
+                      thickness=sc.thickness, resolve_passes=sc.resolve_passes, detect_every=sc.detect_every,
+                      min_normal2=sc.min_normal2, broadphase=sc.broadphase)
+    return cid, sid, cell

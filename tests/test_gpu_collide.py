@@ -172,3 +172,34 @@ def test_autotune_with_collisions_leaves_results_unchanged():
         assert a.last_contacts()[0] > 0
         assert np.array_equal(a.get(c0), b.get(c0))
         assert np.array_equal(a.get(c0, capi.STATE_V), b.get(c0, capi.STATE_V))
+
+
+@pytest.mark.parametrize("mode,bits,cell", [(1, 20, 0.02), (1, 5, 0.05), (0, 0, 0.1), (0, 0, 0.03), (1, 16, 0.5)])
+def test_swept_broadphase_bit_exact(orc, mode, bits, cell):
+    # swept-box pairs and contacts equal the oracle's brute force at every
+    # cell size, table size and dense grid (one smaller than the scene,
+    # whose cells clamp at the border)
+    from test_oracle_collide import falling_patch_scene
+
+    print(f"swept mode={mode} bits={bits} cell={cell}", flush=True)
+    x, p, tris = falling_patch_scene(np.random.default_rng(5), n=14, big=2, drop=0.02)
+    th = 2e-3
+    with capi.Context() as ctx:
+        d = capi.hash_desc(mode, (0.0, 0.0, 0.0), (16, 4, 16), (cell, cell, cell), bits)
+        vf, t, nn, w = ctx.ccd_detect(d, x, p, tris, th, swept=True)
+        pairs = ctx.ccd_pairs()
+    assert np.array_equal(pairs, orc.swept_pairs(x, p, tris, th))
+    arr, n = orc.ccd_detect_swept(x, p, tris, th)
+    ovf, ot, onn, ow = orc.contacts_arrays(arr, n)
+    assert n > 0
+    assert np.array_equal(vf, ovf) and np.array_equal(t, ot) and np.array_equal(nn, onn) and np.array_equal(w, ow)
+
+
+def test_swept_broadphase_rejects_a_cell_too_small():
+    from test_oracle_collide import falling_patch_scene
+
+    x, p, tris = falling_patch_scene(np.random.default_rng(5), n=6, big=1, drop=0.02)
+    with capi.Context() as ctx:
+        d = capi.hash_desc(1, cell=(1e-4, 1e-4, 1e-4), table_bits=16)
+        with pytest.raises(capi.PbdError, match="more than 4096 cells"):
+            ctx.ccd_detect(d, x, p, tris, 2e-3, swept=True)

@@ -1,0 +1,179 @@
+"""Configs 1-3 at their own settings: config 1 for its three seeds over 120
+frames, and configs 2 (cloth with bending dropped on an icosphere) and 3
+(folded cloth with self contact over a box) bit-exact against the oracle at
+reduced size through the same code paths, with the reference's contact
+handling and with the robust one (swept-box candidates, scale-free
+degenerate-face bound), plus size-independent properties at full size
+(finite state, contacts, run-to-run determinism, and with the robust
+handling no vertex inside the static mesh).
+
+At full size the reference's handling lets the cloth through both static
+meshes: its centroid-cell candidates miss faces that extend more than about
+a cell from their centroid (cell 1.5 x mean cloth edge: 5.9 mm at 256^2,
+against 19.7 mm icosphere faces), and its response skips every face with
+|(p2-p1) x (p3-p1)|^2 <= 1e-6 m^4 (CCD.cpp:2008), i.e. every face of
+these meshes.  The full-size tests of that mode check only what it
+guarantees."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from positionbaseddynamicsold_amd import capi, scenes
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_config1_other_seeds_120_frames(orc, seed):
+    # seed 0 runs in test_gpu_solver.test_config1_120_frames_bit_exact
+    say(f"config1 seed {seed}")
+    spec = scenes.config1(seed)
+    with capi.Context() as ctx:
+        cid = scenes.add_to_context(ctx, spec)
+        ctx.init()
+        ctx.set_solver(dt=scenes.DT, iterations=20, graph=1)
+        pos, tris = orc.grid_mesh(32, 32, 1.0, 1.0, jitter=spec.jitter, seed=seed)
+        oc = orc.OracleCloth(pos, tris, orc.grid_pairs(32, 32), pins=spec.pins)
+        prm = oc.params(dt=scenes.DT)
+        ctx.step(1200)
+        for _ in range(1200):
+            oc.substep(prm, cap=20)
+        assert np.array_equal(ctx.get(cid), oc.x)
+        assert np.array_equal(ctx.get(cid, capi.STATE_V), oc.v)
+
+
+def oracle_scene(orc, sc):
+    """The scene as one oracle vertex array: cloth first, static mesh after
+    it with weight 0; prims are the cloth's."""
+    c = sc.cloth
+    cpos, ctris = orc.grid_mesh(c.rx, c.ry, c.sx, c.sy, jitter=c.jitter, seed=c.seed, origin=c.origin)
+    pos = np.concatenate([cpos, sc.static_pos]).astype(np.float32)
+    tris = np.concatenate([ctris, sc.static_tris + len(cpos)]).astype(np.int32)
+    pins = list(c.pins) + list(range(len(cpos), len(pos)))
+    quads = orc.bending_quads(ctris) if sc.bending else None
+    oc = orc.OracleCloth(pos, tris, orc.grid_pairs(c.rx, c.ry), pins=pins, bending=sc.bending,
+                         stiffness=sc.stiffness, iterations=sc.iterations, quads=quads)
+    if sc.fold is not None:
+        folded = scenes.fold_positions(cpos, *sc.fold)
+        oc.x[: len(cpos)] = folded
+        oc.p[: len(cpos)] = folded
+    return oc, tris, len(cpos)
+
+
+def oracle_substep(orc, oc, tris, g, sc, prm):
+    """projectConstraintWithColliCPU order (PBD_cuda.cu:1130-1210): advect;
+    per sweep: detect at ii % every == 0 and the last sweep, sweep, resolve;
+    integrate.  Returns the contacts seen."""
+    oc.advect(prm)
+    arr, nc, seen = None, 0, 0
+    I = sc.iterations
+    for ii in range(I):
+        if ii % sc.detect_every == 0 or ii == I - 1:
+            if sc.broadphase == 1:
+                arr, nc = orc.ccd_detect_swept(oc.x, oc.p, tris, sc.thickness)
+            else:
+                arr, nc = orc.ccd_detect(g, oc.x, oc.p, tris, sc.thickness)
+            seen += nc
+        oc.sweep(ii)
+        if nc:
+            oc.p[:] = orc.ccd_resolve(arr, nc, tris, oc.x, oc.p, oc.w, sc.thickness, sc.resolve_passes,
+                                      sc.min_normal2 if sc.min_normal2 > 0 else 1e-6)
+    oc.L.orc_integrate(oc.nv, prm.dt, orc.F(oc.x), orc.F(oc.v), orc.F(oc.p))
+    return seen
+
+
+def run_parity(orc, sc, substeps, check_at):
+    oc, tris, ncl = oracle_scene(orc, sc)
+    with capi.Context() as ctx:
+        cid, sid, cell = scenes.add_ccd_scene(ctx, sc)
+        g = orc.grid(1, cell=(cell, cell, cell), table_bits=sc.table_bits)
+        prm = oc.params(dt=scenes.DT)
+        assert np.array_equal(ctx.get(cid), oc.x[:ncl])
+        seen = 0
+        for k in range(1, substeps + 1):
+            ctx.step(1)
+            seen += oracle_substep(orc, oc, tris, g, sc, prm)
+            if k in check_at:
+                assert np.array_equal(ctx.get(cid), oc.x[:ncl]), f"substep {k}"
+        assert np.array_equal(ctx.get(cid, capi.STATE_V), oc.v[:ncl])
+        assert np.array_equal(ctx.get(sid), sc.static_pos)
+        return seen
+
+
+@pytest.mark.parametrize("robust", [False, True])
+def test_config2_reduced_bit_exact(orc, robust):
+    # 40 x 40 cloth with bending over a subdivision-3 icosphere: free fall
+    # (0.2 m to the sphere top) and the first 40 substeps of contact
+    say(f"config2 reduced robust={robust}")
+    sc = scenes.config2(n=40, subdiv=3)
+    sc = sc.robust() if robust else sc
+    seen = run_parity(orc, sc, 160, check_at=(60, 120, 140, 160))
+    assert seen > 0, "no contact reached"
+
+
+@pytest.mark.parametrize("robust", [False, True])
+def test_config3_reduced_bit_exact(orc, robust):
+    # 40 x 40 folded cloth (self contact) over a box meshed 8 x 8 per face
+    say(f"config3 reduced robust={robust}")
+    sc = scenes.config3(n=40, box_div=8)
+    sc = sc.robust() if robust else sc
+    seen = run_parity(orc, sc, 24, check_at=(1, 8, 16, 24))
+    assert seen > 0, "no contact reached"
+
+
+def full_run(sc, substeps):
+    with capi.Context() as ctx:
+        cid, _, _ = scenes.add_ccd_scene(ctx, sc)
+        contacts = 0
+        for _ in range(substeps):
+            ctx.step(1)
+            contacts += ctx.last_contacts()[0]
+        return ctx.get(cid), ctx.get(cid, capi.STATE_V), contacts
+
+
+def inside_sphere(x):
+    # the icosphere's faces sit inside the analytic radius by at most the
+    # chord sag of a subdivision-4 face (0.22 mm)
+    return np.linalg.norm(x.astype(np.float64), axis=1) < 0.3 - 1e-3
+
+
+def inside_box(x):
+    return (np.abs(x[:, 0]) < 0.2 - 1e-3) & (np.abs(x[:, 2]) < 0.2 - 1e-3) & (x[:, 1] < 0.4 - 1e-3) & \
+        (x[:, 1] > 1e-3)
+
+
+@pytest.mark.parametrize("robust", [False, True])
+def test_config2_full_size_properties(robust):
+    # 256 x 256 with bending onto the 2,562-vertex icosphere, 30 substeps
+    # into contact: finite, contacts found, identical on a second run; with
+    # the robust handling no vertex inside the sphere
+    say(f"config2 full size robust={robust}")
+    sc = scenes.config2()
+    sc = sc.robust() if robust else sc
+    x, v, contacts = full_run(sc, 150)
+    assert np.isfinite(x).all() and np.isfinite(v).all()
+    assert contacts > 0
+    say(f"  contacts {contacts}, vertices inside the sphere {int(inside_sphere(x).sum())}")
+    if robust:
+        assert not inside_sphere(x).any()
+    x2, v2, _ = full_run(sc, 150)
+    assert np.array_equal(x, x2) and np.array_equal(v, v2)
+
+
+@pytest.mark.parametrize("robust", [False, True])
+def test_config3_full_size_properties(robust):
+    # 512 x 512 folded cloth (self contact) falling onto the 0.4 m box
+    say(f"config3 full size robust={robust}")
+    sc = scenes.config3()
+    sc = sc.robust() if robust else sc
+    x, v, contacts = full_run(sc, 24)
+    assert np.isfinite(x).all() and np.isfinite(v).all()
+    assert contacts > 0
+    say(f"  contacts {contacts}, vertices inside the box {int(inside_box(x).sum())}")
+    if robust:
+        assert not inside_box(x).any()
+    x2, v2, _ = full_run(sc, 24)
+    assert np.array_equal(x, x2) and np.array_equal(v, v2)

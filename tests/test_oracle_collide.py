@@ -181,3 +181,100 @@ def test_resolve_separates_to_thickness(orc):
         assert d >= 2 * h - tol
         if w[1] == 0:
             assert np.array_equal(q[1:4], p[1:4])  # static face untouched
+
+
+def falling_patch_scene(rng, n=7, big=3, drop=0.03):
+    """A static plane of `big` x `big` large quads (two triangles each) and an
+    n x n jittered patch of small triangles moving down through it."""
+    s = np.linspace(-0.5, 0.5, big + 1)
+    gx, gz = np.meshgrid(s, s, indexing="ij")
+    plane = np.stack([gx.ravel(), np.zeros(gx.size), gz.ravel()], 1)
+    ptris = []
+    for i in range(big):
+        for j in range(big):
+            a, b, c, d = i * (big + 1) + j, i * (big + 1) + j + 1, (i + 1) * (big + 1) + j, (i + 1) * (big + 1) + j + 1
+            ptris += [[a, c, b], [b, c, d]]
+    t = np.linspace(-0.3, 0.3, n)
+    hx, hz = np.meshgrid(t, t, indexing="ij")
+    patch = np.stack([hx.ravel(), np.full(hx.size, 0.01), hz.ravel()], 1) + rng.uniform(-0.01, 0.01, (n * n, 3))
+    qtris = []
+    for i in range(n - 1):
+        for j in range(n - 1):
+            a, b, c, d = i * n + j, i * n + j + 1, (i + 1) * n + j, (i + 1) * n + j + 1
+            qtris += [[a, b, c], [b, d, c]]
+    x = np.concatenate([plane, patch]).astype(np.float32)
+    p = x.copy()
+    m = len(plane)
+    p[m:] += rng.uniform(-0.005, 0.005, (n * n, 3)).astype(np.float32)
+    p[m:, 1] -= np.float32(drop)
+    tris = np.concatenate([np.array(ptris), np.array(qtris) + m]).astype(np.int32)
+    return x, p, tris
+
+
+def test_swept_contacts_equal_all_pairs(orc):
+    # the swept-box set keeps every pair the narrowphase accepts: contacts
+    # over it equal contacts over all (v, f); the set is a strict subset
+    rng = np.random.default_rng(21)
+    total = 0
+    for trial in range(6):
+        x, p, tris = falling_patch_scene(rng, n=6 + trial, big=1 + trial % 3, drop=0.005 + 0.01 * trial)
+        for th in (1e-4, 2e-3, 1e-2):
+            a, n = orc.ccd_detect_swept(x, p, tris, th)
+            b, m = orc.ccd_detect_swept(x, p, tris, th, all_pairs=True)
+            ca, cb = orc.contacts_arrays(a, n), orc.contacts_arrays(b, m)
+            for u, v in zip(ca, cb):
+                assert np.array_equal(u, v)
+            total += n
+            cand = orc.swept_pairs(x, p, tris, th)
+            every = orc.swept_pairs(x, p, tris, th, all_pairs=True)
+            assert len(cand) < len(every) and np.isin(cand, every).all()
+    assert total > 0
+    # random soup: skinny triangles, every vertex moving
+    for seed in range(4):
+        r = np.random.default_rng(100 + seed)
+        nv = 90
+        x = r.uniform(-0.2, 0.2, (nv, 3)).astype(np.float32)
+        p = (x + r.normal(0, 0.05, (nv, 3))).astype(np.float32)
+        tris = np.array([r.choice(nv, 3, replace=False) for _ in range(70)], np.int32)
+        for th in (1e-3, 2e-2):
+            a, n = orc.ccd_detect_swept(x, p, tris, th)
+            b, m = orc.ccd_detect_swept(x, p, tris, th, all_pairs=True)
+            assert n == m
+            for u, v in zip(orc.contacts_arrays(a, n), orc.contacts_arrays(b, m)):
+                assert np.array_equal(u, v)
+
+
+def test_swept_pairs_ignore_vertices_outside_triangles(orc):
+    # a vertex in no triangle is never a candidate (candidates come from
+    # triangle vertices, CCD.cpp:1541-1556)
+    x, p, tris = falling_patch_scene(np.random.default_rng(3), n=5, big=1)
+    lone = len(x)
+    x = np.concatenate([x, [[0.0, 0.005, 0.0]]]).astype(np.float32)
+    p = np.concatenate([p, [[0.0, -0.005, 0.0]]]).astype(np.float32)
+    keys = orc.swept_pairs(x, p, tris, 1e-3)
+    assert not np.any((keys >> 32) == lone)
+    every = orc.swept_pairs(x, p, tris, 1e-3, all_pairs=True)
+    assert not np.any((every >> 32) == lone)
+
+
+def test_resolve_degenerate_bound(orc):
+    # VFResolveTest skips a face with |(p2-p1) x (p3-p1)|^2 <= 1e-6: a 1 cm
+    # face (|n|^2 = 1e-8) is never resolved with the reference bound and is
+    # resolved with a bound below its |n|^2
+    h = 1e-3
+    tris, x, p = moving_vertex_scene((0.25, 0.25), 0.05, -0.001)
+    s = np.float32(0.01)
+    x[1:4] *= s
+    p[1:4] *= s
+    x[[0, 4, 5], :2] *= s
+    p[[0, 4, 5], :2] *= s
+    arr, n = orc.ccd_detect_swept(x, p, tris, h)
+    vf, *_ = orc.contacts_arrays(arr, n)
+    keep = [i for i in range(n) if vf[i].tolist() == [0, 0]]
+    assert keep
+    one = (type(arr[0]) * 1)(arr[keep[0]])
+    w = np.array([1, 0, 0, 0, 1, 1], np.float32)
+    q = orc.ccd_resolve(one, 1, tris, x, p, w, h, 4)
+    assert np.array_equal(q, p)
+    q = orc.ccd_resolve(one, 1, tris, x, p, w, h, 4, min_n2=1e-12)
+    assert q[0, 2] >= 2 * h - 1e-6
