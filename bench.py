@@ -206,12 +206,17 @@ def main():
         log(f"[bench] autotune: {tuned}")
 
     rule = {"rule": a.warm_rule}
+    # launch configuration per run: the plain solver's from the tuning above,
+    # the warm-started solver's from a second tuning once its budget is known
+    # (results do not depend on it)
+    tuned_warm = dict(tuned)
 
     def solver(sweeps, ws, graph=1, warm=0):
+        cfg = tuned_warm if ws else tuned
         ctx.ws_enable(ws)
-        ctx.set_solver(dt=scenes.DT, iterations=sweeps, check_every=tuned["tile_sweeps"],
-                       tile_sweeps=tuned["tile_sweeps"], tile_max_verts=tuned["tile_max_verts"],
-                       block_threads=tuned["block_threads"], graph=graph, warm_iterations=warm,
+        ctx.set_solver(dt=scenes.DT, iterations=sweeps, check_every=cfg["tile_sweeps"],
+                       tile_sweeps=cfg["tile_sweeps"], tile_max_verts=cfg["tile_max_verts"],
+                       block_threads=cfg["block_threads"], graph=graph, warm_iterations=warm,
                        warm_rule=rule["rule"])
 
     # warm-up, with the batch's own training samples recorded along the way
@@ -284,12 +289,22 @@ def main():
     other = budget(1 - a.warm_rule)
     head = budget(a.warm_rule)
     warm_k, r_warm, used_frac = head["budget_kept"], head["residual"], head["kept_fraction"]
+    if not a.no_tune:
+        ctx.snapshot(restore=True)
+        solver(PLAIN_SWEEPS, True, warm=warm_k)
+        t = time.perf_counter()
+        best, _, best_ms, trials = ctx.autotune(2)
+        tuned_warm = {"tile_sweeps": best.tile_sweeps, "block_threads": best.block_threads,
+                      "tile_max_verts": best.tile_max_verts, "trials": trials, "best_ms_per_substep": best_ms,
+                      "seconds": time.perf_counter() - t}
+        log(f"[bench] autotune (warm-started run): {tuned_warm}")
     if a.save_network:
         np.savez(a.save_network, params=ctx.ws_params(), norm=ctx.ws_norm(), count=a.count, seed=a.seed,
                  warmup=a.warmup, collect_substeps=a.collect_substeps, target_sweeps=a.target_sweeps, settle=2,
                  plain_sweeps=PLAIN_SWEEPS, warm_k=warm_k, warm_rule=a.warm_rule,
                  tile_sweeps=tuned["tile_sweeps"], tile_max_verts=tuned["tile_max_verts"],
-                 block_threads=tuned["block_threads"])
+                 block_threads=tuned["block_threads"], warm_tile_sweeps=tuned_warm["tile_sweeps"],
+                 warm_tile_max_verts=tuned_warm["tile_max_verts"], warm_block_threads=tuned_warm["block_threads"])
 
     def timed(ws, steps, warm=0):
         ctx.snapshot(restore=True)
@@ -406,6 +421,7 @@ def main():
             "plain_batched_vertex_iterations_per_s": nv_total * PLAIN_SWEEPS * plain_rate,
             "graph_replays_timed": warm_replays,
             "autotuned": tuned,
+            "autotuned_warm": tuned_warm,
             "single_tile_cloths": sum(1 for i in infos if i.whole),
             "tiles": sum(i.ntiles for i in infos),
             "copy_kernel_gbs": copy_gbs,

@@ -42,7 +42,9 @@ struct pbd_ctx {
     TilePlan plan;
     int plan_sweeps = 0, plan_max_load = 0;
     DBuf<TileDev> tiles;
-    DBuf<int> tverts, tcols, order_whole, order_partial, tile_ids;
+    // launch orders: single-tile cloths' tiles, multi-tile cloths' tiles, and
+    // both (the first launch of a substep runs them side by side)
+    DBuf<int> tverts, tcols, order_whole, order_partial, order_first, tile_ids;
     DBuf<uint32_t> tdist, tbend, town;
     DBuf<float> tdist_rest, tbend_rest, town_rest;
     bool any_whole = false, any_partial = false;
@@ -229,6 +231,9 @@ void upload_plan(pbd_ctx* c) {
     c->town_rest.upload(p.own_rest, c->stream);
     c->order_whole.upload(p.whole_order, c->stream);
     c->order_partial.upload(p.partial_order, c->stream);
+    std::vector<int> first(p.whole_order);  // single-tile cloths first: their blocks run longest
+    first.insert(first.end(), p.partial_order.begin(), p.partial_order.end());
+    c->order_first.upload(first, c->stream);
     std::vector<int> ids(p.tiles.size());
     for (size_t t = 0; t < ids.size(); t++) ids[t] = (int)t;
     c->tile_ids.upload(ids, c->stream);
@@ -698,12 +703,15 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     a.copy_spent = any_tol ? 1 : 0;
     const int start = cur;
     int launched = 0;
-    if (c->any_whole) {
+    // Single-tile cloths run all their sweeps in one block and write
+    // pbuf[start ^ 1]; they go into the first chunk's launch when there is one
+    // (both read pbuf[start] and write pbuf[start ^ 1], on disjoint vertices).
+    a.whole_nsweeps = iterations;
+    a.fused = fused_whole ? 1 : 0;
+    if (c->any_whole && !c->any_partial) {
         a.sweep_base = 0;
         a.nsweeps = iterations;
-        a.fused = fused_whole ? 1 : 0;
         a.chunk = 0;
-        // whole tiles write pbuf[start ^ 1]
         tiles((int)c->order_whole.n, c->order_whole.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
     }
     if (c->any_partial) {
@@ -712,9 +720,9 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
             launched++;
             a.sweep_base = sw;
             a.nsweeps = std::min(ts, iterations - sw);
-            a.fused = 0;
             a.chunk = ch;
-            tiles((int)c->order_partial.n, c->order_partial.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
+            const DBuf<int>& ord = (ch == 0 && c->any_whole) ? c->order_first : c->order_partial;
+            tiles((int)ord.n, ord.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
             cur ^= 1;
             sw += a.nsweeps;
             if (any_tol && ch + 1 < nchunks) {
@@ -799,6 +807,7 @@ void step_ccd(pbd_ctx* c, float dt) {
     a.tol = -1.0;
     a.kmax = c->kmax;
     a.nsweeps = 1;
+    a.whole_nsweeps = 1;
     a.fused = 0;
     a.chunk = 0;
     a.no_coll = 1;
@@ -806,7 +815,7 @@ void step_ccd(pbd_ctx* c, float dt) {
         int cur = c->pcur;
         if (ii % every == 0 || ii == I - 1) detect_contacts(c, c->pbuf[cur].ptr);
         a.sweep_base = ii;
-        for (const DBuf<int>* ord : {&c->order_whole, &c->order_partial}) {
+        for (const DBuf<int>* ord : {&c->order_first}) {
             if (!ord->n) continue;
             if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
             launch_tiles(s, (int)ord->n, ord->ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr,
@@ -968,6 +977,7 @@ void step_cloth(pbd_ctx* c, int ci, float dt) {
     if (k.whole) {
         // advect on load, all sweeps, integrate on store: one launch
         a.nsweeps = I;
+        a.whole_nsweeps = I;
         a.fused = 1;
         tiles(c->pbuf[cur ^ 1].ptr, c->pbuf[cur].ptr);  // p_in is not read when fused
     } else {

@@ -414,20 +414,35 @@ __global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict_
     __syncthreads();
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
-    const float h = cl[vcloth[v]].hscale;
+    // Loads in two rounds (own state, neighbour ids and rest lengths, cloth
+    // id; then the neighbour positions and the cloth scale), so a thread waits
+    // for two memory round trips rather than one per neighbour.
+    const int ci = vcloth[v];
     const float4 P = p[v], X = x[v];
+    int nj[6];
+    float nl[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        nj[k] = nbr[(size_t)v * 6 + k];
+        nl[k] = nbr_rest[(size_t)v * 6 + k];
+    }
+    const float h = cl[ci].hscale;
+    float4 nq[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) nq[k] = p[max(nj[k], 0)];  // index 0 stands in for a missing neighbour
     float f[22];
     f[0] = (P.x - X.x) / h;
     f[1] = (P.y - X.y) / h;
     f[2] = (P.z - X.z) / h;
+#pragma unroll
     for (int k = 0; k < 6; k++) {
-        const int j = nbr[(size_t)v * 6 + k];
+        const int j = nj[k];
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         if (j >= 0) {
-            const float4 Q = p[j];
+            const float4 Q = nq[k];
             const float ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
             const float d = sqrtf(ex * ex + ey * ey + ez * ez);
-            const float L = nbr_rest[(size_t)v * 6 + k];
+            const float L = nl[k];
             if (d > 0.0f) {
                 const float sc = (d - L) / (L * d);
                 sx = sc * ex;
@@ -600,29 +615,53 @@ __global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist
     int cl = -1;
     double a = 0.0, b = 0.0;
     unsigned long long sa = 0, sb = 0;
-    for (int k = 0; k < RESID_PER_THREAD; k++) {
-        long long j = base + 256ll * k;
-        if (j >= n) break;
-        int4 q = dist[j];
-        if (q.w != cl) {
-            if (cl >= 0) flush_resid<RULE>(rs, ncl, cl, a, b, sa, sb);
-            cl = q.w;
-            a = b = 0.0;
-            sa = sb = 0;
+    // prims in batches of RESID_BATCH: records first, then their four
+    // gathers each, so a batch waits for two memory round trips
+    constexpr int RESID_BATCH = 4;
+    for (int k0 = 0; k0 < RESID_PER_THREAD; k0 += RESID_BATCH) {
+        int4 qb[RESID_BATCH];
+        float4 P0b[RESID_BATCH], P1b[RESID_BATCH], D0b[RESID_BATCH], D1b[RESID_BATCH];
+#pragma unroll
+        for (int u = 0; u < RESID_BATCH; u++) {
+            const long long j = base + 256ll * (k0 + u);
+            qb[u] = j < n ? dist[j] : make_int4(0, 0, 0, -1);
         }
-        float4 P0 = p[q.x], P1 = p[q.y], D0 = delta[q.x], D1 = delta[q.y];
-        float L = __int_as_float(q.z);
-        const float r0 = rel_res(P0, P1, L);
-        float4 Q0 = make_float4(P0.x + D0.x, P0.y + D0.y, P0.z + D0.z, 0.0f);
-        float4 Q1 = make_float4(P1.x + D1.x, P1.y + D1.y, P1.z + D1.z, 0.0f);
-        const float r1 = rel_res(Q0, Q1, L);
-        if (RULE == 0) {
-            a = fmax(a, (double)r0);
-            b = fmax(b, (double)r1);
-        } else {
-            sa += sq_fixed(r0);
-            sb += sq_fixed(r1);
+#pragma unroll
+        for (int u = 0; u < RESID_BATCH; u++) {
+            P0b[u] = p[qb[u].x];
+            P1b[u] = p[qb[u].y];
+            D0b[u] = delta[qb[u].x];
+            D1b[u] = delta[qb[u].y];
         }
+        bool done = false;
+#pragma unroll
+        for (int u = 0; u < RESID_BATCH; u++) {
+            if (base + 256ll * (k0 + u) >= n) {
+                done = true;
+                break;
+            }
+            const int4 q = qb[u];
+            if (q.w != cl) {
+                if (cl >= 0) flush_resid<RULE>(rs, ncl, cl, a, b, sa, sb);
+                cl = q.w;
+                a = b = 0.0;
+                sa = sb = 0;
+            }
+            float4 P0 = P0b[u], P1 = P1b[u], D0 = D0b[u], D1 = D1b[u];
+            float L = __int_as_float(q.z);
+            const float r0 = rel_res(P0, P1, L);
+            float4 Q0 = make_float4(P0.x + D0.x, P0.y + D0.y, P0.z + D0.z, 0.0f);
+            float4 Q1 = make_float4(P1.x + D1.x, P1.y + D1.y, P1.z + D1.z, 0.0f);
+            const float r1 = rel_res(Q0, Q1, L);
+            if (RULE == 0) {
+                a = fmax(a, (double)r0);
+                b = fmax(b, (double)r1);
+            } else {
+                sa += sq_fixed(r0);
+                sb += sq_fixed(r1);
+            }
+        }
+        if (done) break;
     }
     int c0 = __shfl(cl, 0);
     unsigned long long same = __ballot(cl == c0 || cl < 0);

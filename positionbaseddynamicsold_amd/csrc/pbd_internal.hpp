@@ -106,7 +106,8 @@ struct TileDev {
 struct SolveArgs {
     float dt;
     int sweep_base;      // global sweep index of this launch's first sweep
-    int nsweeps;         // sweeps this launch (partial tiles) / cap (whole tiles)
+    int nsweeps;         // sweeps this launch for multi-tile cloths' tiles
+    int whole_nsweeps;   // sweeps this launch for single-tile cloths' tiles
     int check_every;
     double tol;
     int kmax;            // length of each stiffness table

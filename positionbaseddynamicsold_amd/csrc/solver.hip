@@ -379,7 +379,7 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     // per-cloth sweep budget (warm start: short when the prediction was kept);
     // a cloth whose budget is spent is left where it is (its final buffer is
     // tracked per cloth), one that met the tolerance is carried forward
-    int nsweeps = a.nsweeps;
+    int nsweeps = td.whole ? a.whole_nsweeps : a.nsweeps;
     if (sweep_cap) nsweeps = min(nsweeps, sweep_cap[td.cloth] - a.sweep_base);
     const double tol = cd.tol >= 0.0 ? cd.tol : a.tol;
     bool carry = false;

@@ -34,6 +34,9 @@ def main():
     cfg = {k: int(net[k]) for k in ("count", "seed", "warmup", "collect_substeps", "target_sweeps", "settle",
                                      "plain_sweeps", "warm_k", "warm_rule", "tile_sweeps", "tile_max_verts",
                                      "block_threads")}
+    # the warm-started run's own launch configuration, when the network file holds one
+    wcfg = {k: int(net["warm_" + k]) if "warm_" + k in net else cfg[k]
+            for k in ("tile_sweeps", "tile_max_verts", "block_threads")}
     specs = scenes.config5(seed=cfg["seed"], count=cfg["count"])
     ctx = capi.Context(0)
     for s in specs:
@@ -41,10 +44,11 @@ def main():
     ctx.init()
 
     def solver(ws, graph=0, warm=0):
+        c = wcfg if ws else cfg
         ctx.ws_enable(ws)
-        ctx.set_solver(dt=scenes.DT, iterations=cfg["plain_sweeps"], check_every=cfg["tile_sweeps"],
-                       tile_sweeps=cfg["tile_sweeps"], tile_max_verts=cfg["tile_max_verts"],
-                       block_threads=cfg["block_threads"], graph=graph, warm_iterations=warm,
+        ctx.set_solver(dt=scenes.DT, iterations=cfg["plain_sweeps"], check_every=c["tile_sweeps"],
+                       tile_sweeps=c["tile_sweeps"], tile_max_verts=c["tile_max_verts"],
+                       block_threads=c["block_threads"], graph=graph, warm_iterations=warm,
                        warm_rule=cfg["warm_rule"])
 
     # the bench's substeps before its snapshot
