@@ -87,6 +87,9 @@ def parse():
                     "positionbaseddynamicsold_amd.tilework needs to repeat the profiled passes")
     ap.add_argument("--warm-rule", type=int, default=1, choices=(0, 1),
                     help="fallback rule of the headline run: 0 max |d-L|/L, 1 RMS (the other is reported too)")
+    ap.add_argument("--warm-gate", type=float, default=1e-4,
+                    help="no correction is predicted for a vertex whose one-ring prims are all within this "
+                         "relative strain (0: every vertex); the run without the gate is reported too")
     return ap.parse_args()
 
 
@@ -205,7 +208,7 @@ def main():
                  "seconds": time.perf_counter() - t}
         log(f"[bench] autotune: {tuned}")
 
-    rule = {"rule": a.warm_rule}
+    rule = {"rule": a.warm_rule, "gate": a.warm_gate}
     # launch configuration per run: the plain solver's from the tuning above,
     # the warm-started solver's from a second tuning once its budget is known
     # (results do not depend on it)
@@ -217,7 +220,7 @@ def main():
         ctx.set_solver(dt=scenes.DT, iterations=sweeps, check_every=cfg["tile_sweeps"],
                        tile_sweeps=cfg["tile_sweeps"], tile_max_verts=cfg["tile_max_verts"],
                        block_threads=cfg["block_threads"], graph=graph, warm_iterations=warm,
-                       warm_rule=rule["rule"])
+                       warm_rule=rule["rule"], warm_gate=rule["gate"])
 
     # warm-up, with the batch's own training samples recorded along the way
     solver(PLAIN_SWEEPS, False)
@@ -256,9 +259,9 @@ def main():
     r_plain_c, _ = residuals(False)
     r_plain = float(r_plain_c.mean())
 
-    def budget(rl):
+    def budget(rl, gate):
         """Smallest warm budget k (discarded cloths keep 20) with mean residual <= R."""
-        rule["rule"] = rl
+        rule["rule"], rule["gate"] = rl, gate
         tried = {}
 
         def mean_warm(k):
@@ -275,19 +278,20 @@ def main():
                 lo = mid + 1
         r_c, used = mean_warm(hi)
         ratio = r_c / np.maximum(r_plain_c, 1e-30)
-        log(f"[bench] rule {'RMS' if rl else 'max'}: warm start reaches {r_c.mean():.5e} (R = {r_plain:.5e}) "
+        log(f"[bench] rule {'RMS' if rl else 'max'}, gate {gate:g}: warm start reaches {r_c.mean():.5e} (R = {r_plain:.5e}) "
             f"with budget {hi} (kept) / {PLAIN_SWEEPS} (discarded); tried {dict(sorted(tried.items()))}; kept for "
             f"{100 * used:.0f}% of cloth-substeps; per-cloth residual ratio warm/plain median "
             f"{np.median(ratio):.3f}, p90 {np.quantile(ratio, 0.9):.3f}, max {ratio.max():.3f}")
-        return {"rule": "RMS" if rl else "max", "budget_kept": hi, "budget_discarded": PLAIN_SWEEPS,
+        return {"rule": "RMS" if rl else "max", "gate": gate, "budget_kept": hi, "budget_discarded": PLAIN_SWEEPS,
                 "residual": float(r_c.mean()), "kept_fraction": used,
                 "per_cloth_ratio_median": float(np.median(ratio)),
                 "per_cloth_ratio_p90": float(np.quantile(ratio, 0.9)),
                 "per_cloth_ratio_max": float(ratio.max()),
                 "budgets_tried": {str(k): v for k, v in sorted(tried.items())}}
 
-    other = budget(1 - a.warm_rule)
-    head = budget(a.warm_rule)
+    other = budget(1 - a.warm_rule, a.warm_gate)
+    ungated = budget(a.warm_rule, 0.0) if a.warm_gate > 0 else None
+    head = budget(a.warm_rule, a.warm_gate)
     warm_k, r_warm, used_frac = head["budget_kept"], head["residual"], head["kept_fraction"]
     if not a.no_tune:
         ctx.snapshot(restore=True)
@@ -301,7 +305,7 @@ def main():
     if a.save_network:
         np.savez(a.save_network, params=ctx.ws_params(), norm=ctx.ws_norm(), count=a.count, seed=a.seed,
                  warmup=a.warmup, collect_substeps=a.collect_substeps, target_sweeps=a.target_sweeps, settle=2,
-                 plain_sweeps=PLAIN_SWEEPS, warm_k=warm_k, warm_rule=a.warm_rule,
+                 plain_sweeps=PLAIN_SWEEPS, warm_k=warm_k, warm_rule=a.warm_rule, warm_gate=a.warm_gate,
                  tile_sweeps=tuned["tile_sweeps"], tile_max_verts=tuned["tile_max_verts"],
                  block_threads=tuned["block_threads"], warm_tile_sweeps=tuned_warm["tile_sweeps"],
                  warm_tile_max_verts=tuned_warm["tile_max_verts"], warm_block_threads=tuned_warm["block_threads"])
@@ -352,6 +356,10 @@ def main():
     rule["rule"] = 1 - a.warm_rule
     other["substeps_per_s"], other["ms_per_step"], _ = timed(True, a.steps, warm=other["budget_kept"])
     rule["rule"] = a.warm_rule
+    if ungated is not None:
+        rule["gate"] = 0.0
+        ungated["substeps_per_s"], ungated["ms_per_step"], _ = timed(True, a.steps, warm=ungated["budget_kept"])
+        rule["gate"] = a.warm_gate
     warm_rate, warm_ms, warm_replays = timed(True, a.steps, warm=warm_k)
     log(f"[bench] plain batched {plain_rate:.2f} substeps/s ({plain_ms:.2f} ms); warm start + batched "
         f"{warm_rate:.2f} substeps/s ({warm_ms:.2f} ms): x{warm_rate / unb_rate:.2f} vs plain unbatched, "
@@ -402,6 +410,7 @@ def main():
                                "substeps_compared": a.match_substeps},
             "warm_start_rule": head,
             "warm_start_other_rule": other,
+            "warm_start_without_gate": ungated,
             "plain_substeps_per_s": unb_rate,
             "plain_ms_per_step": unb_ms,
             "plain_definition": "plain solver, unbatched: each cloth stepped alone (its own launches, LDS "

@@ -93,6 +93,9 @@ typedef struct {
                                exactly the plain solver's substep */
     int warm_rule;          /* fallback rule: the prediction is kept iff its residual does not exceed the
                                unwarmed start's, measured as max |d-L|/L (0) or as the RMS of |d-L|/L (1) */
+    float warm_gate;        /* a vertex whose one-ring distance prims all have |d-L|/L < warm_gate gets
+                               no predicted correction (0 = every vertex gets one); a cloth with no strained
+                               vertex then keeps the zero prediction and runs warm_iterations sweeps */
 } pbd_solver_params;
 
 typedef struct {

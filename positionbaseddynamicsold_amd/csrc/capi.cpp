@@ -583,7 +583,7 @@ void ws_apply_prediction(pbd_ctx* c) {
     ws_build_nbr(c);
     float4* p = c->pbuf[c->pcur].ptr;
     ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, c->ws.nbr.ptr, c->ws.nbr_rest.ptr, c->nv_total,
-             c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr);
+             c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr, c->sp.warm_gate);
     launch_fill_u64(s, c->ws.rs.ptr, 4 * (size_t)ncl, 0ull);
     ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.rs.ptr, ncl, c->sp.warm_rule);
     int full = c->sp.iterations;
@@ -1252,6 +1252,7 @@ int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* p) {
                   "block_threads must be a multiple of 64 up to 1024");
         PBD_CHECK(p->warm_iterations >= 0, PBD_ERR_ARG, "warm_iterations must be >= 0");
         PBD_CHECK(p->warm_rule == 0 || p->warm_rule == 1, PBD_ERR_ARG, "warm_rule must be 0 (max) or 1 (RMS)");
+        PBD_CHECK(p->warm_gate >= 0.0f, PBD_ERR_ARG, "warm_gate must be >= 0");
         ctx->sp = *p;
         if (ctx->inited) {
             set_device(ctx);
@@ -1700,7 +1701,7 @@ int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
             advect_all(ctx, dt);
             float4* p = ctx->pbuf[ctx->pcur].ptr;
             ws_infer(s, W.W.ptr, ctx->cl.ptr, ctx->vcloth.ptr, ctx->x.ptr, p, W.nbr.ptr, W.nbr_rest.ptr, ctx->nv_total,
-                     W.delta.ptr, W.X.ptr + 22 * W.ns, W.norm.ptr, W.feat.ptr, W.hs.ptr);
+                     W.delta.ptr, W.X.ptr + 22 * W.ns, W.norm.ptr, W.feat.ptr, W.hs.ptr, 0.0f);
             PBD_HIP(hipMemcpyAsync(W.psave.ptr, p, sizeof(float4) * ctx->nv_total, hipMemcpyDeviceToDevice, s));
             double tol = ctx->sp.tol;
             ctx->sp.tol = -1.0;

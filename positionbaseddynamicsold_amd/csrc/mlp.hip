@@ -395,7 +395,9 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float
 // 16-byte chunk q holds features 4q..4q+3 then 16+4q..16+4q+3 (zero past 21),
 // so lane (r, q) of the network kernel loads both K-steps of its A operand
 // for vertex r with a single 16-byte load.  Also writes the output scale
-// h (0 for pinned vertices) and, when asked, the raw FP32 features.
+// h (0 for pinned vertices, and with a gate > 0 for vertices whose one-ring
+// prims are all within gate of their rest length: no correction is predicted
+// where no constraint is violated) and, when asked, the raw FP32 features.
 //
 // k_ws_mlp: persistent; the weights stay in LDS, each wave takes 16 vertices
 // at a time with the next group's features already in flight, and writes
@@ -405,7 +407,7 @@ __global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict_
                                                      const float4* __restrict__ x, const float4* __restrict__ p,
                                                      const int* __restrict__ nbr, const float* __restrict__ nbr_rest,
                                                      int nv, const float* __restrict__ norm, uint4* __restrict__ feat,
-                                                     float* __restrict__ hs, float* __restrict__ feat_out) {
+                                                     float* __restrict__ hs, float* __restrict__ feat_out, float gate) {
     __shared__ float s_mu[22], s_rsd[22];
     if (threadIdx.x < 22) {
         s_mu[threadIdx.x] = norm[threadIdx.x];
@@ -434,6 +436,7 @@ __global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict_
     f[0] = (P.x - X.x) / h;
     f[1] = (P.y - X.y) / h;
     f[2] = (P.z - X.z) / h;
+    bool strained = !(gate > 0.0f);  // gate 0: every vertex; else some one-ring prim with |d - L| >= gate L
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         const int j = nj[k];
@@ -443,6 +446,7 @@ __global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict_
             const float ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
             const float d = sqrtf(ex * ex + ey * ey + ez * ez);
             const float L = nl[k];
+            strained |= !(fabsf(d - L) < gate * L);
             if (d > 0.0f) {
                 const float sc = (d - L) / (L * d);
                 sx = sc * ex;
@@ -469,7 +473,7 @@ __global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict_
             }
     uint4* row = feat + (size_t)v * 4;
     for (int q = 0; q < 4; q++) row[q] = make_uint4(b[4 * q], b[4 * q + 1], b[4 * q + 2], b[4 * q + 3]);
-    hs[v] = pinned ? 0.0f : h;
+    hs[v] = (pinned || !strained) ? 0.0f : h;
 }
 
 // Weights the network kernel keeps in LDS (the inference copies).
@@ -777,7 +781,7 @@ size_t ws_feature_bytes(int nv) { return (size_t)std::max(nv, 1) * 64; }
 
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
               const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm,
-              void* feat_scratch, float* hs_scratch) {
+              void* feat_scratch, float* hs_scratch, float gate) {
     if (nv <= 0) return;
     static int max_blocks = 0;
     if (!max_blocks) {
@@ -789,7 +793,7 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
     }
     uint4* feat = (uint4*)feat_scratch;
     hipLaunchKernelGGL(k_ws_features, dim3(nb256(nv)), dim3(256), 0, s, cl, vcloth, x, p, nbr, nbr_rest, nv, norm,
-                       feat, hs_scratch, feat_out);
+                       feat, hs_scratch, feat_out, gate);
     PBD_HIP(hipGetLastError());
     int groups = (nv + 15) / 16;
     int blocks = std::min((groups + MLP_WAVES - 1) / MLP_WAVES, max_blocks);

@@ -220,7 +220,7 @@ void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float
               int step);
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
               const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm,
-              void* feat_scratch, float* hs_scratch);
+              void* feat_scratch, float* hs_scratch, float gate);
 size_t ws_feature_bytes(int nv);
 void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm);
 void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* rs,

@@ -49,7 +49,7 @@ def main():
         ctx.set_solver(dt=scenes.DT, iterations=cfg["plain_sweeps"], check_every=c["tile_sweeps"],
                        tile_sweeps=c["tile_sweeps"], tile_max_verts=c["tile_max_verts"],
                        block_threads=c["block_threads"], graph=graph, warm_iterations=warm,
-                       warm_rule=cfg["warm_rule"])
+                       warm_rule=cfg["warm_rule"], warm_gate=float(net["warm_gate"]) if "warm_gate" in net else 0.0)
 
     # the bench's substeps before its snapshot
     solver(False, graph=1)
