@@ -33,23 +33,30 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // bf16 round-to-nearest-even (v_cvt_pk_bf16_f32; NaN stays NaN)
 __device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 
-typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
+typedef float f32v2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16v2 __attribute__((ext_vector_type(2)));
+// Round-to-nearest-even BF16 of four floats; the two-element conversions map
+// to one v_cvt_pk_bf16_f32 per pair.
 __device__ __forceinline__ bf16x4 pack_bf16(float a, float b, float c, float d) {
-    bf16v4 v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
-    return __builtin_bit_cast(bf16x4, v);
+    const bf16v2 lo = __builtin_convertvector((f32v2){a, b}, bf16v2);
+    const bf16v2 hi = __builtin_convertvector((f32v2){c, d}, bf16v2);
+    return __builtin_bit_cast(bf16x4, make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi)));
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // GELU in its tanh form, 0.5 x (1 + tanh(u)) with u = sqrt(2/pi) (x + 0.044715 x^3),
 // evaluated as the identical x * sigmoid(2u) on the hardware exp2 and
-// reciprocal; saturates to 0 / x without clamping.  Training and inference
-// share it.  dy/dx = s + x s (1 - s) 2u'(x).
+// reciprocal, with the constants folded so that exp(-2u) = 2^(x (B + A x^2)):
+// five VALU operations and two transcendentals per value (the inference
+// kernel's bound).  Saturates to 0 / x without clamping.  Training and
+// inference share it.  dy/dx = s + x s (1 - s) 2u'(x).
 __device__ __forceinline__ void gelu(float x, float* y, float* dy) {
     const float c2 = 1.5957691216f;  // 2 sqrt(2/pi)
     const float k = 0.044715f;
+    const float B = -2.3022082f;     // -2 sqrt(2/pi) log2(e)
+    const float A = -0.10294324f;    // 0.044715 B
     const float x2 = x * x;
-    const float z = c2 * x * (1.0f + k * x2);                          // 2u
-    const float e = __builtin_amdgcn_exp2f(-z * 1.4426950409f);        // exp(-2u)
+    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(A, x2, B));  // exp(-2u)
     const float sg = __builtin_amdgcn_rcpf(1.0f + e);
     *y = x * sg;
     if (dy) *dy = sg + x * sg * (1.0f - sg) * c2 * (1.0f + 3.0f * k * x2);
