@@ -410,19 +410,14 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float
 // at a time with the next group's features already in flight, and writes
 // delta = h * (y * sdy) straight from the accumulators.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
-                                                     const float4* __restrict__ x, const float4* __restrict__ p,
-                                                     const int* __restrict__ nbr, const float* __restrict__ nbr_rest,
-                                                     int nv, const float* __restrict__ norm, uint4* __restrict__ feat,
-                                                     float* __restrict__ hs, float* __restrict__ feat_out, float gate) {
-    __shared__ float s_mu[22], s_rsd[22];
-    if (threadIdx.x < 22) {
-        s_mu[threadIdx.x] = norm[threadIdx.x];
-        s_rsd[threadIdx.x] = 1.0f / norm[22 + threadIdx.x];
-    }
-    __syncthreads();
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nv) return;
+// Features of vertex v: the standardised BF16 row in MFMA operand order
+// (b[4q..4q+3] = 16-byte chunk q), the output scale (returned) and, when
+// asked, the raw FP32 features.  s_mu / s_rsd: feature mean and 1 / sd.
+__device__ __forceinline__ float vertex_features(const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
+                                                 const float4* __restrict__ x, const float4* __restrict__ p,
+                                                 const int* __restrict__ nbr, const float* __restrict__ nbr_rest,
+                                                 int v, const float* s_mu, const float* s_rsd, float gate,
+                                                 uint32_t* b, float* __restrict__ feat_out) {
     // Loads in two rounds (own state, neighbour ids and rest lengths, cloth
     // id; then the neighbour positions and the cloth scale), so a thread waits
     // for two memory round trips rather than one per neighbour.
@@ -469,18 +464,39 @@ __global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict_
     f[21] = pinned ? 1.0f : 0.0f;
     if (feat_out)
         for (int k = 0; k < 22; k++) feat_out[(size_t)v * 22 + k] = f[k];
-    uint32_t b[16];  // 32 BF16, two per word, in MFMA operand order
+    // 32 BF16, two per word, in MFMA operand order
+#pragma unroll
     for (int q = 0; q < 4; q++)
+#pragma unroll
         for (int half = 0; half < 2; half++)
+#pragma unroll
             for (int j = 0; j < 4; j += 2) {
                 const int k0 = 16 * half + 4 * q + j;
                 const uint32_t lo = k0 < 22 ? f2bf(norm_feature(f[k0], s_mu[k0], s_rsd[k0])) : 0u;
                 const uint32_t hi = k0 + 1 < 22 ? f2bf(norm_feature(f[k0 + 1], s_mu[k0 + 1], s_rsd[k0 + 1])) : 0u;
                 b[4 * q + 2 * half + j / 2] = lo | (hi << 16);
             }
+    return (pinned || !strained) ? 0.0f : h;
+}
+
+__global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
+                                                     const float4* __restrict__ x, const float4* __restrict__ p,
+                                                     const int* __restrict__ nbr, const float* __restrict__ nbr_rest,
+                                                     int nv, const float* __restrict__ norm, uint4* __restrict__ feat,
+                                                     float* __restrict__ hs, float* __restrict__ feat_out, float gate) {
+    __shared__ float s_mu[22], s_rsd[22];
+    if (threadIdx.x < 22) {
+        s_mu[threadIdx.x] = norm[threadIdx.x];
+        s_rsd[threadIdx.x] = 1.0f / norm[22 + threadIdx.x];
+    }
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    uint32_t b[16];
+    const float h = vertex_features(cl, vcloth, x, p, nbr, nbr_rest, v, s_mu, s_rsd, gate, b, feat_out);
     uint4* row = feat + (size_t)v * 4;
     for (int q = 0; q < 4; q++) row[q] = make_uint4(b[4 * q], b[4 * q + 1], b[4 * q + 2], b[4 * q + 3]);
-    hs[v] = (pinned || !strained) ? 0.0f : h;
+    hs[v] = h;
 }
 
 // Weights the network kernel keeps in LDS (the inference copies).
@@ -510,11 +526,44 @@ __device__ __forceinline__ bf16x4 gelu4(f32x4 a) {
     return pack_bf16(y0, y1, y2, y3);
 }
 
-// Every layer is computed transposed, out^T = W * act^T, weights as the MFMA A
-// operand and activations as B.  Lane (r = l % 16, q = l / 16) then holds,
-// for output tile t, outputs 16t + 4q + i of vertex r -- exactly its B
-// operand for K-step t of the next layer -- so activations never leave the
-// registers; biases enter as the accumulator input.
+// The network on 16 vertices, one wave: lane (r = l % 16, q = l / 16) passes
+// the 16-byte feature chunk q of vertex r and gets back, in lanes q == 0,
+// outputs 0..3 of vertex r (standardised).  Every layer is computed
+// transposed, out^T = W * act^T, weights as the MFMA A operand and
+// activations as B.  Lane (r, q) then holds, for output tile t, outputs
+// 16t + 4q + i of vertex r -- exactly its B operand for K-step t of the next
+// layer -- so activations never leave the registers; biases enter as the
+// accumulator input.
+__device__ __forceinline__ f32x4 mlp_group(const InfW& W, const uint4 cur, int r, int q) {
+    const bf16x4 x0 = as_bf16x4(cur.x, cur.y), x1 = as_bf16x4(cur.z, cur.w);
+    f32x4 acc[4];
+    bf16x4 h1[4], h2[4];
+    // layer 1: K = 32 (features); A operand of K-step st at 8q + 4st
+    for (int t = 0; t < 4; t++) acc[t] = *(const f32x4*)(W.b1 + 16 * t + 4 * q);
+    for (int st = 0; st < 2; st++) {
+        const bf16x4 x = st == 0 ? x0 : x1;
+        bf16x4 a[4];
+        for (int t = 0; t < 4; t++) a[t] = *(const bf16x4*)(W.i1 + (16 * t + r) * 32 + 8 * q + 4 * st);
+        for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], x, acc[t], 0, 0, 0);
+    }
+    for (int t = 0; t < 4; t++) h1[t] = gelu4(acc[t]);
+    // layer 2: K = 64; A operand of K-step st at 16q + 4st
+    for (int t = 0; t < 4; t++) acc[t] = *(const f32x4*)(W.b2 + 16 * t + 4 * q);
+    for (int st = 0; st < 4; st++) {
+        bf16x4 a[4];
+        for (int t = 0; t < 4; t++) a[t] = *(const bf16x4*)(W.i2 + (16 * t + r) * 64 + 16 * q + 4 * st);
+        for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], h1[st], acc[t], 0, 0, 0);
+    }
+    for (int t = 0; t < 4; t++) h2[t] = gelu4(acc[t]);
+    // layer 3: 16 output rows (3 used), K = 64
+    f32x4 o = *(const f32x4*)(W.b3 + 4 * q);
+    for (int st = 0; st < 4; st++) {
+        const bf16x4 a = *(const bf16x4*)(W.i3 + r * 64 + 16 * q + 4 * st);
+        o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[st], o, 0, 0, 0);
+    }
+    return o;
+}
+
 __global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_mlp(const MlpW* __restrict__ gW, const uint4* __restrict__ feat,
                                                            const float* __restrict__ hs,
                                                            const float* __restrict__ norm, int nv,
@@ -547,39 +596,82 @@ __global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_mlp(const MlpW* __rest
         next = fetch(g + stride);
         h_next = fetch_h(g + stride);
         const int v = g * 16 + r;
-        const bf16x4 x0 = as_bf16x4(cur.x, cur.y), x1 = as_bf16x4(cur.z, cur.w);
-        f32x4 acc[4];
-        bf16x4 h1[4], h2[4];
-        // layer 1: K = 32 (features); A operand of K-step st at 8q + 4st
-        for (int t = 0; t < 4; t++) acc[t] = *(const f32x4*)(W.b1 + 16 * t + 4 * q);
-        for (int st = 0; st < 2; st++) {
-            const bf16x4 x = st == 0 ? x0 : x1;
-            bf16x4 a[4];
-            for (int t = 0; t < 4; t++) a[t] = *(const bf16x4*)(W.i1 + (16 * t + r) * 32 + 8 * q + 4 * st);
-            for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], x, acc[t], 0, 0, 0);
-        }
-        for (int t = 0; t < 4; t++) h1[t] = gelu4(acc[t]);
-        // layer 2: K = 64; A operand of K-step st at 16q + 4st
-        for (int t = 0; t < 4; t++) acc[t] = *(const f32x4*)(W.b2 + 16 * t + 4 * q);
-        for (int st = 0; st < 4; st++) {
-            bf16x4 a[4];
-            for (int t = 0; t < 4; t++) a[t] = *(const bf16x4*)(W.i2 + (16 * t + r) * 64 + 16 * q + 4 * st);
-            for (int t = 0; t < 4; t++)
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], h1[st], acc[t], 0, 0, 0);
-        }
-        for (int t = 0; t < 4; t++) h2[t] = gelu4(acc[t]);
-        // layer 3: 16 output rows (3 used), K = 64
-        f32x4 o = *(const f32x4*)(W.b3 + 4 * q);
-        for (int st = 0; st < 4; st++) {
-            const bf16x4 a = *(const bf16x4*)(W.i3 + r * 64 + 16 * q + 4 * st);
-            o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[st], o, 0, 0, 0);
-        }
+        const f32x4 o = mlp_group(W, cur, r, q);
         // lane (r, 0) holds outputs 0..3 of vertex r
         if (q == 0 && v < nv) {
             float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (h != 0.0f) d = make_float4(h * (o[0] * sdy0), h * (o[1] * sdy1), h * (o[2] * sdy2), 0.0f);
             delta[v] = d;
         }
+    }
+}
+
+// Features and network in one kernel (the solver's path, no raw features
+// kept).  Each wave takes 64 vertices at a time: every lane computes one
+// vertex's feature row and output scale into the wave's LDS area (chunk-major,
+// so the 16-byte writes of consecutive lanes are consecutive), then the wave
+// runs the network on the four groups of 16 from there.  Saves writing and
+// re-reading 68 bytes per vertex against the two-kernel form.
+struct alignas(16) InferStage {
+    uint4 chunk[4][64];  // [q][vertex]: 16-byte feature chunk q of each vertex
+    float hs[64];
+};
+
+__global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_infer(
+    const MlpW* __restrict__ gW, const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
+    const float4* __restrict__ x, const float4* __restrict__ p, const int* __restrict__ nbr,
+    const float* __restrict__ nbr_rest, int nv, const float* __restrict__ norm, float gate,
+    float4* __restrict__ delta) {
+    __shared__ InfW W;
+    __shared__ InferStage stage[MLP_WAVES];
+    __shared__ float s_mu[22], s_rsd[22];
+    {
+        const uint4* src = (const uint4*)gW->i1;  // i1, i2, i3, b1, b2, b3 are contiguous
+        uint4* dst = (uint4*)&W;
+        for (int i = threadIdx.x; i < (int)(sizeof(InfW) / 16); i += blockDim.x) dst[i] = src[i];
+        if (threadIdx.x < 22) {
+            s_mu[threadIdx.x] = norm[threadIdx.x];
+            s_rsd[threadIdx.x] = 1.0f / norm[22 + threadIdx.x];
+        }
+        __syncthreads();
+    }
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    InferStage& S = stage[wv];
+    const float sdy0 = norm[44], sdy1 = norm[45], sdy2 = norm[46];
+    const int nblk = (nv + 63) / 64;
+    for (int g = blockIdx.x * MLP_WAVES + wv; g < nblk; g += gridDim.x * MLP_WAVES) {
+        {
+            const int v = g * 64 + l;
+            uint32_t b[16];
+            float h = 0.0f;
+            if (v < nv) {
+                h = vertex_features(cl, vcloth, x, p, nbr, nbr_rest, v, s_mu, s_rsd, gate, b, nullptr);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; k++) b[k] = 0u;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) S.chunk[c][l] = make_uint4(b[4 * c], b[4 * c + 1], b[4 * c + 2], b[4 * c + 3]);
+            S.hs[l] = h;
+        }
+        // the wave's own LDS writes, visible to all its lanes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int s = 0; s < 4; s++) {
+            const int vl = 16 * s + r, v = g * 64 + vl;
+            const f32x4 o = mlp_group(W, S.chunk[q][vl], r, q);
+            if (q == 0 && v < nv) {
+                const float h = S.hs[vl];
+                float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (h != 0.0f) d = make_float4(h * (o[0] * sdy0), h * (o[1] * sdy1), h * (o[2] * sdy2), 0.0f);
+                delta[v] = d;
+            }
+        }
+        // every lane has read the stage before the next group overwrites it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -599,7 +691,7 @@ __device__ __forceinline__ float rel_res(float4 a, float4 b, float L) {
 // RESID_PER_THREAD prims (stride 256) and flushes them with atomics only when
 // its cloth changes or at the end; a wave whose lanes all end on one cloth
 // reduces first.
-constexpr int RESID_PER_THREAD = 8;
+constexpr int RESID_PER_THREAD = 64;  // long runs per thread: few flushes (measured 8: 0.57 ms, 64: 0.30 ms)
 
 __device__ __forceinline__ unsigned long long sq_fixed(float r) {
     const float q = fminf(r * r, 1.0f);
@@ -628,7 +720,7 @@ __global__ __launch_bounds__(256) void k_ws_resid2(const int4* __restrict__ dist
     unsigned long long sa = 0, sb = 0;
     // prims in batches of RESID_BATCH: records first, then their four
     // gathers each, so a batch waits for two memory round trips
-    constexpr int RESID_BATCH = 4;
+    constexpr int RESID_BATCH = 2;
     for (int k0 = 0; k0 < RESID_PER_THREAD; k0 += RESID_BATCH) {
         int4 qb[RESID_BATCH];
         float4 P0b[RESID_BATCH], P1b[RESID_BATCH], D0b[RESID_BATCH], D1b[RESID_BATCH];
@@ -797,6 +889,22 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
         PBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ws_mlp, 64 * MLP_WAVES, 0));
         max_blocks = std::max(1, ncu) * std::max(1, per_cu);
+    }
+    if (!feat_out) {
+        static int max_blocks_fused = 0;
+        if (!max_blocks_fused) {
+            int dev = 0, ncu = 0, per_cu = 0;
+            PBD_HIP(hipGetDevice(&dev));
+            PBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ws_infer, 64 * MLP_WAVES, 0));
+            max_blocks_fused = std::max(1, ncu) * std::max(1, per_cu);
+        }
+        const int nblk = (nv + 63) / 64;
+        const int blocks = std::min((nblk + MLP_WAVES - 1) / MLP_WAVES, max_blocks_fused);
+        hipLaunchKernelGGL(k_ws_infer, dim3(blocks), dim3(64 * MLP_WAVES), 0, s, (const MlpW*)W, cl, vcloth, x, p,
+                           nbr, nbr_rest, nv, norm, gate, delta);
+        PBD_HIP(hipGetLastError());
+        return;
     }
     uint4* feat = (uint4*)feat_scratch;
     hipLaunchKernelGGL(k_ws_features, dim3(nb256(nv)), dim3(256), 0, s, cl, vcloth, x, p, nbr, nbr_rest, nv, norm,
