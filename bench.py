@@ -374,11 +374,14 @@ def main():
     infos = [ctx.info(i) for i in ids]
     log(f"[bench] k_tiles {prof.tile_launches} launches, {prof.tile_ms / max(1, prof.tile_launches):.3f} ms avg, "
         f"{achieved:.0f} GB/s algorithmic (plain run {plain_achieved:.0f}); copy kernel {copy_gbs:.0f} GB/s")
+    # HBM bytes per launch from the committed counter runs of the same launch
+    # configuration (none if the counters were taken on another one)
     traffic = None
     tf = ROOT / "profiles" / "k_tiles_batch512_traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+            key = "ts{tile_sweeps}_bt{block_threads}_mv{tile_max_verts}".format(**tuned_warm)
+            traffic = json.loads(tf.read_text())["warm"][key]["hbm_bytes_per_launch"]
         except Exception:
             traffic = None
     cpu = None

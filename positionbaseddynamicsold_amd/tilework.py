@@ -1,7 +1,7 @@
 """The bench's tile-kernel passes on the 512-cloth batch, without the training
 around them, as a short program for hardware-counter collection.
 
-    python -m positionbaseddynamicsold_amd.tilework --network NET.npz [--steps K]
+    python -m positionbaseddynamicsold_amd.tilework --network NET.npz [--steps K] [--warm-tile-sweeps N]
 
 NET.npz is the batch network bench.py writes with --save-network (parameters,
 normalisation, warm budget, fallback rule, tile configuration).  The program
@@ -29,6 +29,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--network", required=True)
     ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warm-tile-sweeps", type=int, default=0,
+                    help="sweeps per launch of the warm-started pass (default: the network file's)")
     a = ap.parse_args()
     net = np.load(a.network)
     cfg = {k: int(net[k]) for k in ("count", "seed", "warmup", "collect_substeps", "target_sweeps", "settle",
@@ -37,6 +39,8 @@ def main():
     # the warm-started run's own launch configuration, when the network file holds one
     wcfg = {k: int(net["warm_" + k]) if "warm_" + k in net else cfg[k]
             for k in ("tile_sweeps", "tile_max_verts", "block_threads")}
+    if a.warm_tile_sweeps > 0:
+        wcfg["tile_sweeps"] = a.warm_tile_sweeps
     specs = scenes.config5(seed=cfg["seed"], count=cfg["count"])
     ctx = capi.Context(0)
     for s in specs:
@@ -81,7 +85,8 @@ def main():
                 "alg_bytes_per_launch": prof.tile_alg_bytes / max(1, prof.tile_launches),
                 "achieved_gbs": prof.tile_alg_bytes / (prof.tile_ms * 1e-3) / 1e9}
 
-    out = {"warm": profiled(True, warm=cfg["warm_k"]), "plain": profiled(False)}
+    out = {"warm": profiled(True, warm=cfg["warm_k"]), "plain": profiled(False),
+           "warm_config": wcfg, "plain_config": {k: cfg[k] for k in ("tile_sweeps", "tile_max_verts", "block_threads")}}
     out["copy_ms"] = ctx.bench_copy(1 << 30, 10)
     print(json.dumps(out), flush=True)
     ctx.close()

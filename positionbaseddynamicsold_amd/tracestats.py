@@ -16,12 +16,16 @@ runs (kernel filter "k_tiles|k_copy|k_integrate") of bench.py or of
 positionbaseddynamicsold_amd.tilework, which repeats the same two passes, it
 prints the tile kernel's HBM traffic per launch in those passes, scaled by
 what the counters read for the calibration copy (1 GiB read + 1 GiB written
-per launch).
+per launch).  With --tilework (that run's JSON line) and --update-json it
+also records the figures in the JSON file under each pass's launch
+configuration ("ts<sweeps>_bt<threads>_mv<tile vertices>"), which bench.py
+reads for its roofline traffic.
 """
 from __future__ import annotations
 
 import argparse
 import csv
+import json
 import statistics
 from collections import defaultdict
 
@@ -87,6 +91,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--fetch")
     ap.add_argument("--write")
+    ap.add_argument("--tilework", help="JSON line printed by the counted tilework run")
+    ap.add_argument("--update-json", help="traffic JSON to add this configuration's figures to")
     a = ap.parse_args()
     if a.trace:
         ev = load_trace(a.trace)
@@ -115,6 +121,24 @@ def main():
             print(f"k_tiles {label}: FETCH_SIZE {f:.0f} KiB, WRITE_SIZE {wr:.0f} KiB per launch; calibration copy "
                   f"reads {fc:.0f} KiB / writes {wc:.0f} KiB for 1 GiB each (scale {scale_f:.4f}, {scale_w:.4f}); "
                   f"HBM traffic {traffic / 1e9:.4f} GB per launch")
+            if a.tilework and a.update_json:
+                record(a.update_json, label, json.loads(open(a.tilework).read().strip().splitlines()[-1]),
+                       f, wr, scale_f, scale_w, traffic)
+
+
+def record(path, label, tw, f, wr, scale_f, scale_w, traffic):
+    import os
+    data = json.loads(open(path).read()) if os.path.exists(path) else {}
+    cfg = tw[label + "_config"]
+    key = "ts{tile_sweeps}_bt{block_threads}_mv{tile_max_verts}".format(**cfg)
+    data.setdefault(label, {})[key] = {
+        "fetch_kib": round(f), "write_kib": round(wr), "fetch_scale": round(scale_f, 4),
+        "write_scale": round(scale_w, 4), "hbm_bytes_per_launch": traffic,
+        "alg_bytes_per_launch": tw[label]["alg_bytes_per_launch"],
+        "avg_launch_ms_counted": tw[label]["avg_launch_ms"], "launches": tw[label]["launches"]}
+    with open(path, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+        fh.write("\n")
 
 
 if __name__ == "__main__":
