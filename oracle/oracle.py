@@ -298,6 +298,20 @@ class OracleCloth:
     def sweep(self, index):
         self.L.orc_sweep(C.byref(self.c), index)
 
+    def start64(self):
+        """Starts an FP64 twin from the current FP32 state (x, v)."""
+        self.x64 = self.x.astype(np.float64)
+        self.v64 = self.v.astype(np.float64)
+        self.p64 = self.x64.copy()
+
+    def substeps64(self, prm: OrcParams, sweeps=20, n=1):
+        """n plain substeps of the FP64 twin (same topology, order, rest values, stiffness)."""
+        dp = C.POINTER(C.c_double)
+        self.L.orc64_substeps.argtypes = [C.POINTER(OrcCloth), C.POINTER(OrcParams), dp, dp, dp, C.c_int, C.c_int]
+        self.L.orc64_substeps.restype = None
+        self.L.orc64_substeps(C.byref(self.c), C.byref(prm), self.x64.ctypes.data_as(dp),
+                              self.v64.ctypes.data_as(dp), self.p64.ctypes.data_as(dp), sweeps, n)
+
     def residual(self, p=None):
         m, s, n = C.c_double(), C.c_double(), C.c_int()
         arr = self.p if p is None else np.ascontiguousarray(p, np.float32)

@@ -634,6 +634,105 @@ int orc_substep(orc_cloth* c, const orc_params* prm, int cap, int check_every, d
 }
 
 /* ------------------------------------------------------------------------ */
+/* FP64 twin of the plain substep (new): the same steps as orc_advect,       */
+/* orc_sweep, orc_colliders and orc_integrate, on the same topology, colour   */
+/* order, rest values (FP32, widened) and stiffness tables, in double, so the */
+/* FP32 solver's rounding error can be stated against it.  Bending uses the  */
+/* FP64 angle and gradient of orc_dihedral_grad_f64 (atan2 from libm).        */
+/* ------------------------------------------------------------------------ */
+static void orc64_project_distance(const int* ids, double rest, double k, const float* w, double* p) {
+    double* p0 = p + 3 * ids[0];
+    double* p1 = p + 3 * ids[1];
+    double w0 = w[ids[0]], w1 = w[ids[1]], ws = w0 + w1;
+    if (ws == 0.0) return;
+    double vx = p0[0] - p1[0], vy = p0[1] - p1[1], vz = p0[2] - p1[2];
+    double d = sqrt(vx * vx + vy * vy + vz * vz);
+    if (d == 0.0) return;
+    double c = d - rest;
+    double s0 = (-w0 / ws) * c / d, s1 = (w1 / ws) * c / d;
+    p0[0] += s0 * vx * k; p0[1] += s0 * vy * k; p0[2] += s0 * vz * k;
+    p1[0] += s1 * vx * k; p1[1] += s1 * vy * k; p1[2] += s1 * vz * k;
+}
+
+static void orc64_project_bending(const int* ids, double rest, double k, const float* w, double* p) {
+    double* q[4] = {p + 3 * ids[0], p + 3 * ids[1], p + 3 * ids[2], p + 3 * ids[3]};
+    double th, u[12];
+    orc_dihedral_grad_f64(q[0], q[1], q[2], q[3], &th, u);
+    double den = 0.0;
+    for (int j = 0; j < 4; j++)
+        den += w[ids[j]] * (u[3 * j] * u[3 * j] + u[3 * j + 1] * u[3 * j + 1] + u[3 * j + 2] * u[3 * j + 2]);
+    if (!(den > 1e-20)) return;
+    double s = (th - rest) / den * k;
+    for (int j = 0; j < 4; j++)
+        for (int a = 0; a < 3; a++) q[j][a] -= s * w[ids[j]] * u[3 * j + a];
+}
+
+static void orc64_colliders(const orc_params* prm, int nv, const float* w, double* p) {
+    for (int i = 0; i < nv; i++) {
+        if (w[i] == 0.0f) continue;
+        double* q = p + 3 * i;
+        if (prm->sphere_on) {
+            double dx = q[0] - prm->sphere_c[0], dy = q[1] - prm->sphere_c[1], dz = q[2] - prm->sphere_c[2];
+            double d = sqrt(dx * dx + dy * dy + dz * dz);
+            if (!(d - prm->sphere_r > 0.001) && d > 0.0) {
+                double md = prm->sphere_r - d;
+                q[0] += dx / d * md; q[1] += dy / d * md; q[2] += dz / d * md;
+            }
+        }
+        if (prm->box_on) {
+            int inside = 1;
+            for (int k = 0; k < 3; k++) inside &= (q[k] > prm->box_lo[k]) & (q[k] < prm->box_hi[k]);
+            if (inside) {
+                int bk = 0, bs = 0;
+                double best = q[0] - prm->box_lo[0];
+                for (int k = 0; k < 3; k++) {
+                    double dl = q[k] - prm->box_lo[k], dh = prm->box_hi[k] - q[k];
+                    if (dl < best) { best = dl; bk = k; bs = 0; }
+                    if (dh < best) { best = dh; bk = k; bs = 1; }
+                }
+                q[bk] = bs ? prm->box_hi[bk] : prm->box_lo[bk];
+            }
+        }
+    }
+}
+
+/* `substeps` FP64 substeps of `sweeps` sweeps on x, v (3 nv doubles each);
+ * p is scratch of the same size.  Topology, weights, rest values and
+ * stiffness tables come from c. */
+void orc64_substeps(const orc_cloth* c, const orc_params* prm, double* x, double* v, double* p, int sweeps,
+                    int substeps) {
+    const double dt = prm->dt, damp = prm->damp_factor, kd_w = (double)prm->wind_drag * dt;
+    for (int s = 0; s < substeps; s++) {
+        for (int i = 0; i < c->nv; i++)
+            for (int k = 0; k < 3; k++) {
+                double vk = v[3 * i + k] + (double)prm->gravity[k] * dt * c->w[i];
+                if (prm->wind_drag != 0.0f) vk += ((double)prm->wind[k] - vk) * kd_w * c->w[i];
+                vk *= damp;
+                v[3 * i + k] = vk;
+                p[3 * i + k] = x[3 * i + k] + vk * dt;
+            }
+        for (int sw = 0; sw < sweeps; sw++) {
+            int ki = sw < c->k_len ? sw : c->k_len - 1;
+            double kd = c->k_dist[ki], kb = c->k_bend ? c->k_bend[ki] : 1.0;
+            for (int col = 0; col < c->ncolors; col++) {
+                int st0 = c->worksets[2 * col], n = c->worksets[2 * col + 1];
+                for (int j = st0; j < st0 + n; j++) {
+                    int prim = c->sorted[j];
+                    int st = c->prim_list[2 * prim], cnt = c->prim_list[2 * prim + 1];
+                    if (cnt == 2) orc64_project_distance(c->indices + st, c->rest[prim], kd, c->w, p);
+                    else if (cnt == 4) orc64_project_bending(c->indices + st, c->rest[prim], kb, c->w, p);
+                }
+            }
+            if (prm->sphere_on || prm->box_on) orc64_colliders(prm, c->nv, c->w, p);
+        }
+        for (int i = 0; i < 3 * c->nv; i++) {
+            v[i] = (p[i] - x[i]) / dt;
+            x[i] = p[i];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Broadphase                                                                 */
 /* ------------------------------------------------------------------------ */
 

@@ -92,6 +92,8 @@ void orc_sweep(orc_cloth* c, int sweep_index);
 void orc_colliders(const orc_params* prm, int nv, const float* w, float* p);
 void orc_integrate(int nv, float dt, float* x, float* v, const float* p);
 void orc_residual(const orc_cloth* c, const float* p, double* max_rel, double* sum_sq, int* count);
+void orc64_substeps(const orc_cloth* c, const orc_params* prm, double* x, double* v, double* p, int sweeps,
+                    int substeps);
 int orc_substep(orc_cloth* c, const orc_params* prm, int cap, int check_every, double tol,
                 double* final_residual);
 void orc_stiffness_table_ref(float s, int n, float* k);

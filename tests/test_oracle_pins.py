@@ -335,3 +335,28 @@ def test_bending_projection_invariants(orc):
         qr = (q.astype(np.float64) @ Rm.T + [0.3, -2, 1]).astype(np.float32)
         thr = L.orc_dihedral(*[F(np.ascontiguousarray(qr[i])) for i in range(4)])
         assert abs(thr - th) < 1e-5
+
+
+def test_fp64_twin_matches_fp32_oracle_and_free_fall(orc):
+    # the FP64 substep twin: equal to the FP32 oracle to FP32 rounding over
+    # one substep (hanging cloth with bending and a sphere), and to the
+    # closed-form free fall (x0 + g dt^2 n(n+1)/2 for an unpinned cloth, up
+    # to the tiny corrections FP32-rounded rest lengths cause in double)
+    from positionbaseddynamicsold_amd import scenes
+    pos, tris = orc.grid_mesh(16, 16, 0.5, 0.5, jitter=1e-4, seed=7)
+    oc = orc.OracleCloth(pos, tris, orc.grid_pairs(16, 16), pins=[0, 15], bending=True, stiffness=(1.0, 0.1, 1.0))
+    prm = oc.params(dt=scenes.DT, sphere=((0.25, -0.2, 0.25), 0.3))
+    for _ in range(30):
+        oc.substep(prm, cap=20)
+    oc.start64()
+    oc.substep(prm, cap=20)
+    oc.substeps64(prm, 20, 1)
+    assert np.abs(oc.x - oc.x64).max() < 2e-6
+    # a dropped bending term or a wrong sign shows up far above that
+    ff = orc.OracleCloth(pos, tris, orc.grid_pairs(16, 16))
+    ff.start64()
+    n = 40
+    ff.substeps64(ff.params(dt=scenes.DT), 20, n)
+    dt = scenes.DT
+    expect = pos.astype(np.float64) + np.array([0.0, -9.8, 0.0]) * dt * dt * n * (n + 1) / 2
+    assert np.abs(ff.x64 - expect).max() < 1e-6
