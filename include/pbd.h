@@ -293,6 +293,14 @@ typedef struct {
     long long vertex_sweeps; /* sum over cloths and substeps of V x sweeps */
     long long sweeps_max;    /* largest per-cloth sweep count seen */
     long long graph_replays; /* substeps replayed from a captured graph */
+    /* contact detection of the in-solver collision path, summed over detections */
+    long long ccd_detections;
+    long long ccd_triangles;    /* triangles hashed */
+    long long ccd_cell_entries; /* (cell, face) entries (swept-box candidates; one per triangle otherwise) */
+    long long ccd_raw_pairs;    /* candidate pairs emitted before de-duplication */
+    long long ccd_pairs;        /* distinct candidate pairs through the narrowphase */
+    long long ccd_contacts;     /* contacts found */
+    long long ccd_resolved;     /* contacts x passes evaluated by the response */
 } pbd_profile;
 PBD_API int pbd_profile_enable(pbd_ctx* ctx, int on);
 PBD_API int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset);

@@ -65,7 +65,9 @@ class CollisionParams(C.Structure):
 class Profile(C.Structure):
     _fields_ = [("tile_ms", C.c_double), ("tile_launches", C.c_longlong), ("tile_alg_bytes", C.c_double),
                 ("step_ms", C.c_double), ("substeps", C.c_longlong), ("vertex_sweeps", C.c_longlong),
-                ("sweeps_max", C.c_longlong), ("graph_replays", C.c_longlong)]
+                ("sweeps_max", C.c_longlong), ("graph_replays", C.c_longlong), ("ccd_detections", C.c_longlong),
+                ("ccd_triangles", C.c_longlong), ("ccd_cell_entries", C.c_longlong), ("ccd_raw_pairs", C.c_longlong),
+                ("ccd_pairs", C.c_longlong), ("ccd_contacts", C.c_longlong), ("ccd_resolved", C.c_longlong)]
 
 
 class ClothInfo(C.Structure):

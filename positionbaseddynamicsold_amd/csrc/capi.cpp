@@ -334,7 +334,7 @@ void do_init(pbd_ctx* c, int color_rounds) {
         vv.upload(ids, s);
         k2.alloc(np);
         v2.alloc(np);
-        hist.alloc((size_t)256 * ((np + 4095) / 4096));
+        hist.alloc(radix_hist_ints(np));
         radix_sort_pairs(s, kk.ptr, vv.ptr, k2.ptr, v2.ptr, np, bits, hist.ptr);
         std::vector<uint32_t> sorted_ids;
         vv.download(sorted_ids, s);
@@ -1900,7 +1900,19 @@ int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset) {
     return guard([&] {
         PBD_CHECK(ctx && out, PBD_ERR_ARG, "null argument");
         *out = ctx->prof;
-        if (reset) ctx->prof = pbd_profile{};
+        const Broadphase& b = ctx->cbp;
+        out->ccd_detections = b.st_dets;
+        out->ccd_triangles = b.st_tris;
+        out->ccd_cell_entries = b.st_entries;
+        out->ccd_raw_pairs = b.st_raw;
+        out->ccd_pairs = b.st_pairs;
+        out->ccd_contacts = b.st_contacts;
+        out->ccd_resolved = b.st_resolved;
+        if (reset) {
+            ctx->prof = pbd_profile{};
+            ctx->cbp.st_dets = ctx->cbp.st_tris = ctx->cbp.st_entries = 0;
+            ctx->cbp.st_raw = ctx->cbp.st_pairs = ctx->cbp.st_contacts = ctx->cbp.st_resolved = 0;
+        }
     });
 }
 

@@ -645,7 +645,7 @@ void Broadphase::build(hipStream_t s, const HashCfg& g, const float4* p, const i
     tcell.alloc(std::max(1, nt));
     k2.alloc(std::max(1, nt));
     v2.alloc(std::max(1, nt));
-    hist.alloc((size_t)256 * ((std::max(1, nt) + 4095) / 4096));
+    hist.alloc(radix_hist_ints(nt));
     cstart.alloc(std::max(1, g.ncells));
     cend.alloc(std::max(1, g.ncells));
     if (nt > 0) hipLaunchKernelGGL(k_tri_keys, dim3(blocks(nt)), dim3(256), 0, s, g, p, tris, nt, keys.ptr, ids.ptr, raw.ptr,
@@ -673,6 +673,10 @@ long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int
     PBD_HIP(hipMemcpyAsync(&total, off.ptr + nt, sizeof(int), hipMemcpyDeviceToHost, s));
     PBD_HIP(hipStreamSynchronize(s));
     PBD_CHECK(total >= 0, PBD_ERR_LIMIT, "candidate pair count overflows int32");
+    st_dets++;
+    st_tris += nt;
+    st_entries += nt;
+    st_raw += total;
     if (total == 0) return 0;
     pv.alloc(total);
     pf.alloc(total);
@@ -691,7 +695,7 @@ long long Broadphase::unique_pairs(hipStream_t s, int total, int nt, int nv) {
     int fb = 1, vb = 1;
     while (fb < 31 && (1ll << fb) < nt) fb++;
     while (vb < 31 && (1ll << vb) < nv) vb++;
-    hist2.alloc((size_t)256 * ((total + 4095) / 4096));
+    hist2.alloc(radix_hist_ints(total));
     radix_sort_pairs(s, pf.ptr, pv.ptr, t1.ptr, t2.ptr, total, fb, hist2.ptr);
     radix_sort_pairs(s, pv.ptr, pf.ptr, t1.ptr, t2.ptr, total, vb, hist2.ptr);
     hipLaunchKernelGGL(k_unique_flags, dim3(blocks(total)), dim3(256), 0, s, pv.ptr, pf.ptr, total, flag.ptr);
@@ -707,6 +711,7 @@ long long Broadphase::unique_pairs(hipStream_t s, int total, int nt, int nv) {
                        uv.ptr, uf.ptr);
     PBD_HIP(hipGetLastError());
     npairs = uniq;
+    st_pairs += uniq;
     return uniq;
 }
 
@@ -744,7 +749,7 @@ long long Broadphase::pairs_swept(hipStream_t s, const HashCfg& g, const float4*
     grow(ids, ne);
     grow(k2, ne);
     grow(v2, ne);
-    grow(hist, (size_t)256 * ((std::max(1, ne) + 4095) / 4096));
+    grow(hist, radix_hist_ints(ne));
     grow(cstart, std::max(1, g.ncells));
     grow(cend, std::max(1, g.ncells));
     hipLaunchKernelGGL(k_swept_entries, dim3(blocks(nt)), dim3(256), 0, s, g, nt, sw_cell.ptr, off.ptr, keys.ptr, ids.ptr);
@@ -767,6 +772,10 @@ long long Broadphase::pairs_swept(hipStream_t s, const HashCfg& g, const float4*
     PBD_CHECK(!hst[2], PBD_ERR_LIMIT, "a vertex's swept box spans more than 4096 cells: use a larger cell");
     const int total = hst[0];
     PBD_CHECK(total >= 0, PBD_ERR_LIMIT, "candidate pair count overflows int32");
+    st_dets++;
+    st_tris += nt;
+    st_entries += ne;
+    st_raw += total;
     if (total == 0) return 0;
     pv.alloc(total);
     pf.alloc(total);
@@ -805,12 +814,14 @@ long long Broadphase::narrow(hipStream_t s, const float4* x, const float4* p, co
                        ctmp_t.ptr, ctmp_n.ptr, ctmp_w.ptr, cv.ptr, cf.ptr, ct.ptr, cn.ptr, cw.ptr);
     PBD_HIP(hipGetLastError());
     ncontacts = nc;
+    st_contacts += nc;
     return nc;
 }
 
 void Broadphase::resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes,
                          float min_n2) {
     if (ncontacts <= 0) return;
+    st_resolved += ncontacts * passes;
     acc.alloc((size_t)3 * nv);
     vcnt.alloc(nv);
     for (int ps = 0; ps < passes; ps++) {

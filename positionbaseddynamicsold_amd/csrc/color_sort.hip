@@ -101,30 +101,6 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
     hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of n ints by one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_exclusive(int* __restrict__ a, int n) {
-    __shared__ int part[1024];
-    int t = threadIdx.x;
-    int per = (n + 1023) / 1024;
-    int b = t * per, e = min(n, b + per);
-    int s = 0;
-    for (int i = b; i < e; i++) s += a[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int run = part[t] - s;
-    for (int i = b; i < e; i++) {
-        int v = a[i];
-        a[i] = run;
-        run += v;
-    }
-}
-
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout,
@@ -166,17 +142,21 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
 
 // Stable sort of (key, value) pairs on bits [0, bits) of the key.  Uses
 // (k2, v2) as the ping-pong buffer; the result ends in (k, v).  `hist` holds
-// 256 * ceil(n/4096) ints.
+// radix_hist_ints(n) ints: digit counts | their exclusive scan (multi-block)
+// | the scan's block sums.
 void radix_sort_pairs(hipStream_t s, uint32_t* k, uint32_t* v, uint32_t* k2, uint32_t* v2, int n,
                       int bits, int* hist) {
     if (n <= 0) return;
     int nb = (n + RB - 1) / RB;
     int passes = (bits + 7) / 8;
+    int* counts = hist;
+    int* offs = hist + (size_t)256 * nb;
+    int* sums = hist + (size_t)512 * nb;
     uint32_t *ki = k, *vi = v, *ko = k2, *vo = v2;
     for (int ps = 0; ps < passes; ps++) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, ps * 8, hist, nb);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, 256 * nb);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, ki, vi, ko, vo, n, ps * 8, hist, nb);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, ps * 8, counts, nb);
+        device_scan(s, counts, offs, 256 * nb, sums);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, ki, vi, ko, vo, n, ps * 8, offs, nb);
         std::swap(ki, ko);
         std::swap(vi, vo);
     }

@@ -195,6 +195,9 @@ struct Broadphase {
     DBuf<float4> sw_box; // swept-box broadphase: face boxes (lo, hi)
     DBuf<int4> sw_cell;  // and their cell ranges (lo, hi)
     DBuf<int> sw_err, sw_vin;
+    // running counts (host values the drivers read anyway): detections,
+    // triangles, cell entries, raw pairs, distinct pairs, contacts
+    long long st_dets = 0, st_tris = 0, st_entries = 0, st_raw = 0, st_pairs = 0, st_contacts = 0, st_resolved = 0;
     void build(hipStream_t s, const HashCfg& g, const float4* p, const int* tris, int nt);
     long long pairs(hipStream_t s, const float4* p, const int* tris, int nv);
     long long unique_pairs(hipStream_t s, int total, int nt, int nv);
@@ -252,6 +255,13 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps_per_launch,
 // ---------------------------------------------------------------------------
 int launch_color_round(hipStream_t s, const int* prim_npts, const int* adj_sn, const int* adj,
                        int* color, int* prd, int* uncolored, int np);
+// Ints of `hist` that radix_sort_pairs needs for n keys: the per-block digit
+// counts (256 x ceil(n / 4096)), their exclusive scan, and the block sums of
+// that scan.
+inline size_t radix_hist_ints(long long n) {
+    const size_t h = (size_t)256 * (size_t)((std::max(1ll, n) + 4095) / 4096);
+    return 2 * h + (h + 1023) / 1024 + 2;
+}
 void radix_sort_pairs(hipStream_t s, uint32_t* k, uint32_t* v, uint32_t* k2, uint32_t* v2, int n,
                       int bits, int* hist);
 void launch_advect(hipStream_t s, const ClothDev* cloths, const int* vcloth, const float4* x,

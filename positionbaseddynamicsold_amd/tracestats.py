@@ -16,7 +16,12 @@ runs (kernel filter "k_tiles|k_copy|k_integrate") of bench.py or of
 positionbaseddynamicsold_amd.tilework, which repeats the same two passes, it
 prints the tile kernel's HBM traffic per launch in those passes, scaled by
 what the counters read for the calibration copy (1 GiB read + 1 GiB written
-per launch).  With --tilework (that run's JSON line) and --update-json it
+per launch).  With --collide (the JSON line of positionbaseddynamicsold_amd
+.collidework), it summarises that program's trace instead: every kernel's
+time and, with counter files of the same program, HBM bytes and GB/s per
+kernel, plus the algorithmic rate of the kernels whose bytes per unit are
+fixed (hash keys 56 B per triangle, narrowphase 104 B per candidate, response
+160 B per contact and pass).  With --tilework (that run's JSON line) and --update-json it
 also records the figures in the JSON file under each pass's launch
 configuration ("ts<sweeps>_bt<threads>_mv<tile vertices>"), which bench.py
 reads for its roofline traffic.
@@ -92,8 +97,12 @@ def main():
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--tilework", help="JSON line printed by the counted tilework run")
+    ap.add_argument("--collide", help="JSON line printed by the collidework run")
     ap.add_argument("--update-json", help="traffic JSON to add this configuration's figures to")
     a = ap.parse_args()
+    if a.collide:
+        collide_summary(a.trace, a.collide, a.fetch, a.write)
+        return
     if a.trace:
         ev = load_trace(a.trace)
         table(ev, "whole run (all launches)")
@@ -124,6 +133,53 @@ def main():
             if a.tilework and a.update_json:
                 record(a.update_json, label, json.loads(open(a.tilework).read().strip().splitlines()[-1]),
                        f, wr, scale_f, scale_w, traffic)
+
+
+HBM_PEAK = 8.0e12
+
+# algorithmic bytes per unit (work-file count) of the kernels with a fixed per-unit figure
+COLLIDE_ALG = {"k_tri_keys": (56, "triangles"), "k_narrow": (104, "pairs"), "k_resolve_acc": (160, "resolved")}
+
+
+def collide_summary(trace, work_path, fetch=None, write=None):
+    work = json.loads(open(work_path).read().strip().splitlines()[-1])
+    ev = load_trace(trace)
+    per = defaultdict(list)
+    for n, s, e, _ in ev:
+        per[n].append((e - s) / 1e3)
+    tot = sum(sum(v) for v in per.values())
+    hbm = {}
+    if fetch and write:
+        for path, kind in ((fetch, "f"), (write, "w")):
+            seq = counters(path)
+            copies = [v for n, v, g in seq if n == "k_copy"]
+            cal = statistics.fmean(copies[-10:])  # the 1 GiB calibration copies end the run
+            scale = COPY_BYTES / (cal * 1024.0)
+            for n, v, _ in seq:
+                if n == "k_copy":
+                    continue
+                hbm.setdefault(n, {"f": 0.0, "w": 0.0})[kind] += v * 1024.0 * scale
+    print(f"config {work['config']} ({work['mode']} contact handling): {work['cloth_vertices']} cloth vertices, "
+          f"{work['cloth_triangles'] + work['static_triangles']} triangles, table 2^{work['table_bits']}, "
+          f"{work['substeps']} substeps x {work['sweeps']} sweeps, {work['detections']} detections")
+    print(f"per detection: {work['cell_entries'] / max(1, work['detections']):.0f} cell entries, "
+          f"{work['raw_pairs'] / max(1, work['detections']):.0f} raw pairs, "
+          f"{work['pairs'] / max(1, work['detections']):.0f} distinct pairs, "
+          f"{work['contacts'] / max(1, work['detections']):.0f} contacts; {work['ms_per_substep']:.2f} ms per substep")
+    print(f"\n{'kernel':26s} {'calls':>6s} {'total ms':>9s} {'share':>6s} {'HBM GB':>8s} {'HBM GB/s':>9s} "
+          f"{'of peak':>7s} {'alg GB/s':>9s}")
+    for n, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        t = sum(v) * 1e-6  # durations are in microseconds
+        line = f"{n:26s} {len(v):6d} {sum(v) / 1e3:9.3f} {100 * sum(v) / tot:5.1f}%"
+        if n in hbm:
+            b = hbm[n]["f"] + hbm[n]["w"]
+            line += f" {b / 1e9:8.3f} {b / t / 1e9:9.0f} {b / t / HBM_PEAK:6.1%}"
+        else:
+            line += f" {'':8s} {'':9s} {'':7s}"
+        if n in COLLIDE_ALG:
+            per_unit, key = COLLIDE_ALG[n]
+            line += f" {per_unit * work[key] / t / 1e9:9.0f}"
+        print(line)
 
 
 def record(path, label, tw, f, wr, scale_f, scale_w, traffic):
