@@ -636,18 +636,25 @@ __global__ __launch_bounds__(256) void k_resolve_apply(const unsigned long long*
 // ---------------------------------------------------------------------------
 static inline int blocks(long long n) { return (int)((n + 255) / 256); }
 
+// Working buffers only grow: detections of varying size reuse them instead
+// of freeing and allocating (hipFree waits for the device) every time.
+template <typename T>
+static void grow(DBuf<T>& b, size_t n) {
+    if (b.n < n || !b.ptr) b.alloc(std::max<size_t>(n, 1));
+}
+
 void Broadphase::build(hipStream_t s, const HashCfg& g, const float4* p, const int* tris, int nt) {
     cfg = g;
     this->nt = nt;
-    keys.alloc(std::max(1, nt));
-    ids.alloc(std::max(1, nt));
-    raw.alloc(std::max(1, nt));
-    tcell.alloc(std::max(1, nt));
-    k2.alloc(std::max(1, nt));
-    v2.alloc(std::max(1, nt));
-    hist.alloc(radix_hist_ints(nt));
-    cstart.alloc(std::max(1, g.ncells));
-    cend.alloc(std::max(1, g.ncells));
+    grow(keys, std::max(1, nt));
+    grow(ids, std::max(1, nt));
+    grow(raw, std::max(1, nt));
+    grow(tcell, std::max(1, nt));
+    grow(k2, std::max(1, nt));
+    grow(v2, std::max(1, nt));
+    grow(hist, radix_hist_ints(nt));
+    grow(cstart, std::max(1, g.ncells));
+    grow(cend, std::max(1, g.ncells));
     if (nt > 0) hipLaunchKernelGGL(k_tri_keys, dim3(blocks(nt)), dim3(256), 0, s, g, p, tris, nt, keys.ptr, ids.ptr, raw.ptr,
                                   tcell.ptr);
     int bits = 1;
@@ -662,9 +669,9 @@ void Broadphase::build(hipStream_t s, const HashCfg& g, const float4* p, const i
 long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int nv) {
     npairs = 0;
     if (nt <= 0) return 0;
-    cnt.alloc(nt + 1);
-    off.alloc(nt + 1);
-    scan_tmp.alloc((nt + SB) / SB + 2);
+    grow(cnt, nt + 1);
+    grow(off, nt + 1);
+    grow(scan_tmp, (nt + SB) / SB + 2);
     hipLaunchKernelGGL(k_pairs<0>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr, cstart.ptr,
                        cend.ptr, tcell.ptr, cnt.ptr, (uint32_t*)nullptr, (uint32_t*)nullptr);
     PBD_HIP(hipMemsetAsync(cnt.ptr + nt, 0, sizeof(int), s));
@@ -678,12 +685,12 @@ long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int
     st_entries += nt;
     st_raw += total;
     if (total == 0) return 0;
-    pv.alloc(total);
-    pf.alloc(total);
-    t1.alloc(total);
-    t2.alloc(total);
-    flag.alloc(total + 1);
-    pos.alloc(total + 1);
+    grow(pv, total);
+    grow(pf, total);
+    grow(t1, total);
+    grow(t2, total);
+    grow(flag, total + 1);
+    grow(pos, total + 1);
     hipLaunchKernelGGL(k_pairs<1>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr, cstart.ptr,
                        cend.ptr, tcell.ptr, off.ptr, pv.ptr, pf.ptr);
     return unique_pairs(s, total, nt, nv);
@@ -695,29 +702,24 @@ long long Broadphase::unique_pairs(hipStream_t s, int total, int nt, int nv) {
     int fb = 1, vb = 1;
     while (fb < 31 && (1ll << fb) < nt) fb++;
     while (vb < 31 && (1ll << vb) < nv) vb++;
-    hist2.alloc(radix_hist_ints(total));
+    grow(hist2, radix_hist_ints(total));
     radix_sort_pairs(s, pf.ptr, pv.ptr, t1.ptr, t2.ptr, total, fb, hist2.ptr);
     radix_sort_pairs(s, pv.ptr, pf.ptr, t1.ptr, t2.ptr, total, vb, hist2.ptr);
     hipLaunchKernelGGL(k_unique_flags, dim3(blocks(total)), dim3(256), 0, s, pv.ptr, pf.ptr, total, flag.ptr);
     PBD_HIP(hipMemsetAsync(flag.ptr + total, 0, sizeof(int), s));
-    scan_tmp.alloc((total + 1 + SB) / SB + 2);
+    grow(scan_tmp, (total + 1 + SB) / SB + 2);
     device_scan(s, flag.ptr, pos.ptr, total + 1, scan_tmp.ptr);
     int uniq = 0;
     PBD_HIP(hipMemcpyAsync(&uniq, pos.ptr + total, sizeof(int), hipMemcpyDeviceToHost, s));
     PBD_HIP(hipStreamSynchronize(s));
-    uv.alloc(std::max(1, uniq));
-    uf.alloc(std::max(1, uniq));
+    grow(uv, std::max(1, uniq));
+    grow(uf, std::max(1, uniq));
     hipLaunchKernelGGL(k_compact_pairs, dim3(blocks(total)), dim3(256), 0, s, pv.ptr, pf.ptr, total, flag.ptr, pos.ptr,
                        uv.ptr, uf.ptr);
     PBD_HIP(hipGetLastError());
     npairs = uniq;
     st_pairs += uniq;
     return uniq;
-}
-
-template <typename T>
-static void grow(DBuf<T>& b, size_t n) {
-    if (b.n < n || !b.ptr) b.alloc(std::max<size_t>(n, 1));
 }
 
 long long Broadphase::pairs_swept(hipStream_t s, const HashCfg& g, const float4* x, const float4* p, const int* tris,
@@ -732,7 +734,7 @@ long long Broadphase::pairs_swept(hipStream_t s, const HashCfg& g, const float4*
     grow(off, (size_t)std::max(nt, nv) + 1);
     grow(sw_err, 2);
     grow(sw_vin, nv);
-    scan_tmp.alloc((std::max(nt, nv) + SB) / SB + 2);
+    grow(scan_tmp, (std::max(nt, nv) + SB) / SB + 2);
     PBD_HIP(hipMemsetAsync(sw_err.ptr, 0, 2 * sizeof(int), s));
     hipLaunchKernelGGL(k_swept_faces, dim3(blocks(nt)), dim3(256), 0, s, g, x, p, tris, nt, 4.0f * th, sw_box.ptr,
                        sw_cell.ptr, cnt.ptr, sw_err.ptr);
@@ -777,12 +779,12 @@ long long Broadphase::pairs_swept(hipStream_t s, const HashCfg& g, const float4*
     st_entries += ne;
     st_raw += total;
     if (total == 0) return 0;
-    pv.alloc(total);
-    pf.alloc(total);
-    t1.alloc(total);
-    t2.alloc(total);
-    flag.alloc(total + 1);
-    pos.alloc(total + 1);
+    grow(pv, total);
+    grow(pf, total);
+    grow(t1, total);
+    grow(t2, total);
+    grow(flag, total + 1);
+    grow(pos, total + 1);
     hipLaunchKernelGGL(k_swept_pairs<1>, dim3(blocks(nv)), dim3(256), 0, s, g, x, p, sw_vin.ptr, nv, tris, sw_box.ptr,
                        sw_cell.ptr, ids.ptr, cstart.ptr, cend.ptr, off.ptr, pv.ptr, pf.ptr, sw_err.ptr);
     return unique_pairs(s, total, nt, nv);
@@ -792,24 +794,24 @@ long long Broadphase::narrow(hipStream_t s, const float4* x, const float4* p, co
     ncontacts = 0;
     if (npairs <= 0) return 0;
     long long n = npairs;
-    flag.alloc(n + 1);
-    pos.alloc(n + 1);
-    ctmp_t.alloc(n);
-    ctmp_n.alloc(3 * n);
-    ctmp_w.alloc(4 * n);
+    grow(flag, n + 1);
+    grow(pos, n + 1);
+    grow(ctmp_t, n);
+    grow(ctmp_n, 3 * n);
+    grow(ctmp_w, 4 * n);
     hipLaunchKernelGGL(k_narrow, dim3(blocks(n)), dim3(256), 0, s, uv.ptr, uf.ptr, (int)n, tris, x, p, th, flag.ptr,
                        ctmp_t.ptr, ctmp_n.ptr, ctmp_w.ptr);
     PBD_HIP(hipMemsetAsync(flag.ptr + n, 0, sizeof(int), s));
-    scan_tmp.alloc((n + 1 + SB) / SB + 2);
+    grow(scan_tmp, (n + 1 + SB) / SB + 2);
     device_scan(s, flag.ptr, pos.ptr, (int)n + 1, scan_tmp.ptr);
     int nc = 0;
     PBD_HIP(hipMemcpyAsync(&nc, pos.ptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
     PBD_HIP(hipStreamSynchronize(s));
-    cv.alloc(std::max(1, nc));
-    cf.alloc(std::max(1, nc));
-    ct.alloc(std::max(1, nc));
-    cn.alloc(3 * std::max(1, nc));
-    cw.alloc(4 * std::max(1, nc));
+    grow(cv, std::max(1, nc));
+    grow(cf, std::max(1, nc));
+    grow(ct, std::max(1, nc));
+    grow(cn, 3 * std::max(1, nc));
+    grow(cw, 4 * std::max(1, nc));
     hipLaunchKernelGGL(k_compact_contacts, dim3(blocks(n)), dim3(256), 0, s, (int)n, flag.ptr, pos.ptr, uv.ptr, uf.ptr,
                        ctmp_t.ptr, ctmp_n.ptr, ctmp_w.ptr, cv.ptr, cf.ptr, ct.ptr, cn.ptr, cw.ptr);
     PBD_HIP(hipGetLastError());
@@ -822,8 +824,8 @@ void Broadphase::resolve(hipStream_t s, const float4* x, float4* p, const int* t
                          float min_n2) {
     if (ncontacts <= 0) return;
     st_resolved += ncontacts * passes;
-    acc.alloc((size_t)3 * nv);
-    vcnt.alloc(nv);
+    grow(acc, (size_t)3 * nv);
+    grow(vcnt, nv);
     for (int ps = 0; ps < passes; ps++) {
         PBD_HIP(hipMemsetAsync(acc.ptr, 0, sizeof(unsigned long long) * 3 * (size_t)nv, s));
         PBD_HIP(hipMemsetAsync(vcnt.ptr, 0, sizeof(int) * (size_t)nv, s));

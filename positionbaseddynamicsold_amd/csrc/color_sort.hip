@@ -117,15 +117,19 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
         uint32_t k = valid ? kin[i] : 0u, v = valid ? vin[i] : 0u;
         int d = valid ? (int)((k >> shift) & 255u) : -1;
         for (int j = lane; j < 256; j += 64) wcnt[wv][j] = 0;
-        unsigned long long active = __ballot(valid);
+        // lanes with this lane's digit: one ballot per digit bit, so the cost
+        // does not depend on how many distinct digits the wave holds
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1;
+            const unsigned long long bb = __ballot(valid && bit);
+            m &= bit ? bb : ~bb;
+        }
         int rank = 0;
-        while (active) {
-            int leader = __ffsll((long long)active) - 1;
-            int ld = __shfl(d, leader);
-            unsigned long long m = __ballot(valid && d == ld);
-            if (valid && d == ld) rank = __popcll(m & __lanemask_lt());
-            if (lane == leader) wcnt[wv][ld] = __popcll(m);
-            active &= ~m;
+        if (valid) {
+            rank = __popcll(m & __lanemask_lt());
+            if (rank == 0) wcnt[wv][d] = __popcll(m);  // the digit's first lane
         }
         __syncthreads();
         if (valid) {
