@@ -660,9 +660,12 @@ __global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_infer(
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int s = 0; s < 4; s++) {
             const int vl = 16 * s + r, v = g * 64 + vl;
-            const f32x4 o = mlp_group(W, S.chunk[q][vl], r, q);
+            const float h = S.hs[vl];
+            // a group whose 16 output scales are all zero (pinned or quiet
+            // vertices) predicts zero whatever the network says: skip it
+            f32x4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (__any(h != 0.0f)) o = mlp_group(W, S.chunk[q][vl], r, q);
             if (q == 0 && v < nv) {
-                const float h = S.hs[vl];
                 float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (h != 0.0f) d = make_float4(h * (o[0] * sdy0), h * (o[1] * sdy1), h * (o[2] * sdy2), 0.0f);
                 delta[v] = d;
