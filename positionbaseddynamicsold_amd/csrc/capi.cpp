@@ -575,9 +575,11 @@ void ws_require_params(pbd_ctx* c) {
     PBD_CHECK(c->ws.params_ready, PBD_ERR_STATE, "warm-start weights not initialised (pbd_ws_init or pbd_ws_set_params)");
 }
 
-// Applies the prediction to the current p of every cloth, keeping it only
-// for cloths whose residual does not rise above the unwarmed start.
-void ws_apply_prediction(pbd_ctx* c) {
+// Predicts the correction and decides, per cloth, whether to keep it (its
+// residual must not rise above the unwarmed start).  With apply, adds it to
+// the current p of the kept cloths; otherwise the caller passes ws.delta and
+// ws.keep to the tile kernel, which adds it while loading.
+void ws_apply_prediction(pbd_ctx* c, bool apply) {
     hipStream_t s = c->stream;
     int ncl = (int)c->cloths.size();
     ws_build_nbr(c);
@@ -589,7 +591,7 @@ void ws_apply_prediction(pbd_ctx* c) {
     int full = c->sp.iterations;
     int warm = c->sp.warm_iterations > 0 ? std::min(c->sp.warm_iterations, full) : full;
     ws_decide(s, c->ws.rs.ptr, ncl, c->sp.warm_rule, warm, full, c->ws.cap.ptr, c->ws.keep.ptr);
-    ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.keep.ptr, c->nv_total);
+    if (apply) ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.keep.ptr, c->nv_total);
     c->ws.used_pending = true;
 }
 
@@ -634,7 +636,10 @@ void advect_all(pbd_ctx* c, float dt) {
 
 // Sweeps on the current p.  Whole-cloth tiles run unfused (p_in -> p_out).
 // cap: optional per-cloth sweep budgets (device array) below `iterations`.
-void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* cap = nullptr) {
+// delta / keep: a warm-start prediction the first tile launch adds while
+// loading (LDS-tile path only).
+void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* cap = nullptr,
+             const float4* delta = nullptr, const int* keep = nullptr) {
     hipStream_t s = c->stream;
     int ncl = (int)c->cloths.size();
     double tol = c->sp.tol;
@@ -644,6 +649,7 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     launch_fill_int(s, c->sweeps.ptr, ncl, 0);
     if (c->sp.path == PBD_PATH_GLOBAL) {
         PBD_CHECK(!fused_whole, PBD_ERR_STATE, "internal: global path is never fused");
+        PBD_CHECK(!delta, PBD_ERR_STATE, "internal: the global path applies the prediction beforehand");
         c->pfinal_valid = false;
         float4* p = c->pbuf[c->pcur].ptr;
         bool coll = false;
@@ -708,6 +714,8 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     // (both read pbuf[start] and write pbuf[start ^ 1], on disjoint vertices).
     a.whole_nsweeps = iterations;
     a.fused = fused_whole ? 1 : 0;
+    a.delta = delta;
+    a.keep = keep;
     if (c->any_whole && !c->any_partial) {
         a.sweep_base = 0;
         a.nsweeps = iterations;
@@ -723,6 +731,8 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
             a.chunk = ch;
             const DBuf<int>& ord = (ch == 0 && c->any_whole) ? c->order_first : c->order_partial;
             tiles((int)ord.n, ord.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
+            a.delta = nullptr;  // later chunks read what the first one wrote
+            a.keep = nullptr;
             cur ^= 1;
             sw += a.nsweeps;
             if (any_tol && ch + 1 < nchunks) {
@@ -847,8 +857,13 @@ void substep(pbd_ctx* c, float dt) {
         step_ccd(c, dt);
     } else if (c->ws.enabled) {
         advect_all(c, dt);
-        ws_apply_prediction(c);
-        project(c, c->sp.iterations, false, dt, c->ws.cap.ptr);
+        if (c->sp.path == PBD_PATH_GLOBAL) {
+            ws_apply_prediction(c, true);
+            project(c, c->sp.iterations, false, dt, c->ws.cap.ptr);
+        } else {
+            ws_apply_prediction(c, false);
+            project(c, c->sp.iterations, false, dt, c->ws.cap.ptr, c->ws.delta.ptr, c->ws.keep.ptr);
+        }
         integrate_all(c, dt, false);
     } else if (c->sp.path == PBD_PATH_GLOBAL) {
         advect_all(c, dt);

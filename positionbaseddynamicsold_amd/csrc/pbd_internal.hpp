@@ -115,6 +115,10 @@ struct SolveArgs {
     int chunk;           // chunk index (selects the residual slot)
     int no_coll;         // 1 = skip the per-sweep analytic colliders (applied by the caller)
     int copy_spent;      // 1 = a multi-tile cloth past its sweep budget is copied forward
+    // warm start folded into the load (first launch of a substep only):
+    // p + delta for the vertices of cloths with keep[cloth] != 0
+    const float4* delta;
+    const int* keep;
 };
 
 // ---------------------------------------------------------------------------

@@ -408,6 +408,8 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
         }
         return;
     }
+    // the kept warm-start prediction enters here, on the substep's first load
+    const bool add_delta = a.delta != nullptr && a.keep[td.cloth] != 0;
     for (int i0 = tid; i0 < td.n_load; i0 += VERT_PREFETCH * T) {
         int g[VERT_PREFETCH];
 #pragma unroll
@@ -425,6 +427,16 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
                     float4 Vn;
                     P[i0 + u * T] = advect_one(cd, X[u], V[u], a.dt, &Vn);
                 }
+        } else if (add_delta) {
+            float4 q[VERT_PREFETCH], dq[VERT_PREFETCH];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) {
+                q[u] = p_in[max(g[u], 0)];
+                dq[u] = a.delta[max(g[u], 0)];
+            }
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) P[i0 + u * T] = make_float4(q[u].x + dq[u].x, q[u].y + dq[u].y, q[u].z + dq[u].z, q[u].w);
         } else {
             float4 q[VERT_PREFETCH];
 #pragma unroll
