@@ -578,14 +578,17 @@ void ws_require_params(pbd_ctx* c) {
 // Predicts the correction and decides, per cloth, whether to keep it (its
 // residual must not rise above the unwarmed start).  With apply, adds it to
 // the current p of the kept cloths; otherwise the caller passes ws.delta and
-// ws.keep to the tile kernel, which adds it while loading.
-void ws_apply_prediction(pbd_ctx* c, bool apply) {
+// ws.keep to the tile kernel, which adds it while loading.  With advect, the
+// inference kernel also advects (writes p from x and v); otherwise p holds
+// the advected positions already.
+void ws_apply_prediction(pbd_ctx* c, bool apply, bool advect, float dt) {
     hipStream_t s = c->stream;
     int ncl = (int)c->cloths.size();
     ws_build_nbr(c);
     float4* p = c->pbuf[c->pcur].ptr;
-    ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, c->ws.nbr.ptr, c->ws.nbr_rest.ptr, c->nv_total,
-             c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr, c->sp.warm_gate);
+    ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, advect ? c->v.ptr : nullptr, dt, c->ws.nbr.ptr,
+             c->ws.nbr_rest.ptr, c->nv_total, c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr,
+             c->sp.warm_gate);
     launch_fill_u64(s, c->ws.rs.ptr, 4 * (size_t)ncl, 0ull);
     ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.rs.ptr, ncl, c->sp.warm_rule);
     int full = c->sp.iterations;
@@ -624,12 +627,16 @@ bool has_tolerance(const pbd_ctx* c) {
     return false;
 }
 
-void advect_all(pbd_ctx* c, float dt) {
-    // the kernel constants hold g*dt for sp.dt; recompute when dt differs
+// The kernel constants hold g*dt for sp.dt; recompute them when dt differs.
+void advect_prepare(pbd_ctx* c, float dt) {
     if (dt != c->sp.dt) {
         c->sp.dt = dt;
         fill_cloth_dev(c);
     }
+}
+
+void advect_all(pbd_ctx* c, float dt) {
+    advect_prepare(c, dt);
     launch_advect(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, 0,
                   c->nv_total, dt, 0);
 }
@@ -856,12 +863,14 @@ void substep(pbd_ctx* c, float dt) {
     if (c->coll.enabled) {
         step_ccd(c, dt);
     } else if (c->ws.enabled) {
-        advect_all(c, dt);
         if (c->sp.path == PBD_PATH_GLOBAL) {
-            ws_apply_prediction(c, true);
+            advect_all(c, dt);
+            ws_apply_prediction(c, true, false, dt);
             project(c, c->sp.iterations, false, dt, c->ws.cap.ptr);
         } else {
-            ws_apply_prediction(c, false);
+            // advect inside the inference kernel; the prediction is added by the tile load
+            advect_prepare(c, dt);
+            ws_apply_prediction(c, false, true, dt);
             project(c, c->sp.iterations, false, dt, c->ws.cap.ptr, c->ws.delta.ptr, c->ws.keep.ptr);
         }
         integrate_all(c, dt, false);
@@ -1715,8 +1724,8 @@ int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
         for (int k = 0; k < substeps; k++) {
             advect_all(ctx, dt);
             float4* p = ctx->pbuf[ctx->pcur].ptr;
-            ws_infer(s, W.W.ptr, ctx->cl.ptr, ctx->vcloth.ptr, ctx->x.ptr, p, W.nbr.ptr, W.nbr_rest.ptr, ctx->nv_total,
-                     W.delta.ptr, W.X.ptr + 22 * W.ns, W.norm.ptr, W.feat.ptr, W.hs.ptr, 0.0f);
+            ws_infer(s, W.W.ptr, ctx->cl.ptr, ctx->vcloth.ptr, ctx->x.ptr, p, nullptr, 0.0f, W.nbr.ptr, W.nbr_rest.ptr,
+                     ctx->nv_total, W.delta.ptr, W.X.ptr + 22 * W.ns, W.norm.ptr, W.feat.ptr, W.hs.ptr, 0.0f);
             PBD_HIP(hipMemcpyAsync(W.psave.ptr, p, sizeof(float4) * ctx->nv_total, hipMemcpyDeviceToDevice, s));
             double tol = ctx->sp.tol;
             ctx->sp.tol = -1.0;

@@ -413,16 +413,23 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, const float
 // Features of vertex v: the standardised BF16 row in MFMA operand order
 // (b[4q..4q+3] = 16-byte chunk q), the output scale (returned) and, when
 // asked, the raw FP32 features.  s_mu / s_rsd: feature mean and 1 / sd.
+// ADVECT: p is not given; the predicted positions of v and its neighbours are
+// computed here from x and vel (the same advect_one as the advect kernel) and
+// v's is written to p_out.
+template <bool ADVECT>
 __device__ __forceinline__ float vertex_features(const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
                                                  const float4* __restrict__ x, const float4* __restrict__ p,
-                                                 const int* __restrict__ nbr, const float* __restrict__ nbr_rest,
-                                                 int v, const float* s_mu, const float* s_rsd, float gate,
-                                                 uint32_t* b, float* __restrict__ feat_out) {
-    // Loads in two rounds (own state, neighbour ids and rest lengths, cloth
-    // id; then the neighbour positions and the cloth scale), so a thread waits
-    // for two memory round trips rather than one per neighbour.
+                                                 const float4* __restrict__ vel, float4* __restrict__ p_out,
+                                                 float dt, const int* __restrict__ nbr,
+                                                 const float* __restrict__ nbr_rest, int v, const float* s_mu,
+                                                 const float* s_rsd, float gate, uint32_t* b,
+                                                 float* __restrict__ feat_out) {
+    // Loads in rounds (own state, neighbour ids and rest lengths, cloth id;
+    // then the neighbour states and the cloth constants), so a thread waits
+    // for a few memory round trips rather than one per neighbour.
     const int ci = vcloth[v];
-    const float4 P = p[v], X = x[v];
+    const float4 X = x[v];
+    float4 P = ADVECT ? vel[v] : p[v];
     int nj[6];
     float nl[6];
 #pragma unroll
@@ -430,10 +437,25 @@ __device__ __forceinline__ float vertex_features(const ClothDev* __restrict__ cl
         nj[k] = nbr[(size_t)v * 6 + k];
         nl[k] = nbr_rest[(size_t)v * 6 + k];
     }
-    const float h = cl[ci].hscale;
-    float4 nq[6];
+    const ClothDev& cd = cl[ci];
+    const float h = cd.hscale;
+    float4 nq[6], nv[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) nq[k] = p[max(nj[k], 0)];  // index 0 stands in for a missing neighbour
+    for (int k = 0; k < 6; k++) {  // index 0 stands in for a missing neighbour
+        if (ADVECT) {
+            nq[k] = x[max(nj[k], 0)];
+            nv[k] = vel[max(nj[k], 0)];
+        } else {
+            nq[k] = p[max(nj[k], 0)];
+        }
+    }
+    if (ADVECT) {
+        float4 vn;
+        P = advect_one(cd, X, P, dt, &vn);
+        p_out[v] = P;
+#pragma unroll
+        for (int k = 0; k < 6; k++) nq[k] = advect_one(cd, nq[k], nv[k], dt, &vn);
+    }
     float f[22];
     f[0] = (P.x - X.x) / h;
     f[1] = (P.y - X.y) / h;
@@ -493,7 +515,8 @@ __global__ __launch_bounds__(256) void k_ws_features(const ClothDev* __restrict_
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     uint32_t b[16];
-    const float h = vertex_features(cl, vcloth, x, p, nbr, nbr_rest, v, s_mu, s_rsd, gate, b, feat_out);
+    const float h = vertex_features<false>(cl, vcloth, x, p, nullptr, nullptr, 0.0f, nbr, nbr_rest, v, s_mu, s_rsd,
+                                           gate, b, feat_out);
     uint4* row = feat + (size_t)v * 4;
     for (int q = 0; q < 4; q++) row[q] = make_uint4(b[4 * q], b[4 * q + 1], b[4 * q + 2], b[4 * q + 3]);
     hs[v] = h;
@@ -617,11 +640,12 @@ struct alignas(16) InferStage {
     float hs[64];
 };
 
+template <bool ADVECT>
 __global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_infer(
     const MlpW* __restrict__ gW, const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
-    const float4* __restrict__ x, const float4* __restrict__ p, const int* __restrict__ nbr,
-    const float* __restrict__ nbr_rest, int nv, const float* __restrict__ norm, float gate,
-    float4* __restrict__ delta) {
+    const float4* __restrict__ x, const float4* __restrict__ p, const float4* __restrict__ vel,
+    float4* __restrict__ p_out, float dt, const int* __restrict__ nbr, const float* __restrict__ nbr_rest, int nv,
+    const float* __restrict__ norm, float gate, float4* __restrict__ delta) {
     __shared__ InfW W;
     __shared__ InferStage stage[MLP_WAVES];
     __shared__ float s_mu[22], s_rsd[22];
@@ -645,7 +669,8 @@ __global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_infer(
             uint32_t b[16];
             float h = 0.0f;
             if (v < nv) {
-                h = vertex_features(cl, vcloth, x, p, nbr, nbr_rest, v, s_mu, s_rsd, gate, b, nullptr);
+                h = vertex_features<ADVECT>(cl, vcloth, x, p, vel, p_out, dt, nbr, nbr_rest, v, s_mu, s_rsd, gate,
+                                            b, nullptr);
             } else {
 #pragma unroll
                 for (int k = 0; k < 16; k++) b[k] = 0u;
@@ -881,9 +906,10 @@ void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float
 // BF16 feature rows of the network kernel: 64 bytes per vertex.
 size_t ws_feature_bytes(int nv) { return (size_t)std::max(nv, 1) * 64; }
 
-void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
-              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm,
-              void* feat_scratch, float* hs_scratch, float gate) {
+void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, float4* p,
+              const float4* vel, float dt, const int* nbr, const float* nbr_rest, int nv, float4* delta,
+              float* feat_out, const float* norm, void* feat_scratch, float* hs_scratch, float gate) {
+    PBD_CHECK(!(vel && feat_out), PBD_ERR_STATE, "internal: advect-on-load inference keeps no raw features");
     if (nv <= 0) return;
     static int max_blocks = 0;
     if (!max_blocks) {
@@ -899,13 +925,18 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
             int dev = 0, ncu = 0, per_cu = 0;
             PBD_HIP(hipGetDevice(&dev));
             PBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-            PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ws_infer, 64 * MLP_WAVES, 0));
+            PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ws_infer<true>, 64 * MLP_WAVES, 0));
             max_blocks_fused = std::max(1, ncu) * std::max(1, per_cu);
         }
         const int nblk = (nv + 63) / 64;
         const int blocks = std::min((nblk + MLP_WAVES - 1) / MLP_WAVES, max_blocks_fused);
-        hipLaunchKernelGGL(k_ws_infer, dim3(blocks), dim3(64 * MLP_WAVES), 0, s, (const MlpW*)W, cl, vcloth, x, p,
-                           nbr, nbr_rest, nv, norm, gate, delta);
+        if (vel)
+            hipLaunchKernelGGL(k_ws_infer<true>, dim3(blocks), dim3(64 * MLP_WAVES), 0, s, (const MlpW*)W, cl, vcloth,
+                               x, (const float4*)nullptr, vel, p, dt, nbr, nbr_rest, nv, norm, gate, delta);
+        else
+            hipLaunchKernelGGL(k_ws_infer<false>, dim3(blocks), dim3(64 * MLP_WAVES), 0, s, (const MlpW*)W, cl,
+                               vcloth, x, (const float4*)p, (const float4*)nullptr, (float4*)nullptr, 0.0f, nbr,
+                               nbr_rest, nv, norm, gate, delta);
         PBD_HIP(hipGetLastError());
         return;
     }

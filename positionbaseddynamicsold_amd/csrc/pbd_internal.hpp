@@ -85,6 +85,23 @@ struct ClothDev {
     double tol;          // residual tolerance of this cloth (< 0: the solver's)
 };
 
+// AdvectGPUKernel (PBD_cuda.cu:949-979): v += g*dt*m ; v *= damping^dt ;
+// p = x + v*dt, with m -> w; optional wind drag v += ((wind - v)*(drag*dt))*w.
+__device__ __forceinline__ float4 advect_one(const ClothDev& c, float4 X, float4 V, float dt, float4* Vout) {
+    float w = X.w;
+    float vx = V.x + c.gdt[0] * w, vy = V.y + c.gdt[1] * w, vz = V.z + c.gdt[2] * w;
+    if (c.wind_drag != 0.0f) {
+        vx = vx + ((c.wind[0] - vx) * c.kd) * w;
+        vy = vy + ((c.wind[1] - vy) * c.kd) * w;
+        vz = vz + ((c.wind[2] - vz) * c.kd) * w;
+    }
+    vx = vx * c.damp;
+    vy = vy * c.damp;
+    vz = vz * c.damp;
+    *Vout = make_float4(vx, vy, vz, 0.0f);
+    return make_float4(X.x + vx * dt, X.y + vy * dt, X.z + vz * dt, w);
+}
+
 // One LDS tile: a set of loaded vertices (interior first) and the prims whose
 // vertices are all loaded, grouped by colour.
 struct TileDev {
@@ -225,9 +242,11 @@ void mlp_grad(hipStream_t s, const void* W, const float* X, const float* T, cons
               float* partial, float* grad);
 void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
               int step);
-void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, const float4* p,
-              const int* nbr, const float* nbr_rest, int nv, float4* delta, float* feat_out, const float* norm,
-              void* feat_scratch, float* hs_scratch, float gate);
+// Predicted corrections.  With vel, p is written: the advected positions of
+// x and vel (advect on load, no raw features); otherwise p is read.
+void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, float4* p,
+              const float4* vel, float dt, const int* nbr, const float* nbr_rest, int nv, float4* delta,
+              float* feat_out, const float* norm, void* feat_scratch, float* hs_scratch, float gate);
 size_t ws_feature_bytes(int nv);
 void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm);
 void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* rs,

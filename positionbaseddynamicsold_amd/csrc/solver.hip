@@ -11,23 +11,6 @@
 
 namespace pbd {
 
-// AdvectGPUKernel (PBD_cuda.cu:949-979): v += g*dt*m ; v *= damping^dt ;
-// p = x + v*dt, with m -> w; optional wind drag v += ((wind - v)*(drag*dt))*w.
-__device__ __forceinline__ float4 advect_one(const ClothDev& c, float4 X, float4 V, float dt, float4* Vout) {
-    float w = X.w;
-    float vx = V.x + c.gdt[0] * w, vy = V.y + c.gdt[1] * w, vz = V.z + c.gdt[2] * w;
-    if (c.wind_drag != 0.0f) {
-        vx = vx + ((c.wind[0] - vx) * c.kd) * w;
-        vy = vy + ((c.wind[1] - vy) * c.kd) * w;
-        vz = vz + ((c.wind[2] - vz) * c.kd) * w;
-    }
-    vx = vx * c.damp;
-    vy = vy * c.damp;
-    vz = vz * c.damp;
-    *Vout = make_float4(vx, vy, vz, 0.0f);
-    return make_float4(X.x + vx * dt, X.y + vy * dt, X.z + vz * dt, w);
-}
-
 __global__ __launch_bounds__(256) void k_advect(const ClothDev* __restrict__ cl,
                                                 const int* __restrict__ vcloth,
                                                 const float4* __restrict__ x, float4* __restrict__ v,
