@@ -723,6 +723,8 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     a.fused = fused_whole ? 1 : 0;
     a.delta = delta;
     a.keep = keep;
+    // a fused substep advects every cloth in its first launch
+    a.advect_load = fused_whole ? 1 : 0;
     if (c->any_whole && !c->any_partial) {
         a.sweep_base = 0;
         a.nsweeps = iterations;
@@ -740,6 +742,7 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
             tiles((int)ord.n, ord.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
             a.delta = nullptr;  // later chunks read what the first one wrote
             a.keep = nullptr;
+            a.advect_load = 0;
             cur ^= 1;
             sw += a.nsweeps;
             if (any_tol && ch + 1 < nchunks) {
@@ -879,10 +882,8 @@ void substep(pbd_ctx* c, float dt) {
         project(c, c->sp.iterations, false, dt);
         integrate_all(c, dt, false);
     } else {
-        // multi-tile cloths advect separately; whole cloths fuse it
-        if (c->any_partial)
-            launch_advect(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, 0,
-                          c->nv_total, dt, 1);
+        // every tile advects on its first load; whole cloths also integrate on store
+        advect_prepare(c, dt);
         project(c, c->sp.iterations, true, dt);
         integrate_all(c, dt, true);
     }

@@ -136,6 +136,7 @@ struct SolveArgs {
     // p + delta for the vertices of cloths with keep[cloth] != 0
     const float4* delta;
     const int* keep;
+    int advect_load;     // 1: multi-tile cloths' tiles advect x, v on load (first launch only)
 };
 
 // ---------------------------------------------------------------------------

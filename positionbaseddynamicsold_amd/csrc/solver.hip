@@ -397,7 +397,7 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
         int g[VERT_PREFETCH];
 #pragma unroll
         for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < td.n_load ? vl[i0 + u * T] : -1;
-        if (td.whole && a.fused) {
+        if (td.whole ? a.fused : a.advect_load) {
             float4 X[VERT_PREFETCH], V[VERT_PREFETCH];
 #pragma unroll
             for (int u = 0; u < VERT_PREFETCH; u++) {
