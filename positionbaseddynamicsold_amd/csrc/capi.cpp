@@ -90,6 +90,7 @@ struct pbd_ctx {
     // the replay ends in is graph_end
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_end[2] = {0, 0};
+    bool graph_pvalid[2] = {false, false};  // pfinal_valid at the end of the captured substep
     // device-side state snapshot
     DBuf<float4> snap_x, snap_v;
     bool snap_valid = false;
@@ -482,7 +483,8 @@ void collect_events(pbd_ctx* c) {
 }
 
 // Algorithmic bytes per prim per sweep and per vertex (fused advect + integrate).
-constexpr double kDistBytes = 68.0, kBendBytes = 132.0, kVertBytes = 100.0;
+constexpr double kDistBytes = 68.0, kBendBytes = 132.0, kVertBytes = 100.0;  // per prim per sweep; per vertex
+constexpr double kAdvectBytes = 52.0, kIntegrateBytes = 48.0;                // the two parts of kVertBytes
 
 // ---- broadphase configuration ----------------------------------------------
 // Grid start as SpatialHashSystem::InitSH computes it (SpatialHashSystem.cpp:88-98):
@@ -639,15 +641,21 @@ void advect_all(pbd_ctx* c, float dt) {
     advect_prepare(c, dt);
     launch_advect(c->stream, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, 0,
                   c->nv_total, dt, 0);
+    c->pfinal_valid = false;  // p is rewritten whole
 }
 
 // Sweeps on the current p.  Whole-cloth tiles run unfused (p_in -> p_out).
 // cap: optional per-cloth sweep budgets (device array) below `iterations`.
+void consolidate(pbd_ctx* c);
+
 // delta / keep: a warm-start prediction the first tile launch adds while
-// loading (LDS-tile path only).
-void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* cap = nullptr,
-             const float4* delta = nullptr, const int* keep = nullptr) {
+// loading (LDS-tile path only).  integrate_end: the tiles may integrate each
+// cloth when they store its last sweep; returns true if they did (the caller
+// then skips integrate_all, and p is left where pfinal says).
+bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* cap = nullptr,
+             const float4* delta = nullptr, const int* keep = nullptr, bool integrate_end = false) {
     hipStream_t s = c->stream;
+    consolidate(c);  // an earlier substep may have left p unconsolidated
     int ncl = (int)c->cloths.size();
     double tol = c->sp.tol;
     int ce = std::max(1, c->sp.check_every);
@@ -690,11 +698,15 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
             launch_min_sweeps(s, c->sweeps.ptr, cap, sw, ncl);
             c->sweeps_pending = true;
         }
-        return;
+        return false;
     }
     replan_if_needed(c);
     int ts = effective_tile_sweeps(c);
     int nchunks = (iterations + ts - 1) / ts;
+    // Tiles integrate their cloth on storing its last sweep when budgets are
+    // fixed and no launch that can end a cloth reads x: the advect-on-load
+    // launch (the first) ends none when there are at least two chunks.
+    const bool int_store = integrate_end && !any_tol && !(fused_whole && c->any_partial && nchunks < 2);
     ensure_slots(c, nchunks);
     launch_fill_u64(s, c->res_slots.ptr, (size_t)c->nslots * ncl, 0ull);
     size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
@@ -725,6 +737,7 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     a.keep = keep;
     // a fused substep advects every cloth in its first launch
     a.advect_load = fused_whole ? 1 : 0;
+    a.integrate_end = int_store ? 1 : 0;
     if (c->any_whole && !c->any_partial) {
         a.sweep_base = 0;
         a.nsweeps = iterations;
@@ -783,11 +796,14 @@ void project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
         for (auto& k : c->cloths) {
             bytes += (double)k.last_sweeps * (kDistBytes * k.ne + kBendBytes * k.nb);
             if (k.whole && fused_whole) bytes += kVertBytes * k.nv;
+            else if (fused_whole) bytes += kAdvectBytes * k.nv;  // advect on the first load
+            if (int_store && !(k.whole && fused_whole)) bytes += kIntegrateBytes * k.nv;
             c->prof.vertex_sweeps += (long long)k.nv * k.last_sweeps;
             c->prof.sweeps_max = std::max<long long>(c->prof.sweeps_max, k.last_sweeps);
         }
         c->prof.tile_alg_bytes += bytes;
     }
+    return int_store;
 }
 
 void integrate_all(pbd_ctx* c, float dt, bool skip_whole) {
@@ -863,6 +879,9 @@ void step_ccd(pbd_ctx* c, float dt) {
 
 // One substep of the whole batch.
 void substep(pbd_ctx* c, float dt) {
+    // x and v hold the state; p is recomputed from them (a previous substep
+    // whose tiles integrated may have left it unconsolidated: it is dead now)
+    c->pfinal_valid = false;
     if (c->coll.enabled) {
         step_ccd(c, dt);
     } else if (c->ws.enabled) {
@@ -871,10 +890,12 @@ void substep(pbd_ctx* c, float dt) {
             ws_apply_prediction(c, true, false, dt);
             project(c, c->sp.iterations, false, dt, c->ws.cap.ptr);
         } else {
-            // advect inside the inference kernel; the prediction is added by the tile load
+            // advect inside the inference kernel; the prediction is added by the
+            // tile load, and each cloth is integrated by the tiles of its last sweep
             advect_prepare(c, dt);
             ws_apply_prediction(c, false, true, dt);
-            project(c, c->sp.iterations, false, dt, c->ws.cap.ptr, c->ws.delta.ptr, c->ws.keep.ptr);
+            if (project(c, c->sp.iterations, false, dt, c->ws.cap.ptr, c->ws.delta.ptr, c->ws.keep.ptr, true))
+                return;
         }
         integrate_all(c, dt, false);
     } else if (c->sp.path == PBD_PATH_GLOBAL) {
@@ -882,10 +903,9 @@ void substep(pbd_ctx* c, float dt) {
         project(c, c->sp.iterations, false, dt);
         integrate_all(c, dt, false);
     } else {
-        // every tile advects on its first load; whole cloths also integrate on store
+        // every tile advects on its first load and integrates on storing its last sweep
         advect_prepare(c, dt);
-        project(c, c->sp.iterations, true, dt);
-        integrate_all(c, dt, true);
+        if (!project(c, c->sp.iterations, true, dt, nullptr, nullptr, nullptr, true)) integrate_all(c, dt, true);
     }
 }
 
@@ -921,10 +941,12 @@ void graph_substep(pbd_ctx* c, float dt) {
         PBD_HIP(e);
         c->graph[start] = ex;
         c->graph_end[start] = c->pcur;
+        c->graph_pvalid[start] = c->pfinal_valid;
         c->pcur = start;
     }
     PBD_HIP(hipGraphLaunch(c->graph[start], s));
     c->pcur = c->graph_end[start];
+    c->pfinal_valid = c->graph_pvalid[start];
     if (c->ws.enabled) {
         c->ws.used_pending = true;
         c->sweeps_pending = true;
@@ -940,6 +962,7 @@ void graph_substep(pbd_ctx* c, float dt) {
 // PositionBasedDynamics.cpp:263-290).  Same arithmetic and order as the batch.
 void step_cloth(pbd_ctx* c, int ci, float dt) {
     hipStream_t s = c->stream;
+    consolidate(c);  // this cloth's range is rewritten below; the others must be in place
     Cloth& k = c->cloths[ci];
     const int ncl = (int)c->cloths.size();
     const int I = c->sp.iterations, ce = std::max(1, c->sp.check_every);
@@ -1971,6 +1994,7 @@ int pbd_cloth_get_state(pbd_ctx* ctx, int id, int which, float* xyz) {
         PBD_CHECK(xyz, PBD_ERR_ARG, "null output");
         PBD_CHECK(which >= PBD_STATE_X && which <= PBD_STATE_P, PBD_ERR_ARG, "unknown state buffer");
         set_device(ctx);
+        if (which == PBD_STATE_P) consolidate(ctx);
         DBuf<float4>& b = which == PBD_STATE_X ? ctx->x : which == PBD_STATE_V ? ctx->v : ctx->pbuf[ctx->pcur];
         std::vector<float4> h(c.nv);
         PBD_HIP(hipMemcpyAsync(h.data(), b.ptr + c.vbase, sizeof(float4) * c.nv, hipMemcpyDeviceToHost, ctx->stream));
@@ -1990,6 +2014,7 @@ int pbd_cloth_set_state(pbd_ctx* ctx, int id, int which, const float* xyz) {
         PBD_CHECK(xyz, PBD_ERR_ARG, "null input");
         PBD_CHECK(which >= PBD_STATE_X && which <= PBD_STATE_P, PBD_ERR_ARG, "unknown state buffer");
         set_device(ctx);
+        if (which == PBD_STATE_P) consolidate(ctx);  // the other cloths' p must be in place first
         DBuf<float4>& b = which == PBD_STATE_X ? ctx->x : which == PBD_STATE_V ? ctx->v : ctx->pbuf[ctx->pcur];
         std::vector<float4> h(c.nv);
         for (int i = 0; i < c.nv; i++)
@@ -2006,6 +2031,7 @@ int pbd_cloth_residual(pbd_ctx* ctx, int id, double* max_rel, double* rms) {
         set_device(ctx);
         hipStream_t s = ctx->stream;
         int ncl = (int)ctx->cloths.size();
+        consolidate(ctx);
         launch_fill_u64(s, ctx->rmax.ptr, ncl, 0ull);
         PBD_HIP(hipMemsetAsync(ctx->rsum.ptr, 0, sizeof(double) * ncl, s));
         launch_residual(s, ctx->rdist.ptr, (int)ctx->rdist.n, ctx->pbuf[ctx->pcur].ptr, ctx->rmax.ptr, ctx->rsum.ptr,
@@ -2029,6 +2055,7 @@ int pbd_residual_all(pbd_ctx* ctx, double* max_rel, double* rms) {
         set_device(ctx);
         hipStream_t s = ctx->stream;
         int ncl = (int)ctx->cloths.size();
+        consolidate(ctx);
         launch_fill_u64(s, ctx->rmax.ptr, ncl, 0ull);
         PBD_HIP(hipMemsetAsync(ctx->rsum.ptr, 0, sizeof(double) * ncl, s));
         launch_residual(s, ctx->rdist.ptr, (int)ctx->rdist.n, ctx->pbuf[ctx->pcur].ptr, ctx->rmax.ptr, ctx->rsum.ptr,
@@ -2081,6 +2108,7 @@ int pbd_state_load(pbd_ctx* ctx, const float* x, const float* v) {
         ctx->v.upload(V, ctx->stream);
         PBD_HIP(hipMemcpyAsync(ctx->pbuf[ctx->pcur].ptr, ctx->x.ptr, sizeof(float4) * ctx->nv_total,
                                hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->pfinal_valid = false;
         PBD_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
@@ -2103,6 +2131,7 @@ int pbd_snapshot(pbd_ctx* ctx, int restore) {
             PBD_HIP(hipMemcpyAsync(ctx->x.ptr, ctx->snap_x.ptr, b, hipMemcpyDeviceToDevice, s));
             PBD_HIP(hipMemcpyAsync(ctx->v.ptr, ctx->snap_v.ptr, b, hipMemcpyDeviceToDevice, s));
             PBD_HIP(hipMemcpyAsync(ctx->pbuf[ctx->pcur].ptr, ctx->snap_x.ptr, b, hipMemcpyDeviceToDevice, s));
+            ctx->pfinal_valid = false;
         }
         PBD_HIP(hipStreamSynchronize(s));
     });
@@ -2125,6 +2154,7 @@ int pbd_autotune(pbd_ctx* ctx, int trial_substeps, pbd_solver_params* best, int*
             launch_copy(s, sx.ptr, ctx->x.ptr, nv);
             launch_copy(s, sv.ptr, ctx->v.ptr, nv);
             launch_copy(s, sx.ptr, ctx->pbuf[ctx->pcur].ptr, nv);
+            ctx->pfinal_valid = false;
         };
         const pbd_solver_params base = ctx->sp;
         const HashCfg hbase = ctx->ccfg;

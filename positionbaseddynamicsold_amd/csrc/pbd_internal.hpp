@@ -137,6 +137,8 @@ struct SolveArgs {
     const float4* delta;
     const int* keep;
     int advect_load;     // 1: multi-tile cloths' tiles advect x, v on load (first launch only)
+    int integrate_end;   // 1: a tile storing its cloth's last sweep also integrates (x, v); no tile of
+                         //    a launch where this can happen reads x
 };
 
 // ---------------------------------------------------------------------------

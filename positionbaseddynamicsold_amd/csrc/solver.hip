@@ -515,11 +515,15 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
         atomicMax(&res_slots[(size_t)a.chunk * ncl + td.cloth], (unsigned long long)__double_as_longlong(resid));
         atomicMax(&sweeps_out[td.cloth], a.sweep_base + sw);
     }
+    // integrate on store: a fused whole cloth, or the launch holding this
+    // cloth's last sweep when the caller allows it (fixed budgets only)
+    const int budget = sweep_cap ? min(sweep_cap[td.cloth], a.whole_nsweeps) : a.whole_nsweeps;
+    const bool integ = (td.whole && a.fused) || (a.integrate_end && a.sweep_base + nsweeps >= budget);
     for (int i0 = tid; i0 < td.n_int; i0 += VERT_PREFETCH * T) {
         int g[VERT_PREFETCH];
 #pragma unroll
         for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < td.n_int ? vl[i0 + u * T] : -1;
-        if (td.whole && a.fused) {
+        if (integ) {
             float4 X[VERT_PREFETCH];
 #pragma unroll
             for (int u = 0; u < VERT_PREFETCH; u++) X[u] = x[max(g[u], 0)];
