@@ -259,3 +259,31 @@ def test_error_codes():
         with pytest.raises(capi.PbdError) as e:
             ctx.init()
         assert e.value.code == capi.PBD_ERR_STATE
+
+
+def test_step_then_split_api_after_tile_integration():
+    # the tiles integrate each cloth when storing its last sweep and leave p
+    # where each cloth's last sweep wrote it; p readers, the split-loop API
+    # and a later step must still see the same state as stepping throughout
+    def make(ctx):
+        ids = [lib_grid(ctx, 96, pins=(0, 95)), lib_grid(ctx, 20, seed=3, pins=(0, 19))]
+        ctx.init()
+        ctx.set_solver(dt=1.0 / 600.0, iterations=8, check_every=2, tile_sweeps=2, tile_max_verts=2048)
+        return ids
+    with capi.Context() as a, capi.Context() as b:
+        ia, ib = make(a), make(b)
+        assert a.info(ia[0]).whole == 0 and a.info(ia[1]).whole == 1
+        a.step(2)
+        for i in ia:  # consolidated p equals the integrated x
+            assert np.array_equal(a.get(i, capi.STATE_P), a.get(i))
+        a.advect(1.0 / 600.0)
+        a.project(8)
+        a.integrate(1.0 / 600.0)
+        a.step(1)
+        b.step(4)
+        for i, j in zip(ia, ib):
+            assert np.array_equal(a.get(i), b.get(j))
+            assert np.array_equal(a.get(i, capi.STATE_V), b.get(j, capi.STATE_V))
+            assert np.array_equal(a.get(i, capi.STATE_P), b.get(j, capi.STATE_P))
+        ra, rb = a.residual_all(), b.residual_all()
+        assert np.array_equal(ra[0], rb[0])
