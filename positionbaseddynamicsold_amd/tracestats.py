@@ -9,10 +9,11 @@ for the two eager passes bench.py times with HIP events after its headline
 runs -- K warm-started substeps followed by K plain substeps, each preceded
 by one untimed substep, just before the copy-kernel calibration -- so the
 tile kernel's average launch duration can be compared with the bench's own
-figure.  Substeps are delimited by the batch-wide integrate launches.
+figure.  Substeps are delimited by the per-substep k_final_buffers launch (the
+tile path's last kernel of a projection; integrate may run inside the tiles).
 
 With counter files from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
-runs (kernel filter "k_tiles|k_copy|k_integrate") of bench.py or of
+runs (kernel filter "k_tiles|k_copy|k_final_buffers") of bench.py or of
 positionbaseddynamicsold_amd.tilework, which repeats the same two passes, it
 prints the tile kernel's HBM traffic per launch in those passes, scaled by
 what the counters read for the calibration copy (1 GiB read + 1 GiB written
@@ -55,9 +56,7 @@ def passes(seq, steps, name_of=lambda e: e[0], grid_of=lambda e: e[3]):
     c0 = max(i for i in copies if grid_of(seq[i]) == big)
     while c0 > 0 and name_of(seq[c0 - 1]) == "k_copy" and grid_of(seq[c0 - 1]) == big:
         c0 -= 1
-    ints = [i for i in range(c0) if name_of(seq[i]) == "k_integrate"]
-    top = max(grid_of(seq[i]) for i in ints)
-    ints = [i for i in ints if grid_of(seq[i]) == top]
+    ints = [i for i in range(c0) if name_of(seq[i]) == "k_final_buffers"]
     need = 2 * (steps + 1)
     if len(ints) < need:
         raise SystemExit("trace does not hold the two profiled passes")
