@@ -153,7 +153,7 @@ PBD_API int pbd_advect(pbd_ctx* ctx, float dt);
 PBD_API int pbd_project(pbd_ctx* ctx, int iterations);
 /* SolverPBD::Integration(dt) (PBD_cuda.cu:1437-1518) */
 PBD_API int pbd_integrate(pbd_ctx* ctx, float dt);
-/* The driver loop body Advect -> ProjectConstraint -> Integration
+/* The reference's main-loop body Advect -> ProjectConstraint -> Integration
  * (PositionBasedDynamics.cpp:263-290), `substeps` times, fused. */
 PBD_API int pbd_step(pbd_ctx* ctx, int substeps);
 /* SolverPBD::SetTarget(obj) and the same loop body on that one object

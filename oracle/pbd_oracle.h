@@ -5,7 +5,7 @@
  * algorithms (vivienldy/PositionBasedDynamicsOld, directory
  * PositionBasedDynamicsCUDA/PositionBasedDynamicsCUDA/, abbreviated PBD/ below).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load it, and only as the checker or the timed CPU baseline.  The product
+ * load it, and only to check results or as the timed CPU baseline.  The product
  * library (libpbd.so) never links or calls it.
  *
  * Every function cites the reference file:line it follows.  Pinning of each

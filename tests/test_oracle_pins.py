@@ -18,7 +18,7 @@ def F(a):
 @pytest.mark.parametrize("n", [2, 3, 32, 256, 512])
 def test_grid_counts_closed_form(orc, n):
     # E = 2n(n-1) + (n-1)^2 and T = 2(n-1)^2 (reference index order
-    # PBD_cuda.cu:19-54, 854-869); BASELINE config 1 quotes 2945 / 1922.
+    # PBD_cuda.cu:19-54, 854-869); the 32x32 workload is quoted with 2945 / 1922.
     pairs = orc.grid_pairs(n, n)
     _, tris = orc.grid_mesh(n, n)
     assert len(pairs) == 2 * n * (n - 1) + (n - 1) ** 2
@@ -105,7 +105,8 @@ def test_powf_distance_equals_sqrtf_form(orc):
 def test_ground_truth_colouring_kat(orc):
     # PBDObject::groundTruthTest (PBD_cuda.cu:871-881): edges (0,3),(0,1),(0,2),(1,3),(2,3).
     # Worked by hand with the bulk-synchronous assign/resolve rules (ties to the
-    # lower prim id): round 1 commits (0,3)=0; round 2 (0,1)=1; round 3 the rest.
+    # lower prim id): the first colouring pass commits (0,3)=0, the second
+    # (0,1)=1, the third the rest.
     pairs = np.array([[0, 3], [0, 1], [0, 2], [1, 3], [2, 3]], np.int32)
     pos = np.zeros((4, 3), np.float32)
     pl, idx, _ = orc.build_prims(pos + np.arange(4)[:, None], pairs)
