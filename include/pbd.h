@@ -124,6 +124,21 @@ PBD_API int pbd_cloth_create_mesh(pbd_ctx* ctx, const float* pos, int nv, const 
 /* PBDObject::Init(topolFile, distConstrFile) (PBD_cuda.cu:650-658): the reference
  * text formats, IO::ReadTopolFromTxt + IO::ReadDistConstrFromTxt (PBD_Basic.cuh:532-688) */
 PBD_API int pbd_cloth_load_txt(pbd_ctx* ctx, const char* topol_path, const char* constr_path, int* cloth);
+/* PBDObject::ContinueSimInit(meshTopolPath, constrPath) (PBD_cuda.cu:660-667):
+ * IO::ReadTopolFromCache + IO::ReadConstraintFromCache (PBD_Basic.cuh:326-530).
+ * The cloth starts from the cached P, velBuffer and prdPBuffer, with massBuffer
+ * as its weights (0 pins a vertex), dampingRate and gravity, and the cached
+ * prims (DISTANCE pairs, BENDING quads, ANCHOR pins) with their cached rest
+ * values.  One stiffness per constraint type; constr_path may be NULL (mesh
+ * edges from the triangles, rest values from P). */
+PBD_API int pbd_cloth_load_cache(pbd_ctx* ctx, const char* mesh_path, const char* constr_path, int* cloth);
+/* PBDObject::SaveMeshTopol + PBDObject::SaveConstraint (PBD_cuda.cu:907-938):
+ * the cloth's current state and constraints in the reference's cache format
+ * (after pbd_init).  precision = significant digits of every float (0 = 6, the
+ * reference's stream default; 9 makes the round trip exact).  Either path may
+ * be NULL to skip that file. */
+PBD_API int pbd_cloth_save_cache(pbd_ctx* ctx, int cloth, const char* mesh_path, const char* constr_path,
+                                 int precision);
 /* Explicit distance prims (IO::ReadDistConstrFromTxt pairs, PBD_Basic.cuh:654-675);
  * without it a grid cloth uses InitDistanceIndices order (PBD_cuda.cu:19-54) and a
  * mesh cloth uses its triangle edges in first-sighting order. */
@@ -133,6 +148,9 @@ PBD_API int pbd_cloth_set_constraint_option(pbd_ctx* ctx, int cloth, unsigned ct
 /* Pins (replaces InitAnchorConstr, PBD_cuda.cu:125-153): weight 0 for the listed vertices */
 PBD_API int pbd_cloth_set_pins(pbd_ctx* ctx, int cloth, const int* verts, int n);
 PBD_API int pbd_cloth_set_params(pbd_ctx* ctx, int cloth, const pbd_cloth_params* params);
+/* The cloth's current parameters (PBDObject::dampingRate/gravity, PBD_Basic.cuh:190-192),
+ * e.g. those a cache file set, to change one field with pbd_cloth_set_params. */
+PBD_API int pbd_cloth_get_params(pbd_ctx* ctx, int cloth, pbd_cloth_params* params);
 /* initConstr: constraints, GenePoint2PrimsMap/GenePrim2PrimsMap, InitGPUBuffers,
  * EdgeColoring(color_rounds) on the GPU, SortEdgesColors + EvalWorksets
  * (PBD_cuda.cu:798-829,155-202,707-750,487-625), for every cloth created so far. */

@@ -90,10 +90,13 @@ SIGNATURES = {
     "pbd_cloth_create_grid": [_P, C.POINTER(GridDesc), _ip],
     "pbd_cloth_create_mesh": [_P, _fp, _I, _ip, _I, _ip],
     "pbd_cloth_load_txt": [_P, C.c_char_p, C.c_char_p, _ip],
+    "pbd_cloth_load_cache": [_P, C.c_char_p, C.c_char_p, _ip],
+    "pbd_cloth_save_cache": [_P, _I, C.c_char_p, C.c_char_p, _I],
     "pbd_cloth_set_distance_pairs": [_P, _I, _ip, _I],
     "pbd_cloth_set_constraint_option": [_P, _I, C.c_uint, _fp],
     "pbd_cloth_set_pins": [_P, _I, _ip, _I],
     "pbd_cloth_set_params": [_P, _I, C.POINTER(ClothParams)],
+    "pbd_cloth_get_params": [_P, _I, C.POINTER(ClothParams)],
     "pbd_init": [_P, _I],
     "pbd_cloth_get_info": [_P, _I, C.POINTER(ClothInfo)],
     "pbd_cloth_get_prims": [_P, _I, _ip, _ip, _fp],
@@ -249,6 +252,21 @@ class Context:
         self.cloths.append(cid.value)
         return cid.value
 
+    def load_cache(self, mesh: str, constr: str | None = None) -> int:
+        """PBDObject::ContinueSimInit (PBD_cuda.cu:660-667): a cloth continued
+        from the reference's mesh and constraint cache files."""
+        cid = C.c_int()
+        check(self.lib.pbd_cloth_load_cache(self.h, mesh.encode(), constr.encode() if constr else None,
+                                            C.byref(cid)))
+        self.cloths.append(cid.value)
+        return cid.value
+
+    def save_cache(self, cloth: int, mesh: str | None, constr: str | None = None, precision: int = 0):
+        """PBDObject::SaveMeshTopol + SaveConstraint (PBD_cuda.cu:907-938);
+        precision 0 writes the reference's 6 significant digits, 9 is exact."""
+        check(self.lib.pbd_cloth_save_cache(self.h, cloth, mesh.encode() if mesh else None,
+                                            constr.encode() if constr else None, precision))
+
     def set_distance_pairs(self, cloth: int, pairs: np.ndarray):
         pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
         check(self.lib.pbd_cloth_set_distance_pairs(self.h, cloth, iptr(pairs), len(pairs)))
@@ -264,6 +282,22 @@ class Context:
     def set_params(self, cloth: int, params: ClothParams | None = None, **kw):
         p = params if params is not None else default_cloth_params(**kw)
         check(self.lib.pbd_cloth_set_params(self.h, cloth, C.byref(p)))
+
+    def get_params(self, cloth: int) -> ClothParams:
+        p = ClothParams()
+        check(self.lib.pbd_cloth_get_params(self.h, cloth, C.byref(p)))
+        return p
+
+    def update_params(self, cloth: int, **kw):
+        """Changes the named fields, keeping the others (e.g. a cache's damping and gravity)."""
+        p = self.get_params(cloth)
+        for k, v in kw.items():
+            f = getattr(p, k)
+            if hasattr(f, "__len__"):
+                f[:] = v
+            else:
+                setattr(p, k, v)
+        self.set_params(cloth, p)
 
     def init(self, color_rounds: int = 20000):
         check(self.lib.pbd_init(self.h, color_rounds))

@@ -431,19 +431,30 @@ void do_init(pbd_ctx* c, int color_rounds) {
     c->gbrc.upload(gbr, s);
     c->rdist.upload(rd, s);
     // ---- state buffers (InitGPUBuffers, PBD_cuda.cu:707-750)
-    std::vector<float4> X(c->nv_total), V(c->nv_total, make_float4(0, 0, 0, 0));
+    // a cloth continued from a cache starts from its cached x, v and p
+    std::vector<float4> X(c->nv_total), V(c->nv_total, make_float4(0, 0, 0, 0)), P;
     std::vector<int> vc(c->nv_total);
+    bool any_p = false;
+    for (Cloth& k : c->cloths) any_p = any_p || !k.p_start.empty();
+    if (any_p) P.resize(c->nv_total);
     for (size_t ci = 0; ci < c->cloths.size(); ci++) {
         Cloth& k = c->cloths[ci];
+        const float* xs = k.x_start.empty() ? k.pos0.data() : k.x_start.data();
         for (int i = 0; i < k.nv; i++) {
-            X[k.vbase + i] = make_float4(k.pos0[3 * i], k.pos0[3 * i + 1], k.pos0[3 * i + 2], k.w[i]);
+            X[k.vbase + i] = make_float4(xs[3 * i], xs[3 * i + 1], xs[3 * i + 2], k.w[i]);
+            if (!k.v_start.empty())
+                V[k.vbase + i] = make_float4(k.v_start[3 * i], k.v_start[3 * i + 1], k.v_start[3 * i + 2], 0.0f);
+            if (any_p)
+                P[k.vbase + i] = k.p_start.empty() ? X[k.vbase + i]
+                                                   : make_float4(k.p_start[3 * i], k.p_start[3 * i + 1],
+                                                                 k.p_start[3 * i + 2], k.w[i]);
             vc[k.vbase + i] = (int)ci;
         }
     }
     c->x.upload(X, s);
     c->v.upload(V, s);
-    c->pbuf[0].upload(X, s);
-    c->pbuf[1].upload(X, s);
+    c->pbuf[0].upload(any_p ? P : X, s);
+    c->pbuf[1].upload(any_p ? P : X, s);
     c->pcur = 0;
     c->vcloth.upload(vc, s);
     c->rmax.alloc(c->cloths.size());
@@ -1184,6 +1195,43 @@ int pbd_cloth_load_txt(pbd_ctx* ctx, const char* topol, const char* constr, int*
     });
 }
 
+int pbd_cloth_load_cache(pbd_ctx* ctx, const char* mesh_path, const char* constr_path, int* cloth) {
+    return guard([&] {
+        PBD_CHECK(ctx && mesh_path && cloth, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(!ctx->inited, PBD_ERR_STATE, "cloths must be created before pbd_init");
+        Cloth c;
+        c.prm = default_params();
+        read_cache(mesh_path, constr_path, c);
+        ctx->cloths.push_back(std::move(c));
+        *cloth = (int)ctx->cloths.size() - 1;
+    });
+}
+
+int pbd_cloth_save_cache(pbd_ctx* ctx, int id, const char* mesh_path, const char* constr_path, int precision) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(ctx->inited, PBD_ERR_STATE, "call pbd_init first");
+        PBD_CHECK(precision >= 0 && precision <= 17, PBD_ERR_ARG, "precision must be 0 (6 digits) to 17");
+        set_device(ctx);
+        consolidate(ctx);
+        std::vector<float> st[3];
+        const DBuf<float4>* src[3] = {&ctx->x, &ctx->v, &ctx->pbuf[ctx->pcur]};
+        std::vector<float4> h(c.nv);
+        for (int k = 0; k < 3; k++) {
+            PBD_HIP(hipMemcpyAsync(h.data(), src[k]->ptr + c.vbase, sizeof(float4) * c.nv, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            PBD_HIP(hipStreamSynchronize(ctx->stream));
+            st[k].resize((size_t)3 * c.nv);
+            for (int i = 0; i < c.nv; i++) {
+                st[k][3 * i] = h[i].x;
+                st[k][3 * i + 1] = h[i].y;
+                st[k][3 * i + 2] = h[i].z;
+            }
+        }
+        write_cache(c, st[0].data(), st[1].data(), st[2].data(), mesh_path, constr_path, precision);
+    });
+}
+
 int pbd_cloth_set_distance_pairs(pbd_ctx* ctx, int id, const int* pairs, int ne) {
     return guard([&] {
         Cloth& c = cloth_of(ctx, id);
@@ -1229,6 +1277,14 @@ int pbd_cloth_set_params(pbd_ctx* ctx, int id, const pbd_cloth_params* p) {
             set_device(ctx);
             fill_cloth_dev(ctx);
         }
+    });
+}
+
+int pbd_cloth_get_params(pbd_ctx* ctx, int id, pbd_cloth_params* p) {
+    return guard([&] {
+        Cloth& c = cloth_of(ctx, id);
+        PBD_CHECK(p, PBD_ERR_ARG, "null params");
+        *p = c.prm;
     });
 }
 

@@ -152,7 +152,10 @@ void build_prims(Cloth& c) {
     else mesh_edge_pairs(c.tris, pairs);
     if (!(c.ct & PBD_DISTANCE)) pairs.clear();
     std::vector<int> quads;
-    if (c.ct & PBD_BENDING) bending_quads(c.tris, quads);
+    if (c.ct & PBD_BENDING) {
+        if (c.has_user_quads) quads = c.user_quads;
+        else bending_quads(c.tris, quads);
+    }
     c.ne = (int)pairs.size() / 2;
     c.nb = (int)quads.size() / 4;
     c.prim_list.clear();
@@ -173,13 +176,27 @@ void build_prims(Cloth& c) {
     }
     for (int i = 0; i < c.nb; i++) {
         const int* q = &quads[4 * i];
+        for (int k = 0; k < 4; k++) {
+            PBD_CHECK(q[k] >= 0 && q[k] < c.nv, PBD_ERR_ARG, "bending prim vertex out of range");
+            for (int j = 0; j < k; j++) PBD_CHECK(q[j] != q[k], PBD_ERR_ARG, "bending prim repeats a vertex");
+        }
         c.prim_list.push_back((int)c.indices.size());
         c.prim_list.push_back(4);
         for (int k = 0; k < 4; k++) c.indices.push_back(q[k]);
         c.rest.push_back(dihedral_angle(&c.pos0[3 * q[0]], &c.pos0[3 * q[1]], &c.pos0[3 * q[2]],
                                         &c.pos0[3 * q[3]]));
     }
-    c.w.assign(c.nv, 1.0f);
+    if (!c.user_rest.empty()) {
+        PBD_CHECK((int)c.user_rest.size() == c.np(), PBD_ERR_ARG,
+                  "explicit rest values do not match the cloth's constraint set");
+        c.rest = c.user_rest;
+    }
+    if (c.w_in.empty()) {
+        c.w.assign(c.nv, 1.0f);
+    } else {
+        PBD_CHECK((int)c.w_in.size() == c.nv, PBD_ERR_ARG, "explicit weights do not match the vertex count");
+        c.w = c.w_in;
+    }
     for (int v : c.pins) {
         PBD_CHECK(v >= 0 && v < c.nv, PBD_ERR_ARG, "pin index out of range");
         c.w[v] = 0.0f;

@@ -151,6 +151,11 @@ struct Cloth {
     std::vector<int> tris;                // 3*nt
     std::vector<int> user_pairs;          // explicit distance prims
     bool has_user_pairs = false;
+    std::vector<int> user_quads;          // explicit bending prims (e0, e1, o0, o1)
+    bool has_user_quads = false;
+    std::vector<float> user_rest;         // explicit rest values, prim order (empty: from pos0)
+    std::vector<float> w_in;              // explicit inverse-mass weights (empty: 1; pins 0 either way)
+    std::vector<float> x_start, v_start, p_start;  // start state (empty: pos0, 0, x)
     std::vector<int> pins;
     unsigned ct = PBD_DISTANCE;
     float stiff[3] = {1.0f, 1.0f, 1.0f};
@@ -194,6 +199,11 @@ void build_prims(Cloth& c);
 void build_adjacency(Cloth& c);
 void read_txt_cloth(const char* topol, const char* constr, std::vector<float>& pos,
                     std::vector<int>& tris, std::vector<int>& pairs, bool& has_pairs);
+// Cache text format (cache_io.cpp).  x, v, p are 3*nv host arrays of the
+// cloth's current state; precision is the significant digits written (0: 6).
+void write_cache(const Cloth& c, const float* x, const float* v, const float* p, const char* mesh_path,
+                 const char* constr_path, int precision);
+void read_cache(const char* mesh_path, const char* constr_path, Cloth& c);
 
 // Broadphase grid: dense (reference) or hashed table.
 struct HashCfg {

@@ -1,0 +1,138 @@
+"""The reference's text cache format (oracle/cache_format.py) on the CPU.
+
+The oracle's float rule (printf %g at 6 digits of the value widened to double)
+is pinned against the C++ stream the reference writes with: a helper built
+here with g++ writes the same floats through ``std::ofstream << float`` and
+parses the written text with ``std::stof``, as the reference's readers do.
+The reader is pinned on a fixture typed out by hand from the reference's
+writers (tests/golden/make_cache_fixture.py), with its quirks: Windows line
+ends, an empty buffer line, ANCHOR prims, a doubled restPosBuffer.
+"""
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import cache_format as cf
+
+HERE = Path(__file__).resolve().parent
+GOLD = HERE / "golden"
+
+
+@pytest.fixture(scope="module")
+def stream_helper(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path_factory.mktemp("stream") / "stream_format"
+    subprocess.run([gxx, "-O1", "-std=c++17", str(HERE / "stream_format.cpp"), "-o", str(exe)], check=True)
+    return exe
+
+
+def run_helper(exe, tmp_path, requests):
+    out = tmp_path / "out.txt"
+    subprocess.run([str(exe), str(out)], input="".join(f"{op} {arg}\n" for op, arg in requests), text=True,
+                   check=True)
+    return out.read_text().split("\n")[:-1]
+
+
+def sample_floats():
+    rng = np.random.default_rng(7)
+    mags = 10.0 ** rng.uniform(-12, 12, 4000)
+    vals = (mags * rng.choice([-1.0, 1.0], mags.size)).astype(np.float32)
+    special = np.array([0.0, -0.0, 1.0, -1.0, 0.1, 0.5, 1e-5, 1e-4, 123456.0, 1234567.0, 999999.5, 9.8, -9.8,
+                        3.4e38, 1.17549435e-38, 1e-40, 0.9, 1.0 / 3.0, 2.0 / 3.0, 65504.0], np.float32)
+    steps = (np.arange(-50, 50, dtype=np.float32) * np.float32(0.0125)).astype(np.float32)
+    return np.concatenate([special, vals, steps])
+
+
+def test_float_text_matches_cpp_stream(stream_helper, tmp_path):
+    vals = sample_floats()
+    bits = vals.view(np.uint32)
+    got = run_helper(stream_helper, tmp_path, [("w", f"{b:08x}") for b in bits])
+    ours = [cf.fmt_float(v) for v in vals]
+    bad = [(float(v), g, o) for v, g, o in zip(vals, got, ours) if g != o]
+    assert not bad, bad[:10]
+
+
+@pytest.mark.parametrize("digits", [6, 9])
+def test_float_text_parses_like_stof(stream_helper, tmp_path, digits):
+    vals = sample_floats()
+    vals = vals[(np.abs(vals) >= np.float32(1.17549435e-38)) | (vals == 0)]  # std::stof rejects subnormals
+    texts = [cf.fmt_float(v, digits) for v in vals]
+    got = run_helper(stream_helper, tmp_path, [("r", t) for t in texts])
+    tiny = np.float32(1.17549435e-38)
+    # std::stof reports a subnormal result as out of range (the reference's
+    # reader would throw there; this library's reader keeps the subnormal)
+    ours = ["range" if 0 < abs(cf._f32(t)) < tiny else f"{np.array([cf._f32(t)], np.float32).view(np.uint32)[0]:x}"
+            for t in texts]
+    assert got == ours
+    if digits == 9:  # 9 significant digits identify every float
+        assert np.array_equal(np.array([cf._f32(t) for t in texts], np.float32).view(np.uint32),
+                              vals.view(np.uint32))
+
+
+def test_split_rule():
+    # UT::splitString: pieces between separators, an empty last piece dropped
+    assert cf.split("a;b;", ";") == ["a", "b"]
+    assert cf.split("a;;b", ";") == ["a", "", "b"]
+    assert cf.split("", ";") == []
+    assert cf.split("name|type|", "|") == ["name", "type"]
+    assert cf.split("name|type|1;2;", "|") == ["name", "type", "1;2;"]
+    assert cf.split("|x|y", "|") == ["", "x", "y"]
+
+
+def test_reader_skips_lines_without_three_pieces():
+    text = "A|int|1;2;\nB|int|\nC|float|0.5\r\nD|x|y|z\nA|int|3;\n"
+    L = cf.read_lines(text)
+    assert L == {"A": ["1", "2", "3"], "C": ["0.5"]}
+
+
+def test_reference_fixture_reads_as_written():
+    mesh = (GOLD / "grid3x2_meshtopol.cache").read_bytes().decode()
+    constr = (GOLD / "grid3x2_constraint.cache").read_bytes().decode()
+    assert "\r\n" in mesh
+    c = cf.read_cache(mesh, constr)
+    assert c["tris"].tolist() == [[0, 1, 3], [1, 4, 3], [1, 2, 4], [2, 5, 4]]
+    assert c["x"].shape == (6, 3) and c["x"][1].tolist() == [0.5, np.float32(-0.0012), np.float32(0.01)]
+    assert c["v"][1].tolist() == [0.0, np.float32(-0.72), np.float32(0.06)]
+    assert c["w"].tolist() == [1.0] * 6
+    assert c["damping"] == np.float32(0.9) and c["gravity"].tolist() == [0.0, -10.0, 0.0]
+    assert c["pairs"].tolist() == [[0, 1], [0, 3], [1, 2], [1, 4], [2, 5], [3, 4], [4, 5]]
+    assert c["quads"].shape == (0, 4)
+    assert c["pins"] == [0, 2]
+    assert c["rest"].tolist() == [0.5] * 7
+    assert c["stiffness"][0] == 1.0
+    assert c["pos0"][5].tolist() == [1.0, 0.0, 0.5]
+    assert c["p"][3].tolist() == [np.float32(0.0102), np.float32(-0.0508), np.float32(0.499)]
+
+
+def test_writer_then_reader_round_trip():
+    rng = np.random.default_rng(3)
+    nv = 12
+    x = rng.normal(size=(nv, 3)).astype(np.float32)
+    v = rng.normal(size=(nv, 3)).astype(np.float32)
+    p = rng.normal(size=(nv, 3)).astype(np.float32)
+    pos0 = rng.normal(size=(nv, 3)).astype(np.float32)
+    w = np.ones(nv, np.float32)
+    w[[2, 7]] = 0.0
+    tris = np.array([[0, 1, 2], [1, 3, 2], [4, 5, 6], [8, 9, 10]], np.int32)
+    prim_list = np.array([[0, 2], [2, 2], [4, 2], [6, 4]], np.int32)
+    indices = np.array([0, 1, 1, 2, 2, 0, 0, 2, 1, 3], np.int32)
+    rest = rng.uniform(0.1, 1.0, 4).astype(np.float32)
+    mesh = cf.mesh_text(x, tris, v, w, 0.95, (0.0, -9.8, 0.0), digits=9)
+    constr = cf.constraint_text(pos0, p, prim_list, indices, rest, 3, w, (0.5, 0.25, 1.0), digits=9)
+    c = cf.read_cache(mesh, constr)
+    for k, ref in (("x", x), ("v", v), ("p", p), ("pos0", pos0), ("w", w), ("rest", rest)):
+        assert np.array_equal(c[k], ref), k
+    assert c["tris"].tolist() == tris.tolist()
+    assert c["pairs"].tolist() == [[0, 1], [1, 2], [2, 0]]
+    assert c["quads"].tolist() == [[0, 2, 1, 3]]
+    assert c["pins"] == [2, 7]
+    assert c["stiffness"][:2] == [0.5, 0.25]
+    # at the stream default the values are the 6-digit roundings
+    c6 = cf.read_cache(cf.mesh_text(x, tris, v, w, 0.95, (0.0, -9.8, 0.0)), None)
+    assert np.array_equal(c6["x"], np.array([[np.float32(float("%.6g" % float(a))) for a in r] for r in x],
+                                            np.float32))

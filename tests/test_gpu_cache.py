@@ -1,0 +1,185 @@
+"""The reference's cache format through pbd_cloth_save_cache and
+pbd_cloth_load_cache (PBDObject::SaveMeshTopol / SaveConstraint /
+ContinueSimInit, PBD_cuda.cu:660-667, 907-938; IO::ReadTopolFromCache /
+ReadConstraintFromCache, PBD_Basic.cuh:326-530).
+
+- The files the library writes equal, byte for byte, the oracle's writer
+  (oracle/cache_format.py, whose float text is pinned against the C++ stream)
+  applied to the state and prims the library reports.
+- A cloth saved at 9 digits and loaded into a fresh context continues
+  bit-identically to the uninterrupted run (state, prims, rest values,
+  colours, then 15 more substeps).
+- The hand-typed reference-shaped fixture loads to the oracle reader's cloth
+  and steps bit-identically to the oracle from that start.
+- A mesh-only cache (the per-frame IO::SaveToplogy output) loads like the
+  same mesh from memory.
+- Malformed and inconsistent files are PBD_ERR_IO.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import cache_format as cf
+from positionbaseddynamicsold_amd import capi
+
+GOLD = Path(__file__).resolve().parent / "golden"
+DT = 1.0 / 600.0
+RX, RY = 12, 10
+PINS = [0, RX - 1]
+STIFF = (0.9, 0.3, 1.0)
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def bent_cloth(ctx):
+    cid = ctx.add_grid(RX, RY, 1.1, 0.9, jitter=2e-3, seed=5)
+    ctx.set_constraint_option(cid, capi.DISTANCE | capi.BENDING, STIFF)
+    ctx.set_pins(cid, PINS)
+    ctx.set_params(cid, damping=0.97, gravity=(0.5, -9.0, 0.25))
+    return cid
+
+
+def weights(nv, pins):
+    w = np.ones(nv, np.float32)
+    w[pins] = 0.0
+    return w
+
+
+@pytest.mark.parametrize("precision", [0, 9])
+def test_saved_text_equals_oracle_writer(orc, tmp_path, precision):
+    say(f"save_cache precision {precision}")
+    _, tris = orc.grid_mesh(RX, RY, 1.1, 0.9, jitter=2e-3, seed=5)
+    with capi.Context() as ctx:
+        cid = bent_cloth(ctx)
+        ctx.init()
+        pos0 = ctx.get(cid)
+        ctx.set_solver(dt=DT, iterations=20)
+        ctx.step(7)
+        m, k = tmp_path / "mesh.cache", tmp_path / "constr.cache"
+        ctx.save_cache(cid, str(m), str(k), precision)
+        x, v, p = (ctx.get(cid, s) for s in (capi.STATE_X, capi.STATE_V, capi.STATE_P))
+        pl, idx, rest = ctx.prims(cid)
+        inf = ctx.info(cid)
+    digits = precision or 6
+    w = weights(len(x), PINS)
+    exp_m = cf.mesh_text(x, tris, v, w, 0.97, (0.5, -9.0, 0.25), digits)
+    exp_k = cf.constraint_text(pos0, p, pl, idx, rest, inf.ne, w, STIFF, digits)
+    assert m.read_bytes().decode() == exp_m
+    assert k.read_bytes().decode() == exp_k
+    assert inf.nb > 0 and "restAngleBuffer" in exp_k
+
+
+def test_round_trip_continues_bit_identically(tmp_path):
+    say("save at 9 digits, load in a fresh context, continue 15 substeps")
+    m, k = tmp_path / "mesh.cache", tmp_path / "constr.cache"
+    with capi.Context() as a, capi.Context() as b:
+        ca = bent_cloth(a)
+        a.init()
+        a.set_solver(dt=DT, iterations=20)
+        a.step(7)
+        a.save_cache(ca, str(m), str(k), 9)
+        cb = b.load_cache(str(m), str(k))  # damping and gravity come from the mesh file
+        prm = b.get_params(cb)
+        assert prm.damping == np.float32(0.97) and list(prm.gravity) == [0.5, -9.0, 0.25]
+        b.init()
+        b.set_solver(dt=DT, iterations=20)
+        for s in (capi.STATE_X, capi.STATE_V, capi.STATE_P):
+            assert np.array_equal(a.get(ca, s), b.get(cb, s)), s
+        for ga, gb in zip(a.prims(ca), b.prims(cb)):
+            assert np.array_equal(ga, gb)
+        for ga, gb in zip(a.colors(ca), b.colors(cb)):
+            assert np.array_equal(ga, gb)
+        ia, ib = a.info(ca), b.info(cb)
+        assert (ia.nv, ia.nt, ia.ne, ia.nb) == (ib.nv, ib.nt, ib.ne, ib.nb)
+        a.step(15)
+        b.step(15)
+        assert np.array_equal(a.get(ca), b.get(cb))
+        assert np.array_equal(a.get(ca, capi.STATE_V), b.get(cb, capi.STATE_V))
+
+
+def test_reference_fixture_steps_like_oracle(orc):
+    mesh = GOLD / "grid3x2_meshtopol.cache"
+    constr = GOLD / "grid3x2_constraint.cache"
+    say(f"load {mesh.name} + {constr.name}")
+    c = cf.read_cache(mesh.read_bytes().decode(), constr.read_bytes().decode())
+    with capi.Context() as ctx:
+        cid = ctx.load_cache(str(mesh), str(constr))
+        ctx.init()
+        inf = ctx.info(cid)
+        assert (inf.nv, inf.nt, inf.ne, inf.nb) == (6, 4, 7, 0)
+        pl, idx, rest = ctx.prims(cid)
+        assert idx.reshape(-1, 2).tolist() == c["pairs"].tolist()
+        assert np.array_equal(rest, c["rest"])
+        assert np.array_equal(ctx.get(cid), c["x"])
+        assert np.array_equal(ctx.get(cid, capi.STATE_V), c["v"])
+        assert np.array_equal(ctx.get(cid, capi.STATE_P), c["p"])
+        ctx.set_solver(dt=DT, iterations=20)
+        ctx.step(10)
+        x = ctx.get(cid)
+    oc = orc.OracleCloth(c["pos0"], c["tris"], c["pairs"], pins=c["pins"])
+    oc.x[:] = c["x"]
+    oc.v[:] = c["v"]
+    oc.p[:] = c["p"]
+    oc.rest[:] = c["rest"]
+    oc.w[:] = c["w"]
+    oc.w[c["pins"]] = 0.0
+    prm = oc.params(dt=DT, gravity=tuple(c["gravity"]), damping=float(c["damping"]))
+    for _ in range(10):
+        oc.substep(prm, cap=20)
+    assert np.array_equal(x, oc.x)
+    assert np.array_equal(x[c["pins"]], c["x"][c["pins"]])
+
+
+def test_mesh_only_cache_loads_like_memory_mesh(tmp_path):
+    say("mesh-only cache against add_mesh")
+    exp = np.load(GOLD / "grid6x5_expected.npz")
+    pos, tris = exp["pos"], exp["tris"]
+    path = tmp_path / "frame.cache"
+    path.write_text(cf.buffer_line("Indices", cf.T_INT, (str(i) for i in tris.reshape(-1)))
+                    + cf.buffer_line("P", cf.T_FLOAT3, (cf.f3(r, 9) for r in pos))
+                    + cf.buffer_line("primList", cf.T_INT2, (f"{3 * t},3" for t in range(len(tris)))))
+    with capi.Context() as ctx:
+        a = ctx.load_cache(str(path))
+        b = ctx.add_mesh(pos, tris)
+        for c_ in (a, b):
+            ctx.set_pins(c_, [0, 5])
+        ctx.init()
+        for ga, gb in zip(ctx.prims(a), ctx.prims(b)):
+            assert np.array_equal(ga, gb)
+        ctx.set_solver(dt=DT, iterations=20)
+        ctx.step(12)
+        assert np.array_equal(ctx.get(a), ctx.get(b))
+
+
+def _expect_io_error(ctx, mesh, constr=None):
+    with pytest.raises(capi.PbdError) as e:
+        ctx.load_cache(str(mesh), str(constr) if constr else None)
+    assert e.value.code == capi.PBD_ERR_IO, str(e.value)
+
+
+def test_load_cache_errors(tmp_path):
+    good_m = (GOLD / "grid3x2_meshtopol.cache").read_text()
+    good_k = (GOLD / "grid3x2_constraint.cache").read_text()
+
+    def write(name, text):
+        p = tmp_path / name
+        p.write_text(text)
+        return p
+
+    with capi.Context() as ctx:
+        _expect_io_error(ctx, tmp_path / "missing.cache")
+        _expect_io_error(ctx, write("m1", good_m.replace("0.5,-0.0012", "0.5,-0.00x2")))
+        _expect_io_error(ctx, write("m2", good_m.replace("massBuffer|float|1;1;1;1;1;1;", "massBuffer|float|1;1;")))
+        _expect_io_error(ctx, write("m3", good_m.replace("0;1;3;1;4;3;", "0;1;9;1;4;3;")))
+        m = write("m", good_m)
+        _expect_io_error(ctx, m, write("k1", good_k.replace("256;256;", "256;512;")))
+        _expect_io_error(ctx, m, write("k2", good_k.replace("1;1;1;1;1;1;1;-1;-1;", "1;1;0.5;1;1;1;1;-1;-1;")))
+        _expect_io_error(ctx, m, write("k3", good_k.replace("0.5;0.5;0.5;0.5;0.5;0.5;0.5;", "0.5;0.5;")))
+        _expect_io_error(ctx, m, write("k4", good_k.replace("14,1;15,1;", "14,1;15,3;")))
+        # nothing was added by the failed loads
+        assert ctx.cloths == []
