@@ -48,7 +48,9 @@ enum { PBD_STATE_X = 0, PBD_STATE_V = 1, PBD_STATE_P = 2 };
 /* Projection path. */
 enum {
     PBD_PATH_AUTO = 0,      /* LDS tiles (whole-cloth or dependency-cone tiles) */
-    PBD_PATH_GLOBAL = 1     /* one launch per colour over global memory */
+    PBD_PATH_GLOBAL = 1,    /* one launch per colour over global memory */
+    PBD_PATH_JACOBI = 2     /* colour-free Jacobi with constraint averaging, one launch per sweep
+                               (SolverType::JACOBI, declared but not implemented, PBD_Basic.cuh:46-51) */
 };
 
 /* Procedural grid cloth (PBDObject(damping, g, resX, resY, sizeX, sizeY, ht)
@@ -96,6 +98,8 @@ typedef struct {
     float warm_gate;        /* a vertex whose one-ring distance prims all have |d-L|/L < warm_gate gets
                                no predicted correction (0 = every vertex gets one); a cloth with no strained
                                vertex then keeps the zero prediction and runs warm_iterations sweeps */
+    float jacobi_omega;     /* PBD_PATH_JACOBI: a vertex moves by omega / n times the sum of the corrections
+                               its n prims give it (0 = 1; 1 to 2 over-relaxes) */
 } pbd_solver_params;
 
 typedef struct {

@@ -76,6 +76,9 @@ def lib() -> C.CDLL:
             "orc_integrate": (None, [C.c_int, C.c_float, _fp, _fp, _fp]),
             "orc_residual": (None, [C.POINTER(OrcCloth), _fp, _dp, _dp, _ip]),
             "orc_substep": (C.c_int, [C.POINTER(OrcCloth), C.POINTER(OrcParams), C.c_int, C.c_int, C.c_double, _dp]),
+            "orc_substep_jacobi": (C.c_int, [C.POINTER(OrcCloth), C.POINTER(OrcParams), C.c_int, C.c_float, C.c_int,
+                                             C.c_double, _dp]),
+            "orc_jacobi_sweep": (None, [C.POINTER(OrcCloth), C.c_int, C.c_float, _fp, _fp]),
             "orc_stiffness_table_ref": (None, [C.c_float, C.c_int, _fp]),
             "orc_stiffness_table_const": (None, [C.c_float, C.c_int, C.c_int, _fp]),
             "orc_grid_init": (None, [C.POINTER(OrcGrid), _fp, _ip, _fp]),
@@ -297,6 +300,20 @@ class OracleCloth:
 
     def sweep(self, index):
         self.L.orc_sweep(C.byref(self.c), index)
+
+    def substep_jacobi(self, prm: OrcParams, cap=20, omega=1.0, check_every=1, tol=-1.0):
+        """Advect + Jacobi sweeps (fixed or to tolerance) + integrate; returns (sweeps, residual)."""
+        self.advect(prm)
+        r = C.c_double()
+        n = self.L.orc_substep_jacobi(C.byref(self.c), C.byref(prm), cap, omega, check_every, tol, C.byref(r))
+        return n, r.value
+
+    def jacobi_sweep(self, index, omega=1.0, p=None):
+        """One Jacobi sweep of p (default: the cloth's p); returns the new positions."""
+        src = np.ascontiguousarray(self.p if p is None else p, np.float32).reshape(-1, 3)
+        out = np.empty_like(src)
+        self.L.orc_jacobi_sweep(C.byref(self.c), index, omega, F(src), F(out))
+        return out
 
     def start64(self):
         """Starts an FP64 twin from the current FP32 state (x, v)."""

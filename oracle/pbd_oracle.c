@@ -634,6 +634,89 @@ int orc_substep(orc_cloth* c, const orc_params* prm, int cap, int check_every, d
 }
 
 /* ------------------------------------------------------------------------ */
+/* Jacobi sweep with constraint averaging (new).  The reference declares     */
+/* SolverType::JACOBI (PBD/PBD_Basic.cuh:46-51) and never implements it; this */
+/* is the averaged Jacobi of Macklin et al. 2014 ("Unified particle physics  */
+/* for real-time applications", sec. 4.2): every prim is projected alone on  */
+/* the sweep's input positions (the Gauss-Seidel projection arithmetic, on a */
+/* copy), and a free vertex moves by omega / n times the sum of the          */
+/* corrections its n prims give it, summed in ascending prim id.  Pinned     */
+/* vertices (w = 0) keep their position.                                     */
+/* ------------------------------------------------------------------------ */
+void orc_jacobi_sweep(const orc_cloth* c, int sweep_index, float omega, const float* pin, float* pout) {
+    int ki = sweep_index < c->k_len ? sweep_index : c->k_len - 1;
+    float kd = c->k_dist[ki], kb = c->k_bend ? c->k_bend[ki] : 1.0f;
+    int nv = c->nv;
+    int* start = (int*)calloc((size_t)nv + 1, sizeof(int));
+    for (int p = 0; p < c->np; p++)
+        for (int j = 0; j < c->prim_list[2 * p + 1]; j++) start[c->indices[c->prim_list[2 * p] + j] + 1]++;
+    for (int i = 0; i < nv; i++) start[i + 1] += start[i];
+    int* fill = (int*)malloc(sizeof(int) * (size_t)(nv > 0 ? nv : 1));
+    int* inc = (int*)malloc(sizeof(int) * (size_t)(start[nv] > 0 ? start[nv] : 1));
+    for (int i = 0; i < nv; i++) fill[i] = start[i];
+    for (int p = 0; p < c->np; p++) /* ascending prim id per vertex */
+        for (int j = 0; j < c->prim_list[2 * p + 1]; j++) inc[fill[c->indices[c->prim_list[2 * p] + j]]++] = p;
+    static const int ids[4] = {0, 1, 2, 3};
+    for (int i = 0; i < nv; i++) {
+        const float* pi = pin + 3 * i;
+        float* po = pout + 3 * i;
+        po[0] = pi[0]; po[1] = pi[1]; po[2] = pi[2];
+        if (c->w[i] == 0.0f) continue;
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        for (int e = start[i]; e < start[i + 1]; e++) {
+            int p = inc[e];
+            int st = c->prim_list[2 * p], n = c->prim_list[2 * p + 1];
+            float loc[12], wl[4];
+            int role = 0;
+            for (int j = 0; j < n; j++) {
+                int v = c->indices[st + j];
+                loc[3 * j] = pin[3 * v]; loc[3 * j + 1] = pin[3 * v + 1]; loc[3 * j + 2] = pin[3 * v + 2];
+                wl[j] = c->w[v];
+                if (v == i) role = j;
+            }
+            if (n == 2) orc_project_distance(ids, c->rest[p], kd, wl, loc);
+            else orc_project_bending(ids, c->rest[p], kb, wl, loc);
+            for (int k = 0; k < 3; k++) sum[k] = sum[k] + (loc[3 * role + k] - pi[k]);
+        }
+        int m = start[i + 1] - start[i];
+        if (m > 0) {
+            float f = omega / (float)m;
+            for (int k = 0; k < 3; k++) po[k] = pi[k] + f * sum[k];
+        }
+    }
+    free(inc);
+    free(fill);
+    free(start);
+}
+
+/* Sweeps of the Jacobi variant with the analytic colliders after each, then
+ * integrate: orc_substep with orc_jacobi_sweep in place of orc_sweep. */
+int orc_substep_jacobi(orc_cloth* c, const orc_params* prm, int cap, float omega, int check_every, double tol,
+                       double* final_residual) {
+    int sweeps = 0;
+    double r = -1.0;
+    float* tmp = (float*)malloc(sizeof(float) * 3 * (size_t)(c->nv > 0 ? c->nv : 1));
+    if (check_every < 1) check_every = 1;
+    while (sweeps < cap) {
+        orc_jacobi_sweep(c, sweeps, omega, c->p, tmp);
+        memcpy(c->p, tmp, sizeof(float) * 3 * (size_t)c->nv);
+        if (prm->sphere_on || prm->box_on) orc_colliders(prm, c->nv, c->w, c->p);
+        sweeps++;
+        if (tol >= 0.0 && sweeps % check_every == 0) {
+            orc_residual(c, c->p, &r, NULL, NULL);
+            if (r <= tol) break;
+        }
+    }
+    free(tmp);
+    if (final_residual) {
+        if (r < 0.0 || sweeps % check_every != 0) orc_residual(c, c->p, &r, NULL, NULL);
+        *final_residual = r;
+    }
+    orc_integrate(c->nv, prm->dt, c->x, c->v, c->p);
+    return sweeps;
+}
+
+/* ------------------------------------------------------------------------ */
 /* FP64 twin of the plain substep (new): the same steps as orc_advect,       */
 /* orc_sweep, orc_colliders and orc_integrate, on the same topology, colour   */
 /* order, rest values (FP32, widened) and stiffness tables, in double, so the */

@@ -96,6 +96,9 @@ void orc64_substeps(const orc_cloth* c, const orc_params* prm, double* x, double
                     int substeps);
 int orc_substep(orc_cloth* c, const orc_params* prm, int cap, int check_every, double tol,
                 double* final_residual);
+void orc_jacobi_sweep(const orc_cloth* c, int sweep_index, float omega, const float* pin, float* pout);
+int orc_substep_jacobi(orc_cloth* c, const orc_params* prm, int cap, float omega, int check_every, double tol,
+                       double* final_residual);
 void orc_stiffness_table_ref(float s, int n, float* k);
 void orc_stiffness_table_const(float s, int iterations, int n, float* k);
 

@@ -35,6 +35,12 @@ struct pbd_ctx {
     DBuf<float2> gbrc;
     std::vector<int> gd_off, gb_off;
     int ncolors_max = 0;
+    // Jacobi path: vertex -> incident prims (CSR over the packed vertices),
+    // built on first use
+    DBuf<int> joff;
+    DBuf<int4> jent;
+    DBuf<int2> jmeta;
+    bool jac_built = false;
     // stiffness tables
     DBuf<float> ktab_d, ktab_b;
     int kmax = 0;
@@ -659,6 +665,83 @@ void advect_all(pbd_ctx* c, float dt) {
 // cap: optional per-cloth sweep budgets (device array) below `iterations`.
 void consolidate(pbd_ctx* c);
 
+// Vertex -> prim CSR of the Jacobi path: each vertex's prims in ascending
+// prim id (distance prims, then bending prims), with the prim's vertices in
+// its own order and this vertex's role in it.
+void ensure_jacobi(pbd_ctx* c) {
+    if (c->jac_built) return;
+    std::vector<int> off((size_t)c->nv_total + 1, 0);
+    for (const Cloth& k : c->cloths)
+        for (int p = 0; p < k.np(); p++)
+            for (int j = 0; j < k.prim_list[2 * p + 1]; j++) off[k.vbase + k.indices[k.prim_list[2 * p] + j] + 1]++;
+    for (int i = 0; i < c->nv_total; i++) off[i + 1] += off[i];
+    PBD_CHECK(off[c->nv_total] >= 0, PBD_ERR_LIMIT, "Jacobi incidence list too large");
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    std::vector<int4> ent(off[c->nv_total]);
+    std::vector<int2> meta(off[c->nv_total]);
+    for (size_t ci = 0; ci < c->cloths.size(); ci++) {
+        const Cloth& k = c->cloths[ci];
+        for (int p = 0; p < k.np(); p++) {
+            const int st = k.prim_list[2 * p], n = k.prim_list[2 * p + 1];
+            const int* q = &k.indices[st];
+            int4 e = n == 2 ? make_int4(k.vbase + q[0], k.vbase + q[1], -1, -1)
+                            : make_int4(k.vbase + q[0], k.vbase + q[1], k.vbase + q[2], k.vbase + q[3]);
+            for (int j = 0; j < n; j++) {
+                int slot = fill[k.vbase + q[j]]++;
+                ent[slot] = e;
+                int bits;
+                memcpy(&bits, &k.rest[p], 4);
+                meta[slot] = make_int2(bits, (int)ci * 8 + (n == 2 ? 0 : 4) + j);
+            }
+        }
+    }
+    c->joff.upload(off, c->stream);
+    c->jent.upload(ent, c->stream);
+    c->jmeta.upload(meta, c->stream);
+    PBD_HIP(hipStreamSynchronize(c->stream));
+    c->jac_built = true;
+}
+
+// Jacobi sweeps over vertices [vb, ve) (ping-pong between the p buffers),
+// stopping early on the solver tolerance like the per-colour path.
+int project_jacobi(pbd_ctx* c, int iterations, const int* cap, int vb, int ve, double tol, int ci_only) {
+    hipStream_t s = c->stream;
+    ensure_jacobi(c);
+    const float omega = c->sp.jacobi_omega > 0.0f ? c->sp.jacobi_omega : 1.0f;
+    const int ce = std::max(1, c->sp.check_every);
+    const int ncl = (int)c->cloths.size();
+    c->pfinal_valid = false;
+    // one cloth alone: both buffers carry the other cloths' p unchanged
+    if (ve - vb < c->nv_total) launch_copy(s, c->pbuf[c->pcur].ptr, c->pbuf[c->pcur ^ 1].ptr, (size_t)c->nv_total);
+    int sw = 0;
+    while (sw < iterations) {
+        const int cur = c->pcur;
+        launch_jacobi(s, c->joff.ptr, c->jent.ptr, c->jmeta.ptr, c->cl.ptr, c->vcloth.ptr, c->ktab_d.ptr,
+                      c->ktab_b.ptr, c->kmax, sw, cap, omega, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, vb, ve);
+        c->pcur = cur ^ 1;
+        sw++;
+        if (tol >= 0.0 && sw % ce == 0) {
+            const Cloth* one = ci_only >= 0 ? &c->cloths[ci_only] : nullptr;
+            int lo = one ? one->rd_lo : 0, hi = one ? one->rd_hi : (int)c->rdist.n;
+            launch_fill_u64(s, c->rmax.ptr, ncl, 0ull);
+            PBD_HIP(hipMemsetAsync(c->rsum.ptr, 0, sizeof(double) * ncl, s));
+            launch_residual(s, c->rdist.ptr + lo, hi - lo, c->pbuf[c->pcur].ptr, c->rmax.ptr, c->rsum.ptr,
+                            nullptr);
+            std::vector<unsigned long long> r;
+            c->rmax.download(r, s);
+            PBD_HIP(hipStreamSynchronize(s));
+            double m = 0.0;
+            for (auto b : r) {
+                double dv;
+                memcpy(&dv, &b, 8);
+                m = std::max(m, dv);
+            }
+            if (m <= tol) break;
+        }
+    }
+    return sw;
+}
+
 // delta / keep: a warm-start prediction the first tile launch adds while
 // loading (LDS-tile path only).  integrate_end: the tiles may integrate each
 // cloth when they store its last sweep; returns true if they did (the caller
@@ -673,6 +756,16 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     const bool any_tol = has_tolerance(c);
     c->sweeps_pending = false;
     launch_fill_int(s, c->sweeps.ptr, ncl, 0);
+    if (c->sp.path == PBD_PATH_JACOBI) {
+        PBD_CHECK(!fused_whole && !delta, PBD_ERR_STATE, "internal: the Jacobi path is never fused");
+        int sw = project_jacobi(c, iterations, cap, 0, c->nv_total, tol, -1);
+        for (auto& k : c->cloths) k.last_sweeps = sw;
+        if (cap) {
+            launch_min_sweeps(s, c->sweeps.ptr, cap, sw, ncl);
+            c->sweeps_pending = true;
+        }
+        return false;
+    }
     if (c->sp.path == PBD_PATH_GLOBAL) {
         PBD_CHECK(!fused_whole, PBD_ERR_STATE, "internal: global path is never fused");
         PBD_CHECK(!delta, PBD_ERR_STATE, "internal: the global path applies the prediction beforehand");
@@ -894,9 +987,10 @@ void substep(pbd_ctx* c, float dt) {
     // whose tiles integrated may have left it unconsolidated: it is dead now)
     c->pfinal_valid = false;
     if (c->coll.enabled) {
+        PBD_CHECK(c->sp.path != PBD_PATH_JACOBI, PBD_ERR_STATE, "the CCD pipeline runs on the Gauss-Seidel paths");
         step_ccd(c, dt);
     } else if (c->ws.enabled) {
-        if (c->sp.path == PBD_PATH_GLOBAL) {
+        if (c->sp.path != PBD_PATH_AUTO) {
             advect_all(c, dt);
             ws_apply_prediction(c, true, false, dt);
             project(c, c->sp.iterations, false, dt, c->ws.cap.ptr);
@@ -909,7 +1003,7 @@ void substep(pbd_ctx* c, float dt) {
                 return;
         }
         integrate_all(c, dt, false);
-    } else if (c->sp.path == PBD_PATH_GLOBAL) {
+    } else if (c->sp.path != PBD_PATH_AUTO) {
         advect_all(c, dt);
         project(c, c->sp.iterations, false, dt);
         integrate_all(c, dt, false);
@@ -934,6 +1028,8 @@ void graph_substep(pbd_ctx* c, float dt) {
     hipStream_t s = c->stream;
     const int start = c->pcur;
     if (!c->graph[start]) {
+        // buffers built on first use are built before capture (it allows no allocation or wait)
+        if (c->sp.path == PBD_PATH_JACOBI) ensure_jacobi(c);
         hipGraph_t g = nullptr;
         PBD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
         try {
@@ -982,6 +1078,13 @@ void step_cloth(pbd_ctx* c, int ci, float dt) {
     if (dt != c->sp.dt) {
         c->sp.dt = dt;
         fill_cloth_dev(c);
+    }
+    if (c->sp.path == PBD_PATH_JACOBI) {
+        launch_advect(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[c->pcur].ptr, vb, ve, dt, 0);
+        k.last_sweeps = project_jacobi(c, I, nullptr, vb, ve, tol, ci);
+        launch_integrate(s, c->cl.ptr, c->vcloth.ptr, c->x.ptr, c->v.ptr, c->pbuf[0].ptr, c->pbuf[1].ptr, nullptr,
+                         c->pcur, vb, ve, dt, 0);
+        return;
     }
     if (c->sp.path == PBD_PATH_GLOBAL) {
         float4* p = c->pbuf[c->pcur].ptr;
@@ -1350,7 +1453,9 @@ int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* p) {
         PBD_CHECK(p->dt > 0.0f, PBD_ERR_ARG, "dt must be positive");
         PBD_CHECK(p->iterations >= 1 && p->iterations <= 1000000, PBD_ERR_ARG, "iterations out of range");
         PBD_CHECK(p->check_every >= 1, PBD_ERR_ARG, "check_every must be >= 1");
-        PBD_CHECK(p->path == PBD_PATH_AUTO || p->path == PBD_PATH_GLOBAL, PBD_ERR_ARG, "unknown path");
+        PBD_CHECK(p->path == PBD_PATH_AUTO || p->path == PBD_PATH_GLOBAL || p->path == PBD_PATH_JACOBI, PBD_ERR_ARG,
+                  "unknown path");
+        PBD_CHECK(p->jacobi_omega >= 0.0f && p->jacobi_omega < 2.0f, PBD_ERR_ARG, "jacobi_omega must be in [0, 2)");
         PBD_CHECK(p->tile_sweeps >= 0 && p->tile_max_verts >= 0, PBD_ERR_ARG, "negative tile setting");
         PBD_CHECK(p->block_threads == 0 || (p->block_threads % 64 == 0 && p->block_threads <= 1024), PBD_ERR_ARG,
                   "block_threads must be a multiple of 64 up to 1024");

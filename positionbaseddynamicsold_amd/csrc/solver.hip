@@ -178,6 +178,78 @@ void launch_colliders(hipStream_t s, const ClothDev* cloths, const int* vcloth, 
 }
 
 // ---------------------------------------------------------------------------
+// Jacobi sweep with constraint averaging (the reference declares
+// SolverType::JACOBI, PBD_Basic.cuh:46-51, and never implements it; this is
+// the averaged Jacobi of Macklin et al. 2014, "Unified particle physics for
+// real-time applications", sec. 4.2).  One thread per vertex gathers, from
+// p_in, the correction each incident prim (CSR in ascending prim id) would
+// give it when projected alone, with the prim's own vertex order and the
+// Gauss-Seidel projection arithmetic, and moves by omega/n times their sum
+// (n = incident prims).  Pinned vertices (w = 0) and cloths whose sweep
+// budget is spent are copied.  Colliders follow in the same thread.  The
+// result depends on no launch or summation order beyond the CSR's.
+//
+// Entry e: ent[e] = prim vertices (global ids; distance uses .x, .y),
+// meta[e] = (rest bits, cloth * 8 + type * 4 + role), type 0 distance /
+// 1 bending, role = this vertex's position in the prim.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_jacobi(const int* __restrict__ off, const int4* __restrict__ ent,
+                                                const int2* __restrict__ meta, const ClothDev* __restrict__ cl,
+                                                const int* __restrict__ vcloth, const float* __restrict__ ktab_d,
+                                                const float* __restrict__ ktab_b, int kmax, int sweep,
+                                                const int* __restrict__ cap, float omega,
+                                                const float4* __restrict__ pin, float4* __restrict__ pout, int b,
+                                                int e) {
+    int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    float4 pi = pin[i];
+    const int ci = vcloth[i];
+    if (pi.w == 0.0f || (cap && sweep >= cap[ci])) {
+        pout[i] = pi;
+        return;
+    }
+    const int ks = sweep < kmax ? sweep : kmax - 1;
+    const int o0 = off[i], o1 = off[i + 1];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int j = o0; j < o1; j++) {
+        int4 q = ent[j];
+        int2 m = meta[j];
+        const int kc = m.y >> 3, role = m.y & 3;
+        float4 r;
+        if (((m.y >> 2) & 1) == 0) {
+            float4 a = pin[q.x], bb = pin[q.y];
+            project_distance(a, bb, __int_as_float(m.x), ktab_d[kc * kmax + ks]);
+            r = role ? bb : a;
+        } else {
+            float4 q0 = pin[q.x], q1 = pin[q.y], q2 = pin[q.z], q3 = pin[q.w];
+            project_bending(q0, q1, q2, q3, __int_as_float(m.x), ktab_b[kc * kmax + ks]);
+            r = role == 0 ? q0 : role == 1 ? q1 : role == 2 ? q2 : q3;
+        }
+        sx = sx + (r.x - pi.x);
+        sy = sy + (r.y - pi.y);
+        sz = sz + (r.z - pi.z);
+    }
+    if (o1 > o0) {
+        const float f = omega / (float)(o1 - o0);
+        pi.x = pi.x + f * sx;
+        pi.y = pi.y + f * sy;
+        pi.z = pi.z + f * sz;
+    }
+    const ClothDev& c = cl[ci];
+    if (c.sphere_on || c.box_on) collide_point(pi, c.sphere_on, c.sphere_c, c.sphere_r, c.box_on, c.box_lo, c.box_hi);
+    pout[i] = pi;
+}
+
+void launch_jacobi(hipStream_t s, const int* off, const int4* ent, const int2* meta, const ClothDev* cloths,
+                   const int* vcloth, const float* ktab_d, const float* ktab_b, int kmax, int sweep, const int* cap,
+                   float omega, const float4* pin, float4* pout, int b, int e) {
+    if (e <= b) return;
+    hipLaunchKernelGGL(k_jacobi, dim3((e - b + 255) / 256), dim3(256), 0, s, off, ent, meta, cloths, vcloth, ktab_d,
+                       ktab_b, kmax, sweep, cap, omega, pin, pout, b, e);
+    PBD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // FP64 residual r = |d - L| / L per distance prim, d from the FP32 positions
 // widened to double.  Max per cloth via 64-bit atomicMax on the bit pattern
 // (r >= 0, so IEEE order equals unsigned order); sum of r^2 per cloth.

@@ -38,7 +38,7 @@ def assert_same(a, b, ids):
         assert np.array_equal(a.get(i, capi.STATE_P), a.get(i, capi.STATE_X)), i
 
 
-@pytest.mark.parametrize("path", [capi.PATH_AUTO, capi.PATH_GLOBAL])
+@pytest.mark.parametrize("path", [capi.PATH_AUTO, capi.PATH_GLOBAL, capi.PATH_JACOBI])
 def test_step_cloth_matches_batch(path):
     print(f"step_cloth path={path}", flush=True)
     with capi.Context() as a, capi.Context() as b:
@@ -83,7 +83,8 @@ def test_step_cloth_rejects_batch_only_modes():
         assert e.value.code == capi.PBD_ERR_STATE
 
 
-@pytest.mark.parametrize("path,ws", [(capi.PATH_AUTO, False), (capi.PATH_AUTO, True), (capi.PATH_GLOBAL, False)])
+@pytest.mark.parametrize("path,ws", [(capi.PATH_AUTO, False), (capi.PATH_AUTO, True), (capi.PATH_GLOBAL, False),
+                                     (capi.PATH_JACOBI, False)])
 def test_graph_replay_matches_eager(path, ws):
     # 7 sweeps in chunks of 2 end in the other p buffer, so both captured
     # graphs (one per starting buffer) are exercised

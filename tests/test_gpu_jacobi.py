@@ -1,0 +1,105 @@
+"""The Jacobi path (PBD_PATH_JACOBI: SolverType::JACOBI, declared but not
+implemented in the reference, PBD_Basic.cuh:46-51) against the oracle's
+orc_substep_jacobi, bit for bit: pinned grids at two relaxation factors,
+bending with an analytic sphere, a batch of uneven cloths, and the
+tolerance-driven sweep count.  The batch-only and graph checks are in
+test_gpu_batch.py (step_cloth and graph replay parametrized over the path)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from positionbaseddynamicsold_amd import capi
+
+DT = 1.0 / 600.0
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def pair(ctx, orc, n, seed, pins, bending=False, sphere=None, stiffness=(1.0, 1.0, 1.0), iterations=20):
+    cid = ctx.add_grid(n, n, 1.0, 1.0, jitter=1e-4, seed=seed)
+    ctx.set_pins(cid, list(pins))
+    if bending:
+        ctx.set_constraint_option(cid, capi.DISTANCE | capi.BENDING, stiffness)
+    elif stiffness != (1.0, 1.0, 1.0):
+        ctx.set_constraint_option(cid, capi.DISTANCE, stiffness)
+    if sphere is not None:
+        ctx.set_params(cid, sphere_on=1, sphere_c=sphere[0], sphere_r=sphere[1])
+    pos, tris = orc.grid_mesh(n, n, 1.0, 1.0, jitter=1e-4, seed=seed)
+    oc = orc.OracleCloth(pos, tris, orc.grid_pairs(n, n), pins=list(pins), bending=bending, stiffness=stiffness,
+                         iterations=iterations)
+    return cid, oc, oc.params(dt=DT, sphere=sphere)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.5])
+def test_pinned_grid_bit_exact(orc, omega):
+    say(f"jacobi 32x32 omega {omega}")
+    with capi.Context() as ctx:
+        cid, oc, prm = pair(ctx, orc, 32, 0, (0, 31), stiffness=(0.9, 1.0, 1.0))
+        ctx.init()
+        ctx.set_solver(dt=DT, iterations=20, path=capi.PATH_JACOBI, jacobi_omega=omega)
+        for k in range(3):
+            ctx.step(10)
+            for _ in range(10):
+                oc.substep_jacobi(prm, cap=20, omega=omega)
+            assert np.array_equal(ctx.get(cid), oc.x), k
+            assert np.array_equal(ctx.get(cid, capi.STATE_V), oc.v), k
+
+
+def test_bending_sphere_bit_exact(orc):
+    say("jacobi 24x24 bending + sphere")
+    with capi.Context() as ctx:
+        # bending k' = 1 - (1 - s)^(1/I) depends on the sweep count I: the oracle cloth gets the same I
+        cid, oc, prm = pair(ctx, orc, 24, 1, (0,), bending=True, sphere=((0.5, -0.3, 0.5), 0.25),
+                            stiffness=(1.0, 0.2, 1.0), iterations=15)
+        ctx.init()
+        ctx.set_solver(dt=DT, iterations=15, path=capi.PATH_JACOBI, jacobi_omega=1.5)
+        ctx.step(40)
+        for _ in range(40):
+            oc.substep_jacobi(prm, cap=15, omega=1.5)
+        assert np.array_equal(ctx.get(cid), oc.x)
+
+
+def test_batch_bit_exact(orc):
+    say("jacobi batch of 3 uneven cloths")
+    sizes = [(20, 0), (33, 1), (47, 2)]
+    with capi.Context() as ctx:
+        items = [pair(ctx, orc, n, s, (0, n - 1), iterations=12) for n, s in sizes]
+        ctx.init()
+        ctx.set_solver(dt=DT, iterations=12, path=capi.PATH_JACOBI, jacobi_omega=1.2)
+        ctx.step(15)
+        for cid, oc, prm in items:
+            for _ in range(15):
+                oc.substep_jacobi(prm, cap=12, omega=1.2)
+            assert np.array_equal(ctx.get(cid), oc.x), cid
+
+
+def test_tolerance_sweeps_match(orc):
+    say("jacobi to tolerance")
+    with capi.Context() as ctx:
+        cid, oc, prm = pair(ctx, orc, 24, 3, (0, 23), iterations=200)
+        ctx.init()
+        ctx.set_solver(dt=DT, iterations=200, tol=2e-3, check_every=5, path=capi.PATH_JACOBI, jacobi_omega=1.5)
+        for k in range(6):
+            ctx.step(1)
+            n, _ = oc.substep_jacobi(prm, cap=200, omega=1.5, check_every=5, tol=2e-3)
+            assert ctx.last_sweeps(cid) == n, k
+            assert np.array_equal(ctx.get(cid), oc.x), k
+        assert 5 <= n < 200
+
+
+def test_jacobi_errors():
+    with capi.Context() as ctx:
+        cid = ctx.add_grid(8, 8)
+        ctx.init()
+        with pytest.raises(capi.PbdError) as e:
+            ctx.set_solver(path=capi.PATH_JACOBI, jacobi_omega=2.5)
+        assert e.value.code == capi.PBD_ERR_ARG
+        ctx.set_solver(path=capi.PATH_JACOBI)
+        ctx.set_collision(enabled=1)
+        with pytest.raises(capi.PbdError) as e:
+            ctx.step(1)
+        assert e.value.code == capi.PBD_ERR_STATE
+        assert cid == 0
