@@ -30,7 +30,9 @@ Protocol (every timed run starts from the same device snapshot):
   5. timed: the plain solver unbatched (each cloth stepped alone, as the
      reference's per-object SolverPBD), the plain solver batched, and the
      warm-started batched solver, both batched runs replayed from captured
-     graphs.
+     graphs;
+  6. the colour-free Jacobi variant as a baseline (config.jacobi_baseline):
+     the smallest sweep count reaching R, and its substeps/s.
 `value` is the warm-started batched solver's substeps/s;
 config.speedup_vs_plain divides it by the unbatched plain solver's rate and
 config.speedup_vs_plain_batched by the batched plain solver's.
@@ -87,6 +89,10 @@ def parse():
                     "positionbaseddynamicsold_amd.tilework needs to repeat the profiled passes")
     ap.add_argument("--warm-rule", type=int, default=1, choices=(0, 1),
                     help="fallback rule of the headline run: 0 max |d-L|/L, 1 RMS (the other is reported too)")
+    ap.add_argument("--jacobi-omega", type=float, default=1.5,
+                    help="relaxation of the Jacobi baseline (constraint averaging)")
+    ap.add_argument("--jacobi-max-sweeps", type=int, default=400, help="largest Jacobi sweep count tried")
+    ap.add_argument("--no-jacobi", action="store_true", help="skip the Jacobi baseline")
     ap.add_argument("--warm-gate", type=float, default=1e-4,
                     help="no correction is predicted for a vertex whose one-ring prims are all within this "
                          "relative strain (0: every vertex); the run without the gate is reported too")
@@ -175,6 +181,76 @@ def cpu_baseline(specs, sweeps, budget_s):
     desc = (f"CPU oracle (plain solver, {sweeps} sweeps), 1 thread, first {len(sample)} cloths of the batch "
             f"({nv} vertices), {n_sub} substeps each in {el:.1f} s; scaled to the full batch by vertex-sweeps")
     return done_vs / el, desc
+
+
+def jacobi_baseline(a, capi, scenes, ctx, ids, r_plain, warm_rate, plain_solver):
+    """The colour-free Jacobi variant (constraint averaging) on the same
+    constraint problems as the plain solver: every substep of the plain
+    trajectory from the snapshot is solved again from its start state (x, v)
+    with S Jacobi sweeps; S is the smallest count whose mean end-of-substep
+    residual over those problems is <= R (bisection).  Its substeps/s are
+    timed from a captured graph.  (Along its own trajectory the max-strain
+    residual does not compare solver types: a solver that barely propagates
+    the pins' support lets the cloth free-fall with little strain.)"""
+    ctx.snapshot(restore=True)
+    plain_solver()
+    starts = []
+    for _ in range(a.match_substeps):
+        starts.append(ctx.state_save())
+        ctx.step(1)
+
+    def solver(sweeps):
+        ctx.ws_enable(False)
+        ctx.set_solver(dt=scenes.DT, iterations=sweeps, path=capi.PATH_JACOBI, jacobi_omega=a.jacobi_omega,
+                       graph=1)
+
+    tried = {}
+
+    def mean_residual(sweeps):
+        solver(sweeps)
+        acc = 0.0
+        for x, v in starts:
+            ctx.state_load(x, v)
+            ctx.step(1)
+            acc += float(ctx.residual_all()[0].mean())
+        tried[sweeps] = acc / len(starts)
+        return tried[sweeps]
+
+    lo, hi = 1, a.jacobi_max_sweeps
+    reached = mean_residual(hi) <= r_plain
+    if reached:
+        while hi > lo:
+            mid = (lo + hi) // 2
+            if mean_residual(mid) <= r_plain:
+                hi = mid
+            else:
+                lo = mid + 1
+        mean_residual(hi)
+    del starts
+    ctx.snapshot(restore=True)
+    solver(hi)
+    ctx.step(1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(a.steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    infos = [ctx.info(i) for i in ids]
+    nv = sum(i.nv for i in infos)
+    entries = sum(2 * i.ne + 4 * i.nb for i in infos)
+    out = {"omega": a.jacobi_omega, "sweeps": hi, "reaches_R": reached, "residual": tried[hi],
+           "substeps_per_s": a.steps / el, "ms_per_step": el * 1e3 / a.steps,
+           "warm_start_speedup_over_jacobi": warm_rate / (a.steps / el),
+           "sweeps_tried": {str(k): v for k, v in sorted(tried.items())},
+           "alg_bytes_per_sweep": 40 * nv + 24 * entries,
+           "definition": "PBD_PATH_JACOBI: every vertex moves by omega/n times the sum of the corrections its n "
+                         "prims give it on the sweep's input positions.  Sweeps: smallest count whose mean "
+                         "end-of-substep residual, over the plain trajectory's substeps each solved from the "
+                         "same start state, is <= R (stiffness schedule as the plain solver)"}
+    log(f"[bench] Jacobi (omega {a.jacobi_omega}): {hi} sweeps {'reach' if reached else 'do not reach'} R "
+        f"({tried[hi]:.5e} vs {r_plain:.5e}); {out['substeps_per_s']:.2f} substeps/s ({out['ms_per_step']:.2f} ms); "
+        f"tried {out['sweeps_tried']}")
+    return out
 
 
 def main():
@@ -364,6 +440,8 @@ def main():
     log(f"[bench] plain batched {plain_rate:.2f} substeps/s ({plain_ms:.2f} ms); warm start + batched "
         f"{warm_rate:.2f} substeps/s ({warm_ms:.2f} ms): x{warm_rate / unb_rate:.2f} vs plain unbatched, "
         f"x{warm_rate / plain_rate:.2f} vs plain batched (graph replays {plain_replays}, {warm_replays})")
+    jac = None if a.no_jacobi else jacobi_baseline(a, capi, scenes, ctx, ids, r_plain, warm_rate,
+                                                   lambda: solver(PLAIN_SWEEPS, False))
     prof = profiled(True, a.steps, warm=warm_k)
     plain_prof = profiled(False, a.steps)
     achieved = prof.tile_alg_bytes / (prof.tile_ms * 1e-3) / 1e9
@@ -437,6 +515,7 @@ def main():
             "single_tile_cloths": sum(1 for i in infos if i.whole),
             "tiles": sum(i.ntiles for i in infos),
             "copy_kernel_gbs": copy_gbs,
+            "jacobi_baseline": jac,
         },
         "roofline": {
             "bound": "hbm",
