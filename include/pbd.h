@@ -243,9 +243,27 @@ PBD_API int pbd_ccd_detect_swept(pbd_ctx* ctx, const pbd_hash_desc* desc, const 
 PBD_API int pbd_ccd_get_pairs(pbd_ctx* ctx, long long cap, int64_t* keys, long long* count);
 /* ContactData (CCD_Basic.h:12-31) of the last detection, ascending (v, f). */
 PBD_API int pbd_ccd_get_contacts(pbd_ctx* ctx, long long cap, int* vf, float* t, float* n, double* w);
+/* Edge-edge contacts (Contact::EE, declared but not implemented in the
+ * reference: CCD_Basic.h:14, CCD.cpp:925-928), added to the vertex-face
+ * contacts of a pbd_ccd_detect / pbd_ccd_detect_swept call on the same
+ * scene (none without one).  Candidates: every edge pair, sharing no vertex,
+ * of the triangle pairs pbd_ccd_detect visits (centroid cells of desc).
+ * Test: the first time in [0, 1] (coplanarity roots of the four points, then
+ * 0 and 1) at which the segments come within 2 x thickness. */
+PBD_API int pbd_ccd_detect_ee(pbd_ctx* ctx, const pbd_hash_desc* desc, const float* x, const float* p, int nv,
+                              const int* tris, int nt, float thickness, long long* npairs, long long* ncontacts);
+/* Edge-edge candidates of the last pbd_ccd_detect_ee: 4 vertex ids each
+ * (a0 < a1, b0 < b1), ordered by edge id pair. */
+PBD_API int pbd_ccd_get_ee_pairs(pbd_ctx* ctx, long long cap, int* quads, long long* count);
+/* Edge-edge contacts of the last pbd_ccd_detect_ee: vertices (a0, a1, b0, b1),
+ * time t, end-state edge normal n (3), closest-point weights
+ * w = (1 - s, s, 1 - u, u). */
+PBD_API int pbd_ccd_get_ee_contacts(pbd_ctx* ctx, long long cap, int* quads, float* t, float* n, double* w,
+                                    long long* count);
 /* CollisionResolveNew(meshTopol, prdPBuffer, contactData, iterations, thickness)
  * (CCD.cpp:1874-1994) on the last detection, Jacobi form; p is updated in
- * place, w[v] == 0 marks static vertices. */
+ * place, w[v] == 0 marks static vertices.  Edge-edge contacts of a
+ * following pbd_ccd_detect_ee are resolved in the same passes. */
 PBD_API int pbd_ccd_resolve(pbd_ctx* ctx, float* p, const float* w, float thickness, int passes);
 
 /* In-solver collision (SolverPBD::ProjectConstraintWithColli, PBD_cuda.cu:1130-1210):
@@ -269,6 +287,10 @@ typedef struct {
      * equal the all-pairs test (CCD_N2, CCD.cpp:1040-1098) at any cell size
      * (the cell only sets the cost).  No reference counterpart for 1. */
     int broadphase;
+    /* 1: edge-edge contacts too (pbd_ccd_detect_ee's test and response),
+     * from the centroid-cell triangle pairs; no reference counterpart
+     * (Contact::EE is declared but not implemented, CCD_Basic.h:14). */
+    int edge_edge;
 } pbd_collision_params;
 PBD_API int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* params);
 PBD_API int pbd_last_contacts(pbd_ctx* ctx, long long* ncontacts, long long* npairs);
@@ -323,6 +345,9 @@ typedef struct {
     long long ccd_pairs;        /* distinct candidate pairs through the narrowphase */
     long long ccd_contacts;     /* contacts found */
     long long ccd_resolved;     /* contacts x passes evaluated by the response */
+    long long ccd_ee_raw_pairs; /* edge-edge candidates before de-duplication */
+    long long ccd_ee_pairs;     /* distinct edge-edge candidates */
+    long long ccd_ee_contacts;  /* edge-edge contacts found */
 } pbd_profile;
 PBD_API int pbd_profile_enable(pbd_ctx* ctx, int on);
 PBD_API int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset);

@@ -355,6 +355,20 @@ def _sig_collide(L):
     L.orc_ccd_resolve.restype = None
     L.orc_ccd_resolve.argtypes = [C.POINTER(OrcContact), C.c_longlong, _ip, _fp, _fp, _fp, C.c_float, C.c_int,
                                   C.c_float]
+    L.orc_edge_table.restype = C.c_int
+    L.orc_edge_table.argtypes = [_ip, C.c_int, _ip, _ip]
+    L.orc_ee_ccd.restype = C.c_int
+    L.orc_ee_ccd.argtypes = [_fp] * 8 + [C.c_float, _fp, _fp, _dp]
+    L.orc_ee_fix.restype = C.c_int
+    L.orc_ee_fix.argtypes = [_fp] * 8 + [C.c_float, _fp]
+    L.orc_ee_candidates.restype = C.c_longlong
+    L.orc_ee_candidates.argtypes = [C.POINTER(OrcGrid), _fp, _ip, C.c_int, _ip, _ip, C.c_longlong, _ip]
+    L.orc_ccd_detect_ee.restype = C.c_longlong
+    L.orc_ccd_detect_ee.argtypes = [C.POINTER(OrcGrid), _fp, _fp, _ip, C.c_int, C.c_float, C.c_longlong, _ip, _fp,
+                                    _fp, _dp]
+    L.orc_ccd_resolve_ee.restype = None
+    L.orc_ccd_resolve_ee.argtypes = [C.POINTER(OrcContact), C.c_longlong, _ip, C.c_longlong, _ip, _fp, _fp, _fp,
+                                     C.c_int, C.c_float, C.c_int, C.c_float]
 
 
 def grid(mode=0, centre=(0.0, 0.0, 0.0), size=(1, 1, 1), cell=(1.0, 1.0, 1.0), table_bits=16) -> OrcGrid:
@@ -441,6 +455,89 @@ def ccd_detect_swept(x, p, tris, thickness, all_pairs=False):
     arr = (OrcContact * max(n, 1))()
     L.orc_ccd_detect_swept(F(x), F(p), I(tris), len(tris), len(x), thickness, int(all_pairs), n, arr)
     return arr, n
+
+
+def edge_table(tris):
+    """Edges in first-sighting order, (lower, higher); per-triangle edge ids."""
+    L = lib()
+    _sig_collide(L)
+    tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    tedge = np.zeros(3 * max(len(tris), 1), np.int32)
+    edges = np.zeros(6 * max(len(tris), 1), np.int32)
+    ne = L.orc_edge_table(I(tris), len(tris), I(tedge), I(edges))
+    return tedge[:3 * len(tris)], edges[:2 * ne].reshape(-1, 2)
+
+
+def ee_candidates(g, p, tris):
+    """Edge-edge candidate pairs (edge ids, ascending) and the edge table."""
+    L = lib()
+    _sig_collide(L)
+    p = np.ascontiguousarray(p, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    tedge, edges = edge_table(tris)
+    ed = np.ascontiguousarray(edges.reshape(-1), np.int32)
+    n = L.orc_ee_candidates(C.byref(g), F(p), I(tris), len(tris), I(tedge), I(ed), 0, None)
+    out = np.zeros(2 * max(n, 1), np.int32)
+    L.orc_ee_candidates(C.byref(g), F(p), I(tris), len(tris), I(tedge), I(ed), n, I(out))
+    return out[:2 * n].reshape(-1, 2), edges
+
+
+def ee_ccd(x4, p4, thickness):
+    """One edge pair: x4, p4 = (a0, a1, b0, b1) start and end positions.
+    Returns (hit, t, n, w)."""
+    L = lib()
+    _sig_collide(L)
+    xs = [np.ascontiguousarray(v, np.float32) for v in x4]
+    ps = [np.ascontiguousarray(v, np.float32) for v in p4]
+    t = np.zeros(1, np.float32)
+    n = np.zeros(3, np.float32)
+    w = np.zeros(4, np.float64)
+    hit = L.orc_ee_ccd(*[F(v) for v in xs], *[F(v) for v in ps], thickness, F(t), F(n), w.ctypes.data_as(_dp))
+    return bool(hit), float(t[0]), n, w
+
+
+def ee_fix(X4, q4, thickness):
+    """Displacements (4 x 3) of the edge-edge response, or None."""
+    L = lib()
+    _sig_collide(L)
+    xs = [np.ascontiguousarray(v, np.float32) for v in X4]
+    qs = [np.ascontiguousarray(v, np.float32) for v in q4]
+    d = np.zeros(12, np.float32)
+    ok = L.orc_ee_fix(*[F(v) for v in xs], *[F(v) for v in qs], thickness, F(d))
+    return d.reshape(4, 3) if ok else None
+
+
+def ccd_detect_ee(g, x, p, tris, thickness):
+    """Edge-edge contacts: quads (a0, a1, b0, b1), t, n, w, in candidate order."""
+    L = lib()
+    _sig_collide(L)
+    x = np.ascontiguousarray(x, np.float32)
+    p = np.ascontiguousarray(p, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    n = L.orc_ccd_detect_ee(C.byref(g), F(x), F(p), I(tris), len(tris), thickness, 0, None, None, None, None)
+    m = max(n, 1)
+    q = np.zeros((m, 4), np.int32)
+    t = np.zeros(m, np.float32)
+    nn = np.zeros((m, 3), np.float32)
+    w = np.zeros((m, 4), np.float64)
+    L.orc_ccd_detect_ee(C.byref(g), F(x), F(p), I(tris), len(tris), thickness, n, I(q), F(t), F(nn),
+                        w.ctypes.data_as(_dp))
+    return q[:n], t[:n], nn[:n], w[:n]
+
+
+def ccd_resolve_ee(arr, n, quads, tris, x, p, w, thickness, passes, min_n2=1e-6):
+    """Vertex-face contacts (arr, n) and edge-edge contacts (quads) resolved
+    together, `passes` Jacobi passes."""
+    L = lib()
+    _sig_collide(L)
+    p = np.ascontiguousarray(p, np.float32).copy()
+    q = np.ascontiguousarray(quads, np.int32).reshape(-1, 4)
+    if arr is None:
+        arr, n = (OrcContact * 1)(), 0
+    L.orc_ccd_resolve_ee(arr, n, I(q), len(q), I(np.ascontiguousarray(tris, np.int32)),
+                         F(np.ascontiguousarray(x, np.float32)), F(p), F(np.ascontiguousarray(w, np.float32)),
+                         len(p), thickness, passes, min_n2)
+    return p
 
 
 def contacts_arrays(arr, n):

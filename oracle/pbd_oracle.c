@@ -1363,6 +1363,292 @@ void orc_ccd_resolve(const orc_contact* c, long long nc, const int* tris, const 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Edge-edge CCD (new: the reference declares Contact::EE, PBD/CCD_Basic.h:14,*/
+/* and leaves it unimplemented, PBD/CCD.cpp:925-928).                        */
+/* ------------------------------------------------------------------------ */
+
+/* Edges in first-sighting order (triangle by triangle, edges (v0,v1),
+ * (v1,v2), (v2,v0)), endpoints (lower, higher); tedge[3t+k] = edge id of
+ * triangle t's edge k.  Returns the edge count.  Quadratic search: test
+ * sizes only. */
+int orc_edge_table(const int* tris, int nt, int* tedge, int* edges) {
+    int ne = 0;
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++) {
+            int a = tris[3 * t + k], b = tris[3 * t + (k + 1) % 3];
+            int lo = a < b ? a : b, hi = a < b ? b : a, id = -1;
+            for (int e = 0; e < ne; e++)
+                if (edges[2 * e] == lo && edges[2 * e + 1] == hi) { id = e; break; }
+            if (id < 0) { id = ne++; edges[2 * id] = lo; edges[2 * id + 1] = hi; }
+            tedge[3 * t + k] = id;
+        }
+    return ne;
+}
+
+static float clamp01f(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+/* Closest points a0 + s (a1 - a0), b0 + u (b1 - b0), s, u in [0, 1]
+ * (Ericson 2005, Real-Time Collision Detection, sec. 5.1.9). */
+void orc_seg_closest(const float* a0, const float* a1, const float* b0, const float* b1, float* s_out,
+                     float* u_out) {
+    float d1[3], d2[3], r[3];
+    v3sub(a1, a0, d1); v3sub(b1, b0, d2); v3sub(a0, b0, r);
+    float a = v3dot(d1, d1), e = v3dot(d2, d2), f = v3dot(d2, r);
+    float s, u;
+    if (a <= 1e-30f && e <= 1e-30f) { s = 0.0f; u = 0.0f; }
+    else if (a <= 1e-30f) { s = 0.0f; u = clamp01f(f / e); }
+    else {
+        float c = v3dot(d1, r);
+        if (e <= 1e-30f) { u = 0.0f; s = clamp01f(-c / a); }
+        else {
+            float b = v3dot(d1, d2);
+            float den = a * e - b * b;
+            s = den > 0.0f ? clamp01f((b * f - c * e) / den) : 0.0f;
+            u = (b * s + f) / e;
+            if (u < 0.0f) { u = 0.0f; s = clamp01f(-c / a); }
+            else if (u > 1.0f) { u = 1.0f; s = clamp01f((b - c) / a); }
+        }
+    }
+    *s_out = s;
+    *u_out = u;
+}
+
+static void seg_point(const float* a0, const float* a1, float s, float* r) {
+    for (int k = 0; k < 3; k++) r[k] = a0[k] + s * (a1[k] - a0[k]);
+}
+
+/* Edge (x0,x1)->(p0,p1) against edge (x2,x3)->(p2,p3): the coplanarity
+ * times of the four points (the cubic of orc_vf_ccd with the first edge's
+ * start point as origin; Provot 1997) plus 0 and 1, ascending; the first
+ * time in [0, 1] at which the segments are within 2 x thickness is the
+ * contact: t, the end-state edge normal (p1-p0) x (p3-p2) normalised (0 if
+ * parallel), weights (1-s, s, 1-u, u). */
+int orc_ee_ccd(const float* x0, const float* x1, const float* x2, const float* x3,
+               const float* p0, const float* p1, const float* p2, const float* p3,
+               float thickness, float* t_out, float* n_out, double* w_out) {
+    float v0[3], X1[3], X2[3], X3[3], V1[3], V2[3], V3[3], tmp[3];
+    v3sub(p0, x0, v0);
+    v3sub(x1, x0, X1); v3sub(x2, x0, X2); v3sub(x3, x0, X3);
+    v3sub(p1, x1, tmp); v3sub(tmp, v0, V1);
+    v3sub(p2, x2, tmp); v3sub(tmp, v0, V2);
+    v3sub(p3, x3, tmp); v3sub(tmp, v0, V3);
+    double a0 = stp(X1, X2, X3);
+    double a1 = (double)(stp(V1, X2, X3) + stp(X1, V2, X3) + stp(X1, X2, V3));
+    double a2 = (double)(stp(X1, V2, V3) + stp(V1, X2, V3) + stp(V1, V2, X3));
+    double a3 = stp(V1, V2, V3);
+    double t[5];
+    int nsol = orc_solve_cubic(a3, a2, a1, a0, t);
+    int m = 0;
+    for (int i = 0; i < nsol; i++) if (isfinite(t[i])) t[m++] = t[i];
+    t[m] = 1;
+    t[m + 1] = 0;
+    dsort(t, m + 2);
+    for (int i = 0; i < m + 2; i++) {
+        if (t[i] < 0 || t[i] > 1) continue;
+        float c0[3], c1[3], c2[3], c3[3], pa[3], pb[3], s, u;
+        lerp3(x0, p0, t[i], c0); lerp3(x1, p1, t[i], c1);
+        lerp3(x2, p2, t[i], c2); lerp3(x3, p3, t[i], c3);
+        orc_seg_closest(c0, c1, c2, c3, &s, &u);
+        seg_point(c0, c1, s, pa);
+        seg_point(c2, c3, u, pb);
+        float d = orc_distance(pa, pb);
+        if (d - 2.0f * thickness < -1e-6f) {
+            float e1[3], e2[3], cr[3];
+            v3sub(p1, p0, e1); v3sub(p3, p2, e2); v3cross(e1, e2, cr);
+            if (t_out) *t_out = (float)t[i];
+            if (n_out) {
+                if (v3dot(cr, cr) > 0.0f) v3normalize(cr, n_out);
+                else { n_out[0] = 0.0f; n_out[1] = 0.0f; n_out[2] = 0.0f; }
+            }
+            if (w_out) { w_out[0] = 1.0f - s; w_out[1] = s; w_out[2] = 1.0f - u; w_out[3] = u; }
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* Edge-edge response on the current positions q (start positions X orient
+ * it): the closest points are pushed apart along the edge normal, oriented
+ * as their offset at the start, until 2 x thickness apart, half each, spread
+ * over each edge's ends by w_k / sum w^2; near-parallel edges (sin^2 <=
+ * 1e-12) are skipped.  d: 4 displacements (12 floats).  Returns 0 if none. */
+int orc_ee_fix(const float* X0, const float* X1, const float* X2, const float* X3, const float* q0,
+               const float* q1, const float* q2, const float* q3, float thickness, float* d) {
+    float s0, u0, pa[3], pb[3], side[3], ea[3], eb[3], cr[3], nn[3], rel[3], s, u;
+    orc_seg_closest(X0, X1, X2, X3, &s0, &u0);
+    seg_point(X0, X1, s0, pa);
+    seg_point(X2, X3, u0, pb);
+    v3sub(pa, pb, side);
+    v3sub(q1, q0, ea); v3sub(q3, q2, eb); v3cross(ea, eb, cr);
+    if (v3dot(cr, cr) <= 1e-12f * (v3dot(ea, ea) * v3dot(eb, eb))) return 0;
+    v3normalize(cr, nn);
+    if (v3dot(side, nn) < 0.0f) { nn[0] = -nn[0]; nn[1] = -nn[1]; nn[2] = -nn[2]; }
+    orc_seg_closest(q0, q1, q2, q3, &s, &u);
+    seg_point(q0, q1, s, pa);
+    seg_point(q2, q3, u, pb);
+    v3sub(pa, pb, rel);
+    float gap = v3dot(rel, nn);
+    if (!(gap - 2.0f * thickness < -1e-6f)) return 0;
+    float depth = (2.0f * thickness - gap) * 0.5f;
+    float w[4] = {1.0f - s, s, 1.0f - u, u};
+    float sa = w[0] * w[0] + w[1] * w[1], sb = w[2] * w[2] + w[3] * w[3];
+    for (int k = 0; k < 4; k++) {
+        float f = k < 2 ? (depth * w[k]) / sa : -((depth * w[k]) / sb);
+        d[3 * k] = nn[0] * f; d[3 * k + 1] = nn[1] * f; d[3 * k + 2] = nn[2] * f;
+    }
+    return 1;
+}
+
+static int cmp_pair(const void* a, const void* b) {
+    const int* x = (const int*)a;
+    const int* y = (const int*)b;
+    if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+    if (x[1] != y[1]) return x[1] < y[1] ? -1 : 1;
+    return 0;
+}
+
+/* Edge-edge candidates by brute force over triangle pairs t < f whose
+ * centroid cells are both inside the grid and at most one cell apart on
+ * each axis: every edge pair sharing no vertex, as (lower edge id, higher),
+ * distinct and ascending.  Returns the count; writes min(count, cap). */
+long long orc_ee_candidates(const orc_grid* g, const float* pos, const int* tris, int nt, const int* tedge,
+                            const int* edges, long long cap, int* out) {
+    int* keys = (int*)malloc(sizeof(int) * ((size_t)nt + 1));
+    int* ids = (int*)malloc(sizeof(int) * ((size_t)nt + 1));
+    orc_tri_keys(g, pos, tris, nt, keys, ids);
+    int* tc = (int*)malloc(sizeof(int) * 3 * ((size_t)nt + 1));
+    float c[3];
+    for (int t = 0; t < nt; t++) {
+        const int* tv = tris + 3 * t;
+        for (int k = 0; k < 3; k++)
+            c[k] = (pos[3 * tv[0] + k] + pos[3 * tv[1] + k] + pos[3 * tv[2] + k]) / 3;
+        orc_cell_coords(g, c, tc + 3 * t);
+    }
+    size_t bufcap = 1024, n = 0;
+    int* buf = (int*)malloc(sizeof(int) * 2 * bufcap);
+    for (int t = 0; t < nt; t++) {
+        if (keys[t] < 0) continue;
+        for (int f = t + 1; f < nt; f++) {
+            if (keys[f] < 0) continue;
+            int near = 1;
+            for (int k = 0; k < 3; k++) near &= abs(tc[3 * t + k] - tc[3 * f + k]) <= 1;
+            if (!near) continue;
+            for (int kb = 0; kb < 3; kb++)
+                for (int ka = 0; ka < 3; ka++) {
+                    int ea = tedge[3 * t + ka], eb = tedge[3 * f + kb];
+                    const int* av = edges + 2 * ea;
+                    const int* bv = edges + 2 * eb;
+                    if (av[0] == bv[0] || av[0] == bv[1] || av[1] == bv[0] || av[1] == bv[1]) continue;
+                    if (n == bufcap) { bufcap *= 2; buf = (int*)realloc(buf, sizeof(int) * 2 * bufcap); }
+                    buf[2 * n] = ea < eb ? ea : eb;
+                    buf[2 * n + 1] = ea < eb ? eb : ea;
+                    n++;
+                }
+        }
+    }
+    qsort(buf, n, 2 * sizeof(int), cmp_pair);
+    size_t u = 0;
+    for (size_t i = 0; i < n; i++)
+        if (i == 0 || buf[2 * i] != buf[2 * u - 2] || buf[2 * i + 1] != buf[2 * u - 1]) {
+            buf[2 * u] = buf[2 * i];
+            buf[2 * u + 1] = buf[2 * i + 1];
+            u++;
+        }
+    if (out) for (size_t i = 0; i < u && (long long)i < cap; i++) { out[2 * i] = buf[2 * i]; out[2 * i + 1] = buf[2 * i + 1]; }
+    free(buf); free(keys); free(ids); free(tc);
+    return (long long)u;
+}
+
+/* Edge-edge detection: candidates (above) through orc_ee_ccd, in candidate
+ * order; quads = (a0, a1, b0, b1). */
+long long orc_ccd_detect_ee(const orc_grid* g, const float* x, const float* p, const int* tris, int nt,
+                            float thickness, long long cap, int* quads, float* t, float* n, double* w) {
+    int* tedge = (int*)malloc(sizeof(int) * 3 * ((size_t)nt + 1));
+    int* edges = (int*)malloc(sizeof(int) * 6 * ((size_t)nt + 1));
+    orc_edge_table(tris, nt, tedge, edges);
+    long long np = orc_ee_candidates(g, p, tris, nt, tedge, edges, 0, NULL);
+    int* cand = (int*)malloc(sizeof(int) * 2 * (size_t)(np + 1));
+    orc_ee_candidates(g, p, tris, nt, tedge, edges, np, cand);
+    long long nc = 0;
+    for (long long i = 0; i < np; i++) {
+        const int* a = edges + 2 * cand[2 * i];
+        const int* b = edges + 2 * cand[2 * i + 1];
+        float tt, nn[3];
+        double ww[4];
+        if (!orc_ee_ccd(x + 3 * a[0], x + 3 * a[1], x + 3 * b[0], x + 3 * b[1], p + 3 * a[0], p + 3 * a[1],
+                        p + 3 * b[0], p + 3 * b[1], thickness, &tt, nn, ww))
+            continue;
+        if (nc < cap) {
+            if (quads) { quads[4 * nc] = a[0]; quads[4 * nc + 1] = a[1]; quads[4 * nc + 2] = b[0]; quads[4 * nc + 3] = b[1]; }
+            if (t) t[nc] = tt;
+            if (n) for (int k = 0; k < 3; k++) n[3 * nc + k] = nn[k];
+            if (w) for (int k = 0; k < 4; k++) w[4 * nc + k] = ww[k];
+        }
+        nc++;
+    }
+    free(cand); free(edges); free(tedge);
+    return nc;
+}
+
+/* orc_ccd_resolve with edge-edge contacts (quads) summed into the same
+ * fixed-point accumulators in every pass; an edge whose two ends are static
+ * leaves the whole gap to the other. */
+void orc_ccd_resolve_ee(const orc_contact* c, long long nc, const int* quads, long long nee, const int* tris,
+                        const float* x, float* p, const float* w, int nv, float thickness, int passes,
+                        float min_n2) {
+    long long* acc = (long long*)malloc(sizeof(long long) * 3 * (size_t)(nv + 1));
+    int* cnt = (int*)malloc(sizeof(int) * (size_t)(nv + 1));
+    float q[12], d[12];
+    for (int pass = 0; pass < passes; pass++) {
+        memset(acc, 0, sizeof(long long) * 3 * (size_t)nv);
+        memset(cnt, 0, sizeof(int) * (size_t)nv);
+        for (long long i = 0; i < nc; i++) {
+            int id[4] = {c[i].v, tris[3 * c[i].f], tris[3 * c[i].f + 1], tris[3 * c[i].f + 2]};
+            for (int k = 0; k < 4; k++) {
+                q[3 * k] = p[3 * id[k]]; q[3 * k + 1] = p[3 * id[k] + 1]; q[3 * k + 2] = p[3 * id[k] + 2];
+            }
+            const float *x0 = x + 3 * id[0], *x1 = x + 3 * id[1], *x2 = x + 3 * id[2], *x3 = x + 3 * id[3];
+            if (!orc_vf_resolve_test(x0, x1, x2, x3, q, q + 3, q + 6, q + 9, thickness, min_n2)) continue;
+            float before[12];
+            memcpy(before, q, sizeof(before));
+            orc_vf_resolve(x0, x1, x2, x3, q, q + 3, q + 6, q + 9, thickness);
+            float wv = w[id[0]];
+            int face_static = (w[id[1]] == 0.0f) && (w[id[2]] == 0.0f) && (w[id[3]] == 0.0f);
+            for (int k = 0; k < 4; k++) {
+                float sc = 1.0f;
+                if (w[id[k]] == 0.0f) continue;
+                if (k == 0 && face_static) sc = 2.0f;
+                if (k > 0 && wv == 0.0f) sc = 2.0f;
+                for (int a = 0; a < 3; a++) acc[3 * id[k] + a] += to_fixed((q[3 * k + a] - before[3 * k + a]) * sc);
+                cnt[id[k]]++;
+            }
+        }
+        for (long long i = 0; i < nee; i++) {
+            const int* id = quads + 4 * i;
+            for (int k = 0; k < 4; k++) {
+                q[3 * k] = p[3 * id[k]]; q[3 * k + 1] = p[3 * id[k] + 1]; q[3 * k + 2] = p[3 * id[k] + 2];
+            }
+            if (!orc_ee_fix(x + 3 * id[0], x + 3 * id[1], x + 3 * id[2], x + 3 * id[3], q, q + 3, q + 6, q + 9,
+                            thickness, d))
+                continue;
+            int a_static = w[id[0]] == 0.0f && w[id[1]] == 0.0f, b_static = w[id[2]] == 0.0f && w[id[3]] == 0.0f;
+            for (int k = 0; k < 4; k++) {
+                if (w[id[k]] == 0.0f) continue;
+                float sc = (k < 2 ? b_static : a_static) ? 2.0f : 1.0f;
+                for (int a = 0; a < 3; a++) acc[3 * id[k] + a] += to_fixed(d[3 * k + a] * sc);
+                cnt[id[k]]++;
+            }
+        }
+        for (int v = 0; v < nv; v++) {
+            if (!cnt[v]) continue;
+            for (int a = 0; a < 3; a++)
+                p[3 * v + a] = p[3 * v + a] + (float)((double)acc[3 * v + a] / 4294967296.0 / (double)cnt[v]);
+        }
+    }
+    free(acc); free(cnt);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Warm-start MLP in FP64 (new).  22 -> 64 -> 64 -> 3, GELU (tanh form,      */
 /* Hendrycks & Gimpel 2016), weights row-major W[out][in].                   */
 /* Loss = mean over samples of 0.5*|y - t|^2.                                */

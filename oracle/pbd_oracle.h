@@ -151,6 +151,20 @@ long long orc_ccd_detect_swept(const float* x, const float* p, const int* tris, 
                                int all_pairs, long long cap, orc_contact* out);
 long long orc_ccd_detect(const orc_grid* g, const float* x, const float* p, const int* tris, int nt,
                          int nv, float thickness, long long cap, orc_contact* out);
+int orc_edge_table(const int* tris, int nt, int* tedge, int* edges);
+void orc_seg_closest(const float* a0, const float* a1, const float* b0, const float* b1, float* s, float* u);
+int orc_ee_ccd(const float* x0, const float* x1, const float* x2, const float* x3,
+               const float* p0, const float* p1, const float* p2, const float* p3,
+               float thickness, float* t_out, float* n_out, double* w_out);
+int orc_ee_fix(const float* X0, const float* X1, const float* X2, const float* X3, const float* q0,
+               const float* q1, const float* q2, const float* q3, float thickness, float* d);
+long long orc_ee_candidates(const orc_grid* g, const float* pos, const int* tris, int nt, const int* tedge,
+                            const int* edges, long long cap, int* out);
+long long orc_ccd_detect_ee(const orc_grid* g, const float* x, const float* p, const int* tris, int nt,
+                            float thickness, long long cap, int* quads, float* t, float* n, double* w);
+void orc_ccd_resolve_ee(const orc_contact* c, long long nc, const int* quads, long long nee, const int* tris,
+                        const float* x, float* p, const float* w, int nv, float thickness, int passes,
+                        float min_n2);
 void orc_ccd_resolve(const orc_contact* c, long long nc, const int* tris, const float* x, float* p,
                      const float* w, float thickness, int passes, float min_n2);
 

@@ -59,7 +59,7 @@ def hash_desc(mode=0, centre=(0.0, 0.0, 0.0), size=(1, 1, 1), cell=(1.0, 1.0, 1.
 class CollisionParams(C.Structure):
     _fields_ = [("enabled", C.c_int), ("grid", HashDesc), ("cell_scale", C.c_float), ("thickness", C.c_float),
                 ("resolve_passes", C.c_int), ("detect_every", C.c_int), ("min_normal2", C.c_float),
-                ("broadphase", C.c_int)]
+                ("broadphase", C.c_int), ("edge_edge", C.c_int)]
 
 
 class Profile(C.Structure):
@@ -67,7 +67,8 @@ class Profile(C.Structure):
                 ("step_ms", C.c_double), ("substeps", C.c_longlong), ("vertex_sweeps", C.c_longlong),
                 ("sweeps_max", C.c_longlong), ("graph_replays", C.c_longlong), ("ccd_detections", C.c_longlong),
                 ("ccd_triangles", C.c_longlong), ("ccd_cell_entries", C.c_longlong), ("ccd_raw_pairs", C.c_longlong),
-                ("ccd_pairs", C.c_longlong), ("ccd_contacts", C.c_longlong), ("ccd_resolved", C.c_longlong)]
+                ("ccd_pairs", C.c_longlong), ("ccd_contacts", C.c_longlong), ("ccd_resolved", C.c_longlong),
+                ("ccd_ee_raw_pairs", C.c_longlong), ("ccd_ee_pairs", C.c_longlong), ("ccd_ee_contacts", C.c_longlong)]
 
 
 class ClothInfo(C.Structure):
@@ -91,6 +92,10 @@ SIGNATURES = {
     "pbd_cloth_create_mesh": [_P, _fp, _I, _ip, _I, _ip],
     "pbd_cloth_load_txt": [_P, C.c_char_p, C.c_char_p, _ip],
     "pbd_cloth_load_cache": [_P, C.c_char_p, C.c_char_p, _ip],
+    "pbd_ccd_detect_ee": [_P, C.POINTER(HashDesc), _fp, _fp, _I, _ip, _I, C.c_float, C.POINTER(C.c_longlong),
+                          C.POINTER(C.c_longlong)],
+    "pbd_ccd_get_ee_pairs": [_P, C.c_longlong, _ip, C.POINTER(C.c_longlong)],
+    "pbd_ccd_get_ee_contacts": [_P, C.c_longlong, _ip, _fp, _fp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
     "pbd_cloth_save_cache": [_P, _I, C.c_char_p, C.c_char_p, _I],
     "pbd_cloth_set_distance_pairs": [_P, _I, _ip, _I],
     "pbd_cloth_set_constraint_option": [_P, _I, C.c_uint, _fp],
@@ -428,6 +433,33 @@ class Context:
         check(self.lib.pbd_ccd_get_contacts(self.h, m, iptr(vf), fptr(t), fptr(nn), w.ctypes.data_as(_dp)))
         return vf[:m], t[:m], nn[:m], w[:m]
 
+    def ccd_detect_ee(self, desc: HashDesc, x, p, tris, thickness: float):
+        """Edge-edge contacts (quads (a0, a1, b0, b1), t, n, w) of one detection,
+        added to those of the preceding ccd_detect on the same scene."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+        tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        n, npairs = C.c_longlong(), C.c_longlong()
+        check(self.lib.pbd_ccd_detect_ee(self.h, C.byref(desc), fptr(x), fptr(p), len(x), iptr(tris), len(tris),
+                                         thickness, C.byref(npairs), C.byref(n)))
+        m = n.value
+        q = np.zeros((max(m, 1), 4), np.int32)
+        t = np.zeros(max(m, 1), np.float32)
+        nn = np.zeros((max(m, 1), 3), np.float32)
+        w = np.zeros((max(m, 1), 4), np.float64)
+        cnt = C.c_longlong()
+        check(self.lib.pbd_ccd_get_ee_contacts(self.h, m, iptr(q), fptr(t), fptr(nn), w.ctypes.data_as(_dp),
+                                               C.byref(cnt)))
+        return q[:m], t[:m], nn[:m], w[:m]
+
+    def ccd_ee_pairs(self) -> np.ndarray:
+        """Edge-edge candidates of the last ccd_detect_ee, (a0, a1, b0, b1) rows."""
+        cnt = C.c_longlong()
+        check(self.lib.pbd_ccd_get_ee_pairs(self.h, 0, None, C.byref(cnt)))
+        q = np.zeros((max(cnt.value, 1), 4), np.int32)
+        check(self.lib.pbd_ccd_get_ee_pairs(self.h, cnt.value, iptr(q), C.byref(cnt)))
+        return q[:cnt.value]
+
     def ccd_pairs(self) -> np.ndarray:
         """Candidate pairs of the last detection as sorted keys v << 32 | f."""
         n = C.c_longlong()
@@ -443,11 +475,12 @@ class Context:
         return p
 
     def set_collision(self, enabled=1, grid: HashDesc | None = None, cell_scale=1.5, thickness=0.005,
-                      resolve_passes=2, detect_every=10, min_normal2=0.0, broadphase=0):
+                      resolve_passes=2, detect_every=10, min_normal2=0.0, broadphase=0, edge_edge=0):
         """min_normal2 = 0 keeps the reference's degenerate-face bound (1e-6 m^4);
-        broadphase 0 = the reference's centroid cells, 1 = swept boxes."""
+        broadphase 0 = the reference's centroid cells, 1 = swept boxes;
+        edge_edge 1 adds edge-edge contacts."""
         cp = CollisionParams(enabled, grid if grid is not None else hash_desc(1, table_bits=20), cell_scale,
-                             thickness, resolve_passes, detect_every, min_normal2, broadphase)
+                             thickness, resolve_passes, detect_every, min_normal2, broadphase, edge_edge)
         check(self.lib.pbd_set_collision(self.h, C.byref(cp)))
 
     def last_contacts(self):

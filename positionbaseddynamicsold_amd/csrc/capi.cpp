@@ -556,6 +556,15 @@ void detect_contacts(pbd_ctx* c, const float4* p) {
         c->cbp.pairs(s, p, c->gtris.ptr, c->nv_total);
     }
     c->cbp.narrow(s, c->x.ptr, p, c->gtris.ptr, c->coll.thickness);
+    if (c->coll.edge_edge) {
+        // edge-edge candidates come from the centroid-cell table
+        if (c->coll.broadphase == 1) c->cbp.build(s, c->ccfg, p, c->gtris.ptr, c->gnt);
+        c->cbp.ee_pairs(s, p, c->gtris.ptr);
+        c->cbp.ee_narrow(s, c->x.ptr, p, c->coll.thickness);
+    } else {
+        c->cbp.ee_npairs = 0;
+        c->cbp.ee_ncontacts = 0;
+    }
 }
 
 // ---- learned warm start ------------------------------------------------------
@@ -1610,8 +1619,83 @@ int pbd_ccd_detect(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, const f
         ctx->sbp.build(s, g, ctx->sp4.ptr, ctx->stris.ptr, nt);
         ctx->sbp.pairs(s, ctx->sp4.ptr, ctx->stris.ptr, nv);
         *nc = ctx->sbp.narrow(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, th);
+        ctx->sbp.ee_npairs = 0;
+        ctx->sbp.ee_ncontacts = 0;
         ctx->sh_built = true;
         ctx->ccd_built = true;
+    });
+}
+
+int pbd_ccd_detect_ee(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, const float* p, int nv, const int* tris,
+                      int nt, float th, long long* npairs, long long* nc) {
+    return guard([&] {
+        PBD_CHECK(ctx && d && nc, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(th >= 0.0f, PBD_ERR_ARG, "thickness must be >= 0");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        HashCfg g = make_hash_cfg(*d);
+        // vertex-face contacts of an earlier detection on another scene do not carry over
+        bool same = ctx->ccd_built && ctx->snv == nv && ctx->snt == nt;
+        upload_scene(ctx, x, p, nullptr, nv, tris, nt);
+        if (!same) {
+            ctx->sbp.npairs = 0;
+            ctx->sbp.ncontacts = 0;
+        }
+        std::vector<int> te;
+        std::vector<int2> ed;
+        build_edge_table(tris, nt, te, ed);
+        ctx->sbp.set_edges(s, te, ed);
+        ctx->sbp.build(s, g, ctx->sp4.ptr, ctx->stris.ptr, nt);
+        long long np = ctx->sbp.ee_pairs(s, ctx->sp4.ptr, ctx->stris.ptr);
+        if (npairs) *npairs = np;
+        *nc = ctx->sbp.ee_narrow(s, ctx->sx.ptr, ctx->sp4.ptr, th);
+        ctx->sh_built = true;
+        ctx->ccd_built = true;
+    });
+}
+
+int pbd_ccd_get_ee_pairs(pbd_ctx* ctx, long long cap, int* quads, long long* count) {
+    return guard([&] {
+        PBD_CHECK(ctx && count, PBD_ERR_ARG, "null argument");
+        PBD_CHECK(ctx->ccd_built, PBD_ERR_STATE, "call pbd_ccd_detect_ee first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        Broadphase& b = ctx->sbp;
+        *count = b.ee_npairs;
+        long long m = std::min(cap, b.ee_npairs);
+        if (m <= 0 || !quads) return;
+        std::vector<uint32_t> ea(m), eb(m);
+        std::vector<int2> ed;
+        PBD_HIP(hipMemcpyAsync(ea.data(), b.eua.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+        PBD_HIP(hipMemcpyAsync(eb.data(), b.eub.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+        b.edges.download(ed, s);
+        PBD_HIP(hipStreamSynchronize(s));
+        for (long long i = 0; i < m; i++) {
+            int2 a = ed[ea[i]], c = ed[eb[i]];
+            quads[4 * i] = a.x;
+            quads[4 * i + 1] = a.y;
+            quads[4 * i + 2] = c.x;
+            quads[4 * i + 3] = c.y;
+        }
+    });
+}
+
+int pbd_ccd_get_ee_contacts(pbd_ctx* ctx, long long cap, int* quads, float* t, float* n, double* w,
+                            long long* count) {
+    return guard([&] {
+        PBD_CHECK(ctx, PBD_ERR_ARG, "null context");
+        PBD_CHECK(ctx->ccd_built, PBD_ERR_STATE, "call pbd_ccd_detect_ee first");
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        Broadphase& b = ctx->sbp;
+        if (count) *count = b.ee_ncontacts;
+        long long m = std::min(cap, b.ee_ncontacts);
+        if (m <= 0) return;
+        if (quads) PBD_HIP(hipMemcpyAsync(quads, b.ecq.ptr, 16 * m, hipMemcpyDeviceToHost, s));
+        if (t) PBD_HIP(hipMemcpyAsync(t, b.ect.ptr, 4 * m, hipMemcpyDeviceToHost, s));
+        if (n) PBD_HIP(hipMemcpyAsync(n, b.ecn.ptr, 12 * m, hipMemcpyDeviceToHost, s));
+        if (w) PBD_HIP(hipMemcpyAsync(w, b.ecw.ptr, 32 * m, hipMemcpyDeviceToHost, s));
+        PBD_HIP(hipStreamSynchronize(s));
     });
 }
 
@@ -1627,6 +1711,8 @@ int pbd_ccd_detect_swept(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, c
         long long np = ctx->sbp.pairs_swept(s, g, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, nt, nv, th);
         if (npairs) *npairs = np;
         *nc = ctx->sbp.narrow(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, th);
+        ctx->sbp.ee_npairs = 0;
+        ctx->sbp.ee_ncontacts = 0;
         ctx->sh_built = false;  // the cell table holds swept entries, not centroid keys
         ctx->ccd_built = true;
     });
@@ -1717,7 +1803,7 @@ int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* cp) {
             return;
         }
         PBD_CHECK(cp->thickness >= 0.0f && cp->resolve_passes >= 0 && cp->min_normal2 >= 0.0f &&
-                      (cp->broadphase == 0 || cp->broadphase == 1),
+                      (cp->broadphase == 0 || cp->broadphase == 1) && (cp->edge_edge == 0 || cp->edge_edge == 1),
                   PBD_ERR_ARG, "bad collision parameters");
         pbd_hash_desc d = cp->grid;
         if (cp->cell_scale > 0.0f) {
@@ -1739,6 +1825,12 @@ int pbd_set_collision(pbd_ctx* ctx, const pbd_collision_params* cp) {
             for (int i : k.tris) gt.push_back(i + k.vbase);
         ctx->gtris.upload(gt, ctx->stream);
         ctx->gnt = (int)gt.size() / 3;
+        if (cp->edge_edge) {
+            std::vector<int> te;
+            std::vector<int2> ed;
+            build_edge_table(gt.data(), ctx->gnt, te, ed);
+            ctx->cbp.set_edges(ctx->stream, te, ed);
+        }
         ctx->coll = *cp;
         replan_if_needed(ctx);
         PBD_HIP(hipStreamSynchronize(ctx->stream));
@@ -2110,6 +2202,9 @@ int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset) {
         PBD_CHECK(ctx && out, PBD_ERR_ARG, "null argument");
         *out = ctx->prof;
         const Broadphase& b = ctx->cbp;
+        out->ccd_ee_raw_pairs = b.st_ee_raw;
+        out->ccd_ee_pairs = b.st_ee_pairs;
+        out->ccd_ee_contacts = b.st_ee_contacts;
         out->ccd_detections = b.st_dets;
         out->ccd_triangles = b.st_tris;
         out->ccd_cell_entries = b.st_entries;
@@ -2121,6 +2216,7 @@ int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset) {
             ctx->prof = pbd_profile{};
             ctx->cbp.st_dets = ctx->cbp.st_tris = ctx->cbp.st_entries = 0;
             ctx->cbp.st_raw = ctx->cbp.st_pairs = ctx->cbp.st_contacts = ctx->cbp.st_resolved = 0;
+            ctx->cbp.st_ee_raw = ctx->cbp.st_ee_pairs = ctx->cbp.st_ee_contacts = 0;
         }
     });
 }

@@ -194,6 +194,61 @@ __global__ __launch_bounds__(256) void k_pairs(HashCfg g, const float4* __restri
     if (!EMIT) cnt_or_off[t] = n;
 }
 
+// Edge-edge candidates (new: the reference declares Contact::EE, CCD_Basic.h:14,
+// and leaves it unimplemented, CCD.cpp:925-928).  From the triangle pairs
+// k_pairs visits (centroid cells within one cell, the neighbour's own cell
+// checked), taken once (t < f): every edge of t against every edge of f that
+// shares no vertex with it, as (lower edge id, higher edge id).
+template <int EMIT>
+__global__ __launch_bounds__(256) void k_ee_pairs(HashCfg g, const float4* __restrict__ p,
+                                                  const int* __restrict__ tris, int nt, const int* __restrict__ raw,
+                                                  const uint32_t* __restrict__ sids, const int* __restrict__ cstart,
+                                                  const int* __restrict__ cend, const int4* __restrict__ tcell,
+                                                  const int* __restrict__ tedge, const int2* __restrict__ edges,
+                                                  int* __restrict__ cnt_or_off, uint32_t* __restrict__ oa,
+                                                  uint32_t* __restrict__ ob) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int n = 0, off = EMIT ? cnt_or_off[t] : 0;
+    if (raw[t] >= 0) {
+        int gp[3];
+        tri_cell(g, p, tris, t, gp);
+        int te[3] = {tedge[3 * t], tedge[3 * t + 1], tedge[3 * t + 2]};
+        int2 tv[3] = {edges[te[0]], edges[te[1]], edges[te[2]]};
+        for (int z = -1; z <= 1; z++)
+            for (int y = -1; y <= 1; y++)
+                for (int x = -1; x <= 1; x++) {
+                    const int qx = gp[0] + x, qy = gp[1] + y, qz = gp[2] + z;
+                    int h = cell_key(g, qx, qy, qz);
+                    if (h < 0) continue;
+                    int s = cstart[h];
+                    if (s < 0) continue;
+                    int e = cend[h];
+                    for (int i = s; i <= e; i++) {
+                        int f = (int)sids[i];
+                        if (f <= t) continue;
+                        int4 fc = tcell[f];
+                        if (fc.x != qx || fc.y != qy || fc.z != qz) continue;
+                        for (int kb = 0; kb < 3; kb++) {
+                            int eb = tedge[3 * f + kb];
+                            int2 bv = edges[eb];
+                            for (int ka = 0; ka < 3; ka++) {
+                                int2 av = tv[ka];
+                                if (av.x == bv.x || av.x == bv.y || av.y == bv.x || av.y == bv.y) continue;
+                                if (EMIT) {
+                                    int ea = te[ka];
+                                    oa[off + n] = (uint32_t)(ea < eb ? ea : eb);
+                                    ob[off + n] = (uint32_t)(ea < eb ? eb : ea);
+                                }
+                                n++;
+                            }
+                        }
+                    }
+                }
+    }
+    if (!EMIT) cnt_or_off[t] = n;
+}
+
 __global__ __launch_bounds__(256) void k_unique_flags(const uint32_t* __restrict__ hi, const uint32_t* __restrict__ lo,
                                                       int n, int* __restrict__ flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -561,9 +616,202 @@ __global__ __launch_bounds__(256) void k_compact_contacts(int n, const int* __re
 }
 
 // ---------------------------------------------------------------------------
-// response
+// edge-edge narrowphase and response (new; see k_ee_pairs)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ long long to_fixed(float f) { return (long long)rint((double)f * 4294967296.0); }
+__device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+// Closest points a0 + s (a1 - a0), b0 + u (b1 - b0), s, u in [0, 1], of two
+// segments (Ericson 2005, Real-Time Collision Detection, sec. 5.1.9).
+__device__ void seg_closest(V3 a0, V3 a1, V3 b0, V3 b1, float* s_out, float* u_out) {
+    V3 d1 = sub(a1, a0), d2 = sub(b1, b0), r = sub(a0, b0);
+    float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    float s, u;
+    if (a <= 1e-30f && e <= 1e-30f) {
+        s = 0.0f;
+        u = 0.0f;
+    } else if (a <= 1e-30f) {
+        s = 0.0f;
+        u = clamp01(f / e);
+    } else {
+        float c = dot(d1, r);
+        if (e <= 1e-30f) {
+            u = 0.0f;
+            s = clamp01(-c / a);
+        } else {
+            float b = dot(d1, d2);
+            float den = a * e - b * b;
+            s = den > 0.0f ? clamp01((b * f - c * e) / den) : 0.0f;
+            u = (b * s + f) / e;
+            if (u < 0.0f) {
+                u = 0.0f;
+                s = clamp01(-c / a);
+            } else if (u > 1.0f) {
+                u = 1.0f;
+                s = clamp01((b - c) / a);
+            }
+        }
+    }
+    *s_out = s;
+    *u_out = u;
+}
+
+__device__ __forceinline__ V3 seg_point(V3 a0, V3 a1, float s) {
+    return V3{a0.x + s * (a1.x - a0.x), a0.y + s * (a1.y - a0.y), a0.z + s * (a1.z - a0.z)};
+}
+
+// Edge (x0, x1) -> (p0, p1) against edge (x2, x3) -> (p2, p3) over the step:
+// the times the four points are coplanar (the cubic of vf_ccd, with the
+// first edge's start point as origin; Provot 1997) plus t = 0 and 1,
+// ascending; the first time in [0, 1] where the segments come within
+// 2 x thickness is the contact.  Records t, the end-state edge normal
+// (p1 - p0) x (p3 - p2) normalised (0 when parallel) and the weights
+// (1 - s, s, 1 - u, u) of the closest points.
+__device__ bool ee_ccd(V3 x0, V3 x1, V3 x2, V3 x3, V3 p0, V3 p1, V3 p2, V3 p3, float th, float* t_out, V3* n_out,
+                       float* w_out) {
+    V3 v0 = sub(p0, x0);
+    V3 X1 = sub(x1, x0), X2 = sub(x2, x0), X3 = sub(x3, x0);
+    V3 V1 = sub(sub(p1, x1), v0), V2 = sub(sub(p2, x2), v0), V3_ = sub(sub(p3, x3), v0);
+    double a0 = stp(X1, X2, X3);
+    double a1 = (double)(stp(V1, X2, X3) + stp(X1, V2, X3) + stp(X1, X2, V3_));
+    double a2 = (double)(stp(X1, V2, V3_) + stp(V1, X2, V3_) + stp(V1, V2, X3));
+    double a3 = stp(V1, V2, V3_);
+    double t[5];
+    int nsol = d_solve_cubic(a3, a2, a1, a0, t);
+    int m = 0;
+    for (int i = 0; i < nsol; i++)
+        if (isfinite(t[i])) t[m++] = t[i];
+    t[m] = 1;
+    t[m + 1] = 0;
+    int nt = m + 2;
+    for (int i = 1; i < nt; i++) {
+        double v = t[i];
+        int j = i - 1;
+        while (j >= 0 && t[j] > v) {
+            t[j + 1] = t[j];
+            j--;
+        }
+        t[j + 1] = v;
+    }
+    for (int i = 0; i < nt; i++) {
+        if (t[i] < 0 || t[i] > 1) continue;
+        V3 c0 = lerp(x0, p0, t[i]), c1 = lerp(x1, p1, t[i]), c2 = lerp(x2, p2, t[i]), c3 = lerp(x3, p3, t[i]);
+        float s, u;
+        seg_closest(c0, c1, c2, c3, &s, &u);
+        float d = dist3(seg_point(c0, c1, s), seg_point(c2, c3, u));
+        if (d - 2.0f * th < -1e-6f) {
+            V3 cr = cross(sub(p1, p0), sub(p3, p2));
+            *t_out = (float)t[i];
+            *n_out = dot(cr, cr) > 0.0f ? vnormalize(cr) : V3{0.0f, 0.0f, 0.0f};
+            w_out[0] = 1.0f - s;
+            w_out[1] = s;
+            w_out[2] = 1.0f - u;
+            w_out[3] = u;
+            return true;
+        }
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(256) void k_narrow_ee(const uint32_t* __restrict__ ua, const uint32_t* __restrict__ ub,
+                                                   int n, const int2* __restrict__ edges,
+                                                   const float4* __restrict__ x, const float4* __restrict__ p,
+                                                   float th, int* __restrict__ flag, float* __restrict__ ct,
+                                                   float* __restrict__ cn, double* __restrict__ cw) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int2 ea = edges[ua[i]], eb = edges[ub[i]];
+    float t, w[4];
+    V3 nr;
+    bool hit = ee_ccd(v3(x[ea.x]), v3(x[ea.y]), v3(x[eb.x]), v3(x[eb.y]), v3(p[ea.x]), v3(p[ea.y]), v3(p[eb.x]),
+                      v3(p[eb.y]), th, &t, &nr, w);
+    flag[i] = hit ? 1 : 0;
+    if (hit) {
+        ct[i] = t;
+        cn[3 * i] = nr.x;
+        cn[3 * i + 1] = nr.y;
+        cn[3 * i + 2] = nr.z;
+        for (int k = 0; k < 4; k++) cw[4 * i + k] = w[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact_ee(int n, const int* __restrict__ flag, const int* __restrict__ pos,
+                                                    const uint32_t* __restrict__ ua, const uint32_t* __restrict__ ub,
+                                                    const int2* __restrict__ edges, const float* __restrict__ ct,
+                                                    const float* __restrict__ cn, const double* __restrict__ cw,
+                                                    int4* __restrict__ oq, float* __restrict__ ot,
+                                                    float* __restrict__ on, double* __restrict__ ow) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    int o = pos[i];
+    int2 ea = edges[ua[i]], eb = edges[ub[i]];
+    oq[o] = make_int4(ea.x, ea.y, eb.x, eb.y);
+    ot[o] = ct[i];
+    for (int k = 0; k < 3; k++) on[3 * o + k] = cn[3 * i + k];
+    for (int k = 0; k < 4; k++) ow[4 * o + k] = cw[4 * i + k];
+}
+
+// Edge-edge response on the current positions: the closest points of the
+// two edges are pushed apart along the edge normal (oriented as the closest
+// points' offset at the start of the step) until they are 2 x thickness
+// apart, half each, spread over each edge's ends by the closest-point weights
+// (w_k / sum w^2, so the closest point moves by the full half); a static edge
+// leaves the whole gap to the other; near-parallel edges (sin^2 of the angle
+// <= 1e-12) are skipped.  Fixes are summed in the vertex-face response's
+// fixed-point accumulators.
+__device__ bool ee_fix(const V3* X, const V3* q, float th, V3* d) {
+    float s0, u0;
+    seg_closest(X[0], X[1], X[2], X[3], &s0, &u0);
+    V3 side = sub(seg_point(X[0], X[1], s0), seg_point(X[2], X[3], u0));
+    V3 ea = sub(q[1], q[0]), eb = sub(q[3], q[2]);
+    V3 cr = cross(ea, eb);
+    if (dot(cr, cr) <= 1e-12f * (dot(ea, ea) * dot(eb, eb))) return false;
+    V3 nn = vnormalize(cr);
+    if (dot(side, nn) < 0.0f) nn = V3{-nn.x, -nn.y, -nn.z};
+    float s, u;
+    seg_closest(q[0], q[1], q[2], q[3], &s, &u);
+    float gap = dot(sub(seg_point(q[0], q[1], s), seg_point(q[2], q[3], u)), nn);
+    if (!(gap - 2.0f * th < -1e-6f)) return false;
+    float depth = (2.0f * th - gap) * 0.5f;
+    float w[4] = {1.0f - s, s, 1.0f - u, u};
+    float sa = w[0] * w[0] + w[1] * w[1], sb = w[2] * w[2] + w[3] * w[3];
+    for (int k = 0; k < 4; k++) {
+        float f = k < 2 ? (depth * w[k]) / sa : -((depth * w[k]) / sb);
+        d[k] = V3{nn.x * f, nn.y * f, nn.z * f};
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_resolve_ee_acc(const int4* __restrict__ cq, int nc,
+                                                        const float4* __restrict__ x, const float4* __restrict__ p,
+                                                        float th, unsigned long long* __restrict__ acc,
+                                                        int* __restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    int4 c = cq[i];
+    int id[4] = {c.x, c.y, c.z, c.w};
+    float4 P[4];
+    V3 X[4], q[4], d[4];
+    for (int k = 0; k < 4; k++) {
+        P[k] = p[id[k]];
+        X[k] = v3(x[id[k]]);
+        q[k] = v3(P[k]);
+    }
+    if (!ee_fix(X, q, th, d)) return;
+    bool a_static = P[0].w == 0.0f && P[1].w == 0.0f, b_static = P[2].w == 0.0f && P[3].w == 0.0f;
+    for (int k = 0; k < 4; k++) {
+        if (P[k].w == 0.0f) continue;
+        float sc = (k < 2 ? b_static : a_static) ? 2.0f : 1.0f;
+        atomicAdd(&acc[3 * (size_t)id[k]], (unsigned long long)to_fixed(d[k].x * sc));
+        atomicAdd(&acc[3 * (size_t)id[k] + 1], (unsigned long long)to_fixed(d[k].y * sc));
+        atomicAdd(&acc[3 * (size_t)id[k] + 2], (unsigned long long)to_fixed(d[k].z * sc));
+        atomicAdd(&cnt[id[k]], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// response
+// ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_resolve_acc(const int* __restrict__ cvv, const int* __restrict__ cff, int nc,
                                                      const int* __restrict__ tris, const float4* __restrict__ x,
@@ -698,6 +946,15 @@ long long Broadphase::pairs(hipStream_t s, const float4* p, const int* tris, int
 
 // Sorts the `total` raw pairs in pv/pf by (v, f) and keeps one of each in uv/uf.
 long long Broadphase::unique_pairs(hipStream_t s, int total, int nt, int nv) {
+    long long uniq = unique_into(s, total, nt, nv, uv, uf);
+    npairs = uniq;
+    st_pairs += uniq;
+    return uniq;
+}
+
+// Sorts the `total` raw pairs in pv/pf by (pv, pf) (pv < nv, pf < nt) and
+// writes one of each to oa/ob.
+long long Broadphase::unique_into(hipStream_t s, int total, int nt, int nv, DBuf<uint32_t>& oa, DBuf<uint32_t>& ob) {
     // sort by (v, f): by f first, then stably by v
     int fb = 1, vb = 1;
     while (fb < 31 && (1ll << fb) < nt) fb++;
@@ -712,14 +969,80 @@ long long Broadphase::unique_pairs(hipStream_t s, int total, int nt, int nv) {
     int uniq = 0;
     PBD_HIP(hipMemcpyAsync(&uniq, pos.ptr + total, sizeof(int), hipMemcpyDeviceToHost, s));
     PBD_HIP(hipStreamSynchronize(s));
-    grow(uv, std::max(1, uniq));
-    grow(uf, std::max(1, uniq));
+    grow(oa, std::max(1, uniq));
+    grow(ob, std::max(1, uniq));
     hipLaunchKernelGGL(k_compact_pairs, dim3(blocks(total)), dim3(256), 0, s, pv.ptr, pf.ptr, total, flag.ptr, pos.ptr,
-                       uv.ptr, uf.ptr);
+                       oa.ptr, ob.ptr);
     PBD_HIP(hipGetLastError());
-    npairs = uniq;
-    st_pairs += uniq;
     return uniq;
+}
+
+void Broadphase::set_edges(hipStream_t s, const std::vector<int>& te, const std::vector<int2>& ed) {
+    tedge.upload(te, s);
+    edges.upload(ed, s);
+    nedges = (int)ed.size();
+}
+
+// Edge-edge candidates from the cell table of the last build() (see k_ee_pairs).
+long long Broadphase::ee_pairs(hipStream_t s, const float4* p, const int* tris) {
+    ee_npairs = 0;
+    ee_ncontacts = 0;
+    if (nt <= 0) return 0;
+    PBD_CHECK(tedge.n >= (size_t)3 * nt, PBD_ERR_STATE, "internal: edge table does not match the triangles");
+    grow(cnt, nt + 1);
+    grow(off, nt + 1);
+    grow(scan_tmp, (nt + SB) / SB + 2);
+    hipLaunchKernelGGL(k_ee_pairs<0>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr,
+                       cstart.ptr, cend.ptr, tcell.ptr, tedge.ptr, edges.ptr, cnt.ptr, (uint32_t*)nullptr,
+                       (uint32_t*)nullptr);
+    PBD_HIP(hipMemsetAsync(cnt.ptr + nt, 0, sizeof(int), s));
+    device_scan(s, cnt.ptr, off.ptr, nt + 1, scan_tmp.ptr);
+    int total = 0;
+    PBD_HIP(hipMemcpyAsync(&total, off.ptr + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipStreamSynchronize(s));
+    PBD_CHECK(total >= 0, PBD_ERR_LIMIT, "edge-edge candidate count overflows int32");
+    st_ee_raw += total;
+    if (total == 0) return 0;
+    grow(pv, total);
+    grow(pf, total);
+    grow(t1, total);
+    grow(t2, total);
+    grow(flag, total + 1);
+    grow(pos, total + 1);
+    hipLaunchKernelGGL(k_ee_pairs<1>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr,
+                       cstart.ptr, cend.ptr, tcell.ptr, tedge.ptr, edges.ptr, off.ptr, pv.ptr, pf.ptr);
+    ee_npairs = unique_into(s, total, nedges, nedges, eua, eub);
+    st_ee_pairs += ee_npairs;
+    return ee_npairs;
+}
+
+long long Broadphase::ee_narrow(hipStream_t s, const float4* x, const float4* p, float th) {
+    ee_ncontacts = 0;
+    if (ee_npairs <= 0) return 0;
+    long long n = ee_npairs;
+    grow(flag, n + 1);
+    grow(pos, n + 1);
+    grow(ctmp_t, n);
+    grow(ctmp_n, 3 * n);
+    grow(ctmp_w, 4 * n);
+    hipLaunchKernelGGL(k_narrow_ee, dim3(blocks(n)), dim3(256), 0, s, eua.ptr, eub.ptr, (int)n, edges.ptr, x, p, th,
+                       flag.ptr, ctmp_t.ptr, ctmp_n.ptr, ctmp_w.ptr);
+    PBD_HIP(hipMemsetAsync(flag.ptr + n, 0, sizeof(int), s));
+    grow(scan_tmp, (n + 1 + SB) / SB + 2);
+    device_scan(s, flag.ptr, pos.ptr, (int)n + 1, scan_tmp.ptr);
+    int nc = 0;
+    PBD_HIP(hipMemcpyAsync(&nc, pos.ptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    PBD_HIP(hipStreamSynchronize(s));
+    grow(ecq, std::max(1, nc));
+    grow(ect, std::max(1, nc));
+    grow(ecn, 3 * std::max(1, nc));
+    grow(ecw, 4 * std::max(1, nc));
+    hipLaunchKernelGGL(k_compact_ee, dim3(blocks(n)), dim3(256), 0, s, (int)n, flag.ptr, pos.ptr, eua.ptr, eub.ptr,
+                       edges.ptr, ctmp_t.ptr, ctmp_n.ptr, ctmp_w.ptr, ecq.ptr, ect.ptr, ecn.ptr, ecw.ptr);
+    PBD_HIP(hipGetLastError());
+    ee_ncontacts = nc;
+    st_ee_contacts += nc;
+    return nc;
 }
 
 long long Broadphase::pairs_swept(hipStream_t s, const HashCfg& g, const float4* x, const float4* p, const int* tris,
@@ -822,15 +1145,19 @@ long long Broadphase::narrow(hipStream_t s, const float4* x, const float4* p, co
 
 void Broadphase::resolve(hipStream_t s, const float4* x, float4* p, const int* tris, int nv, float th, int passes,
                          float min_n2) {
-    if (ncontacts <= 0) return;
-    st_resolved += ncontacts * passes;
+    if (ncontacts <= 0 && ee_ncontacts <= 0) return;
+    st_resolved += (ncontacts + ee_ncontacts) * passes;
     grow(acc, (size_t)3 * nv);
     grow(vcnt, nv);
     for (int ps = 0; ps < passes; ps++) {
         PBD_HIP(hipMemsetAsync(acc.ptr, 0, sizeof(unsigned long long) * 3 * (size_t)nv, s));
         PBD_HIP(hipMemsetAsync(vcnt.ptr, 0, sizeof(int) * (size_t)nv, s));
-        hipLaunchKernelGGL(k_resolve_acc, dim3(blocks(ncontacts)), dim3(256), 0, s, cv.ptr, cf.ptr, (int)ncontacts, tris, x,
-                           (const float4*)p, th, min_n2, acc.ptr, vcnt.ptr);
+        if (ncontacts > 0)
+            hipLaunchKernelGGL(k_resolve_acc, dim3(blocks(ncontacts)), dim3(256), 0, s, cv.ptr, cf.ptr, (int)ncontacts,
+                               tris, x, (const float4*)p, th, min_n2, acc.ptr, vcnt.ptr);
+        if (ee_ncontacts > 0)
+            hipLaunchKernelGGL(k_resolve_ee_acc, dim3(blocks(ee_ncontacts)), dim3(256), 0, s, ecq.ptr,
+                               (int)ee_ncontacts, x, (const float4*)p, th, acc.ptr, vcnt.ptr);
         hipLaunchKernelGGL(k_resolve_apply, dim3(blocks(nv)), dim3(256), 0, s, acc.ptr, vcnt.ptr, p, nv);
     }
     PBD_HIP(hipGetLastError());

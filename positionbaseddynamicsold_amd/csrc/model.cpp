@@ -113,6 +113,21 @@ void mesh_edge_pairs(const std::vector<int>& tris, std::vector<int>& pairs) {
 // triangle's opposite vertex; emitted at the second sighting.  Bridson,
 // Marino, Fedkiw 2003, sec. 4.1 (the reference's InitBendingConstr is an
 // empty stub, PBD_cuda.cu:110-123).
+void build_edge_table(const int* tris, int nt, std::vector<int>& tedge, std::vector<int2>& edges) {
+    std::unordered_map<uint64_t, int> id;
+    id.reserve((size_t)3 * nt);
+    tedge.assign((size_t)3 * nt, -1);
+    edges.clear();
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++) {
+            int a = tris[3 * t + k], b = tris[3 * t + (k + 1) % 3];
+            int lo = a < b ? a : b, hi = a < b ? b : a;
+            auto it = id.emplace(edge_key(lo, hi), (int)edges.size());
+            if (it.second) edges.push_back(make_int2(lo, hi));
+            tedge[3 * t + k] = it.first->second;
+        }
+}
+
 void bending_quads(const std::vector<int>& tris, std::vector<int>& quads) {
     quads.clear();
     struct E { int a, b, o, n; };

@@ -204,6 +204,10 @@ void read_txt_cloth(const char* topol, const char* constr, std::vector<float>& p
 void write_cache(const Cloth& c, const float* x, const float* v, const float* p, const char* mesh_path,
                  const char* constr_path, int precision);
 void read_cache(const char* mesh_path, const char* constr_path, Cloth& c);
+// Edges of a triangle list in first-sighting order (triangle by triangle,
+// edges (v0,v1), (v1,v2), (v2,v0)), endpoints (lower, higher); tedge[3t+k]
+// is the id of triangle t's edge k.
+void build_edge_table(const int* tris, int nt, std::vector<int>& tedge, std::vector<int2>& edges);
 
 // Broadphase grid: dense (reference) or hashed table.
 struct HashCfg {
@@ -232,9 +236,26 @@ struct Broadphase {
     // running counts (host values the drivers read anyway): detections,
     // triangles, cell entries, raw pairs, distinct pairs, contacts
     long long st_dets = 0, st_tris = 0, st_entries = 0, st_raw = 0, st_pairs = 0, st_contacts = 0, st_resolved = 0;
+    // edge-edge: per-triangle edge ids, edge endpoints (lower, higher), the
+    // distinct candidate edge pairs and the contacts (a0, a1, b0, b1)
+    DBuf<int> tedge;
+    DBuf<int2> edges;
+    int nedges = 0;
+    DBuf<uint32_t> eua, eub;
+    DBuf<int4> ecq;
+    DBuf<float> ect, ecn;
+    DBuf<double> ecw;
+    long long ee_npairs = 0, ee_ncontacts = 0;
+    long long st_ee_raw = 0, st_ee_pairs = 0, st_ee_contacts = 0;
     void build(hipStream_t s, const HashCfg& g, const float4* p, const int* tris, int nt);
     long long pairs(hipStream_t s, const float4* p, const int* tris, int nv);
     long long unique_pairs(hipStream_t s, int total, int nt, int nv);
+    long long unique_into(hipStream_t s, int total, int nt, int nv, DBuf<uint32_t>& oa, DBuf<uint32_t>& ob);
+    void set_edges(hipStream_t s, const std::vector<int>& tedge, const std::vector<int2>& edges);
+    // after build(): edge-edge candidates and contacts (contacts resolved with
+    // the vertex-face ones by resolve())
+    long long ee_pairs(hipStream_t s, const float4* p, const int* tris);
+    long long ee_narrow(hipStream_t s, const float4* x, const float4* p, float th);
     // Swept-box candidate set (replaces build + pairs); x = start of substep.
     long long pairs_swept(hipStream_t s, const HashCfg& g, const float4* x, const float4* p, const int* tris, int nt,
                           int nv, float th);
