@@ -2,7 +2,7 @@
 kernel traces and hardware counters.
 
     python -m positionbaseddynamicsold_amd.collidework [--config 3] [--mode robust|reference]
-        [--settle N] [--substeps K]
+        [--edge-edge] [--settle N] [--substeps K]
 
 Builds the scene (config 3: the 512 x 512 folded cloth with self contact over
 the 0.4 m box mesh; config 2: the 256 x 256 cloth with bending over the
@@ -10,7 +10,8 @@ icosphere), runs N substeps until contact, then K substeps with the contact
 counters reset, copies 1 GiB with the calibration copy kernel, and prints one
 JSON line: scene sizes, hash table, and per-run totals of detections,
 triangles hashed, cell entries, raw and distinct candidate pairs, contacts,
-response passes, plus the wall time per substep.  positionbaseddynamicsold_amd
+response passes (and the edge-edge candidates and contacts with --edge-edge),
+plus the wall time per substep.  positionbaseddynamicsold_amd
 .tracestats --collide turns a kernel trace (and FETCH_SIZE / WRITE_SIZE
 counter runs) of it into per-kernel time and HBM rate.
 """
@@ -19,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import time
+from dataclasses import replace
 
 from . import capi, scenes
 
@@ -30,10 +32,13 @@ def main():
     ap.add_argument("--settle", type=int, default=0, help="substeps before the counted ones (a kernel trace "
                     "then holds more detections than the counts)")
     ap.add_argument("--substeps", type=int, default=24)
+    ap.add_argument("--edge-edge", action="store_true", help="edge-edge contacts too")
     a = ap.parse_args()
     sc = scenes.config3() if a.config == 3 else scenes.config2()
     if a.mode == "robust":
         sc = sc.robust()
+    if a.edge_edge:
+        sc = replace(sc, edge_edge=1)
     ctx = capi.Context(0)
     cid, sid, cell = scenes.add_ccd_scene(ctx, sc)
     if a.settle > 0:
@@ -56,7 +61,8 @@ def main():
         "resolve_passes": sc.resolve_passes, "table_bits": sc.table_bits, "cell": cell,
         "detections": prof.ccd_detections, "triangles": prof.ccd_triangles, "cell_entries": prof.ccd_cell_entries,
         "raw_pairs": prof.ccd_raw_pairs, "pairs": prof.ccd_pairs, "contacts": prof.ccd_contacts,
-        "resolved": prof.ccd_resolved,
+        "resolved": prof.ccd_resolved, "edge_edge": int(a.edge_edge), "ee_raw_pairs": prof.ccd_ee_raw_pairs,
+        "ee_pairs": prof.ccd_ee_pairs, "ee_contacts": prof.ccd_ee_contacts,
         "ms_per_substep": ms, "copy_ms": copy_ms,
     }
     print(json.dumps(out), flush=True)

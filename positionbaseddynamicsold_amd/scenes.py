@@ -156,6 +156,7 @@ class CcdScene:
     min_normal2: float = 0.0
     # candidate pairs: 0 = the reference's centroid cells, 1 = swept boxes
     broadphase: int = 0
+    edge_edge: int = 0            # 1: edge-edge contacts too
 
     def robust(self):
         """The same scene with the swept-box broadphase and a degenerate-face
@@ -266,5 +267,5 @@ def add_ccd_scene(ctx, sc: CcdScene, graph: int = 0):
     ctx.set_solver(dt=DT, iterations=sc.iterations, graph=graph)
     ctx.set_collision(1, capi.hash_desc(1, cell=(cell, cell, cell), table_bits=sc.table_bits), cell_scale=0.0,
                       thickness=sc.thickness, resolve_passes=sc.resolve_passes, detect_every=sc.detect_every,
-                      min_normal2=sc.min_normal2, broadphase=sc.broadphase)
+                      min_normal2=sc.min_normal2, broadphase=sc.broadphase, edge_edge=sc.edge_edge)
     return cid, sid, cell

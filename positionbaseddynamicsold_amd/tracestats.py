@@ -21,8 +21,9 @@ per launch).  With --collide (the JSON line of positionbaseddynamicsold_amd
 .collidework), it summarises that program's trace instead: every kernel's
 time and, with counter files of the same program, HBM bytes and GB/s per
 kernel, plus the algorithmic rate of the kernels whose bytes per unit are
-fixed (hash keys 56 B per triangle, narrowphase 104 B per candidate, response
-160 B per contact and pass).  With --tilework (that run's JSON line) and --update-json it
+fixed (hash keys 56 B per triangle, narrowphase 104 B per candidate,
+edge-edge narrowphase 152 B per candidate, response 160 B per contact and
+pass).  With --tilework (that run's JSON line) and --update-json it
 also records the figures in the JSON file under each pass's launch
 configuration ("ts<sweeps>_bt<threads>_mv<tile vertices>"), which bench.py
 reads for its roofline traffic.
@@ -137,7 +138,8 @@ def main():
 HBM_PEAK = 8.0e12
 
 # algorithmic bytes per unit (work-file count) of the kernels with a fixed per-unit figure
-COLLIDE_ALG = {"k_tri_keys": (56, "triangles"), "k_narrow": (104, "pairs"), "k_resolve_acc": (160, "resolved")}
+COLLIDE_ALG = {"k_tri_keys": (56, "triangles"), "k_narrow": (104, "pairs"), "k_resolve_acc": (160, "resolved"),
+               "k_narrow_ee": (152, "ee_pairs")}
 
 
 def collide_summary(trace, work_path, fetch=None, write=None):
@@ -161,6 +163,10 @@ def collide_summary(trace, work_path, fetch=None, write=None):
     print(f"config {work['config']} ({work['mode']} contact handling): {work['cloth_vertices']} cloth vertices, "
           f"{work['cloth_triangles'] + work['static_triangles']} triangles, table 2^{work['table_bits']}, "
           f"{work['substeps']} substeps x {work['sweeps']} sweeps, {work['detections']} detections")
+    if work.get("edge_edge"):
+        print(f"edge-edge per detection: {work['ee_raw_pairs'] / max(1, work['detections']):.0f} raw pairs, "
+              f"{work['ee_pairs'] / max(1, work['detections']):.0f} distinct pairs, "
+              f"{work['ee_contacts'] / max(1, work['detections']):.0f} contacts")
     print(f"per detection: {work['cell_entries'] / max(1, work['detections']):.0f} cell entries, "
           f"{work['raw_pairs'] / max(1, work['detections']):.0f} raw pairs, "
           f"{work['pairs'] / max(1, work['detections']):.0f} distinct pairs, "
