@@ -1417,15 +1417,36 @@ static void seg_point(const float* a0, const float* a1, float s, float* r) {
     for (int k = 0; k < 3; k++) r[k] = a0[k] + s * (a1[k] - a0[k]);
 }
 
-/* Edge (x0,x1)->(p0,p1) against edge (x2,x3)->(p2,p3): the coplanarity
- * times of the four points (the cubic of orc_vf_ccd with the first edge's
- * start point as origin; Provot 1997) plus 0 and 1, ascending; the first
- * time in [0, 1] at which the segments are within 2 x thickness is the
- * contact: t, the end-state edge normal (p1-p0) x (p3-p2) normalised (0 if
- * parallel), weights (1-s, s, 1-u, u). */
+/* Whether two edges can be in contact: not near parallel (sin^2 > 1e-12)
+ * and, at the start positions X, at least one thickness apart along the
+ * normal of the edges (q1-q0) x (q3-q2).  side = offset of the closest
+ * points at the start, nn = the normal. */
+static int ee_separable(const float* X0, const float* X1, const float* X2, const float* X3, const float* q0,
+                        const float* q1, const float* q2, const float* q3, float thickness, float* side,
+                        float* nn) {
+    float s0, u0, pa[3], pb[3], ea[3], eb[3], cr[3];
+    orc_seg_closest(X0, X1, X2, X3, &s0, &u0);
+    seg_point(X0, X1, s0, pa);
+    seg_point(X2, X3, u0, pb);
+    v3sub(pa, pb, side);
+    v3sub(q1, q0, ea); v3sub(q3, q2, eb); v3cross(ea, eb, cr);
+    if (v3dot(cr, cr) <= 1e-12f * (v3dot(ea, ea) * v3dot(eb, eb))) return 0;
+    v3normalize(cr, nn);
+    if (!(fabsf(v3dot(side, nn)) >= thickness)) return 0;
+    return 1;
+}
+
+/* Edge (x0,x1)->(p0,p1) against edge (x2,x3)->(p2,p3), for separable edges
+ * (ee_separable at the end state): the coplanarity times of the four points
+ * (the cubic of orc_vf_ccd with the first edge's start point as origin;
+ * Provot 1997) plus 0 and 1, ascending; the first time in [0, 1] at which
+ * the segments are within 2 x thickness is the contact: t, the end-state
+ * edge normal (p1-p0) x (p3-p2) normalised, weights (1-s, s, 1-u, u). */
 int orc_ee_ccd(const float* x0, const float* x1, const float* x2, const float* x3,
                const float* p0, const float* p1, const float* p2, const float* p3,
                float thickness, float* t_out, float* n_out, double* w_out) {
+    float side[3], nrm[3];
+    if (!ee_separable(x0, x1, x2, x3, p0, p1, p2, p3, thickness, side, nrm)) return 0;
     float v0[3], X1[3], X2[3], X3[3], V1[3], V2[3], V3[3], tmp[3];
     v3sub(p0, x0, v0);
     v3sub(x1, x0, X1); v3sub(x2, x0, X2); v3sub(x3, x0, X3);
@@ -1453,13 +1474,8 @@ int orc_ee_ccd(const float* x0, const float* x1, const float* x2, const float* x
         seg_point(c2, c3, u, pb);
         float d = orc_distance(pa, pb);
         if (d - 2.0f * thickness < -1e-6f) {
-            float e1[3], e2[3], cr[3];
-            v3sub(p1, p0, e1); v3sub(p3, p2, e2); v3cross(e1, e2, cr);
             if (t_out) *t_out = (float)t[i];
-            if (n_out) {
-                if (v3dot(cr, cr) > 0.0f) v3normalize(cr, n_out);
-                else { n_out[0] = 0.0f; n_out[1] = 0.0f; n_out[2] = 0.0f; }
-            }
+            if (n_out) { n_out[0] = nrm[0]; n_out[1] = nrm[1]; n_out[2] = nrm[2]; }
             if (w_out) { w_out[0] = 1.0f - s; w_out[1] = s; w_out[2] = 1.0f - u; w_out[3] = u; }
             return 1;
         }
@@ -1468,20 +1484,14 @@ int orc_ee_ccd(const float* x0, const float* x1, const float* x2, const float* x
 }
 
 /* Edge-edge response on the current positions q (start positions X orient
- * it): the closest points are pushed apart along the edge normal, oriented
- * as their offset at the start, until 2 x thickness apart, half each, spread
- * over each edge's ends by w_k / sum w^2; near-parallel edges (sin^2 <=
- * 1e-12) are skipped.  d: 4 displacements (12 floats).  Returns 0 if none. */
+ * it), for separable edges (ee_separable): the closest points are pushed
+ * apart along the edge normal, oriented as their offset at the start, until
+ * 2 x thickness apart, half each, spread over each edge's ends by
+ * w_k / sum w^2.  d: 4 displacements (12 floats).  Returns 0 if none. */
 int orc_ee_fix(const float* X0, const float* X1, const float* X2, const float* X3, const float* q0,
                const float* q1, const float* q2, const float* q3, float thickness, float* d) {
-    float s0, u0, pa[3], pb[3], side[3], ea[3], eb[3], cr[3], nn[3], rel[3], s, u;
-    orc_seg_closest(X0, X1, X2, X3, &s0, &u0);
-    seg_point(X0, X1, s0, pa);
-    seg_point(X2, X3, u0, pb);
-    v3sub(pa, pb, side);
-    v3sub(q1, q0, ea); v3sub(q3, q2, eb); v3cross(ea, eb, cr);
-    if (v3dot(cr, cr) <= 1e-12f * (v3dot(ea, ea) * v3dot(eb, eb))) return 0;
-    v3normalize(cr, nn);
+    float pa[3], pb[3], side[3], nn[3], rel[3], s, u;
+    if (!ee_separable(X0, X1, X2, X3, q0, q1, q2, q3, thickness, side, nn)) return 0;
     if (v3dot(side, nn) < 0.0f) { nn[0] = -nn[0]; nn[1] = -nn[1]; nn[2] = -nn[2]; }
     orc_seg_closest(q0, q1, q2, q3, &s, &u);
     seg_point(q0, q1, s, pa);

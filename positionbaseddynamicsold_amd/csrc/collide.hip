@@ -660,15 +660,37 @@ __device__ __forceinline__ V3 seg_point(V3 a0, V3 a1, float s) {
     return V3{a0.x + s * (a1.x - a0.x), a0.y + s * (a1.y - a0.y), a0.z + s * (a1.z - a0.z)};
 }
 
-// Edge (x0, x1) -> (p0, p1) against edge (x2, x3) -> (p2, p3) over the step:
-// the times the four points are coplanar (the cubic of vf_ccd, with the
-// first edge's start point as origin; Provot 1997) plus t = 0 and 1,
-// ascending; the first time in [0, 1] where the segments come within
-// 2 x thickness is the contact.  Records t, the end-state edge normal
-// (p1 - p0) x (p3 - p2) normalised (0 when parallel) and the weights
-// (1 - s, s, 1 - u, u) of the closest points.
+// Whether two edges can be in contact: not near parallel (sin^2 of the
+// angle > 1e-12) and, at the start of the step, at least one thickness apart
+// along the normal n of the edges (q1 - q0) x (q3 - q2).  Edges that start
+// side by side (neighbours in one sheet whose spacing is below the contact
+// gap) have no separation along n to restore and are not contacts.
+__device__ bool ee_separable(V3 X0, V3 X1, V3 X2, V3 X3, V3 q0, V3 q1, V3 q2, V3 q3, float th, V3* side_out,
+                             V3* n_out) {
+    float s0, u0;
+    seg_closest(X0, X1, X2, X3, &s0, &u0);
+    V3 side = sub(seg_point(X0, X1, s0), seg_point(X2, X3, u0));
+    V3 ea = sub(q1, q0), eb = sub(q3, q2);
+    V3 cr = cross(ea, eb);
+    if (dot(cr, cr) <= 1e-12f * (dot(ea, ea) * dot(eb, eb))) return false;
+    V3 nn = vnormalize(cr);
+    if (!(fabsf(dot(side, nn)) >= th)) return false;
+    *side_out = side;
+    *n_out = nn;
+    return true;
+}
+
+// Edge (x0, x1) -> (p0, p1) against edge (x2, x3) -> (p2, p3) over the step,
+// for separable edges (ee_separable at the end state): the times the four
+// points are coplanar (the cubic of vf_ccd, with the first edge's start
+// point as origin; Provot 1997) plus t = 0 and 1, ascending; the first time
+// in [0, 1] where the segments come within 2 x thickness is the contact.
+// Records t, the end-state edge normal (p1 - p0) x (p3 - p2) normalised and
+// the weights (1 - s, s, 1 - u, u) of the closest points.
 __device__ bool ee_ccd(V3 x0, V3 x1, V3 x2, V3 x3, V3 p0, V3 p1, V3 p2, V3 p3, float th, float* t_out, V3* n_out,
                        float* w_out) {
+    V3 side, nrm;
+    if (!ee_separable(x0, x1, x2, x3, p0, p1, p2, p3, th, &side, &nrm)) return false;
     V3 v0 = sub(p0, x0);
     V3 X1 = sub(x1, x0), X2 = sub(x2, x0), X3 = sub(x3, x0);
     V3 V1 = sub(sub(p1, x1), v0), V2 = sub(sub(p2, x2), v0), V3_ = sub(sub(p3, x3), v0);
@@ -700,9 +722,8 @@ __device__ bool ee_ccd(V3 x0, V3 x1, V3 x2, V3 x3, V3 p0, V3 p1, V3 p2, V3 p3, f
         seg_closest(c0, c1, c2, c3, &s, &u);
         float d = dist3(seg_point(c0, c1, s), seg_point(c2, c3, u));
         if (d - 2.0f * th < -1e-6f) {
-            V3 cr = cross(sub(p1, p0), sub(p3, p2));
             *t_out = (float)t[i];
-            *n_out = dot(cr, cr) > 0.0f ? vnormalize(cr) : V3{0.0f, 0.0f, 0.0f};
+            *n_out = nrm;
             w_out[0] = 1.0f - s;
             w_out[1] = s;
             w_out[2] = 1.0f - u;
@@ -751,22 +772,17 @@ __global__ __launch_bounds__(256) void k_compact_ee(int n, const int* __restrict
     for (int k = 0; k < 4; k++) ow[4 * o + k] = cw[4 * i + k];
 }
 
-// Edge-edge response on the current positions: the closest points of the
-// two edges are pushed apart along the edge normal (oriented as the closest
-// points' offset at the start of the step) until they are 2 x thickness
-// apart, half each, spread over each edge's ends by the closest-point weights
-// (w_k / sum w^2, so the closest point moves by the full half); a static edge
-// leaves the whole gap to the other; near-parallel edges (sin^2 of the angle
-// <= 1e-12) are skipped.  Fixes are summed in the vertex-face response's
-// fixed-point accumulators.
+// Edge-edge response on the current positions, for separable edges
+// (ee_separable): the closest points of the two edges are pushed apart along
+// the edge normal (oriented as the closest points' offset at the start of
+// the step) until they are 2 x thickness apart, half each, spread over each
+// edge's ends by the closest-point weights (w_k / sum w^2, so the closest
+// point moves by the full half); a static edge leaves the whole gap to the
+// other.  Fixes are summed in the vertex-face response's fixed-point
+// accumulators.
 __device__ bool ee_fix(const V3* X, const V3* q, float th, V3* d) {
-    float s0, u0;
-    seg_closest(X[0], X[1], X[2], X[3], &s0, &u0);
-    V3 side = sub(seg_point(X[0], X[1], s0), seg_point(X[2], X[3], u0));
-    V3 ea = sub(q[1], q[0]), eb = sub(q[3], q[2]);
-    V3 cr = cross(ea, eb);
-    if (dot(cr, cr) <= 1e-12f * (dot(ea, ea) * dot(eb, eb))) return false;
-    V3 nn = vnormalize(cr);
+    V3 side, nn;
+    if (!ee_separable(X[0], X[1], X[2], X[3], q[0], q[1], q[2], q[3], th, &side, &nn)) return false;
     if (dot(side, nn) < 0.0f) nn = V3{-nn.x, -nn.y, -nn.z};
     float s, u;
     seg_closest(q[0], q[1], q[2], q[3], &s, &u);

@@ -149,3 +149,33 @@ def test_cloth_stays_on_a_thin_rod_only_with_edge_edge(orc):
     # with them the edges across the rod rest on it
     assert mid[0] < -0.1
     assert mid[1] > -0.02
+
+
+def test_config3_full_size_with_edge_edge():
+    # 512 x 512 folded cloth over the 0.4 m box, robust contact handling plus
+    # edge-edge contacts: finite, no vertex inside the box, identical on a
+    # second run
+    from dataclasses import replace
+    say("config3 full size, robust + edge-edge, 24 substeps")
+
+    def run(ee):
+        sc = replace(scenes.config3().robust(), edge_edge=ee)
+        with capi.Context() as ctx:
+            cid, _, _ = scenes.add_ccd_scene(ctx, sc)
+            ctx.profile(True)
+            ctx.step(24)
+            prof = ctx.profile_read()
+            return ctx.get(cid), ctx.get(cid, capi.STATE_V), prof
+
+    x, v, prof = run(1)
+    assert np.isfinite(x).all() and np.isfinite(v).all()
+    inside = (np.abs(x[:, 0]) < 0.2 - 1e-3) & (np.abs(x[:, 2]) < 0.2 - 1e-3) & (x[:, 1] < 0.4 - 1e-3) & \
+        (x[:, 1] > 1e-3)
+    assert not inside.any()
+    assert prof.ccd_ee_contacts > 0
+    x2, v2, _ = run(1)
+    assert np.array_equal(x, x2) and np.array_equal(v, v2)
+    x0, v0, prof0 = run(0)
+    say(f"  edge-edge contacts {prof.ccd_ee_contacts}, vertex-face {prof.ccd_contacts} (without edge-edge "
+        f"{prof0.ccd_contacts}); max |v| {np.abs(v).max():.3f} m/s (without {np.abs(v0).max():.3f}); "
+        f"max |x - x without| {np.abs(x - x0).max():.4f} m")

@@ -7,6 +7,8 @@ mathematics rather than reference vectors:
 - misses: edges that pass beside each other, edges already far apart;
 - the response: a static edge leaves the whole gap to the other edge, which
   ends exactly 2 x thickness away; two free edges move symmetrically;
+- edges side by side in one sheet (no separation along their normal at the
+  start) are neither contacts nor pushed apart;
 - the candidate set equals an independent numpy brute force over triangle
   pairs whose centroid cells are at most one cell apart, and never pairs
   edges that share a vertex.
@@ -87,6 +89,18 @@ def test_response_is_oriented_by_the_start_side(orc):
     q = seg((-1, 0, 0), (1, 0, 0)) + seg((0, 0.002, -1), (0, 0.002, 1))
     d = orc.ee_fix(X, q, th)
     assert d is not None and (d[2:, 1] < 0).all() and (d[:2, 1] > 0).all()
+
+
+def test_side_by_side_edges_are_not_contacts(orc):
+    # two edges of one flat sheet, 1 mm apart in its plane: within the contact
+    # gap, but with no separation along their normal to restore
+    x = seg((0, 0, 0), (1, 0, 0)) + seg((0.5, 0, 0.001), (0.5, 0, 1))
+    p = [v + np.array([0, -0.01, 0], F32) for v in x]
+    assert not orc.ee_ccd(x, p, 0.002)[0]
+    assert orc.ee_fix(x, p, 0.002) is None
+    # the same edges starting 3 mm apart across the sheet normal are
+    x2 = seg((0, 0, 0), (1, 0, 0)) + seg((0.5, 0.003, 0.001), (0.5, 0.003, 1))
+    assert orc.ee_ccd(x2, x2, 0.002)[0]
 
 
 def test_parallel_edges_are_skipped_by_the_response(orc):
