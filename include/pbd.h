@@ -246,10 +246,12 @@ PBD_API int pbd_ccd_get_contacts(pbd_ctx* ctx, long long cap, int* vf, float* t,
 /* Edge-edge contacts (Contact::EE, declared but not implemented in the
  * reference: CCD_Basic.h:14, CCD.cpp:925-928), added to the vertex-face
  * contacts of a pbd_ccd_detect / pbd_ccd_detect_swept call on the same
- * scene (none without one).  Candidates: every edge pair, sharing no vertex,
- * of the triangle pairs pbd_ccd_detect visits (centroid cells of desc).
+ * scene (none without one).  Candidates: every edge pair of the triangle
+ * pairs pbd_ccd_detect visits (centroid cells of desc) that shares no vertex
+ * and has no vertex that is a mesh neighbour of the other edge's.
  * Test: the first time in [0, 1] (coplanarity roots of the four points, then
- * 0 and 1) at which the segments come within 2 x thickness. */
+ * 0 and 1) at which the segments come within 2 x thickness, for edges that
+ * start at least one thickness apart along their normal. */
 PBD_API int pbd_ccd_detect_ee(pbd_ctx* ctx, const pbd_hash_desc* desc, const float* x, const float* p, int nv,
                               const int* tris, int nt, float thickness, long long* npairs, long long* ncontacts);
 /* Edge-edge candidates of the last pbd_ccd_detect_ee: 4 vertex ids each

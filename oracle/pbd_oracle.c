@@ -1517,12 +1517,35 @@ static int cmp_pair(const void* a, const void* b) {
     return 0;
 }
 
+/* Whether a vertex of edge a is a mesh neighbour (joined by an edge) of a
+ * vertex of edge b; off/nbr: vertex -> neighbours. */
+static int ring_neighbours(const int* a, const int* b, const int* off, const int* nbr) {
+    for (int i = 0; i < 2; i++)
+        for (int j = off[a[i]]; j < off[a[i] + 1]; j++)
+            if (nbr[j] == b[0] || nbr[j] == b[1]) return 1;
+    return 0;
+}
+
 /* Edge-edge candidates by brute force over triangle pairs t < f whose
  * centroid cells are both inside the grid and at most one cell apart on
- * each axis: every edge pair sharing no vertex, as (lower edge id, higher),
+ * each axis: every edge pair sharing no vertex and with no vertex of one a
+ * mesh neighbour of a vertex of the other, as (lower edge id, higher),
  * distinct and ascending.  Returns the count; writes min(count, cap). */
 long long orc_ee_candidates(const orc_grid* g, const float* pos, const int* tris, int nt, const int* tedge,
                             const int* edges, long long cap, int* out) {
+    int ne = 0, nv = 0;
+    for (int i = 0; i < 3 * nt; i++) if (tedge[i] + 1 > ne) ne = tedge[i] + 1;
+    for (int i = 0; i < 2 * ne; i++) if (edges[i] + 1 > nv) nv = edges[i] + 1;
+    int* off = (int*)calloc((size_t)nv + 1, sizeof(int));
+    int* nbr = (int*)malloc(sizeof(int) * (size_t)(2 * ne + 1));
+    int* fill = (int*)malloc(sizeof(int) * (size_t)(nv + 1));
+    for (int e = 0; e < ne; e++) { off[edges[2 * e] + 1]++; off[edges[2 * e + 1] + 1]++; }
+    for (int v = 0; v < nv; v++) off[v + 1] += off[v];
+    for (int v = 0; v < nv; v++) fill[v] = off[v];
+    for (int e = 0; e < ne; e++) {
+        nbr[fill[edges[2 * e]]++] = edges[2 * e + 1];
+        nbr[fill[edges[2 * e + 1]]++] = edges[2 * e];
+    }
     int* keys = (int*)malloc(sizeof(int) * ((size_t)nt + 1));
     int* ids = (int*)malloc(sizeof(int) * ((size_t)nt + 1));
     orc_tri_keys(g, pos, tris, nt, keys, ids);
@@ -1549,6 +1572,7 @@ long long orc_ee_candidates(const orc_grid* g, const float* pos, const int* tris
                     const int* av = edges + 2 * ea;
                     const int* bv = edges + 2 * eb;
                     if (av[0] == bv[0] || av[0] == bv[1] || av[1] == bv[0] || av[1] == bv[1]) continue;
+                    if (ring_neighbours(av, bv, off, nbr)) continue;
                     if (n == bufcap) { bufcap *= 2; buf = (int*)realloc(buf, sizeof(int) * 2 * bufcap); }
                     buf[2 * n] = ea < eb ? ea : eb;
                     buf[2 * n + 1] = ea < eb ? eb : ea;
@@ -1565,7 +1589,7 @@ long long orc_ee_candidates(const orc_grid* g, const float* pos, const int* tris
             u++;
         }
     if (out) for (size_t i = 0; i < u && (long long)i < cap; i++) { out[2 * i] = buf[2 * i]; out[2 * i + 1] = buf[2 * i + 1]; }
-    free(buf); free(keys); free(ids); free(tc);
+    free(buf); free(keys); free(ids); free(tc); free(off); free(nbr); free(fill);
     return (long long)u;
 }
 

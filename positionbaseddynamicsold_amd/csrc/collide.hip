@@ -194,17 +194,33 @@ __global__ __launch_bounds__(256) void k_pairs(HashCfg g, const float4* __restri
     if (!EMIT) cnt_or_off[t] = n;
 }
 
+// True when a vertex of edge a is a mesh neighbour of a vertex of edge b
+// (vertex -> neighbour CSR voff / vnbr).
+__device__ __forceinline__ bool ring_neighbours(int2 a, int2 b, const int* __restrict__ voff,
+                                                const int* __restrict__ vnbr) {
+    const int ends[2] = {a.x, a.y};
+    for (int k = 0; k < 2; k++)
+        for (int j = voff[ends[k]]; j < voff[ends[k] + 1]; j++) {
+            int u = vnbr[j];
+            if (u == b.x || u == b.y) return true;
+        }
+    return false;
+}
+
 // Edge-edge candidates (new: the reference declares Contact::EE, CCD_Basic.h:14,
 // and leaves it unimplemented, CCD.cpp:925-928).  From the triangle pairs
 // k_pairs visits (centroid cells within one cell, the neighbour's own cell
 // checked), taken once (t < f): every edge of t against every edge of f that
-// shares no vertex with it, as (lower edge id, higher edge id).
+// shares no vertex with it and has no vertex that is a mesh neighbour of one
+// of its vertices (pairs that close in the mesh are material neighbours, not
+// contacts), as (lower edge id, higher edge id).
 template <int EMIT>
 __global__ __launch_bounds__(256) void k_ee_pairs(HashCfg g, const float4* __restrict__ p,
                                                   const int* __restrict__ tris, int nt, const int* __restrict__ raw,
                                                   const uint32_t* __restrict__ sids, const int* __restrict__ cstart,
                                                   const int* __restrict__ cend, const int4* __restrict__ tcell,
                                                   const int* __restrict__ tedge, const int2* __restrict__ edges,
+                                                  const int* __restrict__ voff, const int* __restrict__ vnbr,
                                                   int* __restrict__ cnt_or_off, uint32_t* __restrict__ oa,
                                                   uint32_t* __restrict__ ob) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -235,6 +251,7 @@ __global__ __launch_bounds__(256) void k_ee_pairs(HashCfg g, const float4* __res
                             for (int ka = 0; ka < 3; ka++) {
                                 int2 av = tv[ka];
                                 if (av.x == bv.x || av.x == bv.y || av.y == bv.x || av.y == bv.y) continue;
+                                if (ring_neighbours(av, bv, voff, vnbr)) continue;
                                 if (EMIT) {
                                     int ea = te[ka];
                                     oa[off + n] = (uint32_t)(ea < eb ? ea : eb);
@@ -997,6 +1014,23 @@ void Broadphase::set_edges(hipStream_t s, const std::vector<int>& te, const std:
     tedge.upload(te, s);
     edges.upload(ed, s);
     nedges = (int)ed.size();
+    // vertex -> neighbour CSR from the edges (each edge once, both directions)
+    int nv = 0;
+    for (const int2& e : ed) nv = std::max(nv, e.y + 1);
+    std::vector<int> off((size_t)nv + 1, 0), nbr((size_t)2 * ed.size());
+    for (const int2& e : ed) {
+        off[e.x + 1]++;
+        off[e.y + 1]++;
+    }
+    for (int v = 0; v < nv; v++) off[v + 1] += off[v];
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (const int2& e : ed) {
+        nbr[fill[e.x]++] = e.y;
+        nbr[fill[e.y]++] = e.x;
+    }
+    voff.upload(off, s);
+    vnbr.upload(nbr, s);
+    PBD_HIP(hipStreamSynchronize(s));
 }
 
 // Edge-edge candidates from the cell table of the last build() (see k_ee_pairs).
@@ -1009,8 +1043,8 @@ long long Broadphase::ee_pairs(hipStream_t s, const float4* p, const int* tris) 
     grow(off, nt + 1);
     grow(scan_tmp, (nt + SB) / SB + 2);
     hipLaunchKernelGGL(k_ee_pairs<0>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr,
-                       cstart.ptr, cend.ptr, tcell.ptr, tedge.ptr, edges.ptr, cnt.ptr, (uint32_t*)nullptr,
-                       (uint32_t*)nullptr);
+                       cstart.ptr, cend.ptr, tcell.ptr, tedge.ptr, edges.ptr, voff.ptr, vnbr.ptr, cnt.ptr,
+                       (uint32_t*)nullptr, (uint32_t*)nullptr);
     PBD_HIP(hipMemsetAsync(cnt.ptr + nt, 0, sizeof(int), s));
     device_scan(s, cnt.ptr, off.ptr, nt + 1, scan_tmp.ptr);
     int total = 0;
@@ -1026,7 +1060,8 @@ long long Broadphase::ee_pairs(hipStream_t s, const float4* p, const int* tris) 
     grow(flag, total + 1);
     grow(pos, total + 1);
     hipLaunchKernelGGL(k_ee_pairs<1>, dim3(blocks(nt)), dim3(256), 0, s, cfg, p, tris, nt, raw.ptr, ids.ptr,
-                       cstart.ptr, cend.ptr, tcell.ptr, tedge.ptr, edges.ptr, off.ptr, pv.ptr, pf.ptr);
+                       cstart.ptr, cend.ptr, tcell.ptr, tedge.ptr, edges.ptr, voff.ptr, vnbr.ptr, off.ptr, pv.ptr,
+                       pf.ptr);
     ee_npairs = unique_into(s, total, nedges, nedges, eua, eub);
     st_ee_pairs += ee_npairs;
     return ee_npairs;

@@ -240,6 +240,7 @@ struct Broadphase {
     // distinct candidate edge pairs and the contacts (a0, a1, b0, b1)
     DBuf<int> tedge;
     DBuf<int2> edges;
+    DBuf<int> voff, vnbr;  // vertex -> mesh neighbours (CSR) of the edge table
     int nedges = 0;
     DBuf<uint32_t> eua, eub;
     DBuf<int4> ecq;

@@ -11,7 +11,7 @@ mathematics rather than reference vectors:
   start) are neither contacts nor pushed apart;
 - the candidate set equals an independent numpy brute force over triangle
   pairs whose centroid cells are at most one cell apart, and never pairs
-  edges that share a vertex.
+  edges that share a vertex or whose vertices are mesh neighbours.
 """
 import numpy as np
 
@@ -127,6 +127,12 @@ def test_candidates_equal_numpy_brute_force(orc):
     cen = ((pos[tris[:, 0]] + pos[tris[:, 1]]) + pos[tris[:, 2]]) / F32(3)
     cells = np.floor((cen - np.array(g.start, F32)) / np.array(g.cell, F32)).astype(np.int64)
     edge_id = {tuple(e): i for i, e in enumerate(edges.tolist())}
+    nbrs = {}
+    for t3 in tris.tolist():
+        for k in range(3):
+            a, b = t3[k], t3[(k + 1) % 3]
+            nbrs.setdefault(a, set()).add(b)
+            nbrs.setdefault(b, set()).add(a)
     ref = set()
     for t in range(len(tris)):
         for f in range(t + 1, len(tris)):
@@ -136,7 +142,7 @@ def test_candidates_equal_numpy_brute_force(orc):
                 a = tuple(sorted((int(tris[t][ka]), int(tris[t][(ka + 1) % 3]))))
                 for kb in range(3):
                     b = tuple(sorted((int(tris[f][kb]), int(tris[f][(kb + 1) % 3]))))
-                    if set(a) & set(b):
+                    if set(a) & set(b) or (nbrs[a[0]] | nbrs[a[1]]) & set(b):
                         continue
                     ea, eb = edge_id[a], edge_id[b]
                     ref.add((min(ea, eb), max(ea, eb)))
