@@ -237,12 +237,14 @@ def jacobi_baseline(a, capi, scenes, ctx, ids, r_plain, warm_rate, plain_solver)
     el = time.perf_counter() - t0
     infos = [ctx.info(i) for i in ids]
     nv = sum(i.nv for i in infos)
-    entries = sum(2 * i.ne + 4 * i.nb for i in infos)
+    dist_inc, bend_inc = sum(2 * i.ne for i in infos), sum(4 * i.nb for i in infos)
     out = {"omega": a.jacobi_omega, "sweeps": hi, "reaches_R": reached, "residual": tried[hi],
            "substeps_per_s": a.steps / el, "ms_per_step": el * 1e3 / a.steps,
            "warm_start_speedup_over_jacobi": warm_rate / (a.steps / el),
            "sweeps_tried": {str(k): v for k, v in sorted(tried.items())},
-           "alg_bytes_per_sweep": 40 * nv + 24 * entries,
+           # k_jacobi: p in/out, cloth id, degree and two offsets per vertex; 8 B per distance
+           # incidence, 24 B per bending incidence (neighbour positions are re-reads)
+           "alg_bytes_per_sweep": 48 * nv + 8 * dist_inc + 24 * bend_inc,
            "definition": "PBD_PATH_JACOBI: every vertex moves by omega/n times the sum of the corrections its n "
                          "prims give it on the sweep's input positions.  Sweeps: smallest count whose mean "
                          "end-of-substep residual, over the plain trajectory's substeps each solved from the "

@@ -35,11 +35,12 @@ struct pbd_ctx {
     DBuf<float2> gbrc;
     std::vector<int> gd_off, gb_off;
     int ncolors_max = 0;
-    // Jacobi path: vertex -> incident prims (CSR over the packed vertices),
-    // built on first use
-    DBuf<int> joff;
-    DBuf<int4> jent;
-    DBuf<int2> jmeta;
+    // Jacobi path: vertex -> incident prims over the packed vertices, built
+    // on first use (layout: JacobiDev)
+    DBuf<int> jddeg, jdoff, jboff;
+    DBuf<int2> jdslot, jdovf, jbmeta;
+    DBuf<int4> jbent;
+    JacobiDev jac{};
     bool jac_built = false;
     // stiffness tables
     DBuf<float> ktab_d, ktab_b;
@@ -674,40 +675,61 @@ void advect_all(pbd_ctx* c, float dt) {
 // cap: optional per-cloth sweep budgets (device array) below `iterations`.
 void consolidate(pbd_ctx* c);
 
-// Vertex -> prim CSR of the Jacobi path: each vertex's prims in ascending
-// prim id (distance prims, then bending prims), with the prim's vertices in
-// its own order and this vertex's role in it.
+// Incidence lists of the Jacobi path: each vertex's prims in ascending prim
+// id (distance prims, then bending prims) with its role in each.
 void ensure_jacobi(pbd_ctx* c) {
     if (c->jac_built) return;
-    std::vector<int> off((size_t)c->nv_total + 1, 0);
+    const int nv = c->nv_total;
+    std::vector<int> ddeg(nv, 0), boff((size_t)nv + 1, 0);
     for (const Cloth& k : c->cloths)
-        for (int p = 0; p < k.np(); p++)
-            for (int j = 0; j < k.prim_list[2 * p + 1]; j++) off[k.vbase + k.indices[k.prim_list[2 * p] + j] + 1]++;
-    for (int i = 0; i < c->nv_total; i++) off[i + 1] += off[i];
-    PBD_CHECK(off[c->nv_total] >= 0, PBD_ERR_LIMIT, "Jacobi incidence list too large");
-    std::vector<int> fill(off.begin(), off.end() - 1);
-    std::vector<int4> ent(off[c->nv_total]);
-    std::vector<int2> meta(off[c->nv_total]);
-    for (size_t ci = 0; ci < c->cloths.size(); ci++) {
-        const Cloth& k = c->cloths[ci];
+        for (int p = 0; p < k.np(); p++) {
+            const int st = k.prim_list[2 * p], n = k.prim_list[2 * p + 1];
+            for (int j = 0; j < n; j++) {
+                const int v = k.vbase + k.indices[st + j];
+                if (n == 2) ddeg[v]++;
+                else boff[v + 1]++;
+            }
+        }
+    std::vector<int> doff((size_t)nv + 1, 0);
+    for (int v = 0; v < nv; v++) {
+        doff[v + 1] = doff[v] + std::max(0, ddeg[v] - JAC_SLOTS);
+        boff[v + 1] += boff[v];
+    }
+    std::vector<int2> dslot((size_t)JAC_SLOTS * nv, make_int2(0, 0)), dovf(std::max(1, doff[nv]));
+    std::vector<int4> bent(std::max(1, boff[nv]));
+    std::vector<int2> bmeta(std::max(1, boff[nv]));
+    std::vector<int> dfill(nv, 0), bfill(boff.begin(), boff.end() - 1);
+    for (const Cloth& k : c->cloths)
         for (int p = 0; p < k.np(); p++) {
             const int st = k.prim_list[2 * p], n = k.prim_list[2 * p + 1];
             const int* q = &k.indices[st];
-            int4 e = n == 2 ? make_int4(k.vbase + q[0], k.vbase + q[1], -1, -1)
-                            : make_int4(k.vbase + q[0], k.vbase + q[1], k.vbase + q[2], k.vbase + q[3]);
+            int bits;
+            memcpy(&bits, &k.rest[p], 4);
             for (int j = 0; j < n; j++) {
-                int slot = fill[k.vbase + q[j]]++;
-                ent[slot] = e;
-                int bits;
-                memcpy(&bits, &k.rest[p], 4);
-                meta[slot] = make_int2(bits, (int)ci * 8 + (n == 2 ? 0 : 4) + j);
+                const int v = k.vbase + q[j];
+                if (n == 2) {
+                    int2 d = make_int2((int)((unsigned)(k.vbase + q[1 - j]) | ((unsigned)j << 31)), bits);
+                    int slot = dfill[v]++;
+                    if (slot < JAC_SLOTS) dslot[(size_t)slot * nv + v] = d;
+                    else dovf[doff[v] + slot - JAC_SLOTS] = d;
+                } else {
+                    int at = bfill[v]++;
+                    bent[at] = make_int4(k.vbase + q[0], k.vbase + q[1], k.vbase + q[2], k.vbase + q[3]);
+                    bmeta[at] = make_int2(bits, j);
+                }
             }
         }
-    }
-    c->joff.upload(off, c->stream);
-    c->jent.upload(ent, c->stream);
-    c->jmeta.upload(meta, c->stream);
-    PBD_HIP(hipStreamSynchronize(c->stream));
+    hipStream_t s = c->stream;
+    c->jddeg.upload(ddeg, s);
+    c->jdslot.upload(dslot, s);
+    c->jdoff.upload(doff, s);
+    c->jdovf.upload(dovf, s);
+    c->jboff.upload(boff, s);
+    c->jbent.upload(bent, s);
+    c->jbmeta.upload(bmeta, s);
+    PBD_HIP(hipStreamSynchronize(s));
+    c->jac = JacobiDev{nv, c->jddeg.ptr, c->jdslot.ptr, c->jdoff.ptr, c->jdovf.ptr, c->jboff.ptr, c->jbent.ptr,
+                       c->jbmeta.ptr};
     c->jac_built = true;
 }
 
@@ -725,8 +747,8 @@ int project_jacobi(pbd_ctx* c, int iterations, const int* cap, int vb, int ve, d
     int sw = 0;
     while (sw < iterations) {
         const int cur = c->pcur;
-        launch_jacobi(s, c->joff.ptr, c->jent.ptr, c->jmeta.ptr, c->cl.ptr, c->vcloth.ptr, c->ktab_d.ptr,
-                      c->ktab_b.ptr, c->kmax, sw, cap, omega, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, vb, ve);
+        launch_jacobi(s, c->jac, c->cl.ptr, c->vcloth.ptr, c->ktab_d.ptr, c->ktab_b.ptr, c->kmax, sw, cap, omega,
+                      c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, vb, ve);
         c->pcur = cur ^ 1;
         sw++;
         if (tol >= 0.0 && sw % ce == 0) {

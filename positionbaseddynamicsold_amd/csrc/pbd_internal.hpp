@@ -337,9 +337,21 @@ void launch_project_colour(hipStream_t s, const int4* dist, int d_begin, int d_e
                            float4* p);
 void launch_colliders(hipStream_t s, const ClothDev* cloths, const int* vcloth, float4* p,
                       int nv_begin, int nv_end, const int* cap = nullptr, int sweep = 0);
-void launch_jacobi(hipStream_t s, const int* off, const int4* ent, const int2* meta, const ClothDev* cloths,
-                   const int* vcloth, const float* ktab_d, const float* ktab_b, int kmax, int sweep, const int* cap,
-                   float omega, const float4* pin, float4* pout, int nv_begin, int nv_end);
+// Incidence lists of the Jacobi path (see k_jacobi in solver.hip).
+constexpr int JAC_SLOTS = 8;
+struct JacobiDev {
+    int nv;
+    const int* ddeg;     // distance incidences per vertex
+    const int2* dslot;   // first JAC_SLOTS of them, slot-major: dslot[k * nv + i]
+    const int* doff;     // nv + 1 offsets of the rest into dovf
+    const int2* dovf;
+    const int* boff;     // nv + 1 offsets into bent / bmeta
+    const int4* bent;    // bending prim vertices (global ids)
+    const int2* bmeta;   // (rest bits, role)
+};
+void launch_jacobi(hipStream_t s, const JacobiDev& J, const ClothDev* cloths, const int* vcloth, const float* ktab_d,
+                   const float* ktab_b, int kmax, int sweep, const int* cap, float omega, const float4* pin,
+                   float4* pout, int nv_begin, int nv_end);
 void launch_residual(hipStream_t s, const int4* dist, int n, const float4* p,
                      unsigned long long* rmax, double* rsum, int* rcnt);
 void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* tiles,

@@ -182,19 +182,22 @@ void launch_colliders(hipStream_t s, const ClothDev* cloths, const int* vcloth, 
 // SolverType::JACOBI, PBD_Basic.cuh:46-51, and never implements it; this is
 // the averaged Jacobi of Macklin et al. 2014, "Unified particle physics for
 // real-time applications", sec. 4.2).  One thread per vertex gathers, from
-// p_in, the correction each incident prim (CSR in ascending prim id) would
-// give it when projected alone, with the prim's own vertex order and the
-// Gauss-Seidel projection arithmetic, and moves by omega/n times their sum
-// (n = incident prims).  Pinned vertices (w = 0) and cloths whose sweep
-// budget is spent are copied.  Colliders follow in the same thread.  The
-// result depends on no launch or summation order beyond the CSR's.
+// p_in, the correction each incident prim (ascending prim id: distance prims,
+// then bending prims) would give it when projected alone, with the prim's
+// own vertex order and the Gauss-Seidel projection arithmetic, and moves by
+// omega/n times their sum (n = incident prims).  Pinned vertices (w = 0) and
+// cloths whose sweep budget is spent are copied.  Colliders follow in the
+// same thread.  The result depends on no launch or summation order beyond
+// the per-vertex prim order.
 //
-// Entry e: ent[e] = prim vertices (global ids; distance uses .x, .y),
-// meta[e] = (rest bits, cloth * 8 + type * 4 + role), type 0 distance /
-// 1 bending, role = this vertex's position in the prim.
+// Distance incidences, 8 B each: (other vertex | role << 31, rest bits),
+// role 1 when this vertex is the prim's second.  The first JAC_SLOTS of a
+// vertex are slot-major (slot k of vertex i at dslot[k * nv + i], so a
+// wave's loads are contiguous), the rest in a per-vertex overflow list
+// (doff).  Bending incidences: boff / bent (prim vertices) / bmeta (rest
+// bits, role).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_jacobi(const int* __restrict__ off, const int4* __restrict__ ent,
-                                                const int2* __restrict__ meta, const ClothDev* __restrict__ cl,
+__global__ __launch_bounds__(256) void k_jacobi(JacobiDev J, const ClothDev* __restrict__ cl,
                                                 const int* __restrict__ vcloth, const float* __restrict__ ktab_d,
                                                 const float* __restrict__ ktab_b, int kmax, int sweep,
                                                 const int* __restrict__ cap, float omega,
@@ -209,28 +212,39 @@ __global__ __launch_bounds__(256) void k_jacobi(const int* __restrict__ off, con
         return;
     }
     const int ks = sweep < kmax ? sweep : kmax - 1;
-    const int o0 = off[i], o1 = off[i + 1];
+    const float kd = ktab_d[ci * kmax + ks];
+    const int deg = J.ddeg[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (int j = o0; j < o1; j++) {
-        int4 q = ent[j];
-        int2 m = meta[j];
-        const int kc = m.y >> 3, role = m.y & 3;
-        float4 r;
-        if (((m.y >> 2) & 1) == 0) {
-            float4 a = pin[q.x], bb = pin[q.y];
-            project_distance(a, bb, __int_as_float(m.x), ktab_d[kc * kmax + ks]);
-            r = role ? bb : a;
-        } else {
-            float4 q0 = pin[q.x], q1 = pin[q.y], q2 = pin[q.z], q3 = pin[q.w];
-            project_bending(q0, q1, q2, q3, __int_as_float(m.x), ktab_b[kc * kmax + ks]);
-            r = role == 0 ? q0 : role == 1 ? q1 : role == 2 ? q2 : q3;
-        }
+    auto dist_one = [&](int2 d) {
+        const int role = (int)((unsigned)d.x >> 31);
+        float4 po = pin[d.x & 0x7fffffff];
+        float4 a = role ? po : pi, bb = role ? pi : po;
+        project_distance(a, bb, __int_as_float(d.y), kd);
+        float4 r = role ? bb : a;
         sx = sx + (r.x - pi.x);
         sy = sy + (r.y - pi.y);
         sz = sz + (r.z - pi.z);
+    };
+    const int ns = deg < JAC_SLOTS ? deg : JAC_SLOTS;
+    for (int k = 0; k < ns; k++) dist_one(J.dslot[(size_t)k * J.nv + i]);
+    for (int j = J.doff[i]; j < J.doff[i + 1]; j++) dist_one(J.dovf[j]);
+    const int b0 = J.boff[i], b1 = J.boff[i + 1];
+    if (b1 > b0) {
+        const float kb = ktab_b[ci * kmax + ks];
+        for (int j = b0; j < b1; j++) {
+            int4 q = J.bent[j];
+            int2 m = J.bmeta[j];
+            float4 q0 = pin[q.x], q1 = pin[q.y], q2 = pin[q.z], q3 = pin[q.w];
+            project_bending(q0, q1, q2, q3, __int_as_float(m.x), kb);
+            float4 r = m.y == 0 ? q0 : m.y == 1 ? q1 : m.y == 2 ? q2 : q3;
+            sx = sx + (r.x - pi.x);
+            sy = sy + (r.y - pi.y);
+            sz = sz + (r.z - pi.z);
+        }
     }
-    if (o1 > o0) {
-        const float f = omega / (float)(o1 - o0);
+    const int n = deg + (b1 - b0);
+    if (n > 0) {
+        const float f = omega / (float)n;
         pi.x = pi.x + f * sx;
         pi.y = pi.y + f * sy;
         pi.z = pi.z + f * sz;
@@ -240,12 +254,12 @@ __global__ __launch_bounds__(256) void k_jacobi(const int* __restrict__ off, con
     pout[i] = pi;
 }
 
-void launch_jacobi(hipStream_t s, const int* off, const int4* ent, const int2* meta, const ClothDev* cloths,
-                   const int* vcloth, const float* ktab_d, const float* ktab_b, int kmax, int sweep, const int* cap,
-                   float omega, const float4* pin, float4* pout, int b, int e) {
+void launch_jacobi(hipStream_t s, const JacobiDev& J, const ClothDev* cloths, const int* vcloth, const float* ktab_d,
+                   const float* ktab_b, int kmax, int sweep, const int* cap, float omega, const float4* pin,
+                   float4* pout, int b, int e) {
     if (e <= b) return;
-    hipLaunchKernelGGL(k_jacobi, dim3((e - b + 255) / 256), dim3(256), 0, s, off, ent, meta, cloths, vcloth, ktab_d,
-                       ktab_b, kmax, sweep, cap, omega, pin, pout, b, e);
+    hipLaunchKernelGGL(k_jacobi, dim3((e - b + 255) / 256), dim3(256), 0, s, J, cloths, vcloth, ktab_d, ktab_b, kmax,
+                       sweep, cap, omega, pin, pout, b, e);
     PBD_HIP(hipGetLastError());
 }
 
