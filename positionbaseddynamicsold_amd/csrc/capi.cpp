@@ -1021,6 +1021,7 @@ void substep(pbd_ctx* c, float dt) {
         PBD_CHECK(c->sp.path != PBD_PATH_JACOBI, PBD_ERR_STATE, "the CCD pipeline runs on the Gauss-Seidel paths");
         step_ccd(c, dt);
     } else if (c->ws.enabled) {
+        PBD_CHECK(c->sp.path != PBD_PATH_JACOBI, PBD_ERR_STATE, "the warm start runs on the Gauss-Seidel paths");
         if (c->sp.path != PBD_PATH_AUTO) {
             advect_all(c, dt);
             ws_apply_prediction(c, true, false, dt);

@@ -98,6 +98,12 @@ def test_jacobi_errors():
             ctx.set_solver(path=capi.PATH_JACOBI, jacobi_omega=2.5)
         assert e.value.code == capi.PBD_ERR_ARG
         ctx.set_solver(path=capi.PATH_JACOBI)
+        ctx.ws_init(0)
+        ctx.ws_enable(True)
+        with pytest.raises(capi.PbdError) as e:
+            ctx.step(1)
+        assert e.value.code == capi.PBD_ERR_STATE
+        ctx.ws_enable(False)
         ctx.set_collision(enabled=1)
         with pytest.raises(capi.PbdError) as e:
             ctx.step(1)
