@@ -62,6 +62,12 @@ HBM_PEAK_GBS = 8000.0
 PLAIN_SWEEPS = 20
 
 
+def traffic_key(cfg):
+    """Launch configuration key of the committed tile-kernel traffic figures."""
+    key = "ts{tile_sweeps}_bt{block_threads}_mv{tile_max_verts}".format(**cfg)
+    return key + ("_sw1" if cfg.get("split_whole") else "")
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -274,7 +280,7 @@ def main():
     t = time.perf_counter()
     ctx.init()
     log(f"[bench] init (constraints, GPU colouring, tile plan) {time.perf_counter() - t:.1f} s")
-    tuned = {"tile_sweeps": a.tile_sweeps, "block_threads": 1024, "tile_max_verts": 0}
+    tuned = {"tile_sweeps": a.tile_sweeps, "block_threads": 1024, "tile_max_verts": 0, "split_whole": 0}
     ctx.set_solver(dt=scenes.DT, iterations=PLAIN_SWEEPS, check_every=a.tile_sweeps, tile_sweeps=a.tile_sweeps,
                    graph=1)
     ctx.step(2)
@@ -282,8 +288,8 @@ def main():
         t = time.perf_counter()
         best, _, best_ms, trials = ctx.autotune(2)
         tuned = {"tile_sweeps": best.tile_sweeps, "block_threads": best.block_threads,
-                 "tile_max_verts": best.tile_max_verts, "trials": trials, "best_ms_per_substep": best_ms,
-                 "seconds": time.perf_counter() - t}
+                 "tile_max_verts": best.tile_max_verts, "split_whole": best.split_whole, "trials": trials,
+                 "best_ms_per_substep": best_ms, "seconds": time.perf_counter() - t}
         log(f"[bench] autotune: {tuned}")
 
     rule = {"rule": a.warm_rule, "gate": a.warm_gate}
@@ -298,7 +304,7 @@ def main():
         ctx.set_solver(dt=scenes.DT, iterations=sweeps, check_every=cfg["tile_sweeps"],
                        tile_sweeps=cfg["tile_sweeps"], tile_max_verts=cfg["tile_max_verts"],
                        block_threads=cfg["block_threads"], graph=graph, warm_iterations=warm,
-                       warm_rule=rule["rule"], warm_gate=rule["gate"])
+                       warm_rule=rule["rule"], warm_gate=rule["gate"], split_whole=cfg["split_whole"])
 
     # warm-up, with the batch's own training samples recorded along the way
     solver(PLAIN_SWEEPS, False)
@@ -377,16 +383,17 @@ def main():
         t = time.perf_counter()
         best, _, best_ms, trials = ctx.autotune(2)
         tuned_warm = {"tile_sweeps": best.tile_sweeps, "block_threads": best.block_threads,
-                      "tile_max_verts": best.tile_max_verts, "trials": trials, "best_ms_per_substep": best_ms,
-                      "seconds": time.perf_counter() - t}
+                      "tile_max_verts": best.tile_max_verts, "split_whole": best.split_whole, "trials": trials,
+                      "best_ms_per_substep": best_ms, "seconds": time.perf_counter() - t}
         log(f"[bench] autotune (warm-started run): {tuned_warm}")
     if a.save_network:
         np.savez(a.save_network, params=ctx.ws_params(), norm=ctx.ws_norm(), count=a.count, seed=a.seed,
                  warmup=a.warmup, collect_substeps=a.collect_substeps, target_sweeps=a.target_sweeps, settle=2,
                  plain_sweeps=PLAIN_SWEEPS, warm_k=warm_k, warm_rule=a.warm_rule, warm_gate=a.warm_gate,
                  tile_sweeps=tuned["tile_sweeps"], tile_max_verts=tuned["tile_max_verts"],
-                 block_threads=tuned["block_threads"], warm_tile_sweeps=tuned_warm["tile_sweeps"],
-                 warm_tile_max_verts=tuned_warm["tile_max_verts"], warm_block_threads=tuned_warm["block_threads"])
+                 block_threads=tuned["block_threads"], split_whole=tuned["split_whole"],
+                 warm_tile_sweeps=tuned_warm["tile_sweeps"], warm_tile_max_verts=tuned_warm["tile_max_verts"],
+                 warm_block_threads=tuned_warm["block_threads"], warm_split_whole=tuned_warm["split_whole"])
 
     def timed(ws, steps, warm=0):
         ctx.snapshot(restore=True)
@@ -460,7 +467,7 @@ def main():
     tf = ROOT / "profiles" / "k_tiles_batch512_traffic.json"
     if tf.exists():
         try:
-            key = "ts{tile_sweeps}_bt{block_threads}_mv{tile_max_verts}".format(**tuned_warm)
+            key = traffic_key(tuned_warm)
             traffic = json.loads(tf.read_text())["warm"][key]["hbm_bytes_per_launch"]
         except Exception:
             traffic = None

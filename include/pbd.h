@@ -100,6 +100,10 @@ typedef struct {
                                vertex then keeps the zero prediction and runs warm_iterations sweeps */
     float jacobi_omega;     /* PBD_PATH_JACOBI: a vertex moves by omega / n times the sum of the corrections
                                its n prims give it (0 = 1; 1 to 2 over-relaxes) */
+    int split_whole;        /* 1: a cloth that fits one LDS tile is stepped tile_sweeps sweeps per launch like a
+                               multi-tile cloth (its one tile loads the whole cloth, no halo) instead of all its
+                               sweeps in one launch, so that no launch waits on one cloth's whole budget
+                               (0 = one launch; results are identical either way) */
 } pbd_solver_params;
 
 typedef struct {

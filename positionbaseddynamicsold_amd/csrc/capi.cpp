@@ -47,7 +47,7 @@ struct pbd_ctx {
     int kmax = 0;
     // tiles
     TilePlan plan;
-    int plan_sweeps = 0, plan_max_load = 0;
+    int plan_sweeps = 0, plan_max_load = 0, plan_split = 0;
     DBuf<TileDev> tiles;
     // launch orders: single-tile cloths' tiles, multi-tile cloths' tiles, and
     // both (the first launch of a substep runs them side by side)
@@ -252,14 +252,15 @@ void upload_plan(pbd_ctx* c) {
 }
 
 void replan_if_needed(pbd_ctx* c) {
-    int s = effective_tile_sweeps(c), ml = effective_max_load(c);
-    if (s == c->plan_sweeps && ml == c->plan_max_load) return;
+    int s = effective_tile_sweeps(c), ml = effective_max_load(c), sp = c->sp.split_whole ? 1 : 0;
+    if (s == c->plan_sweeps && ml == c->plan_max_load && sp == c->plan_split) return;
     // a failed plan leaves no valid one behind: the next call plans again
     c->plan_sweeps = 0;
     c->plan_max_load = 0;
-    plan_tiles(c->cloths, ml, s, c->plan);
+    plan_tiles(c->cloths, ml, s, sp != 0, c->plan);
     c->plan_sweeps = s;
     c->plan_max_load = ml;
+    c->plan_split = sp;
     upload_plan(c);
     fill_cloth_dev(c);
 }
@@ -1488,6 +1489,7 @@ int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* p) {
         PBD_CHECK(p->path == PBD_PATH_AUTO || p->path == PBD_PATH_GLOBAL || p->path == PBD_PATH_JACOBI, PBD_ERR_ARG,
                   "unknown path");
         PBD_CHECK(p->jacobi_omega >= 0.0f && p->jacobi_omega < 2.0f, PBD_ERR_ARG, "jacobi_omega must be in [0, 2)");
+        PBD_CHECK(p->split_whole == 0 || p->split_whole == 1, PBD_ERR_ARG, "split_whole must be 0 or 1");
         PBD_CHECK(p->tile_sweeps >= 0 && p->tile_max_verts >= 0, PBD_ERR_ARG, "negative tile setting");
         PBD_CHECK(p->block_threads == 0 || (p->block_threads % 64 == 0 && p->block_threads <= 1024), PBD_ERR_ARG,
                   "block_threads must be a multiple of 64 up to 1024");
@@ -2494,6 +2496,27 @@ int pbd_autotune(pbd_ctx* ctx, int trial_substeps, pbd_solver_params* best, int*
                                 bb = hb;
                             }
                         }
+            }
+            // then the other way of stepping single-tile cloths, at the best setting
+            if (!ctx->coll.enabled) {
+                ctx->sp = bp;
+                ctx->sp.split_whole = bp.split_whole ? 0 : 1;
+                drop_graphs(ctx);
+                replan_if_needed(ctx);
+                restore();
+                run_substeps(ctx, 1);
+                restore();
+                PBD_HIP(hipEventRecord(e0, s));
+                run_substeps(ctx, trial_substeps);
+                PBD_HIP(hipEventRecord(e1, s));
+                PBD_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                PBD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                n++;
+                if (ms < best_t) {
+                    best_t = ms;
+                    bp = ctx->sp;
+                }
             }
         } catch (...) {
             (void)hipEventDestroy(e0);

@@ -433,20 +433,23 @@ struct Block { int x0, x1, y0, y1; };
 
 }  // namespace
 
-void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, TilePlan& plan) {
+void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split_whole, TilePlan& plan) {
     plan = TilePlan();
     struct Pending { int cloth; std::vector<TileOut> outs; };
     std::vector<Pending> per(cloths.size());
     std::vector<std::thread> pool;
     unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    // Large cloths are planned in parallel; small ones are one whole tile.
+    // Large cloths are planned in parallel; a small one is a single tile of
+    // all its vertices (no halo), whole unless split_whole asks for it to be
+    // stepped like a multi-tile cloth.
     std::vector<int> work;
     for (size_t ci = 0; ci < cloths.size(); ci++) {
         Cloth& c = cloths[ci];
         per[ci].cloth = (int)ci;
-        c.whole = c.nv <= max_load;
+        const bool fits = c.nv <= max_load;
+        c.whole = fits && !split_whole;
         c.max_loaded = 0;
-        if (c.whole) {
+        if (fits) {
             TileOut o;
             o.load.resize(c.nv);
             for (int v = 0; v < c.nv; v++) o.load[v] = v;

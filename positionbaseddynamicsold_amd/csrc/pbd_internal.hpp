@@ -306,7 +306,9 @@ struct TilePlan {
     std::vector<int> partial_order;       // launch order of the partial tiles only
     std::vector<int> whole_order;         // launch order of the whole tiles only
 };
-void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps_per_launch, TilePlan& plan);
+// split_whole: a cloth that fits one tile still gets a (single, halo-free)
+// multi-launch tile instead of a whole tile.
+void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps_per_launch, bool split_whole, TilePlan& plan);
 
 // ---------------------------------------------------------------------------
 // Kernel launchers (implemented in the .hip files)

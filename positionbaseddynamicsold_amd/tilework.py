@@ -36,9 +36,10 @@ def main():
     cfg = {k: int(net[k]) for k in ("count", "seed", "warmup", "collect_substeps", "target_sweeps", "settle",
                                      "plain_sweeps", "warm_k", "warm_rule", "tile_sweeps", "tile_max_verts",
                                      "block_threads")}
+    cfg["split_whole"] = int(net["split_whole"]) if "split_whole" in net else 0
     # the warm-started run's own launch configuration, when the network file holds one
     wcfg = {k: int(net["warm_" + k]) if "warm_" + k in net else cfg[k]
-            for k in ("tile_sweeps", "tile_max_verts", "block_threads")}
+            for k in ("tile_sweeps", "tile_max_verts", "block_threads", "split_whole")}
     if a.warm_tile_sweeps > 0:
         wcfg["tile_sweeps"] = a.warm_tile_sweeps
     specs = scenes.config5(seed=cfg["seed"], count=cfg["count"])
@@ -53,7 +54,8 @@ def main():
         ctx.set_solver(dt=scenes.DT, iterations=cfg["plain_sweeps"], check_every=c["tile_sweeps"],
                        tile_sweeps=c["tile_sweeps"], tile_max_verts=c["tile_max_verts"],
                        block_threads=c["block_threads"], graph=graph, warm_iterations=warm,
-                       warm_rule=cfg["warm_rule"], warm_gate=float(net["warm_gate"]) if "warm_gate" in net else 0.0)
+                       warm_rule=cfg["warm_rule"], warm_gate=float(net["warm_gate"]) if "warm_gate" in net else 0.0,
+                       split_whole=c["split_whole"])
 
     # the bench's substeps before its snapshot
     solver(False, graph=1)
@@ -86,7 +88,8 @@ def main():
                 "achieved_gbs": prof.tile_alg_bytes / (prof.tile_ms * 1e-3) / 1e9}
 
     out = {"warm": profiled(True, warm=cfg["warm_k"]), "plain": profiled(False),
-           "warm_config": wcfg, "plain_config": {k: cfg[k] for k in ("tile_sweeps", "tile_max_verts", "block_threads")}}
+           "warm_config": wcfg,
+           "plain_config": {k: cfg[k] for k in ("tile_sweeps", "tile_max_verts", "block_threads", "split_whole")}}
     out["copy_ms"] = ctx.bench_copy(1 << 30, 10)
     print(json.dumps(out), flush=True)
     ctx.close()

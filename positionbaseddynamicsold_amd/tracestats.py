@@ -25,7 +25,8 @@ fixed (hash keys 56 B per triangle, narrowphase 104 B per candidate,
 edge-edge narrowphase 152 B per candidate, response 160 B per contact and
 pass).  With --tilework (that run's JSON line) and --update-json it
 also records the figures in the JSON file under each pass's launch
-configuration ("ts<sweeps>_bt<threads>_mv<tile vertices>"), which bench.py
+configuration ("ts<sweeps>_bt<threads>_mv<tile vertices>", "_sw1" when
+single-tile cloths are split into per-launch chunks), which bench.py
 reads for its roofline traffic.
 """
 from __future__ import annotations
@@ -191,7 +192,8 @@ def record(path, label, tw, f, wr, scale_f, scale_w, traffic):
     import os
     data = json.loads(open(path).read()) if os.path.exists(path) else {}
     cfg = tw[label + "_config"]
-    key = "ts{tile_sweeps}_bt{block_threads}_mv{tile_max_verts}".format(**cfg)
+    key = "ts{tile_sweeps}_bt{block_threads}_mv{tile_max_verts}".format(**cfg) + ("_sw1" if cfg.get("split_whole")
+                                                                                   else "")
     data.setdefault(label, {})[key] = {
         "fetch_kib": round(f), "write_kib": round(wr), "fetch_scale": round(scale_f, 4),
         "write_scale": round(scale_w, 4), "hbm_bytes_per_launch": traffic,

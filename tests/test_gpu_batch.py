@@ -148,7 +148,9 @@ def test_autotune_leaves_results_unchanged():
         p, bits, ms, n = a.autotune(1)
         print(f"autotune: threads {p.block_threads} tile_sweeps {p.tile_sweeps} max_verts {p.tile_max_verts} "
               f"{ms:.3f} ms/substep over {n} trials", flush=True)
-        assert n == 3 * 4 * 3  # threads x tile_sweeps {1, 2, 4, 5} x tile sizes
+        # threads x tile_sweeps {1, 2, 4, 5} x tile sizes, then the other way of
+        # stepping single-tile cloths (split_whole) at the best of those
+        assert n == 3 * 4 * 3 + 1
         assert bits == -1 and ms > 0.0
         assert p.block_threads in (256, 512, 1024) and p.tile_sweeps in (1, 2, 4, 5)
         assert p.iterations == 7 and p.graph == 1
@@ -157,3 +159,32 @@ def test_autotune_leaves_results_unchanged():
         a.step(5)
         b.step(5)
         assert_same(a, b, ids)
+
+
+@pytest.mark.parametrize("mode", ["plain", "warm", "tol"])
+def test_split_whole_matches(mode):
+    # single-tile cloths stepped a few sweeps per launch (split_whole = 1) give
+    # the same positions, sweep counts and fallback decisions as one launch each
+    print(f"split_whole mode={mode}", flush=True)
+    with capi.Context() as a, capi.Context() as b:
+        ids = mixed_batch(a)
+        mixed_batch(b)
+        kw = dict(iterations=40, tol=3e-3, check_every=2) if mode == "tol" else {}
+        solver(a, graph=1, **kw)
+        solver(b, graph=1, split_whole=1, **kw)
+        assert [a.info(i).whole for i in ids] == [1, 1, 0]
+        assert [b.info(i).whole for i in ids] == [0, 0, 0]
+        if mode == "warm":
+            for c in (a, b):
+                c.ws_init(3)
+                c.ws_enable(True)
+                c.set_solver(dt=1.0 / 600.0, iterations=7, tile_sweeps=2, check_every=2, tile_max_verts=2048,
+                             graph=1, warm_iterations=3, split_whole=1 if c is b else 0)
+        for _ in range(6):
+            a.step(1)
+            b.step(1)
+            assert_same(a, b, ids)
+            if mode == "warm":
+                assert [a.ws_last_used(i) for i in ids] == [b.ws_last_used(i) for i in ids]
+            if mode == "tol":
+                assert [a.last_sweeps(i) for i in ids] == [b.last_sweeps(i) for i in ids]
