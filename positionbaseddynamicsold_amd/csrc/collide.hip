@@ -194,6 +194,8 @@ __global__ __launch_bounds__(256) void k_pairs(HashCfg g, const float4* __restri
     if (!EMIT) cnt_or_off[t] = n;
 }
 
+constexpr int EE_RING = 40;  // LDS slots per thread for a triangle's vertices and their neighbours
+
 // True when a vertex of edge a is a mesh neighbour of a vertex of edge b
 // (vertex -> neighbour CSR voff / vnbr).
 __device__ __forceinline__ bool ring_neighbours(int2 a, int2 b, const int* __restrict__ voff,
@@ -223,6 +225,11 @@ __global__ __launch_bounds__(256) void k_ee_pairs(HashCfg g, const float4* __res
                                                   const int* __restrict__ voff, const int* __restrict__ vnbr,
                                                   int* __restrict__ cnt_or_off, uint32_t* __restrict__ oa,
                                                   uint32_t* __restrict__ ob) {
+    // t's vertices and their mesh neighbours, per thread in LDS: a triangle f
+    // with no vertex in this set shares no vertex with, and is outside the
+    // 2-ring of, every edge of t, so its edge pairs skip the exact tests
+    __shared__ int ring[256][EE_RING + 1];
+    int* rt = ring[threadIdx.x];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     int n = 0, off = EMIT ? cnt_or_off[t] : 0;
@@ -231,6 +238,18 @@ __global__ __launch_bounds__(256) void k_ee_pairs(HashCfg g, const float4* __res
         tri_cell(g, p, tris, t, gp);
         int te[3] = {tedge[3 * t], tedge[3 * t + 1], tedge[3 * t + 2]};
         int2 tv[3] = {edges[te[0]], edges[te[1]], edges[te[2]]};
+        int nr = 0;
+        bool exact_all = false;
+        for (int k = 0; k < 3 && !exact_all; k++) {
+            const int v = tris[3 * t + k];
+            const int b0 = voff[v], b1 = voff[v + 1];
+            if (nr + 1 + (b1 - b0) > EE_RING) {
+                exact_all = true;
+                break;
+            }
+            rt[nr++] = v;
+            for (int j = b0; j < b1; j++) rt[nr++] = vnbr[j];
+        }
         for (int z = -1; z <= 1; z++)
             for (int y = -1; y <= 1; y++)
                 for (int x = -1; x <= 1; x++) {
@@ -245,13 +264,26 @@ __global__ __launch_bounds__(256) void k_ee_pairs(HashCfg g, const float4* __res
                         if (f <= t) continue;
                         int4 fc = tcell[f];
                         if (fc.x != qx || fc.y != qy || fc.z != qz) continue;
+                        bool exact = exact_all;
+                        if (!exact) {
+                            const int f0 = tris[3 * f], f1 = tris[3 * f + 1], f2 = tris[3 * f + 2];
+                            for (int k = 0; k < nr; k++) {
+                                const int u = rt[k];
+                                if (u == f0 || u == f1 || u == f2) {
+                                    exact = true;
+                                    break;
+                                }
+                            }
+                        }
                         for (int kb = 0; kb < 3; kb++) {
                             int eb = tedge[3 * f + kb];
                             int2 bv = edges[eb];
                             for (int ka = 0; ka < 3; ka++) {
                                 int2 av = tv[ka];
-                                if (av.x == bv.x || av.x == bv.y || av.y == bv.x || av.y == bv.y) continue;
-                                if (ring_neighbours(av, bv, voff, vnbr)) continue;
+                                if (exact) {
+                                    if (av.x == bv.x || av.x == bv.y || av.y == bv.x || av.y == bv.y) continue;
+                                    if (ring_neighbours(av, bv, voff, vnbr)) continue;
+                                }
                                 if (EMIT) {
                                     int ea = te[ka];
                                     oa[off + n] = (uint32_t)(ea < eb ? ea : eb);
