@@ -183,3 +183,27 @@ def test_load_cache_errors(tmp_path):
         _expect_io_error(ctx, m, write("k4", good_k.replace("14,1;15,1;", "14,1;15,3;")))
         # nothing was added by the failed loads
         assert ctx.cloths == []
+
+
+def test_cache_of_a_batch_member_round_trips(tmp_path):
+    # save the middle cloth of a 3-cloth batch; load it as the second cloth of
+    # another batch: its state and stepping match the original's
+    say("cache round trip of a batch member")
+    m, k = tmp_path / "mesh.cache", tmp_path / "constr.cache"
+    with capi.Context() as a, capi.Context() as b:
+        a.add_grid(9, 7, 0.5, 0.4, jitter=1e-3, seed=1)
+        ca = bent_cloth(a)
+        a.add_grid(5, 5, 0.2, 0.2, jitter=1e-3, seed=2)
+        a.init()
+        a.set_solver(dt=DT, iterations=20)
+        a.step(5)
+        a.save_cache(ca, str(m), str(k), 9)
+        b.add_grid(17, 4, 0.6, 0.2, jitter=1e-3, seed=3)
+        cb = b.load_cache(str(m), str(k))
+        b.init()
+        b.set_solver(dt=DT, iterations=20)
+        for s in (capi.STATE_X, capi.STATE_V, capi.STATE_P):
+            assert np.array_equal(a.get(ca, s), b.get(cb, s)), s
+        a.step(6)
+        b.step(6)
+        assert np.array_equal(a.get(ca), b.get(cb))
