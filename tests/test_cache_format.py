@@ -136,3 +136,104 @@ def test_writer_then_reader_round_trip():
     c6 = cf.read_cache(cf.mesh_text(x, tris, v, w, 0.95, (0.0, -9.8, 0.0)), None)
     assert np.array_equal(c6["x"], np.array([[np.float32(float("%.6g" % float(a))) for a in r] for r in x],
                                             np.float32))
+
+
+# ---- the library's own reader and writer (cache_io.cpp), host code only ----
+
+ROOT = HERE.parent
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    hipcc = shutil.which("hipcc") or ("/opt/rocm/bin/hipcc" if Path("/opt/rocm/bin/hipcc").exists() else None)
+    if hipcc is None:
+        pytest.skip("hipcc not available")
+    exe = tmp_path_factory.mktemp("harness") / "cache_harness"
+    src = ROOT / "positionbaseddynamicsold_amd" / "csrc"
+    subprocess.run([hipcc, "-O1", "-std=c++17", "-pthread", f"-I{ROOT / 'include'}", str(HERE / "cache_harness.cpp"),
+                    str(src / "cache_io.cpp"), str(src / "model.cpp"), "-o", str(exe)], check=True,
+                   capture_output=True)
+    return exe
+
+
+def run_harness(exe, *args):
+    r = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True)
+    out = {}
+    for line in r.stdout.splitlines():
+        if "=" in line:
+            k, v = line.split("=", 1)
+            out[k] = v
+    return r.returncode, out, r.stdout
+
+
+def arr(s, dtype=np.float32):
+    return np.array([float(t) for t in s.split(",")] if s else [], dtype)
+
+
+def test_library_reader_on_the_reference_fixture(harness):
+    mesh, constr = GOLD / "grid3x2_meshtopol.cache", GOLD / "grid3x2_constraint.cache"
+    rc, out, _ = run_harness(harness, "read", mesh, constr)
+    assert rc == 0
+    c = cf.read_cache(mesh.read_bytes().decode(), constr.read_bytes().decode())
+    assert (int(out["nv"]), int(out["nt"]), int(out["ne"]), int(out["nb"])) == (6, 4, 7, 0)
+    assert int(out["ct"]) == cf.DISTANCE | cf.ANCHOR
+    assert np.array_equal(arr(out["x"]).reshape(-1, 3), c["x"])
+    assert np.array_equal(arr(out["v"]).reshape(-1, 3), c["v"])
+    assert np.array_equal(arr(out["p"]).reshape(-1, 3), c["p"])
+    assert np.array_equal(arr(out["pos0"]).reshape(-1, 3), c["pos0"])
+    assert np.array_equal(arr(out["rest"]), c["rest"])
+    assert arr(out["indices"], np.int64).reshape(-1, 2).tolist() == c["pairs"].tolist()
+    assert arr(out["tris"], np.int64).reshape(-1, 3).tolist() == c["tris"].tolist()
+    assert arr(out["pins"], np.int64).tolist() == c["pins"]
+    w = c["w"].copy()
+    w[c["pins"]] = 0.0
+    assert np.array_equal(arr(out["w"]), w)
+    assert np.float32(float(out["damping"])) == c["damping"]
+    assert np.array_equal(arr(out["gravity"]), c["gravity"])
+
+
+@pytest.mark.parametrize("digits", [0, 9])
+def test_library_writer_equals_oracle_writer(harness, tmp_path, digits):
+    mesh, constr = GOLD / "grid3x2_meshtopol.cache", GOLD / "grid3x2_constraint.cache"
+    om, ok = tmp_path / "m.cache", tmp_path / "k.cache"
+    rc, _, _ = run_harness(harness, "rewrite", mesh, constr, digits, om, ok)
+    assert rc == 0
+    _, out, _ = run_harness(harness, "read", mesh, constr)
+    x, v, p = (arr(out[k]).reshape(-1, 3) for k in ("x", "v", "p"))
+    w = arr(out["w"])
+    d = digits or 6
+    prim_list = arr(out["prim_list"], np.int64).reshape(-1, 2)
+    exp_m = cf.mesh_text(x, arr(out["tris"], np.int64).reshape(-1, 3), v, w, np.float32(float(out["damping"])),
+                         arr(out["gravity"]), d)
+    exp_k = cf.constraint_text(arr(out["pos0"]).reshape(-1, 3), p, prim_list, arr(out["indices"], np.int64),
+                               arr(out["rest"]), int(out["ne"]), w, arr(out["stiffness"]).tolist(), d)
+    assert om.read_bytes().decode() == exp_m
+    assert ok.read_bytes().decode() == exp_k
+    # the rewritten files read back to the same cloth
+    _, again, _ = run_harness(harness, "read", om, ok)
+    for k in ("x", "v", "p", "rest", "w", "indices", "pins") if digits == 9 else ("indices", "pins"):
+        assert again[k] == out[k], k
+
+
+def test_library_reader_rejects_malformed_files(harness, tmp_path):
+    good_m = (GOLD / "grid3x2_meshtopol.cache").read_text()
+    good_k = (GOLD / "grid3x2_constraint.cache").read_text()
+    cases = [
+        (good_m.replace("0.5,-0.0012", "0.5,-0.00x2"), None, "malformed number"),
+        (good_m.replace("massBuffer|float|1;1;1;1;1;1;", "massBuffer|float|1;1;"), None, "massBuffer count"),
+        (good_m.replace("0;1;3;1;4;3;", "0;1;9;1;4;3;"), None, "triangle index"),
+        (good_m, good_k.replace("256;256;", "256;512;"), "unknown constraintType"),
+        (good_m, good_k.replace("1;1;1;1;1;1;1;-1;-1;", "1;1;0.5;1;1;1;1;-1;-1;"), "stiffness differs"),
+        (good_m, good_k.replace("0.5;0.5;0.5;0.5;0.5;0.5;0.5;", "0.5;0.5;"), "fewer rest lengths"),
+        (good_m, good_k.replace("14,1;15,1;", "14,1;15,3;"), "outside the index list"),
+    ]
+    for i, (m, k, msg) in enumerate(cases):
+        mp = tmp_path / f"m{i}.cache"
+        mp.write_text(m)
+        args = [mp]
+        if k is not None:
+            kp = tmp_path / f"k{i}.cache"
+            kp.write_text(k)
+            args.append(kp)
+        rc, _, text = run_harness(harness, "read", *args)
+        assert rc == 3 and text.startswith("error -5") and msg in text, (i, text)
