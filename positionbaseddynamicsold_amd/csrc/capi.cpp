@@ -65,6 +65,7 @@ struct pbd_ctx {
     DBuf<float4> sx, sp4;
     DBuf<int> stris;
     int snv = 0, snt = 0;
+    uint64_t scene_hash = 0;  // of the standalone scene's x, p and triangles (scene_fingerprint)
     bool sh_built = false, ccd_built = false;
     // in-solver collision
     pbd_collision_params coll{};
@@ -532,6 +533,24 @@ HashCfg make_hash_cfg(const pbd_hash_desc& d) {
     return g;
 }
 
+// FNV-1a over the bytes of a standalone scene (x, p, triangles, sizes): tells
+// whether a detection runs on the scene of the previous one.
+uint64_t scene_fingerprint(const float* x, const float* p, int nv, const int* tris, int nt) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* d, size_t n) {
+        const unsigned char* b = (const unsigned char*)d;
+        for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    mix(&nv, sizeof nv);
+    mix(&nt, sizeof nt);
+    if (nv > 0) {
+        mix(x, sizeof(float) * 3 * (size_t)nv);
+        mix(p, sizeof(float) * 3 * (size_t)nv);
+    }
+    if (nt > 0) mix(tris, sizeof(int) * 3 * (size_t)nt);
+    return h;
+}
+
 void upload_scene(pbd_ctx* c, const float* x, const float* p, const float* w, int nv, const int* tris, int nt) {
     PBD_CHECK(nv >= 0 && nt >= 0 && (x || !nv) && (p || !nv) && (tris || !nt), PBD_ERR_ARG, "bad scene arrays");
     for (long long i = 0; i < 3ll * nt; i++) PBD_CHECK(tris[i] >= 0 && tris[i] < nv, PBD_ERR_ARG, "triangle index out of range");
@@ -546,6 +565,7 @@ void upload_scene(pbd_ctx* c, const float* x, const float* p, const float* w, in
     c->stris.upload(std::vector<int>(tris, tris + 3ll * nt), c->stream);
     c->snv = nv;
     c->snt = nt;
+    c->scene_hash = scene_fingerprint(x, p, nv, tris, nt);
 }
 
 // Contact detection for the whole context (all cloths, global vertex ids).
@@ -1659,9 +1679,10 @@ int pbd_ccd_detect_ee(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, cons
         set_device(ctx);
         hipStream_t s = ctx->stream;
         HashCfg g = make_hash_cfg(*d);
+        const uint64_t prev_hash = ctx->scene_hash;
         // vertex-face contacts of an earlier detection on another scene do not carry over
-        bool same = ctx->ccd_built && ctx->snv == nv && ctx->snt == nt;
         upload_scene(ctx, x, p, nullptr, nv, tris, nt);
+        bool same = ctx->ccd_built && ctx->scene_hash == prev_hash;
         if (!same) {
             ctx->sbp.npairs = 0;
             ctx->sbp.ncontacts = 0;

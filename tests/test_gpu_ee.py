@@ -179,3 +179,25 @@ def test_config3_full_size_with_edge_edge():
     say(f"  edge-edge contacts {prof.ccd_ee_contacts}, vertex-face {prof.ccd_contacts} (without edge-edge "
         f"{prof0.ccd_contacts}); max |v| {np.abs(v).max():.3f} m/s (without {np.abs(v0).max():.3f}); "
         f"max |x - x without| {np.abs(x - x0).max():.4f} m")
+
+
+def test_ee_detection_on_another_scene_drops_vertex_face_contacts(orc):
+    # pbd_ccd_detect_ee adds to the vertex-face contacts of the same scene
+    # only: after a vertex-face detection on another scene of the same size,
+    # the response uses the edge-edge contacts alone
+    say("edge-edge detection after a vertex-face detection on another scene")
+    x, p, tris, na = crossing_sheets(orc)
+    p2 = p.copy()
+    p2[na:, 1] += np.float32(0.002)
+    th = 0.004
+    d = capi.hash_desc(1, cell=(0.1, 0.1, 0.1), table_bits=12)
+    g = orc.grid(1, cell=(0.1, 0.1, 0.1), table_bits=12)
+    wts = np.ones(len(x), np.float32)
+    wts[:na] = 0.0
+    with capi.Context() as ctx:
+        ctx.ccd_detect(d, x, p, tris, th)
+        q, _, _, _ = ctx.ccd_detect_ee(d, x, p2, tris, th)
+        oq = orc.ccd_detect_ee(g, x, p2, tris, th)[0]
+        assert len(q) > 50 and np.array_equal(q, oq)
+        r = ctx.ccd_resolve(p2, wts, th, 2)
+        assert np.array_equal(r, orc.ccd_resolve_ee(None, 0, oq, tris, x, p2, wts, th, 2))
