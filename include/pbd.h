@@ -250,7 +250,8 @@ PBD_API int pbd_ccd_get_contacts(pbd_ctx* ctx, long long cap, int* vf, float* t,
 /* Edge-edge contacts (Contact::EE, declared but not implemented in the
  * reference: CCD_Basic.h:14, CCD.cpp:925-928), added to the vertex-face
  * contacts of a pbd_ccd_detect / pbd_ccd_detect_swept call on the same
- * scene: the same x, p and triangles, byte for byte (none otherwise).  Candidates: every edge pair of the triangle
+ * scene and settings: the same x, p and triangles, byte for byte, the same
+ * thickness and the same grid description (none otherwise).  Candidates: every edge pair of the triangle
  * pairs pbd_ccd_detect visits (centroid cells of desc) that shares no vertex
  * and has no vertex that is a mesh neighbour of the other edge's.
  * Test: the first time in [0, 1] (coplanarity roots of the four points, then

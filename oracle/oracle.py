@@ -352,6 +352,10 @@ def _sig_collide(L):
     L.orc_ccd_detect_swept.restype = C.c_longlong
     L.orc_ccd_detect_swept.argtypes = [_fp, _fp, _ip, C.c_int, C.c_int, C.c_float, C.c_int, C.c_longlong,
                                        C.POINTER(OrcContact)]
+    L.orc_vf_cross.restype = C.c_int
+    L.orc_vf_cross.argtypes = [_fp] * 8 + [C.c_float]
+    L.orc_crossings.restype = C.c_longlong
+    L.orc_crossings.argtypes = [_fp, _fp, _ip, C.c_int, C.c_int, C.c_float, _fp, C.c_longlong, _i64p]
     L.orc_ccd_resolve.restype = None
     L.orc_ccd_resolve.argtypes = [C.POINTER(OrcContact), C.c_longlong, _ip, _fp, _fp, _fp, C.c_float, C.c_int,
                                   C.c_float]
@@ -442,6 +446,22 @@ def swept_pairs(x, p, tris, thickness, all_pairs=False):
     out = np.zeros(max(n, 1), np.int64)
     L.orc_swept_pairs(F(x), F(p), I(tris), len(tris), len(x), thickness, int(all_pairs), n, out.ctypes.data_as(_i64p))
     return out[:n]
+
+
+def crossings(x, p, tris, thickness, w=None):
+    """(v, f) pairs whose vertex crosses the face between x and p
+    (NarrowVFCCDTest, the reference's post-resolve check), ascending; with w,
+    pairs of four static (w == 0) vertices are skipped."""
+    L = lib()
+    _sig_collide(L)
+    x = np.ascontiguousarray(x, np.float32)
+    p = np.ascontiguousarray(p, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    wp = None if w is None else F(np.ascontiguousarray(w, np.float32))
+    n = L.orc_crossings(F(x), F(p), I(tris), len(tris), len(x), thickness, wp, 0, None)
+    out = np.zeros(max(n, 1), np.int64)
+    L.orc_crossings(F(x), F(p), I(tris), len(tris), len(x), thickness, wp, n, out.ctypes.data_as(_i64p))
+    return np.stack([out[:n] >> 32, out[:n] & 0xffffffff], 1)
 
 
 def ccd_detect_swept(x, p, tris, thickness, all_pairs=False):

@@ -1144,6 +1144,76 @@ int orc_vf_ccd(const float* x0, const float* x1, const float* x2, const float* x
     return 0;
 }
 
+/* NarrowVFCCDTest  PBD/CCD.cpp:1810-1872: the vertex crosses the face during
+ * the substep.  Same coplanarity cubic as orc_vf_ccd; only its roots are
+ * tried (the reference appends t = 1 but its loop stops before it, :1822-1825),
+ * and at the first root in [0, 1] where the vertex is within 1e-6 of the
+ * face's plane with barycentric weights in [-eps, 1 + eps] (eps = thickness /
+ * mean edge of the end-state face, __USE_EPSILON = 1, CCD_Basic.h:10) the
+ * pair crosses.  Unset or non-finite roots are skipped, as in orc_vf_ccd. */
+int orc_vf_cross(const float* x0, const float* x1, const float* x2, const float* x3,
+                 const float* p0, const float* p1, const float* p2, const float* p3, float thickness) {
+    float v0[3], X1[3], X2[3], X3[3], V1[3], V2[3], V3[3], tmp[3];
+    v3sub(p0, x0, v0);
+    v3sub(x1, x0, X1); v3sub(x2, x0, X2); v3sub(x3, x0, X3);
+    v3sub(p1, x1, tmp); v3sub(tmp, v0, V1);
+    v3sub(p2, x2, tmp); v3sub(tmp, v0, V2);
+    v3sub(p3, x3, tmp); v3sub(tmp, v0, V3);
+    double a0 = stp(X1, X2, X3);
+    double a1 = (double)(stp(V1, X2, X3) + stp(X1, V2, X3) + stp(X1, X2, V3));
+    double a2 = (double)(stp(X1, V2, V3) + stp(V1, X2, V3) + stp(V1, V2, X3));
+    double a3 = stp(V1, V2, V3);
+    double t[3];
+    int nsol = orc_solve_cubic(a3, a2, a1, a0, t);
+    for (int i = 0; i < nsol; i++) {
+        if (!isfinite(t[i]) || t[i] < 0 || t[i] > 1) continue;
+        float c0[3], c1[3], c2[3], c3[3], e1[3], e2[3], cr[3], cn[3], rel[3], wgt[3];
+        lerp3(x0, p0, t[i], c0); lerp3(x1, p1, t[i], c1);
+        lerp3(x2, p2, t[i], c2); lerp3(x3, p3, t[i], c3);
+        v3sub(c2, c1, e1); v3sub(c3, c1, e2); v3cross(e1, e2, cr);
+        v3normalize(cr, cn);
+        v3sub(c0, c1, rel);
+        double d = v3dot(rel, cn);
+        orc_barycentric(c0, c1, c2, c3, wgt);
+        float ed1 = orc_distance(p1, p2), ed2 = orc_distance(p1, p3), ed3 = orc_distance(p2, p3);
+        float eps = thickness / ((ed1 + ed2 + ed3) / 3.0);
+        int in1 = (wgt[0] <= (1 + eps)) && (wgt[0] >= (-eps));
+        int in2 = (wgt[1] <= (1 + eps)) && (wgt[1] >= (-eps));
+        int in3 = (wgt[2] <= (1 + eps)) && (wgt[2] >= (-eps));
+        if (fabs(d) < 1e-6 && in1 && in2 && in3) return 1;
+    }
+    return 0;
+}
+
+/* The reference's post-resolve check, CCD_SH_Narrow (PBD/CCD.cpp:1588-1652,
+ * run after CollisionResolveNew at PositionBasedDynamics.cpp:437-441): the
+ * (v, f) pairs, v not in f, that cross between x and p by orc_vf_cross.
+ * Candidates are the swept-box set, which holds every crossing pair (the
+ * vertex is on the face at the crossing, so the two boxes over the substep
+ * meet).  With w, pairs whose four vertices are all static (w == 0) are
+ * skipped: static meshes with duplicated vertices at their creases (a box
+ * made of six grids) touch themselves without any motion.  Writes up to cap
+ * keys (v << 32 | f), ascending; returns the count. */
+long long orc_crossings(const float* x, const float* p, const int* tris, int nt, int nv, float thickness,
+                        const float* w, long long cap, int64_t* out) {
+    long long nk = orc_swept_pairs(x, p, tris, nt, nv, thickness, 0, 0, NULL);
+    int64_t* keys = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nk + 1));
+    orc_swept_pairs(x, p, tris, nt, nv, thickness, 0, nk, keys);
+    long long n = 0;
+    for (long long i = 0; i < nk; i++) {
+        int v = (int)(keys[i] >> 32), f = (int)(keys[i] & 0xffffffff);
+        const int* fv = tris + 3 * f;
+        if (w && w[v] == 0.0f && w[fv[0]] == 0.0f && w[fv[1]] == 0.0f && w[fv[2]] == 0.0f) continue;
+        if (orc_vf_cross(x + 3 * v, x + 3 * fv[0], x + 3 * fv[1], x + 3 * fv[2], p + 3 * v, p + 3 * fv[0],
+                         p + 3 * fv[1], p + 3 * fv[2], thickness)) {
+            if (out && n < cap) out[n] = keys[i];
+            n++;
+        }
+    }
+    free(keys);
+    return n;
+}
+
 /* VFResolveTest  PBD/CCD.cpp:1996-2032.  min_n2 is the degenerate-face
  * bound on |(p2 - p1) x (p3 - p1)|^2, fixed at 1e-6 in the reference
  * (PBD/CCD.cpp:2008). */

@@ -147,6 +147,10 @@ typedef struct {
 
 long long orc_swept_pairs(const float* x, const float* p, const int* tris, int nt, int nv, float thickness,
                           int all_pairs, long long cap, int64_t* out);
+int orc_vf_cross(const float* x0, const float* x1, const float* x2, const float* x3,
+                 const float* p0, const float* p1, const float* p2, const float* p3, float thickness);
+long long orc_crossings(const float* x, const float* p, const int* tris, int nt, int nv, float thickness,
+                        const float* w, long long cap, int64_t* out);
 long long orc_ccd_detect_swept(const float* x, const float* p, const int* tris, int nt, int nv, float thickness,
                                int all_pairs, long long cap, orc_contact* out);
 long long orc_ccd_detect(const orc_grid* g, const float* x, const float* p, const int* tris, int nt,

@@ -3,7 +3,8 @@
 Usage: python -m positionbaseddynamicsold_amd.build   (or build() from Python)
 
 Every source under csrc/ is compiled by hipcc into _build/ objects in
-parallel and linked into positionbaseddynamicsold_amd/libpbd.so.
+parallel and linked into positionbaseddynamicsold_amd/libpbd.so; then the
+C++ example caller tests/cpp_driver.cpp is linked against it.
 
 -ffp-contract=off keeps every float expression a sequence of IEEE operations
 in source order (no FMA contraction), which is what lets the device results
@@ -61,6 +62,25 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+DRIVER_SRC = ROOT / "tests" / "cpp_driver.cpp"
+DRIVER = ROOT / "tests" / "cpp_driver"
+
+
+def build_driver(verbose: bool = True) -> Path:
+    """The C++ caller of the C ABI (tests/cpp_driver.cpp), linked against
+    libpbd.so with an rpath relative to its own location."""
+    tmp = DRIVER.with_name(DRIVER.name + ".tmp")
+    cmd = [hipcc(), "-x", "c++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(DRIVER_SRC), "-o",
+           str(tmp), f"-L{PKG}", "-lpbd", "-Wl,-rpath,$ORIGIN/../positionbaseddynamicsold_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
+
+
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     srcs = sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
@@ -75,6 +95,7 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_driver(verbose)
     return LIB
 
 

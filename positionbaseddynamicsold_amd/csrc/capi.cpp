@@ -65,7 +65,7 @@ struct pbd_ctx {
     DBuf<float4> sx, sp4;
     DBuf<int> stris;
     int snv = 0, snt = 0;
-    uint64_t scene_hash = 0;  // of the standalone scene's x, p and triangles (scene_fingerprint)
+    uint64_t scene_hash = 0;  // of the standalone scene and its detection settings (scene_fingerprint)
     bool sh_built = false, ccd_built = false;
     // in-solver collision
     pbd_collision_params coll{};
@@ -222,7 +222,7 @@ int effective_tile_sweeps(const pbd_ctx* c) {
 }
 
 int effective_max_load(const pbd_ctx* c) {
-    int lim = (c->lds_max - 256) / 16;
+    int lim = (c->lds_max - TILE_STATIC_LDS) / 16;
     if (c->sp.tile_max_verts > 0) lim = std::min(lim, c->sp.tile_max_verts);
     return std::max(64, lim);
 }
@@ -533,16 +533,33 @@ HashCfg make_hash_cfg(const pbd_hash_desc& d) {
     return g;
 }
 
-// FNV-1a over the bytes of a standalone scene (x, p, triangles, sizes): tells
-// whether a detection runs on the scene of the previous one.
-uint64_t scene_fingerprint(const float* x, const float* p, int nv, const int* tris, int nt) {
+// FNV-1a over 8-byte words of a standalone scene and the settings of its
+// detection (x, p, triangles, sizes, thickness, grid description): tells
+// whether an edge-edge detection runs on the scene and settings of the
+// previous vertex-face one.
+uint64_t scene_fingerprint(const float* x, const float* p, int nv, const int* tris, int nt, float th,
+                           const pbd_hash_desc& d) {
     uint64_t h = 1469598103934665603ull;
-    auto mix = [&](const void* d, size_t n) {
-        const unsigned char* b = (const unsigned char*)d;
-        for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    auto mix = [&](const void* src, size_t n) {
+        const unsigned char* b = (const unsigned char*)src;
+        size_t i = 0;
+        for (; i + 8 <= n; i += 8) {
+            uint64_t w;
+            memcpy(&w, b + i, 8);
+            h = (h ^ w) * 1099511628211ull;
+        }
+        uint64_t tail = 0;
+        memcpy(&tail, b + i, n - i);
+        h = (h ^ tail ^ ((uint64_t)(n - i) << 56)) * 1099511628211ull;
     };
     mix(&nv, sizeof nv);
     mix(&nt, sizeof nt);
+    mix(&th, sizeof th);
+    mix(&d.mode, sizeof d.mode);
+    mix(d.centre, sizeof d.centre);
+    mix(d.size, sizeof d.size);
+    mix(d.cell, sizeof d.cell);
+    mix(&d.table_bits, sizeof d.table_bits);
     if (nv > 0) {
         mix(x, sizeof(float) * 3 * (size_t)nv);
         mix(p, sizeof(float) * 3 * (size_t)nv);
@@ -551,7 +568,8 @@ uint64_t scene_fingerprint(const float* x, const float* p, int nv, const int* tr
     return h;
 }
 
-void upload_scene(pbd_ctx* c, const float* x, const float* p, const float* w, int nv, const int* tris, int nt) {
+void upload_scene(pbd_ctx* c, const float* x, const float* p, const float* w, int nv, const int* tris, int nt,
+                  float th = 0.0f, const pbd_hash_desc* d = nullptr) {
     PBD_CHECK(nv >= 0 && nt >= 0 && (x || !nv) && (p || !nv) && (tris || !nt), PBD_ERR_ARG, "bad scene arrays");
     for (long long i = 0; i < 3ll * nt; i++) PBD_CHECK(tris[i] >= 0 && tris[i] < nv, PBD_ERR_ARG, "triangle index out of range");
     std::vector<float4> X(nv), P(nv);
@@ -565,7 +583,7 @@ void upload_scene(pbd_ctx* c, const float* x, const float* p, const float* w, in
     c->stris.upload(std::vector<int>(tris, tris + 3ll * nt), c->stream);
     c->snv = nv;
     c->snt = nt;
-    c->scene_hash = scene_fingerprint(x, p, nv, tris, nt);
+    c->scene_hash = scene_fingerprint(x, p, nv, tris, nt, th, d ? *d : pbd_hash_desc{});
 }
 
 // Contact detection for the whole context (all cloths, global vertex ids).
@@ -861,8 +879,10 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     int nchunks = (iterations + ts - 1) / ts;
     // Tiles integrate their cloth on storing its last sweep when budgets are
     // fixed and no launch that can end a cloth reads x: the advect-on-load
-    // launch (the first) ends none when there are at least two chunks.
-    const bool int_store = integrate_end && !any_tol && !(fused_whole && c->any_partial && nchunks < 2);
+    // launch (the first) ends none when there are at least two chunks and no
+    // per-cloth budget can end a cloth inside it.
+    const bool int_store =
+        integrate_end && !any_tol && !(fused_whole && c->any_partial && (nchunks < 2 || cap != nullptr));
     ensure_slots(c, nchunks);
     launch_fill_u64(s, c->res_slots.ptr, (size_t)c->nslots * ncl, 0ull);
     size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
@@ -1269,7 +1289,9 @@ int pbd_create(int device, pbd_ctx** out) {
             PBD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             int lds = 0;
             PBD_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-            c->lds_max = lds > 0 ? lds : 65536;
+            // lds_max - TILE_STATIC_LDS is what the tile kernel's dynamic LDS may reach
+            const size_t dyn = tiles_lds_optin((size_t)std::max(lds, 65536) - TILE_STATIC_LDS);
+            c->lds_max = (int)dyn + TILE_STATIC_LDS;
         } catch (...) {
             delete c;
             throw;
@@ -1660,7 +1682,7 @@ int pbd_ccd_detect(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, const f
         set_device(ctx);
         hipStream_t s = ctx->stream;
         HashCfg g = make_hash_cfg(*d);
-        upload_scene(ctx, x, p, nullptr, nv, tris, nt);
+        upload_scene(ctx, x, p, nullptr, nv, tris, nt, th, d);
         ctx->sbp.build(s, g, ctx->sp4.ptr, ctx->stris.ptr, nt);
         ctx->sbp.pairs(s, ctx->sp4.ptr, ctx->stris.ptr, nv);
         *nc = ctx->sbp.narrow(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, th);
@@ -1680,8 +1702,9 @@ int pbd_ccd_detect_ee(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, cons
         hipStream_t s = ctx->stream;
         HashCfg g = make_hash_cfg(*d);
         const uint64_t prev_hash = ctx->scene_hash;
-        // vertex-face contacts of an earlier detection on another scene do not carry over
-        upload_scene(ctx, x, p, nullptr, nv, tris, nt);
+        // vertex-face contacts of an earlier detection on another scene, or
+        // at another thickness or grid, do not carry over
+        upload_scene(ctx, x, p, nullptr, nv, tris, nt, th, d);
         bool same = ctx->ccd_built && ctx->scene_hash == prev_hash;
         if (!same) {
             ctx->sbp.npairs = 0;
@@ -1753,7 +1776,7 @@ int pbd_ccd_detect_swept(pbd_ctx* ctx, const pbd_hash_desc* d, const float* x, c
         set_device(ctx);
         hipStream_t s = ctx->stream;
         HashCfg g = make_hash_cfg(*d);
-        upload_scene(ctx, x, p, nullptr, nv, tris, nt);
+        upload_scene(ctx, x, p, nullptr, nv, tris, nt, th, d);
         long long np = ctx->sbp.pairs_swept(s, g, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, nt, nv, th);
         if (npairs) *npairs = np;
         *nc = ctx->sbp.narrow(s, ctx->sx.ptr, ctx->sp4.ptr, ctx->stris.ptr, th);

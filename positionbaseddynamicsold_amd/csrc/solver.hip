@@ -629,6 +629,20 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     }
 }
 
+// Dynamic LDS the tile kernel may use on the current device: opts in to
+// `bytes` and returns it, or returns the 64 KiB every launch may use without
+// an opt-in when the runtime refuses it.  Every context on a device asks for
+// the same amount (the device's per-workgroup maximum less the kernel's
+// static LDS), so contexts never lower each other's limit.
+size_t tiles_lds_optin(size_t bytes) {
+    if (bytes <= 65536) return bytes;
+    const hipError_t e =
+        hipFuncSetAttribute((const void*)k_tiles, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) return bytes;
+    (void)hipGetLastError();  // the refusal is handled by the smaller limit
+    return 65536;
+}
+
 void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* tiles, const int* verts,
                   const uint32_t* dist, const float* dist_rest, const uint32_t* bend, const float* bend_rest,
                   const uint32_t* own, const float* own_rest, const int* cols, const ClothDev* cloths,
@@ -636,13 +650,6 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   float4* p_out, unsigned long long* res_slots, int ncl, int* sweeps_out, const int* sweep_cap,
                   const SolveArgs& a, int threads, size_t lds_bytes) {
     if (nblocks <= 0) return;
-    static size_t attr_bytes = 0;
-    if (lds_bytes > attr_bytes) {
-        // not every runtime needs or supports the opt-in; the launch reports real failures
-        (void)hipFuncSetAttribute((const void*)k_tiles, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        (void)hipGetLastError();
-        attr_bytes = lds_bytes;
-    }
     hipLaunchKernelGGL(k_tiles, dim3(nblocks), dim3(threads), lds_bytes, s, order, tiles, verts, dist, dist_rest,
                        bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v, p_in, p_out, res_slots,
                        ncl, sweeps_out, sweep_cap, a);

@@ -177,3 +177,45 @@ def test_config3_full_size_properties(robust):
         assert not inside_box(x).any()
     x2, v2, _ = full_run(sc, 24)
     assert np.array_equal(x, x2) and np.array_equal(v, v2)
+
+
+# The reference's own check of the response: CCD_SH_Narrow (CCD.cpp:1588-1652)
+# run after CollisionResolveNew (PositionBasedDynamics.cpp:437-441) reports
+# every vertex that still crosses a face (NarrowVFCCDTest, CCD.cpp:1810-1872).
+# Here it runs on the GPU's positions at the start and end of each collision
+# substep (x, then the resolved and integrated p), over every vertex-face
+# pair that moves.  With the robust handling nothing crosses.  With the
+# reference's handling the reduced config 3 keeps 9 crossings (substeps 21
+# and 24), where the centroid-cell candidates miss a face; the GPU equals the
+# oracle bit for bit, so the set is fixed (keys are 0-based substeps).
+@pytest.mark.parametrize("cfg,robust,expect", [("2", False, {}), ("2", True, {}), ("3", False, {20: None, 23: None}),
+                                               ("3", True, {})])
+def test_no_vertex_crosses_a_face_after_the_response(orc, cfg, robust, expect):
+    say(f"post-response crossings config{cfg} robust={robust}")
+    sc = scenes.config2(n=40, subdiv=3) if cfg == "2" else scenes.config3(n=40, box_div=8)
+    sc = sc.robust() if robust else sc
+    substeps = 160 if cfg == "2" else 24
+    c = sc.cloth
+    _, ctris = orc.grid_mesh(c.rx, c.ry, c.sx, c.sy)
+    ncl = c.rx * c.ry
+    tris = np.concatenate([ctris, sc.static_tris + ncl]).astype(np.int32)
+    w = np.ones(ncl + len(sc.static_pos), np.float32)
+    w[ncl:] = 0.0
+    w[list(c.pins)] = 0.0
+    with capi.Context() as ctx:
+        cid, sid, _ = scenes.add_ccd_scene(ctx, sc)
+        contacts, found = 0, {}
+        x0 = np.concatenate([ctx.get(cid), ctx.get(sid)])
+        for k in range(substeps):
+            ctx.step(1)
+            contacts += ctx.last_contacts()[0]
+            x1 = np.concatenate([ctx.get(cid), ctx.get(sid)])
+            cr = orc.crossings(x0, x1, tris, sc.thickness, w)
+            if len(cr):
+                found[k] = len(cr)
+            x0 = x1
+    say(f"  contacts {contacts}, substeps with crossings {found}")
+    assert contacts > 0
+    assert set(found) == set(expect)
+    if expect:
+        assert sum(found.values()) == 9

@@ -181,11 +181,13 @@ def test_config3_full_size_with_edge_edge():
         f"max |x - x without| {np.abs(x - x0).max():.4f} m")
 
 
-def test_ee_detection_on_another_scene_drops_vertex_face_contacts(orc):
+@pytest.mark.parametrize("differs", ["positions", "thickness", "grid"])
+def test_ee_detection_on_another_scene_drops_vertex_face_contacts(orc, differs):
     # pbd_ccd_detect_ee adds to the vertex-face contacts of the same scene
-    # only: after a vertex-face detection on another scene of the same size,
-    # the response uses the edge-edge contacts alone
-    say("edge-edge detection after a vertex-face detection on another scene")
+    # and settings only: after a vertex-face detection on another scene of the
+    # same size, or on the same scene at another thickness or grid, the
+    # response uses the edge-edge contacts alone
+    say(f"edge-edge detection after a vertex-face detection with other {differs}")
     x, p, tris, na = crossing_sheets(orc)
     p2 = p.copy()
     p2[na:, 1] += np.float32(0.002)
@@ -195,7 +197,14 @@ def test_ee_detection_on_another_scene_drops_vertex_face_contacts(orc):
     wts = np.ones(len(x), np.float32)
     wts[:na] = 0.0
     with capi.Context() as ctx:
-        ctx.ccd_detect(d, x, p, tris, th)
+        if differs == "positions":
+            ctx.ccd_detect(d, x, p, tris, th)
+        elif differs == "thickness":
+            nvf = len(ctx.ccd_detect(d, x, p2, tris, 0.5 * th)[0])
+            assert nvf > 0
+        else:
+            nvf = len(ctx.ccd_detect(capi.hash_desc(1, cell=(0.2, 0.2, 0.2), table_bits=12), x, p2, tris, th)[0])
+            assert nvf > 0
         q, _, _, _ = ctx.ccd_detect_ee(d, x, p2, tris, th)
         oq = orc.ccd_detect_ee(g, x, p2, tris, th)[0]
         assert len(q) > 50 and np.array_equal(q, oq)

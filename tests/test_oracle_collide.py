@@ -183,6 +183,88 @@ def test_resolve_separates_to_thickness(orc):
             assert np.array_equal(q[1:4], p[1:4])  # static face untouched
 
 
+def test_crossings_closed_forms(orc):
+    # the reference's post-resolve check (NarrowVFCCDTest): a vertex through
+    # the interior of a static face crosses it at t = 1/2 (so do vertices 4
+    # and 5, which move with it over interior points); one stopping 1 mm short
+    # of it, or passing beyond the hypotenuse, does not
+    tris, x, p = moving_vertex_scene((0.2, 0.2), 1.0, -1.0)
+    assert orc.crossings(x, p, tris, 0.01).tolist() == [[0, 0], [4, 0], [5, 0]]
+    tris, x, p = moving_vertex_scene((0.2, 0.2), 1.0, 0.001)
+    assert len(orc.crossings(x, p, tris, 0.01)) == 0
+    tris, x, p = moving_vertex_scene((0.8, 0.8), 1.0, -1.0)
+    assert [0, 0] not in orc.crossings(x, p, tris, 0.01).tolist()
+    # a face swept past a static vertex crosses it too (vertex 0 is the static one)
+    tris, x, p = moving_vertex_scene((0.2, 0.2), 0.0, 0.0)
+    x[1:4, 2] = -1.0
+    p[1:4, 2] = 1.0
+    assert [0, 0] in orc.crossings(x, p, tris, 0.01).tolist()
+    # the response leaves a crossing vertex on its start side, 2h off the face
+    h = 0.01
+    tris, x, p = moving_vertex_scene((0.25, 0.25), 0.05, -0.001)
+    assert orc.crossings(x, p, tris, h).tolist() == [[0, 0]]
+    g = orc.grid(1, cell=(2.0, 2.0, 2.0), table_bits=8)
+    arr, n = orc.ccd_detect(g, x, p, tris, h)
+    q = orc.ccd_resolve(arr, n, tris, x, p, np.array([1, 0, 0, 0, 1, 1], np.float32), h, 2)
+    assert len(orc.crossings(x, q, tris, h)) == 0
+
+
+def contacts(orc, vf):
+    arr = (orc.OrcContact * len(vf))()
+    for i, (v, f) in enumerate(vf):
+        arr[i].v, arr[i].f = v, f
+    return arr
+
+
+def test_resolve_averages_two_contacts_of_one_vertex(orc):
+    # One free vertex that ended below two static faces: the plane y = 0
+    # (normal +y) and the plane x = 0 (normal +x).  It started above both.
+    # Each contact moves it to 2h off its plane, the whole way since the face
+    # is static (2 x the half fix); the vertex takes the mean of the two fixes:
+    #   h = 1/8, p = (-1/2, -1/4, 0): fixes (0, 2h + 1/4, 0) = (0, 1/2, 0) and
+    #   (2h + 1/2, 0, 0) = (3/4, 0, 0); mean (3/8, 1/4, 0) -> (-1/8, 0, 0).
+    h = 0.125
+    face_y = [(-1, 0, -1), (-1, 0, 1), (1, 0, -1)]    # (e1 x e2) = (0, 4, 0)
+    face_x = [(0, -1, -1), (0, 1, -1), (0, -1, 1)]    # (e1 x e2) = (4, 0, 0)
+    x = np.array(face_y + face_x + [(0.5, 0.5, 0.0)], np.float32)
+    p = x.copy()
+    p[6] = (-0.5, -0.25, 0.0)
+    tris = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    w = np.array([0, 0, 0, 0, 0, 0, 1], np.float32)
+    q = orc.ccd_resolve(contacts(orc, [(6, 0), (6, 1)]), 2, tris, x, p, w, h, 1)
+    assert q[6].tolist() == [-0.125, 0.0, 0.0]
+    assert np.array_equal(q[:6], p[:6])
+    # each contact alone moves the vertex by its own fix
+    q0 = orc.ccd_resolve(contacts(orc, [(6, 0)]), 1, tris, x, p, w, h, 1)
+    q1 = orc.ccd_resolve(contacts(orc, [(6, 1)]), 1, tris, x, p, w, h, 1)
+    assert q0[6].tolist() == [-0.5, 0.25, 0.0] and q1[6].tolist() == [0.25, -0.25, 0.0]
+
+
+def test_resolve_averages_face_fixes_of_two_contacts(orc):
+    # Two free vertices below one free face (normal +y, all weights 1), both
+    # started above it.  Each contact splits its fix: the vertex moves by half
+    # the gap to 2h, the face vertices by the other half along -n, spread by
+    # the vertex's barycentric weights b_k as b_k / sum(b^2).  A face vertex
+    # that both contacts move takes the mean of the two.
+    #   h = 1/8; face (0,0,0), (0,0,1), (1,0,0)
+    #   v1 at (1/4, -1/8, 1/4): b = (1/2, 1/4, 1/4), sum b^2 = 3/8,
+    #      half gap (2h + 1/8) / 2 = 3/16: v1 +3/16; face -1/4, -1/8, -1/8
+    #   v2 at (0, -3/8, 1/2): b = (1/2, 1/2, 0), sum b^2 = 1/2,
+    #      half gap (2h + 3/8) / 2 = 5/16: v2 +5/16; face -5/16, -5/16, 0
+    #   means: face -9/32, -7/32, -1/16
+    h = 0.125
+    x = np.array([(0, 0, 0), (0, 0, 1), (1, 0, 0), (0.25, 0.125, 0.25), (0.0, 0.125, 0.5)], np.float32)
+    p = x.copy()
+    p[3, 1] = -0.125
+    p[4, 1] = -0.375
+    tris = np.array([[0, 1, 2]], np.int32)
+    w = np.ones(5, np.float32)
+    q = orc.ccd_resolve(contacts(orc, [(3, 0), (4, 0)]), 2, tris, x, p, w, h, 1)
+    dy = (q - p)[:, 1].tolist()
+    assert dy == [-9 / 32, -7 / 32, -1 / 16, 3 / 16, 5 / 16]
+    assert np.array_equal((q - p)[:, [0, 2]], np.zeros((5, 2), np.float32))
+
+
 def falling_patch_scene(rng, n=7, big=3, drop=0.03):
     """A static plane of `big` x `big` large quads (two triangles each) and an
     n x n jittered patch of small triangles moving down through it."""
