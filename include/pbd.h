@@ -224,6 +224,22 @@ typedef struct {
  * (SpatialHashSystem.cpp:88-121): keys, stable sort, cell ranges. */
 PBD_API int pbd_sh_build(pbd_ctx* ctx, const pbd_hash_desc* desc, const float* pos, int nv, const int* tris,
                          int nt);
+/* pbd_sh_build for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It has no counterpart in the reference.  Every pointer is device memory of
+ * `device`: pos (3 floats per vertex), tris (3 ints per triangle) in; out the
+ * sorted keys (-1 first) and triangle ids and the unsorted keys (nt ints
+ * each), and the cell ranges (pbd_hash_desc cells, or 2^table_bits slots,
+ * ints each), as pbd_sh_get returns them.  The work is queued on `stream`
+ * (a hipStream_t; NULL is the null stream) and the call returns without
+ * waiting; the outputs are ready when the stream reaches that point.  It
+ * allocates nothing: the sort's scratch is one static device array of
+ * 8.5 MiB per GPU, held until the process ends, which limits nt to 2^20
+ * (PBD_ERR_LIMIT above).  Calls that overlap on one GPU race on that array.
+ * A triangle with a vertex index outside [0, nv) is keyed -1 (outside). */
+PBD_API int pbd_sh_build_noalloc(int device, void* stream, const pbd_hash_desc* desc, const float* pos, int nv,
+                                 const int* tris, int nt, int* sorted_keys, int* sorted_ids, int* raw_keys,
+                                 int* cell_start, int* cell_end);
 PBD_API int pbd_sh_num_cells(pbd_ctx* ctx, int* ncells);
 /* m_hTriangleHash/m_hTriangleId after sortTrianglesCPU, the unsorted keys of
  * calcHashCPU, and m_hCellStart/m_hCellEnd (SpatialHashSystem.cpp:300-389). */

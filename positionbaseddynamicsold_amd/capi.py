@@ -120,6 +120,7 @@ SIGNATURES = {
     "pbd_cloth_residual": [_P, _I, _dp, _dp],
     "pbd_cloth_last_sweeps": [_P, _I, _ip],
     "pbd_sh_build": [_P, C.POINTER(HashDesc), _fp, _I, _ip, _I],
+    "pbd_sh_build_noalloc": [_I, _P, C.POINTER(HashDesc), _P, _I, _P, _I, _P, _P, _P, _P, _P],
     "pbd_sh_num_cells": [_P, _ip],
     "pbd_sh_get": [_P, _ip, _ip, _ip, _ip, _ip],
     "pbd_sh_pairs": [_P, C.c_longlong, C.POINTER(C.c_int64), C.POINTER(C.c_longlong)],

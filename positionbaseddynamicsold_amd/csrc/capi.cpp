@@ -1623,6 +1623,34 @@ int pbd_sh_build(pbd_ctx* ctx, const pbd_hash_desc* d, const float* pos, int nv,
     });
 }
 
+int pbd_sh_build_noalloc(int device, void* stream, const pbd_hash_desc* d, const float* pos, int nv, const int* tris,
+                         int nt, int* sorted_keys, int* sorted_ids, int* raw_keys, int* cell_start, int* cell_end) {
+    return guard([&] {
+        PBD_CHECK(d, PBD_ERR_ARG, "null grid description");
+        PBD_CHECK(nv >= 0 && nt >= 0, PBD_ERR_ARG, "negative vertex or triangle count");
+        PBD_CHECK(nt <= SH_NOALLOC_MAX_TRIS, PBD_ERR_LIMIT,
+                  "pbd_sh_build_noalloc: " + std::to_string(nt) + " triangles exceed its maximum of " +
+                      std::to_string(SH_NOALLOC_MAX_TRIS));
+        PBD_CHECK(nt == 0 || (pos && tris && sorted_keys && sorted_ids && raw_keys), PBD_ERR_ARG,
+                  "null device array");
+        PBD_CHECK(cell_start && cell_end, PBD_ERR_ARG, "null cell range array");
+        const HashCfg g = make_hash_cfg(*d);
+        int n = 0;
+        PBD_HIP(hipGetDeviceCount(&n));
+        PBD_CHECK(device >= 0 && device < n, PBD_ERR_ARG, "device index out of range");
+        // work on `device`, then give the calling thread its current device back
+        int prev = 0;
+        PBD_HIP(hipGetDevice(&prev));
+        PBD_HIP(hipSetDevice(device));
+        struct Back {
+            int dev;
+            ~Back() { (void)hipSetDevice(dev); }
+        } back{prev};
+        sh_build_noalloc((hipStream_t)stream, device, g, pos, nv, tris, nt, sorted_keys, sorted_ids, raw_keys,
+                         cell_start, cell_end);
+    });
+}
+
 int pbd_sh_num_cells(pbd_ctx* ctx, int* ncells) {
     return guard([&] {
         PBD_CHECK(ctx && ncells, PBD_ERR_ARG, "null argument");

@@ -12,6 +12,8 @@
 //   fixed point (integer sums are order independent, so the result does not
 //   depend on atomic ordering) and applied as the mean fix per vertex.
 // Paths are relative to PositionBasedDynamicsCUDA/PositionBasedDynamicsCUDA/.
+#include <mutex>
+
 #include "pbd_internal.hpp"
 #include "pbd_math.hpp"
 
@@ -101,11 +103,19 @@ __device__ __forceinline__ int cell_key(const HashCfg& g, int x, int y, int z) {
     return (int)(h & ((1u << g.table_bits) - 1u));
 }
 
+// A vertex position from the library's float4 records or from 3 floats per
+// vertex (the C ABI's layout).
+__device__ __forceinline__ float4 vload(const float4* __restrict__ p, int i) { return p[i]; }
+__device__ __forceinline__ float4 vload(const float* __restrict__ p, int i) {
+    return make_float4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0f);
+}
+
 // centroid (a+b+c)/3 per component (evalTriCentroids) and its cell
 // (calcGridPosCPU: floor((c - start)/cell)).
-__device__ __forceinline__ void tri_cell(const HashCfg& g, const float4* __restrict__ p, const int* __restrict__ tris,
+template <typename P>
+__device__ __forceinline__ void tri_cell(const HashCfg& g, const P* __restrict__ p, const int* __restrict__ tris,
                                          int t, int* gp) {
-    float4 a = p[tris[3 * t]], b = p[tris[3 * t + 1]], c = p[tris[3 * t + 2]];
+    float4 a = vload(p, tris[3 * t]), b = vload(p, tris[3 * t + 1]), c = vload(p, tris[3 * t + 2]);
     float cx = (a.x + b.x + c.x) / 3, cy = (a.y + b.y + c.y) / 3, cz = (a.z + b.z + c.z) / 3;
     gp[0] = (int)floorf((cx - g.start[0]) / g.cell[0]);
     gp[1] = (int)floorf((cy - g.start[1]) / g.cell[1]);
@@ -1242,6 +1252,91 @@ void Broadphase::resolve(hipStream_t s, const float4* x, float4* p, const int* t
             hipLaunchKernelGGL(k_resolve_ee_acc, dim3(blocks(ee_ncontacts)), dim3(256), 0, s, ecq.ptr,
                                (int)ee_ncontacts, x, (const float4*)p, th, acc.ptr, vcnt.ptr);
         hipLaunchKernelGGL(k_resolve_apply, dim3(blocks(nv)), dim3(256), 0, s, acc.ptr, vcnt.ptr, p, nv);
+    }
+    PBD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Broadphase build without allocation (pbd_sh_build_noalloc).  The caller
+// brings device positions (3 floats per vertex), triangles and the output
+// arrays; the radix sort runs in the caller's key and id arrays, with its
+// ping-pong copies and digit counts in `scratch`, one static array per GPU
+// that the process keeps until it ends.  Every call overwrites what it uses
+// (digit counts and scans are written in full before they are read), so
+// nothing carries over between calls; calls that overlap on one GPU race on
+// it.
+// ---------------------------------------------------------------------------
+constexpr size_t SH_NOALLOC_HIST_INTS = 2 * ((size_t)256 * (SH_NOALLOC_MAX_TRIS / 4096)) +
+                                        ((size_t)256 * (SH_NOALLOC_MAX_TRIS / 4096) + 1023) / 1024 + 2;
+constexpr size_t SH_NOALLOC_WORDS = 2 * (size_t)SH_NOALLOC_MAX_TRIS + SH_NOALLOC_HIST_INTS;
+static __device__ __align__(256) uint32_t scratch[SH_NOALLOC_WORDS];
+
+size_t sh_noalloc_scratch_bytes() { return sizeof(uint32_t) * SH_NOALLOC_WORDS; }
+
+// Device address of this GPU's `scratch`, resolved once per device with the
+// device current.  The host-side symbol is not a device address: only the
+// resolved pointer is ever passed to a kernel.
+static void* scratch_address(int device) {
+    static std::mutex mu;
+    static std::vector<void*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= device) addr.resize((size_t)device + 1, nullptr);
+    if (!addr[device]) {
+        void* p = nullptr;
+        PBD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PBD_CHECK(p != nullptr, PBD_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        addr[device] = p;
+    }
+    return addr[device];
+}
+
+// k_tri_keys on 3-float positions; a triangle with a vertex index outside
+// [0, nv) is given key -1 (outside the grid) instead of being read.
+__global__ __launch_bounds__(256) void k_tri_keys_noalloc(HashCfg g, const float* __restrict__ p, int nv,
+                                                          const int* __restrict__ tris, int nt,
+                                                          uint32_t* __restrict__ keys1, uint32_t* __restrict__ ids,
+                                                          int* __restrict__ raw) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+    int k = -1;
+    if (a >= 0 && a < nv && b >= 0 && b < nv && c >= 0 && c < nv) {
+        int gp[3];
+        tri_cell(g, p, tris, t, gp);
+        k = cell_key(g, gp[0], gp[1], gp[2]);
+    }
+    keys1[t] = (uint32_t)(k + 1);
+    ids[t] = (uint32_t)t;
+    raw[t] = k;
+}
+
+// keys1 (key + 1, the sort's unsigned form) -> key, in place.
+__global__ __launch_bounds__(256) void k_keys_signed(uint32_t* __restrict__ keys1, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys1[i] = (uint32_t)((int)keys1[i] - 1);
+}
+
+void sh_build_noalloc(hipStream_t s, int device, const HashCfg& g, const float* pos, int nv, const int* tris, int nt,
+                      int* sorted_keys, int* sorted_ids, int* raw_keys, int* cell_start, int* cell_end) {
+    PBD_CHECK(nt <= SH_NOALLOC_MAX_TRIS, PBD_ERR_LIMIT,
+              "pbd_sh_build_noalloc: " + std::to_string(nt) + " triangles exceed its maximum of " +
+                  std::to_string(SH_NOALLOC_MAX_TRIS));
+    uint32_t* scr = (uint32_t*)scratch_address(device);
+    uint32_t* k2 = scr;
+    uint32_t* v2 = scr + SH_NOALLOC_MAX_TRIS;
+    int* hist = (int*)(scr + 2 * (size_t)SH_NOALLOC_MAX_TRIS);
+    uint32_t* k = (uint32_t*)sorted_keys;
+    uint32_t* v = (uint32_t*)sorted_ids;
+    if (nt > 0)
+        hipLaunchKernelGGL(k_tri_keys_noalloc, dim3(blocks(nt)), dim3(256), 0, s, g, pos, nv, tris, nt, k, v, raw_keys);
+    int bits = 1;
+    while (bits < 32 && ((1ll << bits) < (long long)g.ncells + 1)) bits++;
+    radix_sort_pairs(s, k, v, k2, v2, nt, bits, hist);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(g.ncells)), dim3(256), 0, s, cell_start, g.ncells, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(g.ncells)), dim3(256), 0, s, cell_end, g.ncells, -1);
+    if (nt > 0) {
+        hipLaunchKernelGGL(k_cell_ranges, dim3(blocks(nt)), dim3(256), 0, s, k, nt, cell_start, cell_end);
+        hipLaunchKernelGGL(k_keys_signed, dim3(blocks(nt)), dim3(256), 0, s, k, nt);
     }
     PBD_HIP(hipGetLastError());
 }

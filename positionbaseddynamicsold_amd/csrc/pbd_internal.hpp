@@ -267,6 +267,12 @@ struct Broadphase {
                  float min_n2);
 };
 void device_scan(hipStream_t s, const int* in, int* out, int n, int* tmp);
+// Broadphase build into caller device arrays with no allocation: the sort's
+// scratch is a static device array of sh_noalloc_scratch_bytes() per GPU.
+constexpr int SH_NOALLOC_MAX_TRIS = 1 << 20;
+size_t sh_noalloc_scratch_bytes();
+void sh_build_noalloc(hipStream_t s, int device, const HashCfg& g, const float* pos, int nv, const int* tris, int nt,
+                      int* sorted_keys, int* sorted_ids, int* raw_keys, int* cell_start, int* cell_end);
 
 // Warm-start MLP (mlp.hip).
 int mlp_nparam();
