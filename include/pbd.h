@@ -353,8 +353,10 @@ PBD_API int pbd_ws_last_delta(pbd_ctx* ctx, int cloth, float* xyz);
 typedef struct {
     double tile_ms;          /* summed duration of LDS-tile kernel launches (HIP events on the context stream) */
     long long tile_launches;
-    double tile_alg_bytes;   /* algorithmic bytes of those launches (68 B/distance prim/sweep,
-                                132 B/bending prim/sweep, 100 B/vertex for fused advect+integrate) */
+    double tile_alg_bytes;   /* per-sweep model bytes of those launches (68 B/distance prim/sweep,
+                                132 B/bending prim/sweep, 100 B/vertex for fused advect+integrate);
+                                the launches read each loaded vertex once for several sweeps, so
+                                tile_bytes below is what they move */
     double step_ms;          /* summed device-synchronised wall time of pbd_step calls */
     long long substeps;
     long long vertex_sweeps; /* sum over cloths and substeps of V x sweeps */
@@ -371,13 +373,20 @@ typedef struct {
     long long ccd_ee_raw_pairs; /* edge-edge candidates before de-duplication */
     long long ccd_ee_pairs;     /* distinct edge-edge candidates */
     long long ccd_ee_contacts;  /* edge-edge contacts found */
+    /* bytes the LDS-tile launches must move as designed: per launched tile its
+       vertex list and loaded state (16 B per vertex, 32 B when it advects x and
+       v, +16 B with a kept prediction), the prim records of every phase it
+       runs (8 B per distance prim, 12 B per bending prim), its owned prims for
+       the residual (8 B each) and its stores (16 B per interior vertex, 64 B
+       when it integrates); counted from the plan for the launches that ran */
+    double tile_bytes;
 } pbd_profile;
 PBD_API int pbd_profile_enable(pbd_ctx* ctx, int on);
 PBD_API int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset);
 /* Autotuner (no counterpart in the reference).  From the current state, times
  * `trial_substeps` pbd_step substeps (after one untimed substep) for every
- * candidate of block_threads {256, 512, 1024} x tile_sweeps {1, 2, 4, 5, 10,
- * 20 that are <= iterations} x tile_max_verts {device maximum, 4096, 2048}
+ * candidate of block_threads {256, 512, 1024} x tile_sweeps {1, 2, 4, 5
+ * that are <= iterations} x tile_max_verts {device maximum, 4096, 2048}
  * and, with collisions enabled, the hash table size (table_bits - 2 ..
  * table_bits + 2 of the hashed grid), restoring x and v after every trial.
  * The fastest candidate becomes the solver configuration and is returned in

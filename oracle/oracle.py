@@ -76,6 +76,7 @@ def lib() -> C.CDLL:
             "orc_integrate": (None, [C.c_int, C.c_float, _fp, _fp, _fp]),
             "orc_residual": (None, [C.POINTER(OrcCloth), _fp, _dp, _dp, _ip]),
             "orc_substep": (C.c_int, [C.POINTER(OrcCloth), C.POINTER(OrcParams), C.c_int, C.c_int, C.c_double, _dp]),
+            "orc_substep_omp": (C.c_int, [C.POINTER(OrcCloth), C.POINTER(OrcParams), C.c_int, C.c_int]),
             "orc_substep_jacobi": (C.c_int, [C.POINTER(OrcCloth), C.POINTER(OrcParams), C.c_int, C.c_float, C.c_int,
                                              C.c_double, _dp]),
             "orc_jacobi_sweep": (None, [C.POINTER(OrcCloth), C.c_int, C.c_float, _fp, _fp]),
@@ -231,7 +232,9 @@ class OracleCloth:
     """A cloth stepped by the oracle, built in the same order as the library."""
 
     def __init__(self, pos, tris, pairs, pins=(), bending=False, stiffness=(1.0, 1.0, 1.0), iterations=20,
-                 quads=None):
+                 quads=None, colors=None):
+        """colors: per-prim colours computed elsewhere (the oracle's own
+        colouring otherwise); the colour order is derived from them as usual."""
         self.L = lib()
         self.x = np.ascontiguousarray(pos, np.float32).copy()
         self.v = np.zeros_like(self.x)
@@ -243,8 +246,11 @@ class OracleCloth:
         if bending and quads is None:
             quads = bending_quads(self.tris)
         self.prim_list, self.indices, self.rest = build_prims(self.x, pairs, quads if bending else None)
-        self.sn, self.adj = adjacency(self.prim_list, self.indices, self.nv)
-        self.color, self.rounds = color(self.prim_list, self.sn, self.adj)
+        if colors is None:
+            self.sn, self.adj = adjacency(self.prim_list, self.indices, self.nv)
+            self.color, self.rounds = color(self.prim_list, self.sn, self.adj)
+        else:
+            self.color, self.rounds = np.ascontiguousarray(colors, np.int32), -1
         self.sorted, self.worksets = color_sort(self.color)
         self.stiffness = stiffness
         self.set_iterations(iterations)
@@ -297,6 +303,12 @@ class OracleCloth:
         r = C.c_double()
         n = self.L.orc_substep(C.byref(self.c), C.byref(prm), cap, check_every, tol, C.byref(r))
         return n, r.value
+
+    def substep_omp(self, prm: OrcParams, cap=20, threads=1):
+        """Advect + cap sweeps colour-parallel on `threads` host threads +
+        integrate (equal to substep with the same cap, bit for bit)."""
+        self.advect(prm)
+        return self.L.orc_substep_omp(C.byref(self.c), C.byref(prm), cap, threads)
 
     def sweep(self, index):
         self.L.orc_sweep(C.byref(self.c), index)

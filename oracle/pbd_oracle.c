@@ -633,6 +633,43 @@ int orc_substep(orc_cloth* c, const orc_params* prm, int cap, int check_every, d
     return sweeps;
 }
 
+/* The plain substep's sweeps and integrate on nthreads host threads (new:
+ * the reference's CPU path, projectConstraintCPU PBD/PBD_cuda.cu:1090-1127,
+ * is single-threaded).  The prims of one colour share no vertex, so a
+ * colour's prims are split over the threads and each colour ends at the
+ * threads' barrier; colliders and integrate are per vertex.  The result
+ * equals orc_substep's (fixed sweeps) bit for bit.  Advect beforehand, as
+ * for orc_substep.  Returns the sweeps run. */
+int orc_substep_omp(orc_cloth* c, const orc_params* prm, int cap, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    const int coll = prm->sphere_on || prm->box_on;
+    const int nv = c->nv;
+#pragma omp parallel num_threads(nthreads)
+    {
+        for (int sw = 0; sw < cap; sw++) {
+            const int ki = sw < c->k_len ? sw : c->k_len - 1;
+            const float kd = c->k_dist[ki], kb = c->k_bend ? c->k_bend[ki] : 1.0f;
+            for (int col = 0; col < c->ncolors; col++) {
+                const int s = c->worksets[2 * col], n = c->worksets[2 * col + 1];
+#pragma omp for schedule(static)
+                for (int j = s; j < s + n; j++) {
+                    const int prim = c->sorted[j];
+                    const int st = c->prim_list[2 * prim], cnt = c->prim_list[2 * prim + 1];
+                    if (cnt == 2) orc_project_distance(c->indices + st, c->rest[prim], kd, c->w, c->p);
+                    else if (cnt == 4) orc_project_bending(c->indices + st, c->rest[prim], kb, c->w, c->p);
+                }
+            }
+            if (coll) {
+#pragma omp for schedule(static)
+                for (int i = 0; i < nv; i++) orc_colliders(prm, 1, c->w + i, c->p + 3 * i);
+            }
+        }
+#pragma omp for schedule(static)
+        for (int i = 0; i < nv; i++) orc_integrate(1, prm->dt, c->x + 3 * i, c->v + 3 * i, c->p + 3 * i);
+    }
+    return cap;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Jacobi sweep with constraint averaging (new).  The reference declares     */
 /* SolverType::JACOBI (PBD/PBD_Basic.cuh:46-51) and never implements it; this */

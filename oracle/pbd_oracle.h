@@ -94,6 +94,7 @@ void orc_integrate(int nv, float dt, float* x, float* v, const float* p);
 void orc_residual(const orc_cloth* c, const float* p, double* max_rel, double* sum_sq, int* count);
 void orc64_substeps(const orc_cloth* c, const orc_params* prm, double* x, double* v, double* p, int sweeps,
                     int substeps);
+int orc_substep_omp(orc_cloth* c, const orc_params* prm, int cap, int nthreads);
 int orc_substep(orc_cloth* c, const orc_params* prm, int cap, int check_every, double tol,
                 double* final_residual);
 void orc_jacobi_sweep(const orc_cloth* c, int sweep_index, float omega, const float* pin, float* pout);

@@ -69,7 +69,8 @@ class Profile(C.Structure):
                 ("sweeps_max", C.c_longlong), ("graph_replays", C.c_longlong), ("ccd_detections", C.c_longlong),
                 ("ccd_triangles", C.c_longlong), ("ccd_cell_entries", C.c_longlong), ("ccd_raw_pairs", C.c_longlong),
                 ("ccd_pairs", C.c_longlong), ("ccd_contacts", C.c_longlong), ("ccd_resolved", C.c_longlong),
-                ("ccd_ee_raw_pairs", C.c_longlong), ("ccd_ee_pairs", C.c_longlong), ("ccd_ee_contacts", C.c_longlong)]
+                ("ccd_ee_raw_pairs", C.c_longlong), ("ccd_ee_pairs", C.c_longlong), ("ccd_ee_contacts", C.c_longlong),
+                ("tile_bytes", C.c_double)]
 
 
 class ClothInfo(C.Structure):

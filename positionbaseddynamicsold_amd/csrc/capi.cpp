@@ -710,6 +710,38 @@ void advect_all(pbd_ctx* c, float dt) {
     c->pfinal_valid = false;  // p is rewritten whole
 }
 
+// Bytes the tile launches of the last projection moved as designed (the
+// tile_bytes profile field), from each cloth's sweep count and plan sums:
+// ts sweeps per launch for multi-tile cloths (one launch for a single-tile
+// one); fused_whole: the first load advects x and v; kept: the cloth's
+// prediction was added on the first load; int_store: the launch holding a
+// cloth's last sweep integrated it.
+double tile_projection_bytes(const pbd_ctx* c, int ts, bool fused_whole, bool int_store, bool with_delta) {
+    double total = 0.0;
+    for (size_t ci = 0; ci < c->cloths.size(); ci++) {
+        const Cloth& k = c->cloths[ci];
+        const int L = k.last_sweeps;
+        if (L <= 0 || k.tile_record_bytes.size() < 2) continue;
+        const bool kept = with_delta && ci < c->ws.used.size() && c->ws.used[ci] != 0;
+        auto launch = [&](int n, bool first, bool last) {
+            double load = (first && fused_whole) ? 32.0 : 16.0;
+            if (first && kept) load += 16.0;
+            const bool integ = k.whole ? (fused_whole || int_store) : (int_store && last);
+            const double store = integ ? 64.0 : 16.0;
+            const int row = std::min(n, (int)k.tile_record_bytes.size() - 1);
+            const double rec = k.whole ? n * k.tile_record_bytes[1] : k.tile_record_bytes[row];
+            return 4.0 * (double)(k.tiles_loaded + k.tiles_interior) + load * (double)k.tiles_loaded + rec +
+                   8.0 * (double)k.tiles_owned + store * (double)k.tiles_interior;
+        };
+        if (k.whole) {
+            total += launch(L, true, true);
+        } else {
+            for (int sw = 0; sw < L; sw += ts) total += launch(std::min(ts, L - sw), sw == 0, sw + ts >= L);
+        }
+    }
+    return total;
+}
+
 // Sweeps on the current p.  Whole-cloth tiles run unfused (p_in -> p_out).
 // cap: optional per-cloth sweep budgets (device array) below `iterations`.
 void consolidate(pbd_ctx* c);
@@ -968,6 +1000,8 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     }
     if (c->prof_on) {
         collect_events(c);
+        if (delta) ws_read_used(c);
+        c->prof.tile_bytes += tile_projection_bytes(c, ts, fused_whole, int_store, delta != nullptr);
         double bytes = 0.0;
         for (auto& k : c->cloths) {
             bytes += (double)k.last_sweeps * (kDistBytes * k.ne + kBendBytes * k.nb);
@@ -1046,6 +1080,10 @@ void step_ccd(pbd_ctx* c, float dt) {
     for (auto& k : c->cloths) k.last_sweeps = I;
     if (c->prof_on) {
         collect_events(c);
+        // I launches of one sweep each, loading and storing p
+        for (auto& k : c->cloths) k.last_sweeps = 1;
+        c->prof.tile_bytes += (double)I * tile_projection_bytes(c, 1, false, false, false);
+        for (auto& k : c->cloths) k.last_sweeps = I;
         for (auto& k : c->cloths) {
             c->prof.tile_alg_bytes += (double)I * (kDistBytes * k.ne + kBendBytes * k.nb);
             c->prof.vertex_sweeps += (long long)k.nv * I;
@@ -2523,7 +2561,7 @@ int pbd_autotune(pbd_ctx* ctx, int trial_substeps, pbd_solver_params* best, int*
         } else if (has_tolerance(ctx)) {
             sweeps = {std::max(1, base.check_every)};  // chunk = check interval keeps the stopping points
         } else {
-            for (int t : {1, 2, 4, 5, 10, 20})
+            for (int t : {1, 2, 4, 5})
                 if (t <= base.iterations) sweeps.push_back(t);
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -318,8 +318,16 @@ void read_txt_cloth(const char* topol, const char* constr, std::vector<float>& p
 // loaded set L contains the backward dependency cone of I: going backwards
 // over phases k = P-1..0 with colour c_k, D_k = D_{k+1} plus every vertex of a
 // colour-c_k prim incident to D_{k+1} (each vertex has at most one prim per
-// colour).  Prims with all vertices in L are processed; those not incident to
-// D_{k+1} only disturb vertices outside D_{k+1}, which no later phase reads.
+// colour).  Only the colour-c_k prims incident to D_{k+1} need processing at
+// phase k; others only disturb vertices outside D_{k+1}, which no later phase
+// reads.  With entry(v) = the phase at which v joins the cone (P for I), a
+// prim of colour c is needed at phase s*C + c iff some vertex has entry
+// > s*C + c, i.e. for sweeps s <= S(p) = floor((max entry - 1 - c) / C): the
+// needed prims of a colour at sweep s are those with S(p) >= s.  Each colour's
+// prims are stored in decreasing S(p), so phase (s, c) runs a prefix of the
+// colour's list; prims of one colour share no vertex, so their order within
+// the colour changes nothing.  A launch of n < sweeps sweeps has the cone of
+// the last n sweeps: its sweep s uses the prefix of sweep s + sweeps - n.
 // The owned distance prims (first vertex in I) are added to D_P so that the
 // residual can be evaluated on valid values at the end of the launch.
 // ---------------------------------------------------------------------------
@@ -352,18 +360,22 @@ void build_topo(const Cloth& c, ClothTopo& t) {
 struct TileOut {
     std::vector<int> load;   // local -> cloth vertex (interior first)
     int n_int = 0;
-    std::vector<std::vector<int>> dist_by_col, bend_by_col;  // cloth prim ids
+    std::vector<std::vector<int>> dist_by_col, bend_by_col;  // cloth prim ids, by decreasing last needed sweep
+    std::vector<int> dcount, bcount;  // [s * C + c]: prims of colour c needed at sweep s (a prefix); empty: all
     std::vector<int> own;    // owned distance prims
 };
 
 // Computes the loaded set of one tile.  Returns its size; fills `out` only
 // when it fits in max_load.
 int plan_one(const ClothTopo& t, const std::vector<int>& interior, int phases, int max_load,
-             std::vector<int>& mark, int stamp, TileOut* out) {
+             std::vector<int>& mark, std::vector<int>& entry, int stamp, TileOut* out) {
     const Cloth& c = *t.c;
     std::vector<int> D(interior);
     D.reserve((size_t)max_load + 64);
-    for (int v : interior) mark[v] = stamp;
+    for (int v : interior) {
+        mark[v] = stamp;
+        entry[v] = phases;
+    }
     std::vector<int> own;
     {
         std::vector<char> is_int;  // membership via mark (stamp) of interior only
@@ -377,7 +389,7 @@ int plan_one(const ClothTopo& t, const std::vector<int>& interior, int phases, i
         for (int p : own)
             for (int k = 0; k < 2; k++) {
                 int u = c.indices[c.prim_list[2 * p] + k];
-                if (mark[u] != stamp) { mark[u] = stamp; D.push_back(u); }
+                if (mark[u] != stamp) { mark[u] = stamp; entry[u] = phases; D.push_back(u); }
             }
     }
     std::vector<int> frontier(D), nf;
@@ -390,7 +402,7 @@ int plan_one(const ClothTopo& t, const std::vector<int>& interior, int phases, i
                 int st = c.prim_list[2 * p], cnt = c.prim_list[2 * p + 1];
                 for (int q = 0; q < cnt; q++) {
                     int u = c.indices[st + q];
-                    if (mark[u] != stamp) { mark[u] = stamp; D.push_back(u); nf.push_back(u); }
+                    if (mark[u] != stamp) { mark[u] = stamp; entry[u] = k; D.push_back(u); nf.push_back(u); }
                 }
             }
             bool outside = false;
@@ -411,19 +423,43 @@ int plan_one(const ClothTopo& t, const std::vector<int>& interior, int phases, i
     out->own = own;
     out->dist_by_col.assign(t.C, {});
     out->bend_by_col.assign(t.C, {});
+    const int sweeps = t.C > 0 ? phases / t.C : 0;
+    // (last needed sweep, prim) per colour; prims needed at no phase are dropped
+    std::vector<std::vector<std::pair<int, int>>> dl(t.C), bl(t.C);
     for (int v : D)
         for (int j = t.vp_start[v]; j < t.vp_start[v + 1]; j++) {
             int p = t.vp[j];
             int st = c.prim_list[2 * p], cnt = c.prim_list[2 * p + 1];
             if (c.indices[st] != v) continue;  // visit each prim once, from its first vertex
             bool all = true;
-            for (int q = 0; q < cnt; q++)
-                if (mark[c.indices[st + q]] != stamp) { all = false; break; }
+            int emax = -1;
+            for (int q = 0; q < cnt; q++) {
+                const int u = c.indices[st + q];
+                if (mark[u] != stamp) { all = false; break; }
+                emax = std::max(emax, entry[u]);
+            }
             if (!all) continue;
-            (cnt == 2 ? out->dist_by_col : out->bend_by_col)[c.color[p]].push_back(p);
+            const int col = c.color[p];
+            const int last = emax - 1 - col;
+            if (last < 0) continue;
+            (cnt == 2 ? dl : bl)[col].push_back({last / t.C, p});
         }
-    for (auto& v : out->dist_by_col) std::sort(v.begin(), v.end());
-    for (auto& v : out->bend_by_col) std::sort(v.begin(), v.end());
+    auto order = [&](std::vector<std::vector<std::pair<int, int>>>& lists, std::vector<std::vector<int>>& by_col,
+                     std::vector<int>& count) {
+        count.assign((size_t)sweeps * t.C, 0);
+        for (int col = 0; col < t.C; col++) {
+            auto& l = lists[col];
+            std::sort(l.begin(), l.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                return a.first != b.first ? a.first > b.first : a.second < b.second;
+            });
+            for (auto& e : l) {
+                by_col[col].push_back(e.second);
+                for (int s = 0; s <= e.first && s < sweeps; s++) count[(size_t)s * t.C + col]++;
+            }
+        }
+    };
+    order(dl, out->dist_by_col, out->dcount);
+    order(bl, out->bend_by_col, out->bcount);
     return (int)D.size();
 }
 
@@ -473,7 +509,7 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split
         Cloth& c = cloths[ci];
         ClothTopo topo;
         build_topo(c, topo);
-        std::vector<int> mark((size_t)c.nv, -1);
+        std::vector<int> mark((size_t)c.nv, -1), entry((size_t)c.nv, 0);
         int stamp = 0;
         int phases = std::max(1, sweeps) * c.ncolors;
         std::vector<Block> todo;
@@ -485,7 +521,7 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split
                 std::vector<int> in;
                 for (int y = cy; y < std::min(c.ry, cy + T); y++)
                     for (int x = cx; x < std::min(c.rx, cx + T); x++) in.push_back(y * c.rx + x);
-                int n = plan_one(topo, in, phases, max_load, mark, stamp++, nullptr);
+                int n = plan_one(topo, in, phases, max_load, mark, entry, stamp++, nullptr);
                 if (n <= max_load || T <= 4) break;
                 T = std::max(4, (int)(T * 0.85));
             }
@@ -508,7 +544,7 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split
                 for (int v = b.x0; v < b.x1; v++) in.push_back(v);
             }
             TileOut o;
-            int n = plan_one(topo, in, phases, max_load, mark, stamp++, &o);
+            int n = plan_one(topo, in, phases, max_load, mark, entry, stamp++, &o);
             if (n <= max_load) {
                 outs.push_back(std::move(o));
                 c.max_loaded = std::max(c.max_loaded, n);
@@ -553,6 +589,8 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split
     for (size_t ci = 0; ci < cloths.size(); ci++) {
         Cloth& c = cloths[ci];
         c.tile_begin = (int)plan.tiles.size();
+        c.tiles_loaded = c.tiles_interior = c.tiles_owned = 0;
+        c.tile_record_bytes.assign((size_t)std::max(1, sweeps) + 1, 0.0);
         for (auto& o : per[ci].outs) {
             TileDev td{};
             td.cloth = (int)ci;
@@ -583,6 +621,39 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split
                 bsum += (int)o.bend_by_col[col].size();
             }
             plan.cols.push_back(bsum);
+            // per-sweep prefixes of the partial tiles: sweeps x C distance
+            // counts, then sweeps x C bending counts
+            td.plan_sweeps = o.dcount.empty() ? 0 : (int)(o.dcount.size() / std::max(1, c.ncolors));
+            plan.cols.insert(plan.cols.end(), o.dcount.begin(), o.dcount.end());
+            plan.cols.insert(plan.cols.end(), o.bcount.begin(), o.bcount.end());
+            c.tiles_loaded += (long long)o.load.size();
+            c.tiles_interior += o.n_int;
+            c.tiles_owned += (long long)o.own.size();
+            if (o.dcount.empty()) {
+                // every prim at every sweep: the bytes of one sweep
+                double b = 0.0;
+                for (int col = 0; col < c.ncolors; col++)
+                    b += 8.0 * o.dist_by_col[col].size() + 12.0 * o.bend_by_col[col].size();
+                c.tile_record_bytes[1] += b;
+            } else {
+                const int ps = td.plan_sweeps;
+                for (int n = 1; n <= ps && n < (int)c.tile_record_bytes.size(); n++) {
+                    double b = 0.0;
+                    for (int sw = 0; sw < n; sw++)
+                        for (int col = 0; col < c.ncolors; col++) {
+                            const size_t q = (size_t)(sw + ps - n) * c.ncolors + col;
+                            b += 8.0 * o.dcount[q] + 12.0 * o.bcount[q];
+                        }
+                    c.tile_record_bytes[n] += b;
+                }
+            }
+            plan.stat_loaded += (long long)o.load.size();
+            plan.stat_interior += o.n_int;
+            for (int col = 0; col < c.ncolors; col++) plan.stat_dist_loaded += (long long)o.dist_by_col[col].size();
+            for (size_t q = 0; q < o.dcount.size(); q++) plan.stat_dist_phase_needed += o.dcount[q];
+            if (o.dcount.empty())
+                for (int col = 0; col < c.ncolors; col++)
+                    plan.stat_dist_phase_needed += (long long)sweeps * o.dist_by_col[col].size();
             for (int col = 0; col < c.ncolors; col++)
                 for (int p : o.dist_by_col[col]) {
                     const int* ix = &c.indices[c.prim_list[2 * p]];

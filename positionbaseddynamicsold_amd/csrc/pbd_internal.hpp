@@ -114,9 +114,12 @@ struct TileDev {
     int bend_off;        // offset into tile_bend of colour 0
     int own_off;         // offset into tile_own (distance prims owned by the tile)
     int n_own;
-    int col_off;         // offset into tile_cols: (ncolors+1) dist offsets then (ncolors+1) bend offsets
+    int col_off;         // offset into tile_cols: (ncolors+1) dist offsets, (ncolors+1) bend offsets, then for
+                         // partial tiles plan_sweeps x ncolors dist counts and as many bend counts
     int ncolors;
-    int pad;
+    int plan_sweeps;     // partial tiles: sweeps per launch the cone was planned for; sweep s of an n-sweep
+                         // launch runs the first counts[(s + plan_sweeps - n) * ncolors + c] prims of colour c
+                         // (0: whole tiles, every prim of the colour)
 };
 
 // Kernel-wide parameters of one solver launch.
@@ -180,6 +183,11 @@ struct Cloth {
     int rd_lo = 0, rd_hi = 0;             // range in the cloth-ordered residual list
     bool whole = false;
     int max_loaded = 0;
+    // tile plan sums for the bytes a projection moves: loaded, interior and
+    // owned-prim counts over the cloth's tiles, and the prim-record bytes of
+    // one launch of n sweeps (index n; whole tiles: of one sweep, index 1)
+    long long tiles_loaded = 0, tiles_interior = 0, tiles_owned = 0;
+    std::vector<double> tile_record_bytes;
     int last_sweeps = 0;
     double tol = -1.0;                    // per-cloth residual tolerance (< 0: the solver's)
     int np() const { return ne + nb; }
@@ -311,6 +319,9 @@ struct TilePlan {
     std::vector<int> cols;                // per tile colour offsets
     std::vector<int> partial_order;       // launch order of the partial tiles only
     std::vector<int> whole_order;         // launch order of the whole tiles only
+    // plan statistics: loaded and interior vertices, loaded distance prims,
+    // and distance prim updates of one launch of `sweeps` sweeps
+    long long stat_loaded = 0, stat_interior = 0, stat_dist_loaded = 0, stat_dist_phase_needed = 0;
 };
 // split_whole: a cloth that fits one tile still gets a (single, halo-free)
 // multi-launch tile instead of a whole tile.

@@ -74,12 +74,24 @@ PBD_HD void project_distance(float4& a, float4& b, float rest, float k) {
     }
     float s0 = q0 * c;
     float s1 = q1 * c;
-    a.x = a.x + (s0 * rx) * k;
-    a.y = a.y + (s0 * ry) * k;
-    a.z = a.z + (s0 * rz) * k;
-    b.x = b.x + (s1 * rx) * k;
-    b.y = b.y + (s1 * ry) * k;
-    b.z = b.z + (s1 * rz) * k;
+    float ax = s0 * rx, ay = s0 * ry, az = s0 * rz;
+    float bx = s1 * rx, by = s1 * ry, bz = s1 * rz;
+    // (s * r) * k with k == 1 is (s * r) exactly: the full stiffness of the
+    // configs skips six multiplies
+    if (k != 1.0f) {
+        ax = ax * k;
+        ay = ay * k;
+        az = az * k;
+        bx = bx * k;
+        by = by * k;
+        bz = bz * k;
+    }
+    a.x = a.x + ax;
+    a.y = a.y + ay;
+    a.z = a.z + az;
+    b.x = b.x + bx;
+    b.y = b.y + by;
+    b.z = b.z + bz;
 }
 
 // Dihedral terms for hinge (e0,e1) with wings o0, o1 (Bridson et al. 2003):

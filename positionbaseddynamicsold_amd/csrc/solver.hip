@@ -372,7 +372,9 @@ void launch_fill_int(hipStream_t s, int* p, size_t n, int v) {
 // ---------------------------------------------------------------------------
 // LDS tile kernel.  One workgroup per tile: the tile's loaded vertices live
 // in LDS as float4 for the whole launch; each colour phase processes the
-// tile's prims of that colour and ends with a workgroup barrier.
+// tile's prims of that colour that the phase's dependency cone needs (a
+// prefix of the colour's list, the planner's per-sweep counts) and ends with
+// a workgroup barrier.
 //
 // whole tiles (the tile is the entire cloth): optional fused advect on load,
 //   sweeps until the residual check passes or the cap, fused integrate on store.
@@ -520,6 +522,12 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     const int C = td.ncolors;
     const int* dc = cols + td.col_off;
     const int* bc = dc + C + 1;
+    // partial tiles run, at sweep s of this launch, the prefix of each colour
+    // list that the cone of the launch's last nsweeps sweeps needs (the index
+    // is kept within the tile's plan_sweeps rows of counts)
+    const int* dn = bc + C + 1;
+    const int* bn = dn + td.plan_sweeps * C;
+    const int shift = max(0, td.plan_sweeps - nsweeps);
     const uint32_t* D = dist + td.dist_off;
     const float* DR = dist_rest + td.dist_off;
     const uint32_t* B = bend + 2 * (size_t)td.bend_off;
@@ -532,11 +540,13 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
         int ks = gs < a.kmax ? gs : a.kmax - 1;
         float kd = ktab_d[cd.ktab + ks];
         float kb = ktab_b[cd.ktab + ks];
+        const int* dnow = td.plan_sweeps ? dn + min(sw + shift, td.plan_sweeps - 1) * C : nullptr;
+        const int* bnow = td.plan_sweeps ? bn + min(sw + shift, td.plan_sweeps - 1) * C : nullptr;
         for (int c = 0; c < C; c++) {
             // the prim records of a phase are fetched from global memory
             // in batches of PREFETCH per thread before any is processed, so
             // the phase waits for one round trip instead of one per prim
-            const int je = dc[c + 1];
+            const int je = dnow ? dc[c] + dnow[c] : dc[c + 1];
             for (int j0 = dc[c] + tid; j0 < je; j0 += PREFETCH * T) {
                 uint32_t pr[PREFETCH];
                 float rl[PREFETCH];
@@ -556,7 +566,7 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
                     P[i1] = p1;
                 }
             }
-            const int be = bc[c + 1];
+            const int be = bnow ? bc[c] + bnow[c] : bc[c + 1];
             for (int j0 = bc[c] + tid; j0 < be; j0 += BEND_PREFETCH * T) {
                 uint2 bw[BEND_PREFETCH];
                 float br[BEND_PREFETCH];

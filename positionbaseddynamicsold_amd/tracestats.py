@@ -1,7 +1,7 @@
 """Summaries of rocprofv3 CSV output of bench.py.
 
     python -m positionbaseddynamicsold_amd.tracestats [TRACE.csv] [--steps K]
-        [--fetch FETCH_counter_collection.csv --write WRITE_counter_collection.csv]
+        [--fetch FETCH_counter_collection.csv --write WRITE_counter_collection.csv --passes PASSES.json]
 
 TRACE.csv is `rocprofv3 --kernel-trace --output-format csv` output
 (`*_kernel_trace.csv`).  Prints per-kernel statistics for the whole run, then
@@ -13,21 +13,18 @@ figure.  Substeps are delimited by the per-substep k_final_buffers launch (the
 tile path's last kernel of a projection; integrate may run inside the tiles).
 
 With counter files from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`
-runs (kernel filter "k_tiles|k_copy|k_final_buffers") of bench.py or of
-positionbaseddynamicsold_amd.tilework, which repeats the same two passes, it
-prints the tile kernel's HBM traffic per launch in those passes, scaled by
-what the counters read for the calibration copy (1 GiB read + 1 GiB written
-per launch).  With --collide (the JSON line of positionbaseddynamicsold_amd
-.collidework), it summarises that program's trace instead: every kernel's
-time and, with counter files of the same program, HBM bytes and GB/s per
-kernel, plus the algorithmic rate of the kernels whose bytes per unit are
-fixed (hash keys 56 B per triangle, narrowphase 104 B per candidate,
-edge-edge narrowphase 152 B per candidate, response 160 B per contact and
-pass).  With --tilework (that run's JSON line) and --update-json it
-also records the figures in the JSON file under each pass's launch
-configuration ("ts<sweeps>_bt<threads>_mv<tile vertices>", "_sw1" when
-single-tile cloths are split into per-launch chunks), which bench.py
-reads for its roofline traffic.
+runs (kernel filter "k_tiles|k_copy|k_final_buffers") of `bench.py
+--passes-only`, which repeats the same two passes, it prints the tile
+kernel's HBM traffic per launch in those passes, scaled by what the counters
+read for the calibration copy (1 GiB read + 1 GiB written per launch), next
+to the bytes the launches must move as designed (the JSON line that run
+printed, --passes).  With --collide (the JSON line of
+positionbaseddynamicsold_amd.collidework), it summarises that program's
+trace instead: every kernel's time and, with counter files of the same
+program, HBM bytes and GB/s per kernel, plus the algorithmic rate of the
+kernels whose bytes per unit are fixed (hash keys 56 B per triangle,
+narrowphase 104 B per candidate, edge-edge narrowphase 152 B per candidate,
+response 160 B per contact and pass).
 """
 from __future__ import annotations
 
@@ -97,9 +94,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--fetch")
     ap.add_argument("--write")
-    ap.add_argument("--tilework", help="JSON line printed by the counted tilework run")
+    ap.add_argument("--passes", help="JSON line printed by the counted bench.py --passes-only run")
     ap.add_argument("--collide", help="JSON line printed by the collidework run")
-    ap.add_argument("--update-json", help="traffic JSON to add this configuration's figures to")
     a = ap.parse_args()
     if a.collide:
         collide_summary(a.trace, a.collide, a.fetch, a.write)
@@ -131,9 +127,13 @@ def main():
             print(f"k_tiles {label}: FETCH_SIZE {f:.0f} KiB, WRITE_SIZE {wr:.0f} KiB per launch; calibration copy "
                   f"reads {fc:.0f} KiB / writes {wc:.0f} KiB for 1 GiB each (scale {scale_f:.4f}, {scale_w:.4f}); "
                   f"HBM traffic {traffic / 1e9:.4f} GB per launch")
-            if a.tilework and a.update_json:
-                record(a.update_json, label, json.loads(open(a.tilework).read().strip().splitlines()[-1]),
-                       f, wr, scale_f, scale_w, traffic)
+            if a.passes:
+                run = json.loads(open(a.passes).read().strip().splitlines()[-1])[label]
+                need = run["bytes_per_launch"]
+                print(f"  bytes the launches must move as designed: {need / 1e9:.4f} GB per launch "
+                      f"(traffic / design {traffic / need:.3f}); under the counters the launches averaged "
+                      f"{run['avg_launch_ms']:.4f} ms: {traffic / (run['avg_launch_ms'] * 1e-3) / 1e9:.0f} GB/s of "
+                      f"HBM traffic, {need / (run['avg_launch_ms'] * 1e-3) / 1e9:.0f} GB/s by design bytes")
 
 
 HBM_PEAK = 8.0e12
@@ -186,22 +186,6 @@ def collide_summary(trace, work_path, fetch=None, write=None):
             per_unit, key = COLLIDE_ALG[n]
             line += f" {per_unit * work[key] / t / 1e9:9.0f}"
         print(line)
-
-
-def record(path, label, tw, f, wr, scale_f, scale_w, traffic):
-    import os
-    data = json.loads(open(path).read()) if os.path.exists(path) else {}
-    cfg = tw[label + "_config"]
-    key = "ts{tile_sweeps}_bt{block_threads}_mv{tile_max_verts}".format(**cfg) + ("_sw1" if cfg.get("split_whole")
-                                                                                   else "")
-    data.setdefault(label, {})[key] = {
-        "fetch_kib": round(f), "write_kib": round(wr), "fetch_scale": round(scale_f, 4),
-        "write_scale": round(scale_w, 4), "hbm_bytes_per_launch": traffic,
-        "alg_bytes_per_launch": tw[label]["alg_bytes_per_launch"],
-        "avg_launch_ms_counted": tw[label]["avg_launch_ms"], "launches": tw[label]["launches"]}
-    with open(path, "w") as fh:
-        json.dump(data, fh, indent=1, sort_keys=True)
-        fh.write("\n")
 
 
 if __name__ == "__main__":

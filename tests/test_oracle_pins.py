@@ -361,3 +361,23 @@ def test_fp64_twin_matches_fp32_oracle_and_free_fall(orc):
     dt = scenes.DT
     expect = pos.astype(np.float64) + np.array([0.0, -9.8, 0.0]) * dt * dt * n * (n + 1) / 2
     assert np.abs(ff.x64 - expect).max() < 1e-6
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_colour_parallel_substep_equals_serial(orc, threads):
+    # the labelled N-core CPU baseline: each colour's prims split over host
+    # threads give the serial colour-order substep bit for bit (bending,
+    # sphere and pins included); colours given to the constructor reproduce
+    # the oracle's own colouring run
+    n = 40
+    pos, tris = orc.grid_mesh(n, n, 1.0, 1.0, jitter=1e-4, seed=3)
+    kw = dict(pins=[0, n - 1], bending=True, stiffness=(1.0, 0.1, 1.0))
+    a = orc.OracleCloth(pos, tris, orc.grid_pairs(n, n), **kw)
+    b = orc.OracleCloth(pos, tris, orc.grid_pairs(n, n), colors=a.color, **kw)
+    assert np.array_equal(a.sorted, b.sorted) and np.array_equal(a.worksets, b.worksets)
+    prm = a.params(dt=1.0 / 600.0, sphere=((0.5, -0.15, 0.5), 0.3))
+    for _ in range(12):
+        a.substep(prm, cap=20)
+        b.substep_omp(prm, cap=20, threads=threads)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v)
+    assert np.abs(a.x - pos).max() > 1e-3  # it moved
