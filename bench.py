@@ -39,7 +39,8 @@ Protocol (fixed constants, independent of --steps and --warmup):
      sweep on the same start states;
   4. config 4 (32x32 scenes with wind, pins and sphere) the same way:
      train on 24 scenes, choose k on 32 others, time and check 256 held-out
-     scenes batched.
+     scenes batched; config 3 (the folded 512x512 cloth over the box with
+     CCD) trained and tuned on another draw of the scene, timed on seed 0.
 `value` is the warm-started batched solver's substeps/s on the timed batch;
 config.speedup_vs_plain_batched divides it by the plain batched solver's.
 
@@ -81,6 +82,9 @@ WARM_RULE, WARM_GATE = 1, 1e-4   # fallback rule (RMS) and strain gate
 JACOBI_OMEGA = 1.5
 CPU_SAMPLE_VERTICES = 1_000_000
 C4_TRAIN_SCENES, C4_SELECT_SCENES, C4_EVAL_SCENES = 24, 32, 256   # config-4 scene counts
+C3_PROBLEMS = 3        # config-3 substeps on which the budget is chosen, and on which the timed run is checked
+C3_REF_SWEEPS = 1000   # config-3 reference substep (30 plain sweeps; each sweep carries contact detection work)
+C3_COLLECT = 4         # config-3 collision substeps recorded as samples
 
 
 def log(msg):
@@ -97,6 +101,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (4 / 2 sweeps per launch, 1024 threads)")
     ap.add_argument("--no-config4", action="store_true")
+    ap.add_argument("--no-config3", action="store_true")
     ap.add_argument("--save-network", metavar="NPZ", help="write the batch network, warm budget and launch "
                     "configuration of this run")
     ap.add_argument("--load-network", metavar="NPZ", help="with --passes-only: the file --save-network wrote")
@@ -351,6 +356,140 @@ def config4_line(capi, scenes, steps, warmup):
     return out
 
 
+def config3_line(capi, scenes, steps, warmup):
+    """Config 3: the 512 x 512 folded cloth over the 0.4 m box, vertex-face
+    CCD (cloth-box and self) with the robust contact handling, 30 sweeps.
+    The network is trained and the budget chosen on another draw of the
+    scene (seed 1: fold axis and pins); the seed-0 scene is timed and checked."""
+    t0 = time.perf_counter()
+
+    def scene(seed):
+        sc = scenes.config3(seed=seed).robust()
+        ctx = capi.Context(0)
+        cid, sid, cell = scenes.add_ccd_scene(ctx, sc)
+        return sc, ctx, cid, cell
+
+    def solver(ctx, sc, warm=0, sweeps=None):
+        ctx.ws_enable(warm > 0)
+        ctx.set_solver(dt=scenes.DT, iterations=sweeps or sc.iterations, warm_iterations=warm,
+                       warm_rule=WARM_RULE, warm_gate=WARM_GATE)
+
+    def run(ctx, sc, starts, cid, **kw):
+        solver(ctx, sc, **kw)
+        outs = []
+        for x, v in starts:
+            ctx.state_load(x, v)
+            ctx.step(1)
+            outs.append(ctx.state_save()[0])
+        return outs
+
+    def err(outs, refs, ncl):
+        acc = sum(float(((o[:ncl].astype(np.float64) - r[:ncl]) ** 2).sum()) for o, r in zip(outs, refs))
+        return float(np.sqrt(acc / (len(outs) * ncl)))
+
+    def starts_of(ctx, sc, n, warm=0):
+        solver(ctx, sc, warm=warm)
+        st = []
+        for _ in range(n):
+            st.append(ctx.state_save())
+            ctx.step(1)
+        return st
+
+    # training scene: samples of its own contact dynamics, budget on its trajectory
+    sc, ctx, cid, cell = scene(1)
+    ncl = sc.cloth.rx * sc.cloth.ry
+    solver(ctx, sc)
+    ctx.step(SETTLE)
+    ctx.ws_init(0)
+    ctx.set_solver(dt=scenes.DT, iterations=sc.iterations)
+    ctx.ws_collect(TARGET_SWEEPS, C3_COLLECT)
+    loss = ctx.ws_train(epochs=EPOCHS * 4, batch=4096, lr=1e-3, seed=1)
+    prm, norm = ctx.ws_params(), ctx.ws_norm()
+    st = starts_of(ctx, sc, C3_PROBLEMS)
+    t = time.perf_counter()
+    refs = run(ctx, sc, st, cid, sweeps=C3_REF_SWEEPS)
+    log(f"[bench]   config 3: {len(st)} reference substeps of {C3_REF_SWEEPS} sweeps in {time.perf_counter() - t:.1f} s")
+    e_plain = err(run(ctx, sc, st, cid), refs, ncl)
+    tried = {}
+
+    def ratio(q):
+        tried[q] = err(run(ctx, sc, st, cid, warm=q), refs, ncl) / e_plain
+        return tried[q]
+
+    lo, hi = 1, sc.iterations
+    met = ratio(hi) <= 1.0
+    if met:
+        while hi > lo:
+            mid = (lo + hi) // 2
+            if ratio(mid) <= 1.0:
+                hi = mid
+            else:
+                lo = mid + 1
+    k = hi
+    met = met and k < sc.iterations
+    ctx.close()
+    del st, refs
+    # timed scene
+    sc, ctx, cid, cell = scene(0)
+    ctx.ws_set_params(prm)
+    ctx.ws_set_norm(norm)
+    solver(ctx, sc)
+    ctx.step(SETTLE)
+    ctx.sync()
+    ctx.snapshot()
+
+    def timed(warm):
+        solver(ctx, sc, warm=warm)
+        ctx.snapshot(restore=True)
+        for _ in range(max(1, warmup)):
+            ctx.step(1)
+        ctx.snapshot(restore=True)
+        ctx.sync()
+        t = time.perf_counter()
+        ctx.step(steps)
+        ctx.sync()
+        return steps / (time.perf_counter() - t)
+
+    plain_rate, warm_rate = timed(0), timed(k)
+    ctx.snapshot(restore=True)
+    st = starts_of(ctx, sc, C3_PROBLEMS, warm=k)
+    kept = []
+    solver(ctx, sc, warm=k)
+    for x, v in st:
+        ctx.state_load(x, v)
+        ctx.step(1)
+        kept.append(ctx.ws_last_used(cid))
+    refs = run(ctx, sc, st, cid, sweeps=C3_REF_SWEEPS)
+    e_plain_t = err(run(ctx, sc, st, cid), refs, ncl)
+    e_warm_t = err(run(ctx, sc, st, cid, warm=k), refs, ncl)
+    w = np.ones(len(st[0][0]), np.float32)
+    w[ncl:] = 0.0
+    w[list(sc.cloth.pins)] = 0.0
+    dt, g = np.float32(scenes.DT), np.array([0.0, -9.8, 0.0], np.float32)
+    adv = [x + dt * (v + (g * dt) * w[:, None]) * w[:, None] for x, v in st]
+    e_zero_t = err(adv, refs, ncl)
+    ctx.close()
+    out = {"workload": "config 3: 512x512 folded cloth over the 0.4 m box, vertex-face CCD (cloth-box and self, "
+                       "swept-box candidates, scale-free degenerate-face bound), 30 sweeps; one cloth, so the batch "
+                       "is that cloth",
+           "training": {"scene_seed": 1, "samples": (ncl + len(sc.static_pos)) * C3_COLLECT,
+                        "target_sweeps": TARGET_SWEEPS, "epochs": EPOCHS * 4, "standardised_loss": loss,
+                        "source": "collision substeps of the training scene: features at the predicted state, target "
+                                  "= the correction of 60 sweeps with contact detection and response"},
+           "budget": {"selected_on": f"{C3_PROBLEMS} substeps of the training scene", "k": k,
+                      "meets_plain_below_30": met, "tried_E_over_plain": {str(q): v for q, v in sorted(tried.items())}},
+           "plain_substeps_per_s": plain_rate, "warm_substeps_per_s": warm_rate, "speedup": warm_rate / plain_rate,
+           "accuracy_on_timed_substeps": {"problems": len(st), "reference_sweeps": C3_REF_SWEEPS, "E_plain_m": e_plain_t,
+                                          "E_warm_over_plain": e_warm_t / e_plain_t,
+                                          "E_no_projection_over_plain": e_zero_t / e_plain_t, "kept": kept,
+                                          "jacobi": "not run: the CCD pipeline runs on the Gauss-Seidel paths"},
+           "seconds": time.perf_counter() - t0}
+    log(f"[bench] config 3: k = {k} (meets plain below 30: {met}); plain {plain_rate:.1f}, warm {warm_rate:.1f} "
+        f"substeps/s (x{warm_rate / plain_rate:.2f}); E warm/plain {e_warm_t / e_plain_t:.3f}, no projection "
+        f"{e_zero_t / e_plain_t:.3f}; kept {kept}")
+    return out
+
+
 def cpu_sample(b):
     """Cloths evenly spaced over the batch's size ranking, the widest spacing
     that gives >= CPU_SAMPLE_VERTICES vertices."""
@@ -451,6 +590,7 @@ def main():
         passes_only(a, capi, scenes)
         return
     c4 = None if a.no_config4 else config4_line(capi, scenes, max(a.steps, 200), a.warmup)
+    c3 = None if a.no_config3 else config3_line(capi, scenes, a.steps, a.warmup)
 
     # ---- 1. training batch: network, launch configuration, warm budget
     tb = Batch(capi, scenes, scenes.config5(seed=TRAIN_SEED, count=a.count))
@@ -589,6 +729,7 @@ def main():
             "single_tile_cloths": int(sum(i.whole for i in b.info)), "tiles": int(sum(i.ntiles for i in b.info)),
             "copy_kernel_gbs": copy_gbs,
             "config4": c4,
+            "config3": c3,
         },
         "roofline": {
             "bound": "hbm",

@@ -328,7 +328,9 @@ PBD_API int pbd_ws_init(pbd_ctx* ctx, uint64_t seed);
 PBD_API int pbd_ws_get_params(pbd_ctx* ctx, float* params);
 PBD_API int pbd_ws_set_params(pbd_ctx* ctx, const float* params);
 /* Records (features, correction of `sweeps` sweeps) for every vertex over
- * `substeps` plain substeps of the current scene. */
+ * `substeps` plain substeps of the current scene.  With collisions enabled
+ * (pbd_set_collision) each recorded substep runs the collision sweeps, so the
+ * correction includes the contact response. */
 PBD_API int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps);
 PBD_API int pbd_ws_num_samples(pbd_ctx* ctx, long long* n);
 PBD_API int pbd_ws_get_samples(pbd_ctx* ctx, long long first, long long n, float* X, float* Y);

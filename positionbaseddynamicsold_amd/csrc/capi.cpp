@@ -1031,15 +1031,15 @@ void consolidate(pbd_ctx* c) {
     c->pfinal_valid = false;
 }
 
-// Substep with collisions, in the order of projectConstraintWithColliCPU
-// (PBD_cuda.cu:1130-1210): per sweep ii, [detect if ii % every == 0 or last],
-// one Gauss-Seidel sweep, contact response, analytic colliders.
-void step_ccd(pbd_ctx* c, float dt) {
+// Sweeps ii = 0 .. n-1 of the collision substep on the current p: contacts
+// detected at ii % detect_every == 0 and at ii == n - 1, one Gauss-Seidel
+// sweep (cap: per-cloth budgets, a spent cloth is carried forward), the
+// contact response, the analytic colliders.
+void ccd_sweeps(pbd_ctx* c, int n, float dt, const int* cap) {
     hipStream_t s = c->stream;
-    advect_all(c, dt);
     replan_if_needed(c);
-    int I = c->sp.iterations, ncl = (int)c->cloths.size();
-    int every = std::max(1, c->coll.detect_every);
+    const int ncl = (int)c->cloths.size();
+    const int every = std::max(1, c->coll.detect_every);
     ensure_slots(c, 1);
     launch_fill_u64(s, c->res_slots.ptr, (size_t)c->nslots * ncl, 0ull);
     PBD_HIP(hipMemsetAsync(c->sweeps.ptr, 0, sizeof(int) * ncl, s));
@@ -1057,36 +1057,76 @@ void step_ccd(pbd_ctx* c, float dt) {
     a.fused = 0;
     a.chunk = 0;
     a.no_coll = 1;
-    for (int ii = 0; ii < I; ii++) {
+    a.copy_spent = cap ? 1 : 0;  // a cloth past its budget is carried into the other p buffer
+    c->pfinal_valid = false;
+    for (int ii = 0; ii < n; ii++) {
         int cur = c->pcur;
-        if (ii % every == 0 || ii == I - 1) detect_contacts(c, c->pbuf[cur].ptr);
+        if (ii % every == 0 || ii == n - 1) detect_contacts(c, c->pbuf[cur].ptr);
         a.sweep_base = ii;
-        for (const DBuf<int>* ord : {&c->order_first}) {
-            if (!ord->n) continue;
+        if (c->order_first.n) {
             if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
-            launch_tiles(s, (int)ord->n, ord->ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr,
-                         c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr,
-                         c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr,
-                         c->res_slots.ptr, ncl, c->sweeps.ptr, nullptr, a, threads, lds);
+            launch_tiles(s, (int)c->order_first.n, c->order_first.ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr,
+                         c->tdist_rest.ptr, c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr,
+                         c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->pbuf[cur].ptr,
+                         c->pbuf[cur ^ 1].ptr, c->res_slots.ptr, ncl, c->sweeps.ptr, cap, a, threads, lds);
             if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
         }
         c->pcur = cur ^ 1;
         c->cbp.resolve(s, c->x.ptr, c->pbuf[c->pcur].ptr, c->gtris.ptr, c->nv_total, c->coll.thickness,
                        c->coll.resolve_passes, c->coll.min_normal2 > 0.0f ? c->coll.min_normal2 : 1e-6f);
-        if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, c->pbuf[c->pcur].ptr, 0, c->nv_total);
+        if (coll) launch_colliders(s, c->cl.ptr, c->vcloth.ptr, c->pbuf[c->pcur].ptr, 0, c->nv_total, cap, ii);
     }
+}
+
+// Substep with collisions, in the order of projectConstraintWithColliCPU
+// (PBD_cuda.cu:1130-1210): per sweep ii, [detect if ii % every == 0 or last],
+// one Gauss-Seidel sweep, contact response, analytic colliders.
+//
+// With the warm start on, the prediction is made and decided per cloth after
+// advect and added to the kept cloths' p; a kept cloth then runs
+// warm_iterations sweeps, a discarded one `iterations` (exactly the plain
+// collision substep).  The sweep loop runs to the largest budget of a cloth
+// with prims: a cloth whose budget is spent is carried forward unprojected
+// (contacts are still resolved on it), and contacts are detected every
+// detect_every sweeps and at the loop's last sweep.
+void step_ccd(pbd_ctx* c, float dt) {
+    hipStream_t s = c->stream;
+    advect_all(c, dt);
+    replan_if_needed(c);
+    int I = c->sp.iterations, ncl = (int)c->cloths.size();
+    const int* cap = nullptr;
+    std::vector<int> budget(ncl, I);
+    if (c->ws.enabled) {
+        ws_apply_prediction(c, true, false, dt);
+        ws_read_used(c);
+        const int warm = c->sp.warm_iterations > 0 ? std::min(c->sp.warm_iterations, I) : I;
+        int n = 0;
+        for (int ci = 0; ci < ncl; ci++) {
+            budget[ci] = c->ws.used[ci] ? warm : I;
+            if (c->cloths[ci].np() > 0) n = std::max(n, budget[ci]);
+        }
+        I = n > 0 ? n : I;
+        cap = c->ws.cap.ptr;
+    }
+    ccd_sweeps(c, I, dt, cap);
     integrate_all(c, dt, false);
     PBD_HIP(hipStreamSynchronize(s));
-    for (auto& k : c->cloths) k.last_sweeps = I;
+    for (int ci = 0; ci < ncl; ci++) c->cloths[ci].last_sweeps = std::min(budget[ci], I);
+    c->sweeps_pending = false;
     if (c->prof_on) {
         collect_events(c);
         // I launches of one sweep each, loading and storing p
-        for (auto& k : c->cloths) k.last_sweeps = 1;
+        std::vector<int> ran(ncl);
+        for (int ci = 0; ci < ncl; ci++) {
+            ran[ci] = c->cloths[ci].last_sweeps;
+            c->cloths[ci].last_sweeps = 1;
+        }
         c->prof.tile_bytes += (double)I * tile_projection_bytes(c, 1, false, false, false);
-        for (auto& k : c->cloths) k.last_sweeps = I;
-        for (auto& k : c->cloths) {
-            c->prof.tile_alg_bytes += (double)I * (kDistBytes * k.ne + kBendBytes * k.nb);
-            c->prof.vertex_sweeps += (long long)k.nv * I;
+        for (int ci = 0; ci < ncl; ci++) {
+            Cloth& k = c->cloths[ci];
+            k.last_sweeps = ran[ci];
+            c->prof.tile_alg_bytes += (double)ran[ci] * (kDistBytes * k.ne + kBendBytes * k.nb);
+            c->prof.vertex_sweeps += (long long)k.nv * ran[ci];
         }
     }
 }
@@ -1098,6 +1138,7 @@ void substep(pbd_ctx* c, float dt) {
     c->pfinal_valid = false;
     if (c->coll.enabled) {
         PBD_CHECK(c->sp.path != PBD_PATH_JACOBI, PBD_ERR_STATE, "the CCD pipeline runs on the Gauss-Seidel paths");
+        if (c->ws.enabled) ws_require_params(c);
         step_ccd(c, dt);
     } else if (c->ws.enabled) {
         PBD_CHECK(c->sp.path != PBD_PATH_JACOBI, PBD_ERR_STATE, "the warm start runs on the Gauss-Seidel paths");
@@ -2094,7 +2135,6 @@ int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
     return guard([&] {
         PBD_CHECK(ctx && ctx->inited, PBD_ERR_STATE, "call pbd_init first");
         PBD_CHECK(sweeps >= 1 && substeps >= 0, PBD_ERR_ARG, "bad sweep / substep count");
-        PBD_CHECK(!ctx->coll.enabled, PBD_ERR_STATE, "sample collection runs without CCD");
         ws_require_params(ctx);
         PBD_CHECK(!ctx->ws.samples_normalized, PBD_ERR_STATE, "samples were standardised by training; collect before pbd_ws_train");
         set_device(ctx);
@@ -2141,8 +2181,12 @@ int pbd_ws_collect(pbd_ctx* ctx, int sweeps, int substeps) {
             PBD_HIP(hipMemcpyAsync(W.psave.ptr, p, sizeof(float4) * ctx->nv_total, hipMemcpyDeviceToDevice, s));
             double tol = ctx->sp.tol;
             ctx->sp.tol = -1.0;
-            project(ctx, sweeps, false, dt);
-            consolidate(ctx);
+            if (ctx->coll.enabled) {
+                ccd_sweeps(ctx, sweeps, dt, nullptr);  // the target includes the contact response
+            } else {
+                project(ctx, sweeps, false, dt);
+                consolidate(ctx);
+            }
             ctx->sp.tol = tol;
             ws_target(s, ctx->cl.ptr, ctx->vcloth.ptr, ctx->pbuf[ctx->pcur].ptr, W.psave.ptr, ctx->nv_total,
                       W.Y.ptr + 3 * W.ns);

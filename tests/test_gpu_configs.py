@@ -219,3 +219,67 @@ def test_no_vertex_crosses_a_face_after_the_response(orc, cfg, robust, expect):
     assert set(found) == set(expect)
     if expect:
         assert sum(found.values()) == 9
+
+
+def oracle_substep_warm(orc, oc, tris, sc, prm, delta, n):
+    """oracle_substep with a kept prediction added after advect and n sweeps
+    (the loop's last sweep detects too), as the warm-started collision
+    substep runs a kept cloth."""
+    oc.advect(prm)
+    if delta is not None:
+        oc.p[: len(delta)] = oc.p[: len(delta)] + delta
+    arr, nc = None, 0
+    for ii in range(n):
+        if ii % sc.detect_every == 0 or ii == n - 1:
+            arr, nc = orc.ccd_detect_swept(oc.x, oc.p, tris, sc.thickness)
+        oc.sweep(ii)
+        if nc:
+            oc.p[:] = orc.ccd_resolve(arr, nc, tris, oc.x, oc.p, oc.w, sc.thickness, sc.resolve_passes,
+                                      sc.min_normal2 if sc.min_normal2 > 0 else 1e-6)
+    oc.L.orc_integrate(oc.nv, prm.dt, orc.F(oc.x), orc.F(oc.v), orc.F(oc.p))
+
+
+@pytest.mark.parametrize("weights", ["trained", "zero", "corrupt"])
+def test_warm_start_with_collisions_bit_exact(orc, weights):
+    # the warm start inside the collision substep (reduced config 3, robust
+    # handling): a kept prediction is added after advect and the cloth runs
+    # the warm budget (5 sweeps, detecting at its first and last), a discarded
+    # one runs the plain 30; every substep equals the oracle bit for bit
+    say(f"warm start + CCD, {weights} network")
+    sc = scenes.config3(n=40, box_div=8).robust()
+    oc, tris, ncl = oracle_scene(orc, sc)
+    with capi.Context() as ctx:
+        cid, sid, cell = scenes.add_ccd_scene(ctx, sc)
+        prm = oc.params(dt=scenes.DT)
+        ctx.ws_init(0)
+        if weights == "trained":
+            # samples of the scene's own contact dynamics: each recorded
+            # substep is a collision substep of 40 sweeps, as the oracle's
+            ctx.ws_collect(40, 2)
+            for _ in range(2):
+                oracle_substep_warm(orc, oc, tris, sc, prm, None, 40)
+            assert np.array_equal(ctx.get(cid), oc.x[:ncl])
+            assert np.array_equal(ctx.get(cid, capi.STATE_V), oc.v[:ncl])
+            ctx.ws_train(epochs=20, batch=1024, lr=1e-3, seed=0)
+        else:
+            w = ctx.ws_params()
+            w[5632:5827] = 0.0 if weights == "zero" else 50.0  # W3 / b3: zero or wild predictions
+            ctx.ws_set_params(w)
+        ctx.set_solver(dt=scenes.DT, iterations=sc.iterations, warm_iterations=5)
+        ctx.ws_enable(True)
+        used = []
+        for k in range(6):
+            ctx.step(1)
+            u = ctx.ws_last_used(cid)
+            used.append(u)
+            delta = ctx.ws_last_delta(cid) if u else None
+            oracle_substep_warm(orc, oc, tris, sc, prm, delta, 5 if u else sc.iterations)
+            assert ctx.last_sweeps(cid) == (5 if u else sc.iterations)
+            assert np.array_equal(ctx.get(cid), oc.x[:ncl]), f"substep {k}"
+            assert np.array_equal(ctx.get(cid, capi.STATE_V), oc.v[:ncl]), f"substep {k}"
+        assert np.array_equal(ctx.get(sid), sc.static_pos)
+        say(f"  kept {used}")
+        if weights == "zero":
+            assert all(used)
+        if weights == "corrupt":
+            assert not any(used)
