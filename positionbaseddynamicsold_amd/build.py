@@ -4,7 +4,8 @@ Usage: python -m positionbaseddynamicsold_amd.build   (or build() from Python)
 
 Every source under csrc/ is compiled by hipcc into _build/ objects in
 parallel and linked into positionbaseddynamicsold_amd/libpbd.so; then the
-C++ example caller tests/cpp_driver.cpp is linked against it.
+C++ example caller tests/cpp_driver.cpp is linked against it, and the
+exhaustive rounding check tests/rounding_check.hip is built beside it.
 
 -ffp-contract=off keeps every float expression a sequence of IEEE operations
 in source order (no FMA contraction), which is what lets the device results
@@ -81,6 +82,26 @@ def build_driver(verbose: bool = True) -> Path:
     return DRIVER
 
 
+ROUNDING_SRC = ROOT / "tests" / "rounding_check.hip"
+ROUNDING = ROOT / "tests" / "rounding_check"
+
+
+def build_rounding_check(verbose: bool = True) -> Path:
+    """The exhaustive check of the short correctly rounded square root and
+    reciprocal in csrc/pbd_math.hpp (tests/rounding_check.hip), built with
+    the library's flags."""
+    tmp = ROUNDING.with_name(ROUNDING.name + ".tmp")
+    flags = [f for f in FLAGS if f not in ("-fPIC", "-fvisibility=hidden")]
+    cmd = [hipcc(), "-x", "hip", str(ROUNDING_SRC), "-o", str(tmp)] + flags
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"rounding check build failed:\n{r.stderr}")
+    os.replace(tmp, ROUNDING)
+    if verbose:
+        print(f"built {ROUNDING}")
+    return ROUNDING
+
+
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     srcs = sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
@@ -96,6 +117,7 @@ def build(verbose: bool = True) -> Path:
     if verbose:
         print(f"built {LIB}")
     build_driver(verbose)
+    build_rounding_check(verbose)
     return LIB
 
 

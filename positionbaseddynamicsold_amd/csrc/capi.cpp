@@ -925,7 +925,7 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
         launch_tiles(s, nblocks, order, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr, c->tbend.ptr,
                      c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr,
                      c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, cap, sa,
-                     threads, lds);
+                     threads, lds, !c->plan.bend.empty());
         if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
     };
     SolveArgs a{};
@@ -1068,7 +1068,8 @@ void ccd_sweeps(pbd_ctx* c, int n, float dt, const int* cap) {
             launch_tiles(s, (int)c->order_first.n, c->order_first.ptr, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr,
                          c->tdist_rest.ptr, c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr,
                          c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->pbuf[cur].ptr,
-                         c->pbuf[cur ^ 1].ptr, c->res_slots.ptr, ncl, c->sweeps.ptr, cap, a, threads, lds);
+                         c->pbuf[cur ^ 1].ptr, c->res_slots.ptr, ncl, c->sweeps.ptr, cap, a, threads, lds,
+                         !c->plan.bend.empty());
             if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
         }
         c->pcur = cur ^ 1;
@@ -1285,7 +1286,7 @@ void step_cloth(pbd_ctx* c, int ci, float dt) {
         launch_tiles(s, nblocks, order, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr, c->tbend.ptr,
                      c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr,
                      c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, nullptr, a,
-                     threads, lds);
+                     threads, lds, !c->plan.bend.empty());
     };
     int cur = c->pcur;
     if (k.whole) {
@@ -2594,7 +2595,10 @@ int pbd_autotune(pbd_ctx* ctx, int trial_substeps, pbd_solver_params* best, int*
         };
         const pbd_solver_params base = ctx->sp;
         const HashCfg hbase = ctx->ccfg;
+        // tile sizes: the whole LDS, half of it (two workgroups per CU), and two fixed caps
+        const int half = (ctx->lds_max / 2 - TILE_STATIC_LDS) / 16;
         std::vector<int> threads = {256, 512, 1024}, sweeps, maxv = {0, 4096, 2048}, bits = {-1};
+        if (half > 4096 && half < effective_max_load(ctx)) maxv.insert(maxv.begin() + 1, half);
         if (ctx->coll.enabled) {
             sweeps = {base.tile_sweeps};  // contacts force one sweep per launch
             if (hbase.mode == 1) {

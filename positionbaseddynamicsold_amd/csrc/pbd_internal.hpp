@@ -380,9 +380,9 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   const float* ktab_d, const float* ktab_b, float4* x, float4* v,
                   const float4* p_in, float4* p_out, unsigned long long* res_slots,
                   int nslots_cloth, int* sweeps_out, const int* sweep_cap, const SolveArgs& a,
-                  int threads, size_t lds_bytes);
-// Opts the tile kernel in to `bytes` of dynamic LDS on the current device;
-// returns what it may use (bytes, or 64 KiB if the runtime refuses).
+                  int threads, size_t lds_bytes, bool with_bend);
+// Opts the tile kernels in to `bytes` of dynamic LDS on the current device;
+// returns what they may use (bytes, or 64 KiB if the runtime refuses).
 size_t tiles_lds_optin(size_t bytes);
 constexpr int TILE_STATIC_LDS = 256;  // bytes reserved for the tile kernel's static LDS (block reductions)
 void launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);

@@ -42,17 +42,53 @@ PBD_HD V3 cross(V3 a, V3 b) {
 }
 PBD_HD V3 scale(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
 
+// Correctly rounded square root for x in [2^-96, 2^96]: the hardware
+// estimate s (within 1 ulp) is replaced by its neighbour s - ulp or s + ulp
+// when the fused residual x - s' * s shows that neighbour is nearer
+// (Markstein's correction).  In that range it is the sequence a correctly
+// rounded sqrtf runs without its rescaling of small inputs, so the result
+// equals sqrtf bit for bit; an exhaustive test over every float of the range
+// checks it.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    s = rd <= 0.0f ? sd : s;
+    s = ru > 0.0f ? su : s;
+    return s;
+}
+
+// Correctly rounded 1/d for d in [2^-48, 2^48]: one Newton step on the
+// hardware reciprocal r (within 1 ulp), r + r (1 - d r) with the residual
+// fused and exact (Markstein 1990).  Equal to the IEEE quotient 1.0f / d on
+// every float of the range, checked exhaustively.
+__device__ __forceinline__ float rcp_rn(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, r, 1.0f);
+    return __builtin_fmaf(e, r, r);
+}
+
 // Distance projection on two (x, y, z, w) records: the reference's CPU
 // formula r = normalize(p0-p1), dp0 = -w0/(w0+w1)(d-L) r, dp1 = +w1/(w0+w1)(d-L) r
 // (PBD_cuda.cu:1105-1124) scaled by k (PBD_cuda.cu:1282-1287).
-PBD_HD void project_distance(float4& a, float4& b, float rest, float k) {
+__device__ __forceinline__ void project_distance(float4& a, float4& b, float rest, float k) {
     float w0 = a.w, w1 = b.w;
     float ws = w0 + w1;
     if (ws == 0.0f) return;
     float vx = a.x - b.x, vy = a.y - b.y, vz = a.z - b.z;
-    float d = sqrtf(vx * vx + vy * vy + vz * vz);
-    if (d == 0.0f) return;
-    float inv = 1.0f / d;
+    const float d2 = vx * vx + vy * vy + vz * vz;
+    float d, inv;
+    if (d2 >= 0x1p-96f && d2 <= 0x1p96f) {
+        // lengths in [2^-48, 2^48]: the short forms, equal to sqrtf and 1.0f / d there
+        d = sqrt_rn(d2);
+        inv = rcp_rn(d);
+    } else {
+        d = sqrtf(d2);
+        if (d == 0.0f) return;
+        inv = 1.0f / d;
+    }
     float rx = vx * inv, ry = vy * inv, rz = vz * inv;
     float c = d - rest;
     // -w0/ws and w1/ws; equal or zero weights give exactly representable

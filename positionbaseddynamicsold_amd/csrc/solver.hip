@@ -394,52 +394,60 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     return m;
 }
 
-constexpr int PREFETCH = 8;       // distance records a thread loads ahead per colour phase
-constexpr int BEND_PREFETCH = 4;  // bending records a thread loads ahead per colour phase
-constexpr int VERT_PREFETCH = 4;  // vertices a thread gathers ahead when loading or storing a tile
+// How many records or vertices a thread fetches ahead: distance records per
+// colour phase, bending records per colour phase, and vertices when loading
+// or storing a tile.  The distance-only kernel keeps them small so that it
+// fits 64 VGPRs, which lets two 1024-thread workgroups share a CU (one
+// loading or storing its tile while the other projects).
+template <bool BEND>
+struct TileFetch {
+    static constexpr int dist = BEND ? 8 : 4;
+    static constexpr int bend = 4;
+    static constexpr int vert = 4;
+};
 
 // This thread's max relative residual over the tile's owned distance prims
-// (records fetched PREFETCH at a time).
+// (records fetched PF at a time).
+template <int PF>
 __device__ __forceinline__ double own_residual(const float4* P, const uint32_t* __restrict__ own,
                                                const float* __restrict__ rest, int n) {
     const int T = blockDim.x;
     double m = 0.0;
-    for (int j0 = threadIdx.x; j0 < n; j0 += PREFETCH * T) {
-        uint32_t pr[PREFETCH];
-        float rl[PREFETCH];
+    for (int j0 = threadIdx.x; j0 < n; j0 += PF * T) {
+        uint32_t pr[PF];
+        float rl[PF];
 #pragma unroll
-        for (int u = 0; u < PREFETCH; u++) {
+        for (int u = 0; u < PF; u++) {
             const int j = j0 + u * T;
             pr[u] = j < n ? own[j] : 0u;
             rl[u] = j < n ? rest[j] : 1.0f;
         }
 #pragma unroll
-        for (int u = 0; u < PREFETCH; u++)
+        for (int u = 0; u < PF; u++)
             if (j0 + u * T < n) m = fmax(m, rel_residual(P[pr[u] & 0xffffu], P[pr[u] >> 16], rl[u]));
     }
     return m;
 }
 
-__global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
-                                                const TileDev* __restrict__ tiles,
-                                                const int* __restrict__ verts,
-                                                const uint32_t* __restrict__ dist,
-                                                const float* __restrict__ dist_rest,
-                                                const uint32_t* __restrict__ bend,
-                                                const float* __restrict__ bend_rest,
-                                                const uint32_t* __restrict__ own,
-                                                const float* __restrict__ own_rest,
-                                                const int* __restrict__ cols,
-                                                const ClothDev* __restrict__ cloths,
-                                                const float* __restrict__ ktab_d,
-                                                const float* __restrict__ ktab_b, float4* __restrict__ x,
-                                                float4* __restrict__ v, const float4* __restrict__ p_in,
-                                                float4* __restrict__ p_out,
-                                                unsigned long long* __restrict__ res_slots, int ncl,
-                                                int* __restrict__ sweeps_out, const int* __restrict__ sweep_cap,
-                                                SolveArgs a) {
-    extern __shared__ float4 P[];
-    __shared__ double red[16];
+#define TILE_PARAMS                                                                                              \
+    const int *__restrict__ order, const TileDev *__restrict__ tiles, const int *__restrict__ verts,              \
+        const uint32_t *__restrict__ dist, const float *__restrict__ dist_rest, const uint32_t *__restrict__ bend, \
+        const float *__restrict__ bend_rest, const uint32_t *__restrict__ own, const float *__restrict__ own_rest, \
+        const int *__restrict__ cols, const ClothDev *__restrict__ cloths, const float *__restrict__ ktab_d,       \
+        const float *__restrict__ ktab_b, float4 *__restrict__ x, float4 *__restrict__ v,                          \
+        const float4 *__restrict__ p_in, float4 *__restrict__ p_out, unsigned long long *__restrict__ res_slots,   \
+        int ncl, int *__restrict__ sweeps_out, const int *__restrict__ sweep_cap, SolveArgs a
+#define TILE_ARGS                                                                                               \
+    order, tiles, verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v, p_in, \
+        p_out, res_slots, ncl, sweeps_out, sweep_cap, a
+
+// The tile kernel's body; BEND = false leaves the bending phases out (a plan
+// with no bending prims).
+template <bool BEND>
+__device__ __forceinline__ void tiles_body(TILE_PARAMS, float4* P, double* red) {
+    constexpr int PREFETCH = TileFetch<BEND>::dist;
+    constexpr int BEND_PREFETCH = TileFetch<BEND>::bend;
+    constexpr int VERT_PREFETCH = TileFetch<BEND>::vert;
     int slot = order[blockIdx.x];
     if (slot < 0) return;
     const TileDev td = tiles[slot];
@@ -566,8 +574,8 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
                     P[i1] = p1;
                 }
             }
-            const int be = bnow ? bc[c] + bnow[c] : bc[c + 1];
-            for (int j0 = bc[c] + tid; j0 < be; j0 += BEND_PREFETCH * T) {
+            const int be = !BEND ? bc[c] : bnow ? bc[c] + bnow[c] : bc[c + 1];
+            for (int j0 = bc[c] + tid; BEND && j0 < be; j0 += BEND_PREFETCH * T) {
                 uint2 bw[BEND_PREFETCH];
                 float br[BEND_PREFETCH];
 #pragma unroll
@@ -601,12 +609,12 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
         }
         sw++;
         if (td.whole && tol >= 0.0 && sw % a.check_every == 0) {
-            resid = block_max(own_residual(P, own + td.own_off, own_rest + td.own_off, td.n_own), red);
+            resid = block_max(own_residual<PREFETCH>(P, own + td.own_off, own_rest + td.own_off, td.n_own), red);
             if (resid <= tol) break;
         }
     }
     if (!td.whole || resid < 0.0 || (sw % a.check_every) != 0)
-        resid = block_max(own_residual(P, own + td.own_off, own_rest + td.own_off, td.n_own), red);
+        resid = block_max(own_residual<PREFETCH>(P, own + td.own_off, own_rest + td.own_off, td.n_own), red);
     if (tid == 0) {
         atomicMax(&res_slots[(size_t)a.chunk * ncl + td.cloth], (unsigned long long)__double_as_longlong(resid));
         atomicMax(&sweeps_out[td.cloth], a.sweep_base + sw);
@@ -639,6 +647,23 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
     }
 }
 
+// Plans without bending prims: at most 64 VGPRs (eight waves per SIMD), so
+// two 1024-thread workgroups fit a CU when their tiles fit half its LDS.
+__global__ __launch_bounds__(1024, 8) void k_tiles(TILE_PARAMS) {
+    extern __shared__ float4 P[];
+    __shared__ double red[16];
+    tiles_body<false>(TILE_ARGS, P, red);
+}
+
+__global__ __launch_bounds__(1024) void k_tiles_bend(TILE_PARAMS) {
+    extern __shared__ float4 P[];
+    __shared__ double red[16];
+    tiles_body<true>(TILE_ARGS, P, red);
+}
+
+#undef TILE_PARAMS
+#undef TILE_ARGS
+
 // Dynamic LDS the tile kernel may use on the current device: opts in to
 // `bytes` and returns it, or returns the 64 KiB every launch may use without
 // an opt-in when the runtime refuses it.  Every context on a device asks for
@@ -646,11 +671,14 @@ __global__ __launch_bounds__(1024) void k_tiles(const int* __restrict__ order,
 // static LDS), so contexts never lower each other's limit.
 size_t tiles_lds_optin(size_t bytes) {
     if (bytes <= 65536) return bytes;
-    const hipError_t e =
-        hipFuncSetAttribute((const void*)k_tiles, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e == hipSuccess) return bytes;
-    (void)hipGetLastError();  // the refusal is handled by the smaller limit
-    return 65536;
+    for (const void* k : {(const void*)k_tiles, (const void*)k_tiles_bend}) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // the refusal is handled by the smaller limit
+            return 65536;
+        }
+    }
+    return bytes;
 }
 
 void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* tiles, const int* verts,
@@ -658,11 +686,11 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   const uint32_t* own, const float* own_rest, const int* cols, const ClothDev* cloths,
                   const float* ktab_d, const float* ktab_b, float4* x, float4* v, const float4* p_in,
                   float4* p_out, unsigned long long* res_slots, int ncl, int* sweeps_out, const int* sweep_cap,
-                  const SolveArgs& a, int threads, size_t lds_bytes) {
+                  const SolveArgs& a, int threads, size_t lds_bytes, bool with_bend) {
     if (nblocks <= 0) return;
-    hipLaunchKernelGGL(k_tiles, dim3(nblocks), dim3(threads), lds_bytes, s, order, tiles, verts, dist, dist_rest,
-                       bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v, p_in, p_out, res_slots,
-                       ncl, sweeps_out, sweep_cap, a);
+    hipLaunchKernelGGL(with_bend ? k_tiles_bend : k_tiles, dim3(nblocks), dim3(threads), lds_bytes, s, order, tiles,
+                       verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v,
+                       p_in, p_out, res_slots, ncl, sweeps_out, sweep_cap, a);
     PBD_HIP(hipGetLastError());
 }
 

@@ -1,11 +1,11 @@
 // A C++ caller of libpbd.so: the reference's driver loop
 // (PositionBasedDynamics.cpp:256-295 -- PBDObject + SolverPBD, then per frame
 // and substep Advect / ProjectConstraint(GAUSSSEIDEL, iterations) /
-// Integration) written against include/pbd.h, as INTEGRATION.md shows it.
+// Integration) written against include/pbd.h.
 //
 //   cpp_driver OUT.bin [substeps]
 //
-// The config-1 cloth (32 x 32, 1 m, pins at the two top corners, jitter
+// A 32 x 32 cloth (1 m, pins at the two top corners, jitter
 // 1e-4 m from seed 0, dt 1/600 s, 20 sweeps) is stepped twice from the same
 // start: with pbd_step (substeps replayed from a captured graph) and with the
 // split loop pbd_advect / pbd_project / pbd_integrate, the reference's three
@@ -36,7 +36,7 @@ constexpr int kRes = 32;
 constexpr float kDt = 1.0f / 600.0f;
 constexpr int kIterations = 20;
 
-// Builds the config-1 cloth in a new context (PBDObject(...).Init() and
+// Builds the 32 x 32 cloth in a new context (PBDObject(...).Init() and
 // InitAnchorConstr, PBD_cuda.cu:125-153,831-869).
 bool make_cloth(pbd_ctx** ctx, int* cloth, int graph) {
     CHECK(pbd_create(0, ctx));

@@ -148,9 +148,10 @@ def test_autotune_leaves_results_unchanged():
         p, bits, ms, n = a.autotune(1)
         print(f"autotune: threads {p.block_threads} tile_sweeps {p.tile_sweeps} max_verts {p.tile_max_verts} "
               f"{ms:.3f} ms/substep over {n} trials", flush=True)
-        # threads x tile_sweeps {1, 2, 4, 5} x tile sizes, then the other way of
-        # stepping single-tile cloths (split_whole) at the best of those
-        assert n == 3 * 4 * 3 + 1
+        # threads x tile_sweeps {1, 2, 4, 5} x tile sizes (the whole LDS, half of
+        # it, 4096 and 2048 vertices), then the other way of stepping
+        # single-tile cloths (split_whole) at the best of those
+        assert n == 3 * 4 * 4 + 1
         assert bits == -1 and ms > 0.0
         assert p.block_threads in (256, 512, 1024) and p.tile_sweeps in (1, 2, 4, 5)
         assert p.iterations == 7 and p.graph == 1
