@@ -52,6 +52,14 @@ struct pbd_ctx {
     // launch orders: single-tile cloths' tiles, multi-tile cloths' tiles, and
     // both (the first launch of a substep runs them side by side)
     DBuf<int> tverts, tcols, order_whole, order_partial, order_first, tile_ids;
+    // later chunks under per-cloth budgets: the multi-tile tiles still
+    // needed, compacted on the device, and the list's length
+    DBuf<int> order_follow, follow_len, order_partial_cloth;
+    int num_cu = 0;
+    // workgroups the tile kernel keeps resident on the whole GPU, for the
+    // (kernel, threads, LDS) it was last asked for
+    int walk_key_threads = 0, walk_key_bend = -1, walk_slots = 0;
+    size_t walk_key_lds = 0;
     DBuf<uint32_t> tdist, tbend, town;
     DBuf<float> tdist_rest, tbend_rest, town_rest;
     bool any_whole = false, any_partial = false;
@@ -243,6 +251,11 @@ void upload_plan(pbd_ctx* c) {
     std::vector<int> first(p.whole_order);  // single-tile cloths first: their blocks run longest
     first.insert(first.end(), p.partial_order.begin(), p.partial_order.end());
     c->order_first.upload(first, c->stream);
+    std::vector<int> pcl(p.partial_order.size());
+    for (size_t i = 0; i < pcl.size(); i++) pcl[i] = p.partial_order[i] >= 0 ? p.tiles[p.partial_order[i]].cloth : -1;
+    c->order_partial_cloth.upload(pcl, c->stream);
+    c->order_follow.alloc(std::max<size_t>(1, p.partial_order.size()));
+    c->follow_len.alloc(1);
     std::vector<int> ids(p.tiles.size());
     for (size_t t = 0; t < ids.size(); t++) ids[t] = (int)t;
     c->tile_ids.upload(ids, c->stream);
@@ -250,6 +263,21 @@ void upload_plan(pbd_ctx* c) {
     c->any_whole = false;
     c->any_partial = false;
     for (auto& t : p.tiles) (t.whole ? c->any_whole : c->any_partial) = true;
+}
+
+// Grid of the list-walking tile launches: every resident workgroup slot of
+// the GPU once, rounded up to a multiple of 8 (the XCD layout of the lists).
+int walk_grid(pbd_ctx* c) {
+    const bool bend = !c->plan.bend.empty();
+    const int threads = c->sp.block_threads > 0 ? c->sp.block_threads : 1024;
+    const size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
+    if (threads != c->walk_key_threads || lds != c->walk_key_lds || (int)bend != c->walk_key_bend) {
+        c->walk_slots = std::max(1, c->num_cu) * tiles_per_cu(bend, threads, lds);
+        c->walk_key_threads = threads;
+        c->walk_key_lds = lds;
+        c->walk_key_bend = (int)bend;
+    }
+    return (c->walk_slots + 7) / 8 * 8;
 }
 
 void replan_if_needed(pbd_ctx* c) {
@@ -723,6 +751,8 @@ double tile_projection_bytes(const pbd_ctx* c, int ts, bool fused_whole, bool in
         const int L = k.last_sweeps;
         if (L <= 0 || k.tile_record_bytes.size() < 2) continue;
         const bool kept = with_delta && ci < c->ws.used.size() && c->ws.used[ci] != 0;
+        // owned prims are read for the residual only under a tolerance
+        const double own = (k.tol >= 0.0 ? k.tol : c->sp.tol) >= 0.0 ? 8.0 * (double)k.tiles_owned : 0.0;
         auto launch = [&](int n, bool first, bool last) {
             double load = (first && fused_whole) ? 32.0 : 16.0;
             if (first && kept) load += 16.0;
@@ -730,8 +760,10 @@ double tile_projection_bytes(const pbd_ctx* c, int ts, bool fused_whole, bool in
             const double store = integ ? 64.0 : 16.0;
             const int row = std::min(n, (int)k.tile_record_bytes.size() - 1);
             const double rec = k.whole ? n * k.tile_record_bytes[1] : k.tile_record_bytes[row];
-            return 4.0 * (double)(k.tiles_loaded + k.tiles_interior) + load * (double)k.tiles_loaded + rec +
-                   8.0 * (double)k.tiles_owned + store * (double)k.tiles_interior;
+            // a launch of n sweeps loads the prefix of each tile's list its cone needs
+            const double loaded = k.whole ? (double)k.tiles_loaded : k.tile_loaded_by_sweeps[row];
+            return 4.0 * (loaded + (double)k.tiles_interior) + load * loaded + rec + own +
+                   store * (double)k.tiles_interior;
         };
         if (k.whole) {
             total += launch(L, true, true);
@@ -920,14 +952,29 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
     int threads = c->sp.block_threads > 0 ? c->sp.block_threads : 1024;
     int cur = c->pcur;
-    auto tiles = [&](int nblocks, const int* order, const float4* pin, float4* pout, const SolveArgs& sa) {
+    const bool bend = !c->plan.bend.empty();
+    auto tiles = [&](int nblocks, const int* order, const float4* pin, float4* pout, const SolveArgs& sa,
+                     const int* n_items = nullptr) {
         if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
         launch_tiles(s, nblocks, order, c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr, c->tbend.ptr,
                      c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr, c->ktab_d.ptr,
                      c->ktab_b.ptr, c->x.ptr, c->v.ptr, pin, pout, c->res_slots.ptr, ncl, c->sweeps.ptr, cap, sa,
-                     threads, lds, !c->plan.bend.empty());
+                     threads, lds, bend, n_items);
         if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
     };
+    // Under per-cloth budgets and no tolerance, a later chunk needs only the
+    // tiles of cloths whose budget outlasts the first chunk (with a warm
+    // start: the discarded cloths).  Their list is compacted on the device
+    // once, and the later chunks walk it with a grid that fills the GPU once
+    // instead of one workgroup per multi-tile tile, most of which would
+    // find their cloth done.
+    const bool follow = cap && !any_tol && c->any_partial && nchunks > 1;
+    int follow_grid = 0;
+    if (follow) {
+        launch_follow_list(s, c->order_partial.ptr, c->order_partial_cloth.ptr, (int)c->order_partial.n, cap, ts,
+                           c->order_follow.ptr, c->follow_len.ptr);
+        follow_grid = std::min(walk_grid(c), (int)c->order_partial.n);
+    }
     SolveArgs a{};
     a.dt = dt;
     a.check_every = ce;
@@ -959,8 +1006,12 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
             a.sweep_base = sw;
             a.nsweeps = std::min(ts, iterations - sw);
             a.chunk = ch;
-            const DBuf<int>& ord = (ch == 0 && c->any_whole) ? c->order_first : c->order_partial;
-            tiles((int)ord.n, ord.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
+            if (ch > 0 && follow) {
+                tiles(follow_grid, c->order_follow.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a, c->follow_len.ptr);
+            } else {
+                const DBuf<int>& ord = (ch == 0 && c->any_whole) ? c->order_first : c->order_partial;
+                tiles((int)ord.n, ord.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
+            }
             a.delta = nullptr;  // later chunks read what the first one wrote
             a.keep = nullptr;
             a.advect_load = 0;
@@ -1183,6 +1234,10 @@ void graph_substep(pbd_ctx* c, float dt) {
     if (!c->graph[start]) {
         // buffers built on first use are built before capture (it allows no allocation or wait)
         if (c->sp.path == PBD_PATH_JACOBI) ensure_jacobi(c);
+        if (c->sp.path != PBD_PATH_JACOBI && c->sp.path != PBD_PATH_GLOBAL) {
+            replan_if_needed(c);
+            (void)walk_grid(c);  // an occupancy query, made outside the capture
+        }
         hipGraph_t g = nullptr;
         PBD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
         try {
@@ -1369,6 +1424,7 @@ int pbd_create(int device, pbd_ctx** out) {
             PBD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             int lds = 0;
             PBD_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+            PBD_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device));
             // lds_max - TILE_STATIC_LDS is what the tile kernel's dynamic LDS may reach
             const size_t dyn = tiles_lds_optin((size_t)std::max(lds, 65536) - TILE_STATIC_LDS);
             c->lds_max = (int)dyn + TILE_STATIC_LDS;

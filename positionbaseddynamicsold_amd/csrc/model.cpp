@@ -330,6 +330,11 @@ void read_txt_cloth(const char* topol, const char* constr, std::vector<float>& p
 // the last n sweeps: its sweep s uses the prefix of sweep s + sweeps - n.
 // The owned distance prims (first vertex in I) are added to D_P so that the
 // residual can be evaluated on valid values at the end of the launch.
+// L is listed in the order vertices join the cone (interior and owned first,
+// then by decreasing entry).  A prim needed at phase k joins its vertices to
+// the cone at phase k at the latest, so both have entry >= k: a launch of
+// the last n sweeps reads only vertices with entry >= (sweeps - n) * C, a
+// prefix of L whose length the tile records.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -362,6 +367,7 @@ struct TileOut {
     int n_int = 0;
     std::vector<std::vector<int>> dist_by_col, bend_by_col;  // cloth prim ids, by decreasing last needed sweep
     std::vector<int> dcount, bcount;  // [s * C + c]: prims of colour c needed at sweep s (a prefix); empty: all
+    std::vector<int> lcount;  // [n - 1]: loaded vertices a launch of the last n sweeps needs (a prefix of load)
     std::vector<int> own;    // owned distance prims
 };
 
@@ -460,6 +466,15 @@ int plan_one(const ClothTopo& t, const std::vector<int>& interior, int phases, i
     };
     order(dl, out->dist_by_col, out->dcount);
     order(bl, out->bend_by_col, out->bcount);
+    for (size_t m = 1; m < D.size(); m++)
+        PBD_CHECK(entry[D[m]] <= entry[D[m - 1]], PBD_ERR_STATE, "internal: cone list not in entry order");
+    out->lcount.assign((size_t)sweeps, (int)D.size());
+    for (int n = 1; n < sweeps; n++) {
+        const int from = (sweeps - n) * t.C;
+        int m = 0;
+        while (m < (int)D.size() && entry[D[m]] >= from) m++;
+        out->lcount[(size_t)n - 1] = m;
+    }
     return (int)D.size();
 }
 
@@ -591,6 +606,7 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split
         c.tile_begin = (int)plan.tiles.size();
         c.tiles_loaded = c.tiles_interior = c.tiles_owned = 0;
         c.tile_record_bytes.assign((size_t)std::max(1, sweeps) + 1, 0.0);
+        c.tile_loaded_by_sweeps.assign((size_t)std::max(1, sweeps) + 1, 0.0);
         for (auto& o : per[ci].outs) {
             TileDev td{};
             td.cloth = (int)ci;
@@ -622,10 +638,20 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split
             }
             plan.cols.push_back(bsum);
             // per-sweep prefixes of the partial tiles: sweeps x C distance
-            // counts, then sweeps x C bending counts
+            // counts, sweeps x C bending counts, then the loaded vertices of
+            // a launch of 1 .. sweeps sweeps
             td.plan_sweeps = o.dcount.empty() ? 0 : (int)(o.dcount.size() / std::max(1, c.ncolors));
             plan.cols.insert(plan.cols.end(), o.dcount.begin(), o.dcount.end());
             plan.cols.insert(plan.cols.end(), o.bcount.begin(), o.bcount.end());
+            if (td.plan_sweeps) {
+                PBD_CHECK((int)o.lcount.size() == td.plan_sweeps, PBD_ERR_STATE, "internal: load prefix count");
+                plan.cols.insert(plan.cols.end(), o.lcount.begin(), o.lcount.end());
+                for (int n = 1; n <= td.plan_sweeps && n < (int)c.tile_loaded_by_sweeps.size(); n++)
+                    c.tile_loaded_by_sweeps[n] += o.lcount[(size_t)n - 1];
+            } else {
+                for (size_t n = 1; n < c.tile_loaded_by_sweeps.size(); n++)
+                    c.tile_loaded_by_sweeps[n] += (double)o.load.size();
+            }
             c.tiles_loaded += (long long)o.load.size();
             c.tiles_interior += o.n_int;
             c.tiles_owned += (long long)o.own.size();

@@ -184,10 +184,11 @@ struct Cloth {
     bool whole = false;
     int max_loaded = 0;
     // tile plan sums for the bytes a projection moves: loaded, interior and
-    // owned-prim counts over the cloth's tiles, and the prim-record bytes of
-    // one launch of n sweeps (index n; whole tiles: of one sweep, index 1)
+    // owned-prim counts over the cloth's tiles, the prim-record bytes of one
+    // launch of n sweeps (index n; whole tiles: of one sweep, index 1) and
+    // the vertices such a launch loads (index n; whole tiles: all, index 1)
     long long tiles_loaded = 0, tiles_interior = 0, tiles_owned = 0;
-    std::vector<double> tile_record_bytes;
+    std::vector<double> tile_record_bytes, tile_loaded_by_sweeps;
     int last_sweeps = 0;
     double tol = -1.0;                    // per-cloth residual tolerance (< 0: the solver's)
     int np() const { return ne + nb; }
@@ -380,7 +381,13 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   const float* ktab_d, const float* ktab_b, float4* x, float4* v,
                   const float4* p_in, float4* p_out, unsigned long long* res_slots,
                   int nslots_cloth, int* sweeps_out, const int* sweep_cap, const SolveArgs& a,
-                  int threads, size_t lds_bytes, bool with_bend);
+                  int threads, size_t lds_bytes, bool with_bend, const int* n_items = nullptr);
+// Compacts the multi-tile order (layout order[j * 8 + g]; order_cloth: each
+// entry's cloth, -1 for padding) to the tiles whose cloth's budget cap
+// exceeds min_cap, rebalanced over the 8 groups, length into *n_out.
+void launch_follow_list(hipStream_t s, const int* order, const int* order_cloth, int n_order, const int* cap,
+                        int min_cap, int* out, int* n_out);
+int tiles_per_cu(bool with_bend, int threads, size_t lds_bytes);
 // Opts the tile kernels in to `bytes` of dynamic LDS on the current device;
 // returns what they may use (bytes, or 64 KiB if the runtime refuses).
 size_t tiles_lds_optin(size_t bytes);

@@ -444,11 +444,11 @@ __device__ __forceinline__ double own_residual(const float4* P, const uint32_t* 
 // The tile kernel's body; BEND = false leaves the bending phases out (a plan
 // with no bending prims).
 template <bool BEND>
-__device__ __forceinline__ void tiles_body(TILE_PARAMS, float4* P, double* red) {
+__device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, double* red) {
     constexpr int PREFETCH = TileFetch<BEND>::dist;
     constexpr int BEND_PREFETCH = TileFetch<BEND>::bend;
     constexpr int VERT_PREFETCH = TileFetch<BEND>::vert;
-    int slot = order[blockIdx.x];
+    int slot = order[item];
     if (slot < 0) return;
     const TileDev td = tiles[slot];
     const ClothDev& cd = cloths[td.cloth];
@@ -487,12 +487,24 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, float4* P, double* red) 
         }
         return;
     }
+    const int C = td.ncolors;
+    const int* dc = cols + td.col_off;
+    const int* bc = dc + C + 1;
+    // partial tiles run, at sweep s of this launch, the prefix of each colour
+    // list that the cone of the launch's last nsweeps sweeps needs (the index
+    // is kept within the tile's plan_sweeps rows of counts), and load the
+    // prefix of the vertex list that cone reads
+    const int* dn = bc + C + 1;
+    const int* bn = dn + td.plan_sweeps * C;
+    const int* ln = bn + td.plan_sweeps * C;
+    const int shift = max(0, td.plan_sweeps - nsweeps);
+    const int n_load = (td.plan_sweeps && nsweeps < td.plan_sweeps) ? ln[nsweeps - 1] : td.n_load;
     // the kept warm-start prediction enters here, on the substep's first load
     const bool add_delta = a.delta != nullptr && a.keep[td.cloth] != 0;
-    for (int i0 = tid; i0 < td.n_load; i0 += VERT_PREFETCH * T) {
+    for (int i0 = tid; i0 < n_load; i0 += VERT_PREFETCH * T) {
         int g[VERT_PREFETCH];
 #pragma unroll
-        for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < td.n_load ? vl[i0 + u * T] : -1;
+        for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < n_load ? vl[i0 + u * T] : -1;
         if (td.whole ? a.fused : a.advect_load) {
             float4 X[VERT_PREFETCH], V[VERT_PREFETCH];
 #pragma unroll
@@ -527,15 +539,6 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, float4* P, double* red) 
     }
     __syncthreads();
 
-    const int C = td.ncolors;
-    const int* dc = cols + td.col_off;
-    const int* bc = dc + C + 1;
-    // partial tiles run, at sweep s of this launch, the prefix of each colour
-    // list that the cone of the launch's last nsweeps sweeps needs (the index
-    // is kept within the tile's plan_sweeps rows of counts)
-    const int* dn = bc + C + 1;
-    const int* bn = dn + td.plan_sweeps * C;
-    const int shift = max(0, td.plan_sweeps - nsweeps);
     const uint32_t* D = dist + td.dist_off;
     const float* DR = dist_rest + td.dist_off;
     const uint32_t* B = bend + 2 * (size_t)td.bend_off;
@@ -600,7 +603,7 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, float4* P, double* red) 
             __syncthreads();
         }
         if (coll) {
-            for (int i = tid; i < td.n_load; i += T) {
+            for (int i = tid; i < n_load; i += T) {
                 float4 q = P[i];
                 collide_point(q, cd.sphere_on, cd.sphere_c, cd.sphere_r, cd.box_on, cd.box_lo, cd.box_hi);
                 P[i] = q;
@@ -613,10 +616,13 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, float4* P, double* red) 
             if (resid <= tol) break;
         }
     }
-    if (!td.whole || resid < 0.0 || (sw % a.check_every) != 0)
+    // the owned prims' residual decides the tolerance stops; with no
+    // tolerance (a fixed budget) nothing reads it, and it is not computed
+    if (tol >= 0.0 && (!td.whole || resid < 0.0 || (sw % a.check_every) != 0))
         resid = block_max(own_residual<PREFETCH>(P, own + td.own_off, own_rest + td.own_off, td.n_own), red);
     if (tid == 0) {
-        atomicMax(&res_slots[(size_t)a.chunk * ncl + td.cloth], (unsigned long long)__double_as_longlong(resid));
+        if (tol >= 0.0)
+            atomicMax(&res_slots[(size_t)a.chunk * ncl + td.cloth], (unsigned long long)__double_as_longlong(resid));
         atomicMax(&sweeps_out[td.cloth], a.sweep_base + sw);
     }
     // integrate on store: a fused whole cloth, or the launch holding this
@@ -647,18 +653,98 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, float4* P, double* red) 
     }
 }
 
-// Plans without bending prims: at most 64 VGPRs (eight waves per SIMD), so
-// two 1024-thread workgroups fit a CU when their tiles fit half its LDS.
-__global__ __launch_bounds__(1024, 8) void k_tiles(TILE_PARAMS) {
-    extern __shared__ float4 P[];
-    __shared__ double red[16];
-    tiles_body<false>(TILE_ARGS, P, red);
+// One tile per workgroup (n_items == nullptr: order[blockIdx.x]), or a
+// fixed grid walking a list whose length the device wrote (*n_items): the
+// workgroup takes entries blockIdx.x, blockIdx.x + gridDim.x, ... .  With
+// gridDim.x a multiple of 8 every entry a workgroup takes sits at its own
+// position mod 8, the XCD the list's layout meant it for.
+template <bool BEND>
+__device__ __forceinline__ void tiles_walk(TILE_PARAMS, const int* __restrict__ n_items, float4* P, double* red) {
+    // one loop for both forms, so the body is inlined once
+    const int n = n_items ? *n_items : (int)gridDim.x;
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        tiles_body<BEND>(TILE_ARGS, it, P, red);
+        __syncthreads();  // the tile's stores have read LDS before the next tile loads
+    }
 }
 
-__global__ __launch_bounds__(1024) void k_tiles_bend(TILE_PARAMS) {
+// Plans without bending prims: at most 64 VGPRs (eight waves per SIMD), so
+// two 1024-thread workgroups fit a CU when their tiles fit half its LDS.
+__global__ __launch_bounds__(1024, 8) void k_tiles(TILE_PARAMS, const int* __restrict__ n_items) {
     extern __shared__ float4 P[];
     __shared__ double red[16];
-    tiles_body<true>(TILE_ARGS, P, red);
+    tiles_walk<false>(TILE_ARGS, n_items, P, red);
+}
+
+__global__ __launch_bounds__(1024) void k_tiles_bend(TILE_PARAMS, const int* __restrict__ n_items) {
+    extern __shared__ float4 P[];
+    __shared__ double red[16];
+    tiles_walk<true>(TILE_ARGS, n_items, P, red);
+}
+
+// The tiles of a later chunk: the entries of `order` (the multi-tile launch
+// order, laid out as order[j * 8 + g] for XCD group g, a group being a
+// contiguous run of the plan's tiles; order_cloth holds each entry's cloth,
+// -1 for padding) whose cloth's sweep budget cap exceeds min_cap.  They keep
+// their order and are dealt again into 8 contiguous groups of equal size in
+// the same layout, so neighbouring tiles still share an XCD and every XCD
+// gets an equal share however the remaining cloths lie; the last group is
+// padded with -1, and *n_out is the list's length.  One workgroup of 512
+// threads: wave g ranks group g's entries with ballots, FOLLOW_BATCH entries
+// per lane per round so that a round waits for two memory round trips.
+constexpr int FOLLOW_BATCH = 8;
+
+__global__ __launch_bounds__(512) void k_follow_list(const int* __restrict__ order, const int* __restrict__ order_cloth,
+                                                     int n_order, const int* __restrict__ cap, int min_cap,
+                                                     int* __restrict__ out, int* __restrict__ n_out) {
+    __shared__ int counts[8];
+    const int g = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int per_g = n_order / 8;
+    // needed entries of this lane's batch starting at j0: the slots (or -1)
+    auto batch = [&](int j0, int* slot) {
+        int cl[FOLLOW_BATCH];
+#pragma unroll
+        for (int u = 0; u < FOLLOW_BATCH; u++) {
+            const int j = j0 + u * 64 + lane;
+            cl[u] = j < per_g ? order_cloth[j * 8 + g] : -1;
+            slot[u] = j < per_g ? order[j * 8 + g] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < FOLLOW_BATCH; u++)
+            if (cl[u] < 0 || cap[cl[u]] <= min_cap) slot[u] = -1;
+    };
+    int count = 0;
+    for (int j0 = 0; j0 < per_g; j0 += 64 * FOLLOW_BATCH) {
+        int slot[FOLLOW_BATCH];
+        batch(j0, slot);
+#pragma unroll
+        for (int u = 0; u < FOLLOW_BATCH; u++) count += __popcll(__ballot(slot[u] >= 0));
+    }
+    if (lane == 0) counts[g] = count;
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int k = 0; k < 8; k++) {
+        base += k < g ? counts[k] : 0;
+        total += counts[k];
+    }
+    const int per = (total + 7) / 8;  // entries per new group
+    int r = base;
+    for (int j0 = 0; j0 < per_g; j0 += 64 * FOLLOW_BATCH) {
+        int slot[FOLLOW_BATCH];
+        batch(j0, slot);
+#pragma unroll
+        for (int u = 0; u < FOLLOW_BATCH; u++) {
+            const unsigned long long m = __ballot(slot[u] >= 0);
+            if (slot[u] >= 0) {
+                const int q = r + __popcll(m & ((1ull << lane) - 1ull));
+                out[(q % per) * 8 + q / per] = slot[u];
+            }
+            r += __popcll(m);
+        }
+    }
+    // pad: the positions of ranks total .. 8 * per - 1 (all in the last groups)
+    for (int q = total + (int)threadIdx.x; q < 8 * per; q += (int)blockDim.x) out[(q % per) * 8 + q / per] = -1;
+    if (threadIdx.x == 0) *n_out = 8 * per;
 }
 
 #undef TILE_PARAMS
@@ -686,12 +772,28 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   const uint32_t* own, const float* own_rest, const int* cols, const ClothDev* cloths,
                   const float* ktab_d, const float* ktab_b, float4* x, float4* v, const float4* p_in,
                   float4* p_out, unsigned long long* res_slots, int ncl, int* sweeps_out, const int* sweep_cap,
-                  const SolveArgs& a, int threads, size_t lds_bytes, bool with_bend) {
+                  const SolveArgs& a, int threads, size_t lds_bytes, bool with_bend, const int* n_items) {
     if (nblocks <= 0) return;
     hipLaunchKernelGGL(with_bend ? k_tiles_bend : k_tiles, dim3(nblocks), dim3(threads), lds_bytes, s, order, tiles,
                        verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v,
-                       p_in, p_out, res_slots, ncl, sweeps_out, sweep_cap, a);
+                       p_in, p_out, res_slots, ncl, sweeps_out, sweep_cap, a, n_items);
     PBD_HIP(hipGetLastError());
+}
+
+void launch_follow_list(hipStream_t s, const int* order, const int* order_cloth, int n_order, const int* cap,
+                        int min_cap, int* out, int* n_out) {
+    hipLaunchKernelGGL(k_follow_list, dim3(1), dim3(512), 0, s, order, order_cloth, n_order, cap, min_cap, out,
+                       n_out);
+    PBD_HIP(hipGetLastError());
+}
+
+// Workgroups of the tile kernel one CU holds at once, for `threads` threads
+// and `lds_bytes` of dynamic LDS.
+int tiles_per_cu(bool with_bend, int threads, size_t lds_bytes) {
+    int n = 0;
+    PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, with_bend ? k_tiles_bend : k_tiles, threads,
+                                                         lds_bytes));
+    return std::max(1, n);
 }
 
 }  // namespace pbd
