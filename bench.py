@@ -765,9 +765,10 @@ def main():
             "bytes_per_launch": prof.tile_bytes / max(1, prof.tile_launches),
             "avg_launch_ms": avg_ms,
             "launches": prof.tile_launches,
-            "bytes_definition": "per launched tile: vertex list 4 B and state 16 B per loaded vertex (32 B when "
-                                "advecting x and v, +16 B with a kept prediction), 8 B per distance prim record of "
-                                "every colour phase run, 8 B per owned prim, 16 B per interior vertex stored (64 B "
+            "bytes_definition": "per launched tile: vertex list 4 B and state 16 B per loaded vertex of the "
+                                "cone the launch's sweeps read (32 B when advecting x and v, +16 B with a kept "
+                                "prediction), 8 B per distance prim record of every colour phase run, 8 B per "
+                                "owned prim under a tolerance (none here), 16 B per interior vertex stored (64 B "
                                 "when integrating)",
             "plain_pass": {"achieved": plain_achieved, "avg_launch_ms": pprof.tile_ms / max(1, pprof.tile_launches),
                            "bytes_per_launch": pprof.tile_bytes / max(1, pprof.tile_launches)},
