@@ -80,6 +80,7 @@ REF_SWEEPS = 4000      # sweeps of the reference substep
 TRAIN_SEED, TIME_SEED = 1, 0
 WARM_RULE, WARM_GATE = 1, 1e-4   # fallback rule (RMS) and strain gate
 JACOBI_OMEGA = 1.5
+AUTOTUNE_SUBSTEPS = 4  # substeps timed per autotuner trial (one is within the noise of close candidates)
 CPU_SAMPLE_VERTICES = 1_000_000
 C4_TRAIN_SCENES, C4_SELECT_SCENES, C4_EVAL_SCENES = 24, 32, 256   # config-4 scene counts
 C3_PROBLEMS = 3        # config-3 substeps on which the budget is chosen, and on which the timed run is checked
@@ -152,7 +153,7 @@ class Batch:
     def tune(self, kind, warm=0):
         self.solver(kind, warm=warm)
         t = time.perf_counter()
-        best, _, ms, trials = self.ctx.autotune(1)
+        best, _, ms, trials = self.ctx.autotune(AUTOTUNE_SUBSTEPS)
         self.cfg[kind] = {"tile_sweeps": best.tile_sweeps, "block_threads": best.block_threads,
                           "tile_max_verts": best.tile_max_verts, "split_whole": best.split_whole}
         return dict(self.cfg[kind], trials=trials, best_ms_per_substep=ms, seconds=time.perf_counter() - t)

@@ -340,7 +340,7 @@ void do_init(pbd_ctx* c, int color_rounds) {
     std::vector<int> color;
     int rounds = 0;
     if (np > 0) {
-        DBuf<int> d_npts, d_sn, d_adj, d_color, d_prd, d_unc, d_list[2];
+        DBuf<int> d_npts, d_sn, d_adj, d_color, d_prd, d_unc;
         d_npts.upload(npts, s);
         d_sn.upload(adj_sn, s);
         d_adj.upload(adj, s);
@@ -348,20 +348,13 @@ void do_init(pbd_ctx* c, int color_rounds) {
         d_color.upload(minus1, s);
         d_prd.upload(minus1, s);
         d_unc.alloc(1);
-        d_list[0].alloc(np);
-        d_list[1].alloc(np);
-        // each round runs over the prims the previous one left uncoloured
-        int unc = np, cur = 0;
-        const int* active = nullptr;
+        int unc = np;
         while (unc > 0) {
             PBD_CHECK(rounds < color_rounds, PBD_ERR_COLOR, "colouring did not converge in the allowed rounds");
             PBD_HIP(hipMemsetAsync(d_unc.ptr, 0, sizeof(int), s));
-            launch_color_round(s, d_npts.ptr, d_sn.ptr, d_adj.ptr, d_color.ptr, d_prd.ptr, active, unc,
-                               d_list[cur].ptr, d_unc.ptr);
+            launch_color_round(s, d_npts.ptr, d_sn.ptr, d_adj.ptr, d_color.ptr, d_prd.ptr, d_unc.ptr, np);
             PBD_HIP(hipMemcpyAsync(&unc, d_unc.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
             PBD_HIP(hipStreamSynchronize(s));
-            active = d_list[cur].ptr;
-            cur ^= 1;
             rounds++;
         }
         // ---- device stable colour sort (SortEdgesColors, PBD_cuda.cu:546-587)

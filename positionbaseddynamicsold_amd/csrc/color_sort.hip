@@ -12,19 +12,12 @@
 
 namespace pbd {
 
-// Both kernels of a round run over the prims still uncoloured: the first
-// round over 0 .. np-1 (active == nullptr), later rounds over the list the
-// previous round's resolve appended (its order is that of the atomics, which
-// changes nothing: each prim's outcome depends only on the arrays the
-// previous pass wrote).
 __global__ __launch_bounds__(256) void k_color_assign(const int2* __restrict__ adj_sn,
                                                       const int* __restrict__ adj,
                                                       const int* __restrict__ color,
-                                                      int* __restrict__ prd, const int* __restrict__ active,
-                                                      int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int p = active ? active[i] : i;
+                                                      int* __restrict__ prd, int np) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
     if (color[p] >= 0) return;
     int2 sn = adj_sn[p];
     int c = -1, offset = 0;
@@ -45,26 +38,26 @@ __global__ __launch_bounds__(256) void k_color_assign(const int2* __restrict__ a
 }
 
 // Keeps the proposal of every prim that has no conflicting neighbour of
-// higher priority (more points, then lower id) and appends the others to
-// `next`; *n_next counts them (one atomic per workgroup).
+// higher priority (more points, then lower id); the prims left uncoloured
+// are counted into *uncolored with one atomic per workgroup (one per wave
+// had the atomics on that one address take most of the round).
 __global__ __launch_bounds__(256) void k_color_resolve(const int* __restrict__ npts,
                                                        const int2* __restrict__ adj_sn,
                                                        const int* __restrict__ adj,
                                                        int* __restrict__ color,
-                                                       const int* __restrict__ prd, const int* __restrict__ active,
-                                                       int n, int* __restrict__ next, int* __restrict__ n_next) {
-    __shared__ int wave_left[4], block_base;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int p = i < n ? (active ? active[i] : i) : -1;
+                                                       const int* __restrict__ prd, int np,
+                                                       int* __restrict__ uncolored) {
+    __shared__ int wave_left[4];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int left = 0;
-    if (p >= 0 && color[p] < 0) {
+    if (p < np && color[p] < 0) {
         int2 sn = adj_sn[p];
         int c = prd[p], conflict = 0, npt = npts[p];
-        for (int k = sn.x; k < sn.x + sn.y; ++k) {
-            int q = adj[k];
-            if (prd[q] == c) {
-                int nnpt = npts[q];
-                if (nnpt > npt || (nnpt == npt && q < p)) {
+        for (int i = sn.x; i < sn.x + sn.y; ++i) {
+            int n = adj[i];
+            if (prd[n] == c) {
+                int nnpt = npts[n];
+                if (nnpt > npt || (nnpt == npt && n < p)) {
                     conflict = 1;
                     break;
                 }
@@ -73,29 +66,22 @@ __global__ __launch_bounds__(256) void k_color_resolve(const int* __restrict__ n
         if (conflict) left = 1;
         else color[p] = c;
     }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const unsigned long long m = __ballot(left);
-    if (lane == 0) wave_left[wv] = (int)__popcll(m);
+    if ((threadIdx.x & 63) == 0) wave_left[threadIdx.x >> 6] = (int)__popcll(m);
     __syncthreads();
     if (threadIdx.x == 0) {
         const int tot = wave_left[0] + wave_left[1] + wave_left[2] + wave_left[3];
-        block_base = tot ? atomicAdd(n_next, tot) : 0;
-    }
-    __syncthreads();
-    if (left) {
-        int off = block_base;
-        for (int w = 0; w < wv; w++) off += wave_left[w];
-        next[off + (int)__popcll(m & ((1ull << lane) - 1ull))] = p;
+        if (tot) atomicAdd(uncolored, tot);
     }
 }
 
 int launch_color_round(hipStream_t s, const int* npts, const int* adj_sn, const int* adj, int* color, int* prd,
-                       const int* active, int n, int* next, int* n_next) {
-    int nb = (n + 255) / 256;
+                       int* uncolored, int np) {
+    int nb = (np + 255) / 256;
     if (nb == 0) return 0;
-    hipLaunchKernelGGL(k_color_assign, dim3(nb), dim3(256), 0, s, (const int2*)adj_sn, adj, color, prd, active, n);
+    hipLaunchKernelGGL(k_color_assign, dim3(nb), dim3(256), 0, s, (const int2*)adj_sn, adj, color, prd, np);
     hipLaunchKernelGGL(k_color_resolve, dim3(nb), dim3(256), 0, s, npts, (const int2*)adj_sn, adj, color, prd,
-                       active, n, next, n_next);
+                       np, uncolored);
     PBD_HIP(hipGetLastError());
     return 0;
 }

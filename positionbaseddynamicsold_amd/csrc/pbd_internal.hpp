@@ -331,10 +331,10 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps_per_launch,
 // ---------------------------------------------------------------------------
 // Kernel launchers (implemented in the .hip files)
 // ---------------------------------------------------------------------------
-// One colouring round over the n prims of `active` (nullptr: prims 0 .. n-1);
-// the prims left uncoloured are appended to `next`, their count to *n_next.
+// One colouring round over all np prims; the prims left uncoloured are
+// counted into *uncolored.
 int launch_color_round(hipStream_t s, const int* prim_npts, const int* adj_sn, const int* adj, int* color,
-                       int* prd, const int* active, int n, int* next, int* n_next);
+                       int* prd, int* uncolored, int np);
 // Ints of `hist` that radix_sort_pairs needs for n keys: the per-block digit
 // counts (256 x ceil(n / 4096)), their exclusive scan, and the block sums of
 // that scan.
