@@ -5,7 +5,10 @@ reduced size through the same code paths, with the reference's contact
 handling and with the robust one (swept-box candidates, scale-free
 degenerate-face bound), plus size-independent properties at full size
 (finite state, contacts, run-to-run determinism, and with the robust
-handling no vertex inside the static mesh).
+handling no vertex inside the static mesh over the first substeps of
+contact), and longer robust runs through the contact phase, in which a few
+vertices do end inside (bounded here: at most 0.5% of the cloth, under
+1 cm deep).
 
 At full size the reference's handling lets the cloth through both static
 meshes: its centroid-cell candidates miss faces that extend more than about
@@ -14,6 +17,8 @@ against 19.7 mm icosphere faces), and its response skips every face with
 |(p2-p1) x (p3-p1)|^2 <= 1e-6 m^4 (CCD.cpp:2008), i.e. every face of
 these meshes.  The full-size tests of that mode check only what it
 guarantees."""
+import time
+
 import numpy as np
 import pytest
 
@@ -177,6 +182,57 @@ def test_config3_full_size_properties(robust):
         assert not inside_box(x).any()
     x2, v2, _ = full_run(sc, 24)
     assert np.array_equal(x, x2) and np.array_equal(v, v2)
+
+
+def depth_sphere(x):
+    # depth below the icosphere's analytic radius (its faces sit inside it by
+    # at most the chord sag of a subdivision-4 face, 0.22 mm)
+    return 0.3 - np.linalg.norm(x.astype(np.float64), axis=1)
+
+
+def depth_box(x):
+    x = x.astype(np.float64)
+    return np.minimum(np.minimum(0.2 - np.abs(x[:, 0]), 0.2 - np.abs(x[:, 2])), np.minimum(0.4 - x[:, 1], x[:, 1]))
+
+
+def horizon_run(sc, substeps, depth, every):
+    """Steps the scene `substeps` substeps, checking every `every`: finite
+    state, speeds within free fall from rest plus 10 m/s, and the vertices
+    more than 0.22 mm inside the static mesh: at most 0.5% of the cloth, at
+    most 1 cm deep.  Returns the contact counts at the checkpoints."""
+    contacts = []
+    with capi.Context() as ctx:
+        cid, _, _ = scenes.add_ccd_scene(ctx, sc)
+        t0 = time.perf_counter()
+        for k in range(every, substeps + 1, every):
+            ctx.step(every)
+            contacts.append(ctx.last_contacts()[0])
+            x, v = ctx.get(cid), ctx.get(cid, capi.STATE_V)
+            speed = float(np.linalg.norm(v, axis=1).max())
+            d = depth(x)
+            inside = int((d > 2.2e-4).sum())
+            say(f"  substep {k}: contacts {contacts[-1]}, max speed {speed:.3f} m/s, inside {inside}, "
+                f"deepest {1e3 * d.max():.2f} mm ({time.perf_counter() - t0:.0f} s)")
+            assert np.isfinite(x).all() and np.isfinite(v).all()
+            assert speed < 9.8 * k * scenes.DT + 10.0
+            assert inside <= 0.005 * len(x) and d.max() < 0.01
+    return contacts
+
+
+def test_config2_long_run_robust():
+    # 1,000 substeps (100 frames) of the 256 x 256 cloth with bending on the
+    # icosphere: through the contact phase until the cloth slides off
+    say("config2 full size, 1000 substeps, robust contacts")
+    contacts = horizon_run(scenes.config2().robust(), 1000, depth_sphere, 250)
+    assert max(contacts) > 0
+
+
+def test_config3_long_run_robust():
+    # 500 substeps (50 frames) of the folded 512 x 512 cloth over the box:
+    # the contact phase, in which a few vertices end inside the box
+    say("config3 full size, 500 substeps, robust contacts")
+    contacts = horizon_run(scenes.config3().robust(), 500, depth_box, 125)
+    assert max(contacts) > 0
 
 
 # The reference's own check of the response: CCD_SH_Narrow (CCD.cpp:1588-1652)
