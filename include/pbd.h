@@ -104,6 +104,11 @@ typedef struct {
                                multi-tile cloth (its one tile loads the whole cloth, no halo) instead of all its
                                sweeps in one launch, so that no launch waits on one cloth's whole budget
                                (0 = one launch; results are identical either way) */
+    int per_chunk_launches; /* under per-cloth budgets (the warm start) with fixed sweeps: 0 = the substep's
+                               tile work is one launch whose workgroups take (chunk, tile) items from a queue
+                               built on the device, a tile's later chunk waiting only for its neighbour tiles'
+                               previous chunk; 1 = one launch per chunk of tile_sweeps sweeps (results are
+                               identical either way) */
 } pbd_solver_params;
 
 typedef struct {

@@ -45,7 +45,7 @@ class SolverParams(C.Structure):
                 ("path", C.c_int), ("tile_sweeps", C.c_int), ("tile_max_verts", C.c_int),
                 ("block_threads", C.c_int), ("graph", C.c_int), ("warm_iterations", C.c_int),
                 ("warm_rule", C.c_int), ("warm_gate", C.c_float), ("jacobi_omega", C.c_float),
-                ("split_whole", C.c_int)]
+                ("split_whole", C.c_int), ("per_chunk_launches", C.c_int)]
 
 
 class HashDesc(C.Structure):
@@ -341,9 +341,9 @@ class Context:
     # ---- solver --------------------------------------------------------
     def set_solver(self, dt=1.0 / 600.0, iterations=20, tol=-1.0, check_every=1, path=PATH_AUTO,
                    tile_sweeps=0, tile_max_verts=0, block_threads=0, graph=0, warm_iterations=0, warm_rule=0,
-                   warm_gate=0.0, jacobi_omega=0.0, split_whole=0):
+                   warm_gate=0.0, jacobi_omega=0.0, split_whole=0, per_chunk_launches=0):
         p = SolverParams(dt, iterations, tol, check_every, path, tile_sweeps, tile_max_verts, block_threads, graph,
-                         warm_iterations, warm_rule, warm_gate, jacobi_omega, split_whole)
+                         warm_iterations, warm_rule, warm_gate, jacobi_omega, split_whole, per_chunk_launches)
         check(self.lib.pbd_set_solver_params(self.h, C.byref(p)))
 
     def set_solver_params(self, p: SolverParams):

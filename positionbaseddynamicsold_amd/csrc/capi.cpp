@@ -55,6 +55,9 @@ struct pbd_ctx {
     // later chunks under per-cloth budgets: the multi-tile tiles still
     // needed, compacted on the device, and the list's length
     DBuf<int> order_follow, follow_len, order_partial_cloth;
+    // the one-launch form of the same (TileQueue): tile neighbours, per-tile
+    // done chunks, the queue's items and scratch, and the wait timeouts
+    DBuf<int> tnbr_off, tnbr, tdone, q_tile, q_chunk, q_tmp, q_count, q_timeouts;
     int num_cu = 0;
     // workgroups the tile kernel keeps resident on the whole GPU, for the
     // (kernel, threads, LDS) it was last asked for
@@ -256,6 +259,15 @@ void upload_plan(pbd_ctx* c) {
     c->order_partial_cloth.upload(pcl, c->stream);
     c->order_follow.alloc(std::max<size_t>(1, p.partial_order.size()));
     c->follow_len.alloc(1);
+    c->tnbr_off.upload(p.nbr_off, c->stream);
+    c->tnbr.upload(p.nbr.empty() ? std::vector<int>(1, 0) : p.nbr, c->stream);
+    c->tdone.alloc(std::max<size_t>(1, p.tiles.size()));
+    c->q_tmp.alloc(2 * std::max<size_t>(1, first.size()));
+    c->q_count.alloc(2);  // item count, head
+    if (!c->q_timeouts.n) {
+        c->q_timeouts.alloc(1);
+        PBD_HIP(hipMemsetAsync(c->q_timeouts.ptr, 0, sizeof(int), c->stream));
+    }
     std::vector<int> ids(p.tiles.size());
     for (size_t t = 0; t < ids.size(); t++) ids[t] = (int)t;
     c->tile_ids.upload(ids, c->stream);
@@ -263,6 +275,20 @@ void upload_plan(pbd_ctx* c) {
     c->any_whole = false;
     c->any_partial = false;
     for (auto& t : p.tiles) (t.whole ? c->any_whole : c->any_partial) = true;
+}
+
+// Items the substep's tile queue can hold: every tile of the first chunk and
+// every multi-tile tile in each later chunk.  Allocated outside graph
+// capture (allocation is not allowed inside it).
+void ensure_queue(pbd_ctx* c) {
+    const int ts = std::max(1, effective_tile_sweeps(c));
+    const size_t nchunks = (size_t)(std::max(1, c->sp.iterations) + ts - 1) / ts;
+    const size_t need = std::max<size_t>(1, c->order_first.n + nchunks * c->order_partial.n);
+    if (c->q_tile.n < need) {
+        drop_graphs(c);
+        c->q_tile.alloc(need);
+        c->q_chunk.alloc(need);
+    }
 }
 
 // Grid of the list-walking tile launches: every resident workgroup slot of
@@ -970,7 +996,7 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     // find their cloth done.
     const bool follow = cap && !any_tol && c->any_partial && nchunks > 1;
     int follow_grid = 0;
-    if (follow) {
+    if (follow && c->sp.per_chunk_launches) {
         launch_follow_list(s, c->order_partial.ptr, c->order_partial_cloth.ptr, (int)c->order_partial.n, cap, ts,
                            c->order_follow.ptr, c->follow_len.ptr);
         follow_grid = std::min(walk_grid(c), (int)c->order_partial.n);
@@ -999,7 +1025,42 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
         a.chunk = 0;
         tiles((int)c->order_whole.n, c->order_whole.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
     }
-    if (c->any_partial) {
+    if (follow && !c->sp.per_chunk_launches) {
+        // every chunk in one launch: workgroups walk a queue of (tile, chunk)
+        // items built on the device after the keep decision
+        ensure_queue(c);
+        PBD_CHECK(c->q_tile.n >= c->order_first.n + (size_t)nchunks * c->order_partial.n, PBD_ERR_STATE,
+                  "internal: tile queue too small (allocated before graph capture)");
+        const DBuf<int>& ord = c->any_whole ? c->order_first : c->order_partial;
+        launch_queue_schedule(s, ord.ptr, (int)ord.n, c->tiles.ptr, (int)c->tiles.n, cap, ts, nchunks, c->q_tmp.ptr,
+                              c->q_tmp.ptr + c->q_tmp.n / 2, c->q_tile.ptr, c->q_chunk.ptr, c->q_count.ptr,
+                              c->q_count.ptr + 1, c->tdone.ptr);
+        TileQueue q{};
+        q.item_tile = c->q_tile.ptr;
+        q.item_chunk = c->q_chunk.ptr;
+        q.n_items = c->q_count.ptr;
+        q.head = c->q_count.ptr + 1;
+        q.done = c->tdone.ptr;
+        q.nbr_off = c->tnbr_off.ptr;
+        q.nbr = c->tnbr.ptr;
+        q.pbuf0 = c->pbuf[0].ptr;
+        q.pbuf1 = c->pbuf[1].ptr;
+        q.start = cur;
+        q.ts = ts;
+        q.iterations = iterations;
+        q.timeouts = c->q_timeouts.ptr;
+        a.sweep_base = 0;
+        a.nsweeps = std::min(ts, iterations);
+        a.chunk = 0;
+        if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
+        launch_tiles_queue(s, walk_grid(c), c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr,
+                           c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr,
+                           c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->res_slots.ptr, ncl, c->sweeps.ptr,
+                           cap, a, q, threads, lds, bend);
+        if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
+        launched = nchunks;
+        if (nchunks & 1) cur ^= 1;
+    } else if (c->any_partial) {
         int sw = 0;
         for (int ch = 0; ch < nchunks; ch++) {
             launched++;
@@ -1237,6 +1298,7 @@ void graph_substep(pbd_ctx* c, float dt) {
         if (c->sp.path != PBD_PATH_JACOBI && c->sp.path != PBD_PATH_GLOBAL) {
             replan_if_needed(c);
             (void)walk_grid(c);  // an occupancy query, made outside the capture
+            ensure_queue(c);
         }
         hipGraph_t g = nullptr;
         PBD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
@@ -1668,6 +1730,8 @@ int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* p) {
                   "unknown path");
         PBD_CHECK(p->jacobi_omega >= 0.0f && p->jacobi_omega < 2.0f, PBD_ERR_ARG, "jacobi_omega must be in [0, 2)");
         PBD_CHECK(p->split_whole == 0 || p->split_whole == 1, PBD_ERR_ARG, "split_whole must be 0 or 1");
+        PBD_CHECK(p->per_chunk_launches == 0 || p->per_chunk_launches == 1, PBD_ERR_ARG,
+                  "per_chunk_launches must be 0 or 1");
         PBD_CHECK(p->tile_sweeps >= 0 && p->tile_max_verts >= 0, PBD_ERR_ARG, "negative tile setting");
         PBD_CHECK(p->block_threads == 0 || (p->block_threads % 64 == 0 && p->block_threads <= 1024), PBD_ERR_ARG,
                   "block_threads must be a multiple of 64 up to 1024");

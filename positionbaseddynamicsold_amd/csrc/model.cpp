@@ -702,6 +702,41 @@ void plan_tiles(std::vector<Cloth>& cloths, int max_load, int sweeps, bool split
         }
         c.tile_end = (int)plan.tiles.size();
     }
+    // Tile neighbours: t and n are neighbours when one loads a vertex of the
+    // other's interior (a later chunk of t reads what n wrote, or writes what
+    // n read).  Only tiles of the same cloth share vertices.
+    {
+        const size_t nt = plan.tiles.size();
+        std::vector<std::vector<int>> nb(nt);
+        int nv_all = 0;
+        for (const Cloth& c : cloths) nv_all = std::max(nv_all, c.vbase + c.nv);
+        std::vector<int> owner((size_t)nv_all, -1);
+        for (size_t t = 0; t < nt; t++) {
+            const TileDev& td = plan.tiles[t];
+            if (td.whole) continue;
+            for (int k = 0; k < td.n_int; k++) owner[plan.verts[(size_t)td.load_off + k]] = (int)t;
+        }
+        for (size_t t = 0; t < nt; t++) {
+            const TileDev& td = plan.tiles[t];
+            if (td.whole) continue;
+            nb[t].push_back((int)t);
+            for (int k = td.n_int; k < td.n_load; k++) {
+                const int n = owner[plan.verts[(size_t)td.load_off + k]];
+                if (n >= 0 && n != (int)t) {
+                    nb[t].push_back(n);
+                    nb[n].push_back((int)t);
+                }
+            }
+        }
+        plan.nbr_off.assign(nt + 1, 0);
+        plan.nbr.clear();
+        for (size_t t = 0; t < nt; t++) {
+            std::sort(nb[t].begin(), nb[t].end());
+            nb[t].erase(std::unique(nb[t].begin(), nb[t].end()), nb[t].end());
+            plan.nbr.insert(plan.nbr.end(), nb[t].begin(), nb[t].end());
+            plan.nbr_off[t + 1] = (int)plan.nbr.size();
+        }
+    }
     // XCD-aware launch order: consecutive (spatially adjacent) tiles go to the
     // same XCD.  Workgroups are dealt to the 8 XCDs round-robin by block id, so
     // the j-th tile of group g gets block id j*8 + g.

@@ -320,6 +320,10 @@ struct TilePlan {
     std::vector<int> cols;                // per tile colour offsets
     std::vector<int> partial_order;       // launch order of the partial tiles only
     std::vector<int> whole_order;         // launch order of the whole tiles only
+    // neighbours of each partial tile (CSR over tiles): the tiles of its
+    // cloth whose interior it loads or which load its interior, itself
+    // included -- the tiles a later chunk of it must wait for
+    std::vector<int> nbr_off, nbr;
     // plan statistics: loaded and interior vertices, loaded distance prims,
     // and distance prim updates of one launch of `sweeps` sweeps
     long long stat_loaded = 0, stat_interior = 0, stat_dist_loaded = 0, stat_dist_phase_needed = 0;
@@ -384,6 +388,39 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
                   const float4* p_in, float4* p_out, unsigned long long* res_slots,
                   int nslots_cloth, int* sweeps_out, const int* sweep_cap, const SolveArgs& a,
                   int threads, size_t lds_bytes, bool with_bend, const int* n_items = nullptr);
+// One launch for every chunk of a substep under per-cloth budgets: the
+// workgroups take (tile, chunk) items from a queue in which every item comes
+// after the items it depends on (its neighbour tiles' previous chunk).
+struct TileQueue {
+    const int* item_tile;   // items: tile slot and chunk
+    const int* item_chunk;
+    const int* n_items;     // device count
+    int* head;              // next item to take (0 at the start)
+    int* done;              // per tile: last chunk finished (-1 at the start)
+    const int* nbr_off;     // tile neighbours (CSR), the tile itself included
+    const int* nbr;
+    float4* pbuf0;          // chunk k reads pbuf[start ^ (k & 1)], writes the other
+    float4* pbuf1;
+    int start;
+    int ts;                 // sweeps per chunk
+    int iterations;
+    int* timeouts;          // waits that gave up (bounded spins; 0 when every wait ended normally)
+};
+void launch_tiles_queue(hipStream_t s, int nblocks, const TileDev* tiles, const int* verts, const uint32_t* dist,
+                        const float* dist_rest, const uint32_t* bend, const float* bend_rest, const uint32_t* own,
+                        const float* own_rest, const int* cols, const ClothDev* cloths, const float* ktab_d,
+                        const float* ktab_b, float4* x, float4* v, unsigned long long* res_slots, int ncl,
+                        int* sweeps_out, const int* sweep_cap, const SolveArgs& a, const TileQueue& q, int threads,
+                        size_t lds_bytes, bool with_bend);
+// Builds the queue of a substep on the device (one workgroup): resets head
+// and every tile's done, splits the first-chunk order into the tiles of
+// cloths whose budget outlasts the first chunk (the chain) and the others,
+// and lays the items out as [chain, chunk 0][others, part 0][chain, chunk 1]
+// [others, part 1] ... so that the chain's chunks run beside the others'
+// work.  chain_tmp / bulk_tmp: scratch of n_order ints each.
+void launch_queue_schedule(hipStream_t s, const int* order, int n_order, const TileDev* tiles, int n_tiles,
+                           const int* cap, int ts, int nchunks, int* chain_tmp, int* bulk_tmp, int* item_tile,
+                           int* item_chunk, int* n_items, int* head, int* done);
 // Compacts the multi-tile order (layout order[j * 8 + g]; order_cloth: each
 // entry's cloth, -1 for padding) to the tiles whose cloth's budget cap
 // exceeds min_cap, rebalanced over the 8 groups, length into *n_out.

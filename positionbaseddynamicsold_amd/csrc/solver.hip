@@ -767,6 +767,149 @@ __global__ __launch_bounds__(512) void k_follow_list(const int* __restrict__ ord
     if (threadIdx.x == 0) *n_out = 8 * per;
 }
 
+// Queue walk (see TileQueue).  A workgroup takes the next item; for a later
+// chunk it first waits until every neighbour tile has finished the previous
+// chunk (it read what they wrote, and writes what they read).  Deadlock-free
+// whatever the number of resident workgroups: an item waits only for items
+// taken before it, and a taken item is held by a running workgroup, so the
+// oldest unfinished item can always proceed.  A tile whose cloth has a
+// further chunk publishes its chunk after its stores (release at agent
+// scope, so that workgroups on other XCDs see them).  A wait gives up after
+// QUEUE_SPIN polls and counts a timeout instead of hanging.
+constexpr int QUEUE_SPIN = 1 << 22;
+
+template <bool BEND>
+__device__ __forceinline__ void tiles_queue(TILE_PARAMS, const TileQueue& q, float4* P, double* red) {
+    __shared__ int s_item;
+    const int n = *q.n_items;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(q.head, 1);
+        __syncthreads();
+        const int idx = s_item;
+        if (idx >= n) break;
+        const int slot = q.item_tile[idx], c = q.item_chunk[idx];
+        const TileDev& td = tiles[slot];
+        // a later chunk of a cloth whose budget is already spent has nothing to
+        // do (its neighbours publish no such chunk): no wait, no work
+        if (c > 0 && sweep_cap && sweep_cap[td.cloth] <= c * q.ts) {
+            __syncthreads();  // s_item read by all before the next take
+            continue;
+        }
+        if (c > 0) {
+            // poll relaxed (no cache invalidation per poll), then one acquire
+            const int e = q.nbr_off[slot + 1];
+            for (int k = q.nbr_off[slot] + (int)threadIdx.x; k < e; k += blockDim.x) {
+                int spins = 0;
+                while (__hip_atomic_load(&q.done[q.nbr[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < c - 1) {
+                    if (++spins > QUEUE_SPIN) {
+                        atomicAdd(q.timeouts, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // invalidates the CU's caches
+            __syncthreads();
+        }
+        SolveArgs b = a;
+        b.sweep_base = c * q.ts;
+        b.nsweeps = min(q.ts, q.iterations - b.sweep_base);
+        b.chunk = c;
+        if (c > 0) {
+            b.delta = nullptr;
+            b.keep = nullptr;
+            b.advect_load = 0;
+        }
+        const bool odd = ((q.start ^ c) & 1) != 0;
+        tiles_body<BEND>(q.item_tile, tiles, verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths,
+                         ktab_d, ktab_b, x, v, odd ? q.pbuf1 : q.pbuf0, odd ? q.pbuf0 : q.pbuf1, res_slots, ncl,
+                         sweeps_out, sweep_cap, b, idx, P, red);
+        const bool more = !td.whole && sweep_cap && sweep_cap[td.cloth] > (c + 1) * q.ts;
+        // every thread's stores complete (s_waitcnt 0), then one device-wide
+        // release for all of them (the L2 write-back is cache-wide)
+        if (more) __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();  // also: s_item read by all before the next take
+        if (more && threadIdx.x == 0) __hip_atomic_store(&q.done[slot], c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(1024, 8) void k_tiles_queue(TILE_PARAMS, TileQueue q) {
+    extern __shared__ float4 P[];
+    __shared__ double red[16];
+    tiles_queue<false>(TILE_ARGS, q, P, red);
+}
+
+__global__ __launch_bounds__(1024) void k_tiles_queue_bend(TILE_PARAMS, TileQueue q) {
+    extern __shared__ float4 P[];
+    __shared__ double red[16];
+    tiles_queue<true>(TILE_ARGS, q, P, red);
+}
+
+// Block-wide exclusive prefix of a 0/1 flag over up to 1024 threads (16
+// waves); returns this thread's offset and writes the total to *total.
+__device__ __forceinline__ int block_flag_scan(bool f, int* wave_sum, int* total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    const unsigned long long m = __ballot(f);
+    if (lane == 0) wave_sum[wv] = (int)__popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < nw; w++) {
+        off += w < wv ? wave_sum[w] : 0;
+        tot += wave_sum[w];
+    }
+    __syncthreads();  // wave_sum read by all before its next use
+    *total = tot;
+    return off + (int)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+__global__ __launch_bounds__(1024) void k_queue_schedule(const int* __restrict__ order, int n_order,
+                                                         const TileDev* __restrict__ tiles, int n_tiles,
+                                                         const int* __restrict__ cap, int ts, int nchunks,
+                                                         int* __restrict__ chain_tmp, int* __restrict__ bulk_tmp,
+                                                         int* __restrict__ item_tile, int* __restrict__ item_chunk,
+                                                         int* __restrict__ n_items, int* __restrict__ head,
+                                                         int* __restrict__ done) {
+    __shared__ int wave_sum[16];
+    const int T = blockDim.x;
+    for (int t = threadIdx.x; t < n_tiles; t += T) done[t] = -1;
+    if (threadIdx.x == 0) *head = 0;
+    int nc = 0, nb = 0;
+    for (int j0 = 0; j0 < n_order; j0 += T) {
+        const int j = j0 + (int)threadIdx.x;
+        const int slot = j < n_order ? order[j] : -1;
+        bool chain = false, bulk = false;
+        if (slot >= 0) {
+            const TileDev td = tiles[slot];
+            chain = !td.whole && cap[td.cloth] > ts;
+            bulk = !chain;
+        }
+        int tc, tb;
+        const int oc = block_flag_scan(chain, wave_sum, &tc);
+        const int ob = block_flag_scan(bulk, wave_sum, &tb);
+        if (chain) chain_tmp[nc + oc] = slot;
+        if (bulk) bulk_tmp[nb + ob] = slot;
+        nc += tc;
+        nb += tb;
+    }
+    __syncthreads();  // the lists written above, read below by other threads
+    int base = 0;
+    for (int k = 0; k < nchunks; k++) {
+        for (int i = threadIdx.x; i < nc; i += T) {
+            item_tile[base + i] = chain_tmp[i];
+            item_chunk[base + i] = k;
+        }
+        base += nc;
+        const int lo = (int)((long long)k * nb / nchunks), hi = (int)((long long)(k + 1) * nb / nchunks);
+        for (int i = lo + (int)threadIdx.x; i < hi; i += T) {
+            item_tile[base + i - lo] = bulk_tmp[i];
+            item_chunk[base + i - lo] = 0;
+        }
+        base += hi - lo;
+    }
+    if (threadIdx.x == 0) *n_items = base;
+}
+
 #undef TILE_PARAMS
 #undef TILE_ARGS
 
@@ -777,7 +920,8 @@ __global__ __launch_bounds__(512) void k_follow_list(const int* __restrict__ ord
 // static LDS), so contexts never lower each other's limit.
 size_t tiles_lds_optin(size_t bytes) {
     if (bytes <= 65536) return bytes;
-    for (const void* k : {(const void*)k_tiles, (const void*)k_tiles_bend}) {
+    for (const void* k : {(const void*)k_tiles, (const void*)k_tiles_bend, (const void*)k_tiles_queue,
+                          (const void*)k_tiles_queue_bend}) {
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();  // the refusal is handled by the smaller limit
@@ -797,6 +941,30 @@ void launch_tiles(hipStream_t s, int nblocks, const int* order, const TileDev* t
     hipLaunchKernelGGL(with_bend ? k_tiles_bend : k_tiles, dim3(nblocks), dim3(threads), lds_bytes, s, order, tiles,
                        verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v,
                        p_in, p_out, res_slots, ncl, sweeps_out, sweep_cap, a, n_items);
+    PBD_HIP(hipGetLastError());
+}
+
+void launch_tiles_queue(hipStream_t s, int nblocks, const TileDev* tiles, const int* verts, const uint32_t* dist,
+                        const float* dist_rest, const uint32_t* bend, const float* bend_rest, const uint32_t* own,
+                        const float* own_rest, const int* cols, const ClothDev* cloths, const float* ktab_d,
+                        const float* ktab_b, float4* x, float4* v, unsigned long long* res_slots, int ncl,
+                        int* sweeps_out, const int* sweep_cap, const SolveArgs& a, const TileQueue& q, int threads,
+                        size_t lds_bytes, bool with_bend) {
+    if (nblocks <= 0) return;
+    // order, p_in and p_out of the common parameter list are unused: the
+    // queue carries the items and the two buffers
+    hipLaunchKernelGGL(with_bend ? k_tiles_queue_bend : k_tiles_queue, dim3(nblocks), dim3(threads), lds_bytes, s,
+                       q.item_tile, tiles, verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths,
+                       ktab_d, ktab_b, x, v, (const float4*)q.pbuf0, q.pbuf1, res_slots, ncl, sweeps_out, sweep_cap,
+                       a, q);
+    PBD_HIP(hipGetLastError());
+}
+
+void launch_queue_schedule(hipStream_t s, const int* order, int n_order, const TileDev* tiles, int n_tiles,
+                           const int* cap, int ts, int nchunks, int* chain_tmp, int* bulk_tmp, int* item_tile,
+                           int* item_chunk, int* n_items, int* head, int* done) {
+    hipLaunchKernelGGL(k_queue_schedule, dim3(1), dim3(1024), 0, s, order, n_order, tiles, n_tiles, cap, ts, nchunks,
+                       chain_tmp, bulk_tmp, item_tile, item_chunk, n_items, head, done);
     PBD_HIP(hipGetLastError());
 }
 

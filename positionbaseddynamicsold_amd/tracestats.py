@@ -107,7 +107,8 @@ def main():
         w = table(ev[wa:wb], f"profiled warm-started pass ({a.steps} substeps)")
         p = table(ev[pa:pb], f"profiled plain pass ({a.steps} substeps)")
         for label, per in (("warm", w), ("plain", p)):
-            t = per.get("k_tiles", [])
+            # the tile kernel in any of its forms (per-chunk launches, the queued launch, with bending)
+            t = [d for n, v in per.items() if n.startswith("k_tiles") for d in v]
             if t:
                 print(f"k_tiles {label}: {len(t)} launches, average {statistics.fmean(t) / 1e3:.4f} ms")
     if a.fetch and a.write:
@@ -118,7 +119,7 @@ def main():
                 (wa2, wb2), (pa2, pb2), c0 = passes(seq, a.steps, grid_of=lambda e: e[2])
                 cal = statistics.fmean(v for n, v, g in seq[c0:] if n == "k_copy")
                 a2, b2 = (wa2, wb2) if rng == 0 else (pa2, pb2)
-                tiles = [v for n, v, _ in seq[a2:b2] if n == "k_tiles"]
+                tiles = [v for n, v, _ in seq[a2:b2] if n.startswith("k_tiles")]
                 vals.append((statistics.fmean(tiles), cal))
             (f, fc), (wr, wc) = vals
             scale_f = COPY_BYTES / (fc * 1024.0)
