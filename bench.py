@@ -44,12 +44,13 @@ Protocol (fixed constants, independent of --steps and --warmup):
 `value` is the warm-started batched solver's substeps/s on the timed batch;
 config.speedup_vs_plain_batched divides it by the plain batched solver's.
 
-`roofline` is for the LDS-tile Gauss-Seidel kernel (k_tiles) in a second,
-eager pass over the warm-started substeps with HIP events around every
-launch on the library's own stream: the bytes the launches must move as the
-kernel is designed (tile loads with their dependency-cone halo, the prim
-records of every colour phase, owned prims, stores; pbd_profile.tile_bytes)
-over the summed launch time.  `cpu_baseline` times the CPU oracle on a
+`roofline` is for the LDS-tile Gauss-Seidel kernel in a second, eager pass
+over the warm-started substeps with HIP events around every launch on the
+library's own stream (warm start: k_tiles_queue, one launch per substep
+walking a queue of (tile, chunk) items; plain: k_tiles, one launch per chunk
+of sweeps): the bytes the launches must move as the kernel is designed (tile
+loads with their dependency-cone halo, the prim records of every colour
+phase, stores; pbd_profile.tile_bytes) over the summed launch time.  `cpu_baseline` times the CPU oracle on a
 sample of >= 1 M vertices spread over the batch's size range: one thread
 (the reference's CPU path is single-threaded) and, labelled separately, all
 host cores colour-parallel; both scaled to the whole batch by vertex-sweeps.
@@ -762,7 +763,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_tiles",
+            "kernel": "k_tiles_queue (the warm-started substep's tile work; the plain pass: k_tiles)",
             "bytes_per_launch": prof.tile_bytes / max(1, prof.tile_launches),
             "avg_launch_ms": avg_ms,
             "launches": prof.tile_launches,

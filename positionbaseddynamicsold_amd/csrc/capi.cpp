@@ -703,20 +703,29 @@ void ws_require_params(pbd_ctx* c) {
 // ws.keep to the tile kernel, which adds it while loading.  With advect, the
 // inference kernel also advects (writes p from x and v); otherwise p holds
 // the advected positions already.
-void ws_apply_prediction(pbd_ctx* c, bool apply, bool advect, float dt) {
+// The prediction and its residuals.  defer: the keep decision is left to the
+// projection (dec, filled here), which folds it into its first kernel when
+// it can; otherwise it is made here (and applied to p if `apply`).
+void ws_apply_prediction(pbd_ctx* c, bool apply, bool advect, float dt, QueueDecide* dec = nullptr) {
     hipStream_t s = c->stream;
     int ncl = (int)c->cloths.size();
     ws_build_nbr(c);
     float4* p = c->pbuf[c->pcur].ptr;
+    // the fused inference clears the residual sums itself
+    const bool fused = advect;
     ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, advect ? c->v.ptr : nullptr, dt, c->ws.nbr.ptr,
              c->ws.nbr_rest.ptr, c->nv_total, c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr,
-             c->sp.warm_gate);
-    launch_fill_u64(s, c->ws.rs.ptr, 4 * (size_t)ncl, 0ull);
+             c->sp.warm_gate, fused ? c->ws.rs.ptr : nullptr, fused ? 4 * ncl : 0);
+    if (!fused) launch_fill_u64(s, c->ws.rs.ptr, 4 * (size_t)ncl, 0ull);
     ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.rs.ptr, ncl, c->sp.warm_rule);
     int full = c->sp.iterations;
     int warm = c->sp.warm_iterations > 0 ? std::min(c->sp.warm_iterations, full) : full;
-    ws_decide(s, c->ws.rs.ptr, ncl, c->sp.warm_rule, warm, full, c->ws.cap.ptr, c->ws.keep.ptr);
-    if (apply) ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.keep.ptr, c->nv_total);
+    if (dec && !apply) {
+        *dec = QueueDecide{c->ws.rs.ptr, c->sp.warm_rule, warm, full, c->ws.keep.ptr, c->ws.cap.ptr};
+    } else {
+        ws_decide(s, c->ws.rs.ptr, ncl, c->sp.warm_rule, warm, full, c->ws.cap.ptr, c->ws.keep.ptr);
+        if (apply) ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.keep.ptr, c->nv_total);
+    }
     c->ws.used_pending = true;
 }
 
@@ -907,7 +916,8 @@ int project_jacobi(pbd_ctx* c, int iterations, const int* cap, int vb, int ve, d
 // cloth when they store its last sweep; returns true if they did (the caller
 // then skips integrate_all, and p is left where pfinal says).
 bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* cap = nullptr,
-             const float4* delta = nullptr, const int* keep = nullptr, bool integrate_end = false) {
+             const float4* delta = nullptr, const int* keep = nullptr, bool integrate_end = false,
+             const QueueDecide* dec = nullptr) {
     hipStream_t s = c->stream;
     consolidate(c);  // an earlier substep may have left p unconsolidated
     int ncl = (int)c->cloths.size();
@@ -915,7 +925,19 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     int ce = std::max(1, c->sp.check_every);
     const bool any_tol = has_tolerance(c);
     c->sweeps_pending = false;
-    launch_fill_int(s, c->sweeps.ptr, ncl, 0);
+    // A deferred keep decision is folded into the tile queue's schedule kernel
+    // (with the sweep and residual-slot clears and the final-buffer record)
+    // when the queued launch runs; otherwise it is made now.
+    bool fold = false;
+    if (dec) {
+        if (c->sp.path == PBD_PATH_AUTO) {
+            replan_if_needed(c);
+            const int ts0 = effective_tile_sweeps(c);
+            fold = cap && !any_tol && c->any_partial && (iterations + ts0 - 1) / ts0 > 1 && !c->sp.per_chunk_launches;
+        }
+        if (!fold) ws_decide(s, dec->rs, ncl, dec->rule, dec->warm_sweeps, dec->full_sweeps, dec->cap, dec->keep);
+    }
+    if (!fold) launch_fill_int(s, c->sweeps.ptr, ncl, 0);
     if (c->sp.path == PBD_PATH_JACOBI) {
         PBD_CHECK(!fused_whole && !delta, PBD_ERR_STATE, "internal: the Jacobi path is never fused");
         int sw = project_jacobi(c, iterations, cap, 0, c->nv_total, tol, -1);
@@ -974,7 +996,7 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     const bool int_store =
         integrate_end && !any_tol && !(fused_whole && c->any_partial && (nchunks < 2 || cap != nullptr));
     ensure_slots(c, nchunks);
-    launch_fill_u64(s, c->res_slots.ptr, (size_t)c->nslots * ncl, 0ull);
+    if (!fold) launch_fill_u64(s, c->res_slots.ptr, (size_t)c->nslots * ncl, 0ull);
     size_t lds = (size_t)std::max(1, c->plan_max_load) * sizeof(float4);
     int threads = c->sp.block_threads > 0 ? c->sp.block_threads : 1024;
     int cur = c->pcur;
@@ -1032,9 +1054,18 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
         PBD_CHECK(c->q_tile.n >= c->order_first.n + (size_t)nchunks * c->order_partial.n, PBD_ERR_STATE,
                   "internal: tile queue too small (allocated before graph capture)");
         const DBuf<int>& ord = c->any_whole ? c->order_first : c->order_partial;
-        launch_queue_schedule(s, ord.ptr, (int)ord.n, c->tiles.ptr, (int)c->tiles.n, cap, ts, nchunks, c->q_tmp.ptr,
-                              c->q_tmp.ptr + c->q_tmp.n / 2, c->q_tile.ptr, c->q_chunk.ptr, c->q_count.ptr,
-                              c->q_count.ptr + 1, c->tdone.ptr);
+        if (fold) {
+            PBD_CHECK(dec->cap == cap, PBD_ERR_STATE, "internal: deferred decision for another budget array");
+            launch_queue_schedule_decide(s, *dec, c->cl.ptr, ncl, dec->cap, c->sweeps.ptr, c->res_slots.ptr,
+                                         c->nslots * ncl, c->pfinal.ptr, cur, iterations, ord.ptr, (int)ord.n,
+                                         c->tiles.ptr, (int)c->tiles.n, ts, nchunks, c->q_tmp.ptr,
+                                         c->q_tmp.ptr + c->q_tmp.n / 2, c->q_tile.ptr, c->q_chunk.ptr, c->q_count.ptr,
+                                         c->q_count.ptr + 1, c->tdone.ptr);
+        } else {
+            launch_queue_schedule(s, ord.ptr, (int)ord.n, c->tiles.ptr, (int)c->tiles.n, cap, ts, nchunks,
+                                  c->q_tmp.ptr, c->q_tmp.ptr + c->q_tmp.n / 2, c->q_tile.ptr, c->q_chunk.ptr,
+                                  c->q_count.ptr, c->q_count.ptr + 1, c->tdone.ptr);
+        }
         TileQueue q{};
         q.item_tile = c->q_tile.ptr;
         q.item_chunk = c->q_chunk.ptr;
@@ -1098,8 +1129,9 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     } else {
         cur ^= 1;
     }
-    launch_final_buffers(s, c->cl.ptr, ncl, start, launched, ts, iterations, any_tol ? nullptr : cap,
-                         c->pfinal.ptr);
+    if (!fold)  // the folded schedule kernel recorded them
+        launch_final_buffers(s, c->cl.ptr, ncl, start, launched, ts, iterations, any_tol ? nullptr : cap,
+                             c->pfinal.ptr);
     c->pfinal_valid = true;
     c->pcur = cur;
     if (any_tol || cap) {
@@ -1263,8 +1295,9 @@ void substep(pbd_ctx* c, float dt) {
             // advect inside the inference kernel; the prediction is added by the
             // tile load, and each cloth is integrated by the tiles of its last sweep
             advect_prepare(c, dt);
-            ws_apply_prediction(c, false, true, dt);
-            if (project(c, c->sp.iterations, false, dt, c->ws.cap.ptr, c->ws.delta.ptr, c->ws.keep.ptr, true))
+            QueueDecide dec{};
+            ws_apply_prediction(c, false, true, dt, &dec);
+            if (project(c, c->sp.iterations, false, dt, c->ws.cap.ptr, c->ws.delta.ptr, c->ws.keep.ptr, true, &dec))
                 return;
         }
         integrate_all(c, dt, false);

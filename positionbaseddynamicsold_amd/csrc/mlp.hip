@@ -645,9 +645,12 @@ __global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_infer(
     const MlpW* __restrict__ gW, const ClothDev* __restrict__ cl, const int* __restrict__ vcloth,
     const float4* __restrict__ x, const float4* __restrict__ p, const float4* __restrict__ vel,
     float4* __restrict__ p_out, float dt, const int* __restrict__ nbr, const float* __restrict__ nbr_rest, int nv,
-    const float* __restrict__ norm, float gate, float4* __restrict__ delta) {
+    const float* __restrict__ norm, float gate, float4* __restrict__ delta, unsigned long long* __restrict__ rs_zero,
+    int n_rs) {
     __shared__ InfW W;
     __shared__ InferStage stage[MLP_WAVES];
+    if (rs_zero)
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_rs; i += gridDim.x * blockDim.x) rs_zero[i] = 0ull;
     __shared__ float s_mu[22], s_rsd[22];
     {
         const uint4* src = (const uint4*)gW->i1;  // i1, i2, i3, b1, b2, b3 are contiguous
@@ -821,13 +824,7 @@ __global__ __launch_bounds__(256) void k_ws_decide(const unsigned long long* __r
                                                    int* __restrict__ keep) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncl) return;
-    bool k;
-    if (rule == 1) {
-        k = rs[3 * ncl + c] <= rs[2 * ncl + c];
-    } else {
-        double a = __longlong_as_double((long long)rs[c]), b = __longlong_as_double((long long)rs[ncl + c]);
-        k = b <= a;
-    }
+    const bool k = ws_keep(rs, ncl, c, rule);
     keep[c] = k ? 1 : 0;
     cap[c] = k ? warm_sweeps : full_sweeps;
 }
@@ -908,8 +905,10 @@ size_t ws_feature_bytes(int nv) { return (size_t)std::max(nv, 1) * 64; }
 
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, float4* p,
               const float4* vel, float dt, const int* nbr, const float* nbr_rest, int nv, float4* delta,
-              float* feat_out, const float* norm, void* feat_scratch, float* hs_scratch, float gate) {
+              float* feat_out, const float* norm, void* feat_scratch, float* hs_scratch, float gate,
+              unsigned long long* rs_zero, int n_rs) {
     PBD_CHECK(!(vel && feat_out), PBD_ERR_STATE, "internal: advect-on-load inference keeps no raw features");
+    PBD_CHECK(!(rs_zero && feat_out), PBD_ERR_STATE, "internal: only the fused inference clears the sums");
     if (nv <= 0) return;
     static int max_blocks = 0;
     if (!max_blocks) {
@@ -932,11 +931,12 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
         const int blocks = std::min((nblk + MLP_WAVES - 1) / MLP_WAVES, max_blocks_fused);
         if (vel)
             hipLaunchKernelGGL(k_ws_infer<true>, dim3(blocks), dim3(64 * MLP_WAVES), 0, s, (const MlpW*)W, cl, vcloth,
-                               x, (const float4*)nullptr, vel, p, dt, nbr, nbr_rest, nv, norm, gate, delta);
+                               x, (const float4*)nullptr, vel, p, dt, nbr, nbr_rest, nv, norm, gate, delta, rs_zero,
+                               n_rs);
         else
             hipLaunchKernelGGL(k_ws_infer<false>, dim3(blocks), dim3(64 * MLP_WAVES), 0, s, (const MlpW*)W, cl,
                                vcloth, x, (const float4*)p, (const float4*)nullptr, (float4*)nullptr, 0.0f, nbr,
-                               nbr_rest, nv, norm, gate, delta);
+                               nbr_rest, nv, norm, gate, delta, rs_zero, n_rs);
         PBD_HIP(hipGetLastError());
         return;
     }

@@ -294,9 +294,30 @@ void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float
               int step);
 // Predicted corrections.  With vel, p is written: the advected positions of
 // x and vel (advect on load, no raw features); otherwise p is read.
+// rs_zero (fused form only): n_rs words the kernel clears on the way (the
+// fallback's residual sums, so that no separate fill runs before them).
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, float4* p,
               const float4* vel, float dt, const int* nbr, const float* nbr_rest, int nv, float4* delta,
-              float* feat_out, const float* norm, void* feat_scratch, float* hs_scratch, float gate);
+              float* feat_out, const float* norm, void* feat_scratch, float* hs_scratch, float gate,
+              unsigned long long* rs_zero = nullptr, int n_rs = 0);
+// The fallback rule on one cloth's residual sums rs = [max0 | max1 | sq0 | sq1]
+// (ncl entries each): keep the prediction iff its residual does not exceed
+// the unwarmed start's (rule 0: max |C|/L, bit patterns of non-negative
+// doubles; rule 1: sums of squares in fixed point).
+__device__ __forceinline__ bool ws_keep(const unsigned long long* rs, int ncl, int c, int rule) {
+    if (rule == 1) return rs[3 * ncl + c] <= rs[2 * ncl + c];
+    const double a = __longlong_as_double((long long)rs[c]), b = __longlong_as_double((long long)rs[ncl + c]);
+    return b <= a;
+}
+// The keep decision folded into the tile queue's schedule kernel (it also
+// clears the sweep counters and residual slots and records each cloth's
+// final p buffer, the work of separate small kernels otherwise).
+struct QueueDecide {
+    const unsigned long long* rs;
+    int rule, warm_sweeps, full_sweeps;
+    int* keep;
+    int* cap;       // the budgets the decision writes (the projection's cap)
+};
 size_t ws_feature_bytes(int nv);
 void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm);
 void ws_resid2(hipStream_t s, const int4* dist, int n, const float4* p, const float4* delta, unsigned long long* rs,
@@ -421,6 +442,14 @@ void launch_tiles_queue(hipStream_t s, int nblocks, const TileDev* tiles, const 
 void launch_queue_schedule(hipStream_t s, const int* order, int n_order, const TileDev* tiles, int n_tiles,
                            const int* cap, int ts, int nchunks, int* chain_tmp, int* bulk_tmp, int* item_tile,
                            int* item_chunk, int* n_items, int* head, int* done);
+// The same with the keep decision (writing cap) and the substep's small
+// bookkeeping in the one workgroup: sweeps[ncl] and res_slots[n_slots]
+// cleared, pfinal[c] = the p buffer each cloth ends in.
+void launch_queue_schedule_decide(hipStream_t s, const QueueDecide& d, const ClothDev* cl, int ncl, int* cap,
+                                  int* sweeps, unsigned long long* res_slots, int n_slots, int* pfinal, int start,
+                                  int iterations, const int* order, int n_order, const TileDev* tiles, int n_tiles,
+                                  int ts, int nchunks, int* chain_tmp, int* bulk_tmp, int* item_tile, int* item_chunk,
+                                  int* n_items, int* head, int* done);
 // Compacts the multi-tile order (layout order[j * 8 + g]; order_cloth: each
 // entry's cloth, -1 for padding) to the tiles whose cloth's budget cap
 // exceeds min_cap, rebalanced over the 8 groups, length into *n_out.

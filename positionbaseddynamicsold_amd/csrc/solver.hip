@@ -863,14 +863,13 @@ __device__ __forceinline__ int block_flag_scan(bool f, int* wave_sum, int* total
     return off + (int)__popcll(m & ((1ull << lane) - 1ull));
 }
 
-__global__ __launch_bounds__(1024) void k_queue_schedule(const int* __restrict__ order, int n_order,
-                                                         const TileDev* __restrict__ tiles, int n_tiles,
-                                                         const int* __restrict__ cap, int ts, int nchunks,
-                                                         int* __restrict__ chain_tmp, int* __restrict__ bulk_tmp,
-                                                         int* __restrict__ item_tile, int* __restrict__ item_chunk,
-                                                         int* __restrict__ n_items, int* __restrict__ head,
-                                                         int* __restrict__ done) {
-    __shared__ int wave_sum[16];
+__device__ __forceinline__ void queue_schedule_body(const int* __restrict__ order, int n_order,
+                                                    const TileDev* __restrict__ tiles, int n_tiles,
+                                                    const int* __restrict__ cap, int ts, int nchunks,
+                                                    int* __restrict__ chain_tmp, int* __restrict__ bulk_tmp,
+                                                    int* __restrict__ item_tile, int* __restrict__ item_chunk,
+                                                    int* __restrict__ n_items, int* __restrict__ head,
+                                                    int* __restrict__ done, int* wave_sum) {
     const int T = blockDim.x;
     for (int t = threadIdx.x; t < n_tiles; t += T) done[t] = -1;
     if (threadIdx.x == 0) *head = 0;
@@ -908,6 +907,42 @@ __global__ __launch_bounds__(1024) void k_queue_schedule(const int* __restrict__
         base += hi - lo;
     }
     if (threadIdx.x == 0) *n_items = base;
+}
+
+__global__ __launch_bounds__(1024) void k_queue_schedule(const int* __restrict__ order, int n_order,
+                                                         const TileDev* __restrict__ tiles, int n_tiles,
+                                                         const int* __restrict__ cap, int ts, int nchunks,
+                                                         int* __restrict__ chain_tmp, int* __restrict__ bulk_tmp,
+                                                         int* __restrict__ item_tile, int* __restrict__ item_chunk,
+                                                         int* __restrict__ n_items, int* __restrict__ head,
+                                                         int* __restrict__ done) {
+    __shared__ int wave_sum[16];
+    queue_schedule_body(order, n_order, tiles, n_tiles, cap, ts, nchunks, chain_tmp, bulk_tmp, item_tile, item_chunk,
+                        n_items, head, done, wave_sum);
+}
+
+// The keep decision and the substep's bookkeeping, then the schedule.
+__global__ __launch_bounds__(1024) void k_queue_schedule_decide(
+    QueueDecide d, const ClothDev* __restrict__ cl, int ncl, int* __restrict__ cap, int* __restrict__ sweeps,
+    unsigned long long* __restrict__ res_slots, int n_slots, int* __restrict__ pfinal, int start, int iterations,
+    const int* __restrict__ order, int n_order, const TileDev* __restrict__ tiles, int n_tiles, int ts, int nchunks,
+    int* __restrict__ chain_tmp, int* __restrict__ bulk_tmp, int* __restrict__ item_tile, int* __restrict__ item_chunk,
+    int* __restrict__ n_items, int* __restrict__ head, int* __restrict__ done) {
+    __shared__ int wave_sum[16];
+    for (int c = threadIdx.x; c < ncl; c += blockDim.x) {
+        const bool k = ws_keep(d.rs, ncl, c, d.rule);
+        d.keep[c] = k ? 1 : 0;
+        const int b = k ? d.warm_sweeps : d.full_sweeps;
+        cap[c] = b;
+        sweeps[c] = 0;
+        // the buffer the cloth's last chunk writes (a single-tile cloth: its one launch)
+        const int run = (min(b, iterations) + ts - 1) / ts;
+        pfinal[c] = cl[c].whole ? (start ^ 1) : (start ^ (run & 1));
+    }
+    for (int i = threadIdx.x; i < n_slots; i += blockDim.x) res_slots[i] = 0ull;
+    __syncthreads();  // cap written above, read by every thread below
+    queue_schedule_body(order, n_order, tiles, n_tiles, cap, ts, nchunks, chain_tmp, bulk_tmp, item_tile, item_chunk,
+                        n_items, head, done, wave_sum);
 }
 
 #undef TILE_PARAMS
@@ -965,6 +1000,17 @@ void launch_queue_schedule(hipStream_t s, const int* order, int n_order, const T
                            int* item_chunk, int* n_items, int* head, int* done) {
     hipLaunchKernelGGL(k_queue_schedule, dim3(1), dim3(1024), 0, s, order, n_order, tiles, n_tiles, cap, ts, nchunks,
                        chain_tmp, bulk_tmp, item_tile, item_chunk, n_items, head, done);
+    PBD_HIP(hipGetLastError());
+}
+
+void launch_queue_schedule_decide(hipStream_t s, const QueueDecide& d, const ClothDev* cl, int ncl, int* cap,
+                                  int* sweeps, unsigned long long* res_slots, int n_slots, int* pfinal, int start,
+                                  int iterations, const int* order, int n_order, const TileDev* tiles, int n_tiles,
+                                  int ts, int nchunks, int* chain_tmp, int* bulk_tmp, int* item_tile, int* item_chunk,
+                                  int* n_items, int* head, int* done) {
+    hipLaunchKernelGGL(k_queue_schedule_decide, dim3(1), dim3(1024), 0, s, d, cl, ncl, cap, sweeps, res_slots, n_slots,
+                       pfinal, start, iterations, order, n_order, tiles, n_tiles, ts, nchunks, chain_tmp, bulk_tmp,
+                       item_tile, item_chunk, n_items, head, done);
     PBD_HIP(hipGetLastError());
 }
 
