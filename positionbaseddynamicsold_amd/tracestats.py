@@ -55,7 +55,10 @@ def passes(seq, steps, name_of=lambda e: e[0], grid_of=lambda e: e[3]):
     c0 = max(i for i in copies if grid_of(seq[i]) == big)
     while c0 > 0 and name_of(seq[c0 - 1]) == "k_copy" and grid_of(seq[c0 - 1]) == big:
         c0 -= 1
-    ints = [i for i in range(c0) if name_of(seq[i]) == "k_final_buffers"]
+    # the last kernel of each substep: k_final_buffers (plain, per-chunk
+    # launches) or the queued tile launch (warm start, which records the
+    # final buffers in its schedule kernel)
+    ints = [i for i in range(c0) if name_of(seq[i]) in ("k_final_buffers", "k_tiles_queue")]
     need = 2 * (steps + 1)
     if len(ints) < need:
         raise SystemExit("trace does not hold the two profiled passes")
