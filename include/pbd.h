@@ -384,9 +384,14 @@ typedef struct {
        vertex list and loaded state (16 B per vertex, 32 B when it advects x and
        v, +16 B with a kept prediction), the prim records of every phase it
        runs (8 B per distance prim, 12 B per bending prim), its owned prims for
-       the residual (8 B each) and its stores (16 B per interior vertex, 64 B
-       when it integrates); counted from the plan for the launches that ran */
+       the residual (8 B each, under a tolerance only) and its stores (16 B per
+       interior vertex, 64 B when it integrates); counted from the plan for the
+       launches that ran */
     double tile_bytes;
+    /* waits of the queued tile launch (per_chunk_launches = 0) that gave up
+       after their bounded spin instead of seeing the neighbour tile's chunk;
+       0 in a correct run (the count is kept since the context was created) */
+    long long queue_wait_timeouts;
 } pbd_profile;
 PBD_API int pbd_profile_enable(pbd_ctx* ctx, int on);
 PBD_API int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset);

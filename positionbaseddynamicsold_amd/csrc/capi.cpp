@@ -2533,6 +2533,7 @@ int pbd_profile_enable(pbd_ctx* ctx, int on) {
 int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset) {
     return guard([&] {
         PBD_CHECK(ctx && out, PBD_ERR_ARG, "null argument");
+        set_device(ctx);
         *out = ctx->prof;
         const Broadphase& b = ctx->cbp;
         out->ccd_ee_raw_pairs = b.st_ee_raw;
@@ -2545,6 +2546,13 @@ int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset) {
         out->ccd_pairs = b.st_pairs;
         out->ccd_contacts = b.st_contacts;
         out->ccd_resolved = b.st_resolved;
+        out->queue_wait_timeouts = 0;
+        if (ctx->q_timeouts.n) {
+            int t = 0;
+            PBD_HIP(hipMemcpyAsync(&t, ctx->q_timeouts.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            PBD_HIP(hipStreamSynchronize(ctx->stream));
+            out->queue_wait_timeouts = t;
+        }
         if (reset) {
             ctx->prof = pbd_profile{};
             ctx->cbp.st_dets = ctx->cbp.st_tris = ctx->cbp.st_entries = 0;
