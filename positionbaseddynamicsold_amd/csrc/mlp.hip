@@ -14,6 +14,8 @@
 //   W1[64][22] b1[64] W2[64][64] b2[64] W3[3][64] b3[3]
 #include <algorithm>
 #include <cstddef>
+#include <map>
+#include <mutex>
 #include <cmath>
 
 #include "pbd_internal.hpp"
@@ -903,6 +905,25 @@ void mlp_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float
 // BF16 feature rows of the network kernel: 64 bytes per vertex.
 size_t ws_feature_bytes(int nv) { return (size_t)std::max(nv, 1) * 64; }
 
+// Workgroups of `kernel` (at `threads`, no dynamic LDS) the current device
+// holds at once: CUs x resident workgroups per CU, queried once per device
+// and kernel.
+int resident_blocks(const void* kernel, int threads) {
+    static std::mutex m;
+    static std::map<std::pair<int, const void*>, int> cache;
+    int dev = 0;
+    PBD_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(m);
+    auto it = cache.find({dev, kernel});
+    if (it != cache.end()) return it->second;
+    int ncu = 0, per_cu = 0;
+    PBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    const int n = std::max(1, ncu) * std::max(1, per_cu);
+    cache[{dev, kernel}] = n;
+    return n;
+}
+
 void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vcloth, const float4* x, float4* p,
               const float4* vel, float dt, const int* nbr, const float* nbr_rest, int nv, float4* delta,
               float* feat_out, const float* norm, void* feat_scratch, float* hs_scratch, float gate,
@@ -910,23 +931,8 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
     PBD_CHECK(!(vel && feat_out), PBD_ERR_STATE, "internal: advect-on-load inference keeps no raw features");
     PBD_CHECK(!(rs_zero && feat_out), PBD_ERR_STATE, "internal: only the fused inference clears the sums");
     if (nv <= 0) return;
-    static int max_blocks = 0;
-    if (!max_blocks) {
-        int dev = 0, ncu = 0, per_cu = 0;
-        PBD_HIP(hipGetDevice(&dev));
-        PBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ws_mlp, 64 * MLP_WAVES, 0));
-        max_blocks = std::max(1, ncu) * std::max(1, per_cu);
-    }
     if (!feat_out) {
-        static int max_blocks_fused = 0;
-        if (!max_blocks_fused) {
-            int dev = 0, ncu = 0, per_cu = 0;
-            PBD_HIP(hipGetDevice(&dev));
-            PBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-            PBD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ws_infer<true>, 64 * MLP_WAVES, 0));
-            max_blocks_fused = std::max(1, ncu) * std::max(1, per_cu);
-        }
+        const int max_blocks_fused = resident_blocks((const void*)k_ws_infer<true>, 64 * MLP_WAVES);
         const int nblk = (nv + 63) / 64;
         const int blocks = std::min((nblk + MLP_WAVES - 1) / MLP_WAVES, max_blocks_fused);
         if (vel)
@@ -945,7 +951,7 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
                        feat, hs_scratch, feat_out, gate);
     PBD_HIP(hipGetLastError());
     int groups = (nv + 15) / 16;
-    int blocks = std::min((groups + MLP_WAVES - 1) / MLP_WAVES, max_blocks);
+    int blocks = std::min((groups + MLP_WAVES - 1) / MLP_WAVES, resident_blocks((const void*)k_ws_mlp, 64 * MLP_WAVES));
     hipLaunchKernelGGL(k_ws_mlp, dim3(blocks), dim3(64 * MLP_WAVES), 0, s, (const MlpW*)W, (const uint4*)feat,
                        (const float*)hs_scratch, norm, nv, delta);
     PBD_HIP(hipGetLastError());
