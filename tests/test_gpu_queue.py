@@ -15,12 +15,14 @@ pytestmark = pytest.mark.gpu
 from positionbaseddynamicsold_amd import capi, scenes
 
 
-@pytest.fixture(scope="module")
-def batch():
+@pytest.fixture(scope="module", params=[False, True], ids=["distance", "bending"])
+def batch(request):
+    # the bending batch (dihedral prims at stiffness 0.1) runs the bending
+    # form of the tile kernels
     specs = scenes.config5(seed=7, count=40)
     ctx = capi.Context()
     for s in specs:
-        scenes.add_to_context(ctx, s)
+        scenes.add_to_context(ctx, s, bending=request.param, stiffness=(1.0, 0.1, 1.0))
     ctx.init()
     ctx.set_solver(dt=scenes.DT, iterations=20)
     ctx.step(12)
@@ -29,6 +31,10 @@ def batch():
     ctx.ws_train(epochs=8, batch=4096, lr=1e-3, seed=1)
     ctx.sync()
     ctx.snapshot()
+    # a dihedral prim spans two triangles, so a vertex's dependency cone over
+    # the same sweeps is several times larger: the bending batch gets four
+    # times the tile budget (0 keeps the default budget)
+    ctx.budget_scale = 4 if request.param else 1
     yield ctx
     ctx.close()
 
@@ -49,6 +55,7 @@ def run(ctx, per_chunk, ts, warm, graph, max_verts, steps=6):
 @pytest.mark.parametrize("ts,warm,max_verts", [(2, 1, 2048), (1, 1, 1024), (2, 3, 2048), (3, 1, 0)])
 @pytest.mark.parametrize("graph", [1, 0])
 def test_queued_launch_equals_per_chunk_launches(batch, ts, warm, max_verts, graph):
+    max_verts *= batch.budget_scale
     print(f"queue vs per-chunk: ts {ts} warm {warm} max_verts {max_verts} graph {graph}", flush=True)
     x0, v0, k0 = run(batch, 1, ts, warm, graph, max_verts)
     x1, v1, k1 = run(batch, 0, ts, warm, graph, max_verts)
