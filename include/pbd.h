@@ -106,9 +106,14 @@ typedef struct {
                                (0 = one launch; results are identical either way) */
     int per_chunk_launches; /* under per-cloth budgets (the warm start) with fixed sweeps: 0 = the substep's
                                tile work is one launch whose workgroups take (chunk, tile) items from a queue
-                               built on the device, a tile's later chunk waiting only for its neighbour tiles'
-                               previous chunk; 1 = one launch per chunk of tile_sweeps sweeps (results are
-                               identical either way) */
+                               built on the device after a residual kernel has made the keep decision, a
+                               tile's later chunk waiting only for its neighbour tiles' previous chunk;
+                               1 = one launch per chunk of tile_sweeps sweeps; 2 = as 0, but in a warm
+                               substep the queued launch computes the fallback residuals itself and keeps or
+                               discards each cloth's prediction (no residual kernel; a cloth's tiles wait for
+                               each other, and later chunks are appended once a cloth is decided; when a
+                               multi-tile cloth has as many tiles as the GPU holds workgroups, as 0).
+                               Results are identical in all three; 0 measured fastest */
 } pbd_solver_params;
 
 typedef struct {
@@ -388,10 +393,15 @@ typedef struct {
        interior vertex, 64 B when it integrates); counted from the plan for the
        launches that ran */
     double tile_bytes;
-    /* waits of the queued tile launch (per_chunk_launches = 0) that gave up
-       after their bounded spin instead of seeing the neighbour tile's chunk;
-       0 in a correct run (the count is kept since the context was created) */
+    /* waits of the queued tile launch (per_chunk_launches 0 or 2) that gave up
+       after their bounded spin instead of seeing what they waited for (a
+       neighbour tile's chunk, a cloth's decision, the next item); 0 in a
+       correct run (the count is kept since the context was created) */
     long long queue_wait_timeouts;
+    /* queued tile launches that made the warm start's keep decision
+       themselves (per_chunk_launches 0; counted on the device, graph replays
+       included, since the context was created) */
+    long long queue_deciding_launches;
 } pbd_profile;
 PBD_API int pbd_profile_enable(pbd_ctx* ctx, int on);
 PBD_API int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset);

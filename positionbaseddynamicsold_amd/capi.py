@@ -70,7 +70,8 @@ class Profile(C.Structure):
                 ("ccd_triangles", C.c_longlong), ("ccd_cell_entries", C.c_longlong), ("ccd_raw_pairs", C.c_longlong),
                 ("ccd_pairs", C.c_longlong), ("ccd_contacts", C.c_longlong), ("ccd_resolved", C.c_longlong),
                 ("ccd_ee_raw_pairs", C.c_longlong), ("ccd_ee_pairs", C.c_longlong), ("ccd_ee_contacts", C.c_longlong),
-                ("tile_bytes", C.c_double), ("queue_wait_timeouts", C.c_longlong)]
+                ("tile_bytes", C.c_double), ("queue_wait_timeouts", C.c_longlong),
+                ("queue_deciding_launches", C.c_longlong)]
 
 
 class ClothInfo(C.Structure):

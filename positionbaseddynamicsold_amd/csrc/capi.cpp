@@ -58,6 +58,13 @@ struct pbd_ctx {
     // the one-launch form of the same (TileQueue): tile neighbours, per-tile
     // done chunks, the queue's items and scratch, and the wait timeouts
     DBuf<int> tnbr_off, tnbr, tdone, q_tile, q_chunk, q_tmp, q_count, q_timeouts;
+    // the queued launch that makes the keep decision itself: its fixed
+    // first-chunk items, the block it needs zeroed at its start (residual
+    // sums, per-cloth and per-tile counters), its multi-tile cloths and the
+    // most tiles one of them has
+    DBuf<int> q_stat;
+    DBuf<unsigned long long> q_zero;
+    int q_n_multi = 0, q_tiles_max = 0;
     int num_cu = 0;
     // workgroups the tile kernel keeps resident on the whole GPU, for the
     // (kernel, threads, LDS) it was last asked for
@@ -194,6 +201,8 @@ void fill_cloth_dev(pbd_ctx* c) {
         d.nv = k.nv;
         d.ktab = (int)i * c->kmax;
         d.whole = k.whole ? 1 : 0;
+        d.tile_begin = k.tile_begin;
+        d.tile_end = k.tile_end;
         double sum = 0.0;
         int ne = 0;
         for (int p = 0; p < k.np(); p++)
@@ -265,12 +274,34 @@ void upload_plan(pbd_ctx* c) {
     c->q_tmp.alloc(2 * std::max<size_t>(1, first.size()));
     c->q_count.alloc(2);  // item count, head
     if (!c->q_timeouts.n) {
-        c->q_timeouts.alloc(1);
-        PBD_HIP(hipMemsetAsync(c->q_timeouts.ptr, 0, sizeof(int), c->stream));
+        c->q_timeouts.alloc(2);  // wait timeouts, deciding launches
+        PBD_HIP(hipMemsetAsync(c->q_timeouts.ptr, 0, 2 * sizeof(int), c->stream));
     }
     std::vector<int> ids(p.tiles.size());
     for (size_t t = 0; t < ids.size(); t++) ids[t] = (int)t;
     c->tile_ids.upload(ids, c->stream);
+    {
+        // first chunks of the deciding queue: single-tile cloths' tiles (their
+        // blocks run longest), then the multi-tile cloths' tiles cloth by cloth
+        std::vector<int> st;
+        st.reserve(p.tiles.size());
+        for (size_t t = 0; t < p.tiles.size(); t++)
+            if (p.tiles[t].whole) st.push_back((int)t);
+        for (size_t t = 0; t < p.tiles.size(); t++)
+            if (!p.tiles[t].whole) st.push_back((int)t);
+        c->q_stat.upload(st.empty() ? std::vector<int>(1, 0) : st, c->stream);
+        c->q_n_multi = 0;
+        c->q_tiles_max = 0;
+        for (const Cloth& k : c->cloths)
+            if (!k.whole && k.tile_end > k.tile_begin) {
+                c->q_n_multi++;
+                c->q_tiles_max = std::max(c->q_tiles_max, k.tile_end - k.tile_begin);
+            }
+        // [rs: 4 ncl words, padded to 128 bytes][counters][decision flags][posted][finished]
+        const size_t ncl = c->cloths.size();
+        const size_t ints = QC_COUNT + QC_DECIDED_STRIDE * ncl + ncl + p.tiles.size();
+        c->q_zero.alloc((4 * ncl + 15) / 16 * 16 + (ints + 1) / 2);
+    }
     drop_graphs(c);
     c->any_whole = false;
     c->any_partial = false;
@@ -697,6 +728,24 @@ void ws_require_params(pbd_ctx* c) {
     PBD_CHECK(c->ws.params_ready, PBD_ERR_STATE, "warm-start weights not initialised (pbd_ws_init or pbd_ws_set_params)");
 }
 
+// Whether a warm substep of `iterations` sweeps under per-cloth budgets folds
+// the keep decision into the queued tile launch's schedule (true), or makes
+// it before the projection (false).
+bool has_tolerance(const pbd_ctx* c);
+bool queue_folds(pbd_ctx* c, int iterations) {
+    if (c->sp.path != PBD_PATH_AUTO || has_tolerance(c) || c->sp.per_chunk_launches == 1) return false;
+    replan_if_needed(c);
+    const int ts = effective_tile_sweeps(c);
+    return c->any_partial && (iterations + ts - 1) / ts > 1;
+}
+
+// Whether that launch also computes the residuals and decides
+// (per_chunk_launches 2), when every multi-tile cloth's tiles fit the GPU's
+// resident workgroups at once (they wait for each other).
+bool queue_decides(pbd_ctx* c, int iterations) {
+    return queue_folds(c, iterations) && c->sp.per_chunk_launches == 2 && c->q_tiles_max < walk_grid(c);
+}
+
 // Predicts the correction and decides, per cloth, whether to keep it (its
 // residual must not rise above the unwarmed start).  With apply, adds it to
 // the current p of the kept cloths; otherwise the caller passes ws.delta and
@@ -711,17 +760,23 @@ void ws_apply_prediction(pbd_ctx* c, bool apply, bool advect, float dt, QueueDec
     int ncl = (int)c->cloths.size();
     ws_build_nbr(c);
     float4* p = c->pbuf[c->pcur].ptr;
-    // the fused inference clears the residual sums itself
+    // the queued tile launch may compute the residuals and decide itself;
+    // then the block it needs zeroed takes the place of the residual sums
+    const bool in_launch = dec && !apply && queue_decides(c, c->sp.iterations);
+    unsigned long long* zero = in_launch ? c->q_zero.ptr : c->ws.rs.ptr;
+    const size_t nzero = in_launch ? c->q_zero.n : 4 * (size_t)ncl;
+    // the fused inference clears them itself
     const bool fused = advect;
     ws_infer(s, c->ws.W.ptr, c->cl.ptr, c->vcloth.ptr, c->x.ptr, p, advect ? c->v.ptr : nullptr, dt, c->ws.nbr.ptr,
              c->ws.nbr_rest.ptr, c->nv_total, c->ws.delta.ptr, nullptr, c->ws.norm.ptr, c->ws.feat.ptr, c->ws.hs.ptr,
-             c->sp.warm_gate, fused ? c->ws.rs.ptr : nullptr, fused ? 4 * ncl : 0);
-    if (!fused) launch_fill_u64(s, c->ws.rs.ptr, 4 * (size_t)ncl, 0ull);
-    ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.rs.ptr, ncl, c->sp.warm_rule);
+             c->sp.warm_gate, fused ? zero : nullptr, fused ? (int)nzero : 0);
+    if (!fused) launch_fill_u64(s, zero, nzero, 0ull);
+    if (!in_launch)
+        ws_resid2(s, c->rdist.ptr, (int)c->rdist.n, p, c->ws.delta.ptr, c->ws.rs.ptr, ncl, c->sp.warm_rule);
     int full = c->sp.iterations;
     int warm = c->sp.warm_iterations > 0 ? std::min(c->sp.warm_iterations, full) : full;
     if (dec && !apply) {
-        *dec = QueueDecide{c->ws.rs.ptr, c->sp.warm_rule, warm, full, c->ws.keep.ptr, c->ws.cap.ptr};
+        *dec = QueueDecide{c->ws.rs.ptr, c->sp.warm_rule, warm, full, c->ws.keep.ptr, c->ws.cap.ptr, in_launch};
     } else {
         ws_decide(s, c->ws.rs.ptr, ncl, c->sp.warm_rule, warm, full, c->ws.cap.ptr, c->ws.keep.ptr);
         if (apply) ws_apply(s, c->vcloth.ptr, p, c->ws.delta.ptr, c->ws.keep.ptr, c->nv_total);
@@ -779,26 +834,43 @@ void advect_all(pbd_ctx* c, float dt) {
 // one); fused_whole: the first load advects x and v; kept: the cloth's
 // prediction was added on the first load; int_store: the launch holding a
 // cloth's last sweep integrated it.
-double tile_projection_bytes(const pbd_ctx* c, int ts, bool fused_whole, bool int_store, bool with_delta) {
+double tile_projection_bytes(const pbd_ctx* c, int ts, bool fused_whole, bool int_store, bool with_delta,
+                             bool decide) {
     double total = 0.0;
+    const int warm = c->sp.warm_iterations > 0 ? c->sp.warm_iterations : c->sp.iterations;
     for (size_t ci = 0; ci < c->cloths.size(); ci++) {
         const Cloth& k = c->cloths[ci];
         const int L = k.last_sweeps;
         if (L <= 0 || k.tile_record_bytes.size() < 2) continue;
         const bool kept = with_delta && ci < c->ws.used.size() && c->ws.used[ci] != 0;
-        // owned prims are read for the residual only under a tolerance
+        // owned prims are read for the residual under a tolerance, and by a
+        // first chunk that decides
         const double own = (k.tol >= 0.0 ? k.tol : c->sp.tol) >= 0.0 ? 8.0 * (double)k.tiles_owned : 0.0;
+        auto loaded_at = [&](int n) {
+            const int row = std::min(n, (int)k.tile_record_bytes.size() - 1);
+            return k.whole ? (double)k.tiles_loaded : k.tile_loaded_by_sweeps[row];
+        };
         auto launch = [&](int n, bool first, bool last) {
             double load = (first && fused_whole) ? 32.0 : 16.0;
-            if (first && kept) load += 16.0;
+            if (first && (kept || decide)) load += 16.0;
             const bool integ = k.whole ? (fused_whole || int_store) : (int_store && last);
             const double store = integ ? 64.0 : 16.0;
             const int row = std::min(n, (int)k.tile_record_bytes.size() - 1);
             const double rec = k.whole ? n * k.tile_record_bytes[1] : k.tile_record_bytes[row];
             // a launch of n sweeps loads the prefix of each tile's list its cone needs
-            const double loaded = k.whole ? (double)k.tiles_loaded : k.tile_loaded_by_sweeps[row];
-            return 4.0 * (loaded + (double)k.tiles_interior) + load * loaded + rec + own +
-                   store * (double)k.tiles_interior;
+            const double loaded = loaded_at(n);
+            double b = 4.0 * (loaded + (double)k.tiles_interior) + load * loaded + rec + own +
+                       store * (double)k.tiles_interior;
+            if (first && decide) {
+                // the owned prims' records; a discarded cloth first loaded the
+                // kept budget's cone with the prediction, then loads its own
+                b += own > 0.0 ? 0.0 : 8.0 * (double)k.tiles_owned;
+                if (!kept) {
+                    const double lk = loaded_at(std::min(n, warm));
+                    b += 36.0 * lk - 16.0 * loaded;
+                }
+            }
+            return b;
         };
         if (k.whole) {
             total += launch(L, true, true);
@@ -930,11 +1002,9 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     // when the queued launch runs; otherwise it is made now.
     bool fold = false;
     if (dec) {
-        if (c->sp.path == PBD_PATH_AUTO) {
-            replan_if_needed(c);
-            const int ts0 = effective_tile_sweeps(c);
-            fold = cap && !any_tol && c->any_partial && (iterations + ts0 - 1) / ts0 > 1 && !c->sp.per_chunk_launches;
-        }
+        fold = cap && queue_folds(c, iterations);
+        PBD_CHECK(!dec->in_launch || (fold && queue_decides(c, iterations)), PBD_ERR_STATE,
+                  "internal: the residuals were left to a launch that does not decide");
         if (!fold) ws_decide(s, dec->rs, ncl, dec->rule, dec->warm_sweeps, dec->full_sweeps, dec->cap, dec->keep);
     }
     if (!fold) launch_fill_int(s, c->sweeps.ptr, ncl, 0);
@@ -1018,7 +1088,7 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     // find their cloth done.
     const bool follow = cap && !any_tol && c->any_partial && nchunks > 1;
     int follow_grid = 0;
-    if (follow && c->sp.per_chunk_launches) {
+    if (follow && c->sp.per_chunk_launches == 1) {
         launch_follow_list(s, c->order_partial.ptr, c->order_partial_cloth.ptr, (int)c->order_partial.n, cap, ts,
                            c->order_follow.ptr, c->follow_len.ptr);
         follow_grid = std::min(walk_grid(c), (int)c->order_partial.n);
@@ -1047,7 +1117,53 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
         a.chunk = 0;
         tiles((int)c->order_whole.n, c->order_whole.ptr, c->pbuf[cur].ptr, c->pbuf[cur ^ 1].ptr, a);
     }
-    if (follow && !c->sp.per_chunk_launches) {
+    if (follow && fold && dec->in_launch) {
+        // every chunk in one launch that also decides: fixed first chunks,
+        // later chunks appended per discarded cloth (TileQueue)
+        ensure_queue(c);
+        PBD_CHECK(c->q_tile.n >= (size_t)nchunks * c->order_partial.n, PBD_ERR_STATE,
+                  "internal: tile queue too small (allocated before graph capture)");
+        PBD_CHECK(dec->cap == cap && delta && dec->warm_sweeps >= 1 && dec->full_sweeps == iterations && !fused_whole,
+                  PBD_ERR_STATE, "internal: deciding launch arguments");
+        TileQueue q{};
+        q.nbr_off = c->tnbr_off.ptr;
+        q.nbr = c->tnbr.ptr;
+        q.pbuf0 = c->pbuf[0].ptr;
+        q.pbuf1 = c->pbuf[1].ptr;
+        q.start = cur;
+        q.ts = ts;
+        q.iterations = iterations;
+        q.timeouts = c->q_timeouts.ptr;
+        q.launches = c->q_timeouts.ptr + 1;
+        q.stat_item = c->q_stat.ptr;
+        q.n_static = (int)c->tiles.n;
+        q.n_multi = c->q_n_multi;
+        q.dyn_tile = c->q_tile.ptr;
+        q.dyn_chunk = c->q_chunk.ptr;
+        q.rs = c->q_zero.ptr;
+        int* zi = reinterpret_cast<int*>(c->q_zero.ptr + (4 * (size_t)ncl + 15) / 16 * 16);
+        q.ctr = zi;
+        q.decided = zi + QC_COUNT;
+        q.posted = q.decided + (size_t)QC_DECIDED_STRIDE * ncl;
+        q.finished = q.posted + ncl;
+        q.rule = dec->rule;
+        q.warm_sweeps = dec->warm_sweeps;
+        q.full_sweeps = dec->full_sweeps;
+        q.keep = dec->keep;
+        q.cap = dec->cap;
+        q.pfinal = c->pfinal.ptr;
+        a.sweep_base = 0;
+        a.nsweeps = std::min(ts, iterations);
+        a.chunk = 0;
+        if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
+        launch_tiles_decide(s, walk_grid(c), c->tiles.ptr, c->tverts.ptr, c->tdist.ptr, c->tdist_rest.ptr,
+                            c->tbend.ptr, c->tbend_rest.ptr, c->town.ptr, c->town_rest.ptr, c->tcols.ptr, c->cl.ptr,
+                            c->ktab_d.ptr, c->ktab_b.ptr, c->x.ptr, c->v.ptr, c->res_slots.ptr, ncl, c->sweeps.ptr,
+                            cap, a, q, threads, lds, bend);
+        if (c->prof_on) PBD_HIP(hipEventRecord(next_event(c), s));
+        launched = nchunks;
+        if (nchunks & 1) cur ^= 1;
+    } else if (follow && c->sp.per_chunk_launches != 1) {
         // every chunk in one launch: workgroups walk a queue of (tile, chunk)
         // items built on the device after the keep decision
         ensure_queue(c);
@@ -1145,7 +1261,8 @@ bool project(pbd_ctx* c, int iterations, bool fused_whole, float dt, const int* 
     if (c->prof_on) {
         collect_events(c);
         if (delta) ws_read_used(c);
-        c->prof.tile_bytes += tile_projection_bytes(c, ts, fused_whole, int_store, delta != nullptr);
+        c->prof.tile_bytes +=
+            tile_projection_bytes(c, ts, fused_whole, int_store, delta != nullptr, fold && dec->in_launch);
         double bytes = 0.0;
         for (auto& k : c->cloths) {
             bytes += (double)k.last_sweeps * (kDistBytes * k.ne + kBendBytes * k.nb);
@@ -1266,7 +1383,7 @@ void step_ccd(pbd_ctx* c, float dt) {
             ran[ci] = c->cloths[ci].last_sweeps;
             c->cloths[ci].last_sweeps = 1;
         }
-        c->prof.tile_bytes += (double)I * tile_projection_bytes(c, 1, false, false, false);
+        c->prof.tile_bytes += (double)I * tile_projection_bytes(c, 1, false, false, false, false);
         for (int ci = 0; ci < ncl; ci++) {
             Cloth& k = c->cloths[ci];
             k.last_sweeps = ran[ci];
@@ -1763,8 +1880,8 @@ int pbd_set_solver_params(pbd_ctx* ctx, const pbd_solver_params* p) {
                   "unknown path");
         PBD_CHECK(p->jacobi_omega >= 0.0f && p->jacobi_omega < 2.0f, PBD_ERR_ARG, "jacobi_omega must be in [0, 2)");
         PBD_CHECK(p->split_whole == 0 || p->split_whole == 1, PBD_ERR_ARG, "split_whole must be 0 or 1");
-        PBD_CHECK(p->per_chunk_launches == 0 || p->per_chunk_launches == 1, PBD_ERR_ARG,
-                  "per_chunk_launches must be 0 or 1");
+        PBD_CHECK(p->per_chunk_launches >= 0 && p->per_chunk_launches <= 2, PBD_ERR_ARG,
+                  "per_chunk_launches must be 0, 1 or 2");
         PBD_CHECK(p->tile_sweeps >= 0 && p->tile_max_verts >= 0, PBD_ERR_ARG, "negative tile setting");
         PBD_CHECK(p->block_threads == 0 || (p->block_threads % 64 == 0 && p->block_threads <= 1024), PBD_ERR_ARG,
                   "block_threads must be a multiple of 64 up to 1024");
@@ -2547,11 +2664,13 @@ int pbd_profile_read(pbd_ctx* ctx, pbd_profile* out, int reset) {
         out->ccd_contacts = b.st_contacts;
         out->ccd_resolved = b.st_resolved;
         out->queue_wait_timeouts = 0;
+        out->queue_deciding_launches = 0;
         if (ctx->q_timeouts.n) {
-            int t = 0;
-            PBD_HIP(hipMemcpyAsync(&t, ctx->q_timeouts.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            int t[2] = {0, 0};
+            PBD_HIP(hipMemcpyAsync(t, ctx->q_timeouts.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             PBD_HIP(hipStreamSynchronize(ctx->stream));
-            out->queue_wait_timeouts = t;
+            out->queue_wait_timeouts = t[0];
+            out->queue_deciding_launches = t[1];
         }
         if (reset) {
             ctx->prof = pbd_profile{};

@@ -708,14 +708,6 @@ __global__ __launch_bounds__(64 * MLP_WAVES, 4) void k_ws_infer(
     }
 }
 
-// Per-prim relative residual |d - L| / L in FP32 (it only decides whether the
-// prediction is kept; the reductions are exact).
-__device__ __forceinline__ float rel_res(float4 a, float4 b, float L) {
-    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    float d = sqrtf(dx * dx + dy * dy + dz * dz);
-    return fabsf(d - L) / L;
-}
-
 // Residual of every distance prim with and without the predicted delta, per
 // cloth: the max (FP64 bit patterns, r >= 0) and the sum of squares in
 // 2^-40 fixed point (r^2 clamped to 1; integer sums do not depend on the
@@ -725,11 +717,6 @@ __device__ __forceinline__ float rel_res(float4 a, float4 b, float L) {
 // its cloth changes or at the end; a wave whose lanes all end on one cloth
 // reduces first.
 constexpr int RESID_PER_THREAD = 64;  // long runs per thread: few flushes (measured 8: 0.57 ms, 64: 0.30 ms)
-
-__device__ __forceinline__ unsigned long long sq_fixed(float r) {
-    const float q = fminf(r * r, 1.0f);
-    return (unsigned long long)((double)q * 1099511627776.0);  // 2^40
-}
 
 template <int RULE>
 __device__ __forceinline__ void flush_resid(unsigned long long* rs, int ncl, int c, double a, double b,

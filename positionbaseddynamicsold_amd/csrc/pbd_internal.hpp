@@ -81,6 +81,7 @@ struct ClothDev {
     int vbase, nv;       // vertex range in the packed arrays
     int ktab;            // offset of this cloth's per-sweep stiffness tables
     int whole;           // 1 if the cloth is a single LDS tile
+    int tile_begin, tile_end;  // its tiles' slots in the current plan
     float hscale;        // mean rest length (warm-start feature scale)
     double tol;          // residual tolerance of this cloth (< 0: the solver's)
 };
@@ -142,6 +143,10 @@ struct SolveArgs {
     int advect_load;     // 1: multi-tile cloths' tiles advect x, v on load (first launch only)
     int integrate_end;   // 1: a tile storing its cloth's last sweep also integrates (x, v); no tile of
                          //    a launch where this can happen reads x
+    int coherent_load;   // 1: p_in was written earlier in this launch by other workgroups (a queued
+                         //    later chunk): loaded past every cache
+    int queue_ts;        // queued launches: sweeps per chunk; a tile whose cloth has a further chunk
+                         //    stores p_out write-through for it (0: not queued)
 };
 
 // ---------------------------------------------------------------------------
@@ -300,6 +305,19 @@ void ws_infer(hipStream_t s, const void* W, const ClothDev* cl, const int* vclot
               const float4* vel, float dt, const int* nbr, const float* nbr_rest, int nv, float4* delta,
               float* feat_out, const float* norm, void* feat_scratch, float* hs_scratch, float gate,
               unsigned long long* rs_zero = nullptr, int n_rs = 0);
+// Per-prim relative residual |d - L| / L in FP32 (it only decides whether the
+// prediction is kept; the reductions over it are exact).
+__device__ __forceinline__ float rel_res(float4 a, float4 b, float L) {
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    return fabsf(d - L) / L;
+}
+// r^2 (clamped to 1) in 2^-40 fixed point: sums of these do not depend on the
+// order in which they are added.
+__device__ __forceinline__ unsigned long long sq_fixed(float r) {
+    const float q = fminf(r * r, 1.0f);
+    return (unsigned long long)((double)q * 1099511627776.0);  // 2^40
+}
 // The fallback rule on one cloth's residual sums rs = [max0 | max1 | sq0 | sq1]
 // (ncl entries each): keep the prediction iff its residual does not exceed
 // the unwarmed start's (rule 0: max |C|/L, bit patterns of non-negative
@@ -317,6 +335,7 @@ struct QueueDecide {
     int rule, warm_sweeps, full_sweeps;
     int* keep;
     int* cap;       // the budgets the decision writes (the projection's cap)
+    bool in_launch; // the queued tile launch computes the residuals and decides (rs unused)
 };
 size_t ws_feature_bytes(int nv);
 void ws_normalize(hipStream_t s, float* X, float* Y, long long n, const float* norm);
@@ -426,7 +445,55 @@ struct TileQueue {
     int ts;                 // sweeps per chunk
     int iterations;
     int* timeouts;          // waits that gave up (bounded spins; 0 when every wait ended normally)
+    // The warm start's keep decision made inside the launch (launch_tiles_decide).
+    // Every tile's first chunk is a fixed item (stat_item: the single-tile
+    // cloths' tiles, then the multi-tile cloths' tiles cloth by cloth); the
+    // later chunks of a cloth whose budget outlasts the first are appended
+    // (dyn_tile / dyn_chunk) by the tile that completes its cloth's decision.
+    // Everything below except the lists is zero when the launch starts.
+    const int* stat_item;
+    int n_static;
+    int n_multi;            // multi-tile cloths: each is decided once
+    int* dyn_tile;
+    int* dyn_chunk;
+    int* ctr;               // [next fixed item, next appended item, appended reserved, appended published,
+                            //  multi-tile cloths decided]
+    int* posted;            // per cloth: tiles whose residual sums are in
+    int* decided;           // per cloth c at [c * QC_DECIDED_STRIDE]: 0 until decided, then 1 + keep
+    int* finished;          // per tile: chunks finished (chunk c done: c + 1)
+    unsigned long long* rs; // per cloth: [max0 | max1 | sq0 | sq1] as ws_keep reads them
+    int rule, warm_sweeps, full_sweeps;
+    int* keep;              // written per cloth by the decision, with cap, pfinal and a zeroed sweeps
+    int* cap;
+    int* pfinal;
+    int* launches;          // incremented once per deciding launch
 };
+// Counter slots of TileQueue::ctr, one 128-byte line each (the workgroups'
+// polls of one do not stall the atomics on another), and the spacing of the
+// per-cloth decision flags for the same reason.
+enum {
+    QC_STAT = 0,
+    QC_DYN_HEAD = 32,
+    QC_DYN_TAIL = 64,
+    QC_DYN_READY = 96,
+    QC_DECIDED = 128,
+    QC_ABORT = 160,
+    QC_COUNT = 192,
+    QC_DECIDED_STRIDE = 32
+};
+// The queued launch with the keep decision inside it: a tile's first chunk
+// loads the unwarmed start of its cone, adds its owned distance prims' relative
+// residuals (with and without the prediction) to its cloth's sums, waits
+// until every tile of the cloth has done so, applies the fallback rule, and
+// goes on from the kept prediction or from the unwarmed start.  A cloth's
+// tiles wait for each other, so the launch needs every multi-tile cloth's
+// tiles resident at once (at most nblocks - 1 tiles per cloth).
+void launch_tiles_decide(hipStream_t s, int nblocks, const TileDev* tiles, const int* verts, const uint32_t* dist,
+                         const float* dist_rest, const uint32_t* bend, const float* bend_rest, const uint32_t* own,
+                         const float* own_rest, const int* cols, const ClothDev* cloths, const float* ktab_d,
+                         const float* ktab_b, float4* x, float4* v, unsigned long long* res_slots, int ncl,
+                         int* sweeps_out, const int* sweep_cap, const SolveArgs& a, const TileQueue& q, int threads,
+                         size_t lds_bytes, bool with_bend);
 void launch_tiles_queue(hipStream_t s, int nblocks, const TileDev* tiles, const int* verts, const uint32_t* dist,
                         const float* dist_rest, const uint32_t* bend, const float* bend_rest, const uint32_t* own,
                         const float* own_rest, const int* cols, const ClothDev* cloths, const float* ktab_d,

@@ -461,15 +461,226 @@ __device__ __forceinline__ double own_residual(const float4* P, const uint32_t* 
     order, tiles, verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths, ktab_d, ktab_b, x, v, p_in, \
         p_out, res_slots, ncl, sweeps_out, sweep_cap, a
 
+// The hand-off of positions between chunks of one queued launch, whose
+// workgroups may sit on different XCDs (each with its own L2): the tile that
+// publishes a chunk writes its interior through to memory (store_coherent),
+// every wave waits for its stores to complete, then one thread stores the
+// flag; the next chunk loads its cone past the caches (load_coherent).  No
+// cache-wide write-back or invalidation is needed, so the other work of the
+// launch keeps its L2.  (An agent-scope acquire in one wave plus ordinary
+// loads measured rare stale halo values.)
+__device__ __forceinline__ float4 load_coherent(const float4* p) {
+    const unsigned long long* w = reinterpret_cast<const unsigned long long*>(p);
+    const unsigned long long lo = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const unsigned long long hi = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return make_float4(__uint_as_float((unsigned)lo), __uint_as_float((unsigned)(lo >> 32)),
+                       __uint_as_float((unsigned)hi), __uint_as_float((unsigned)(hi >> 32)));
+}
+__device__ __forceinline__ void store_coherent(float4* p, float4 q) {
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(p);
+    const unsigned long long lo = (unsigned long long)__float_as_uint(q.x) | ((unsigned long long)__float_as_uint(q.y) << 32);
+    const unsigned long long hi = (unsigned long long)__float_as_uint(q.z) | ((unsigned long long)__float_as_uint(q.w) << 32);
+    __hip_atomic_store(w, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(w + 1, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Loads P[0, n) = p_in at the tile's loaded vertices vl (VERT_PREFETCH per
+// thread per batch: indices first, then the gathers).
+template <int VERT_PREFETCH>
+__device__ __forceinline__ void tile_load_p(float4* P, const int* __restrict__ vl, const float4* __restrict__ p_in,
+                                            int n) {
+    const int T = blockDim.x;
+    for (int i0 = threadIdx.x; i0 < n; i0 += VERT_PREFETCH * T) {
+        int g[VERT_PREFETCH];
+        float4 q[VERT_PREFETCH];
+#pragma unroll
+        for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < n ? vl[i0 + u * T] : -1;
+#pragma unroll
+        for (int u = 0; u < VERT_PREFETCH; u++) q[u] = p_in[max(g[u], 0)];
+#pragma unroll
+        for (int u = 0; u < VERT_PREFETCH; u++)
+            if (g[u] >= 0) P[i0 + u * T] = q[u];
+    }
+}
+
+// This thread's share of the owned distance prims' relative residuals on P:
+// the max and the sum of squares in fixed point (the fallback rule's two
+// measures, computed as the separate residual kernel computes them).
+template <int PF>
+__device__ __forceinline__ void own_resid_parts(const float4* P, const uint32_t* __restrict__ own,
+                                                const float* __restrict__ rest, int n, double* m,
+                                                unsigned long long* sq) {
+    const int T = blockDim.x;
+    for (int j0 = threadIdx.x; j0 < n; j0 += PF * T) {
+        uint32_t pr[PF];
+        float rl[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int j = j0 + u * T;
+            pr[u] = j < n ? own[j] : 0u;
+            rl[u] = j < n ? rest[j] : 1.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (j0 + u * T < n) {
+                const float r = rel_res(P[pr[u] & 0xffffu], P[pr[u] >> 16], rl[u]);
+                *m = fmax(*m, (double)r);
+                *sq += sq_fixed(r);
+            }
+    }
+}
+
+constexpr int QUEUE_SPIN = 1 << 22;
+constexpr int DECIDE_SPIN = 1 << 20;
+
+// A bounded wait gave up: count it and raise the launch's abort flag, so
+// that every other wait of the launch ends at once instead of timing out
+// in turn (the results are then wrong, and the count says so).
+__device__ __forceinline__ void queue_give_up(const TileQueue& q) {
+    atomicAdd(q.timeouts, 1);
+    __hip_atomic_store(&q.ctr[QC_ABORT], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ bool queue_aborted(const TileQueue& q) {
+    return __hip_atomic_load(&q.ctr[QC_ABORT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+}
+
+// The cloth's budget, keep flag, final p buffer and a zeroed sweep count,
+// written through to memory by the tile that decides and complete before it
+// publishes the decision (other tiles of the launch read the budget and add
+// to the sweep count after seeing it).
+__device__ __forceinline__ void record_decision(const TileQueue& q, const ClothDev& cd, int c, bool k,
+                                                int* sweeps_out) {
+    const int b = k ? q.warm_sweeps : q.full_sweeps;
+    const int run = (min(b, q.iterations) + q.ts - 1) / q.ts;
+    __hip_atomic_store(&q.keep[c], k ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&q.cap[c], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&q.pfinal[c], cd.whole ? (q.start ^ 1) : (q.start ^ (run & 1)), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&sweeps_out[c], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
+// The keep decision of the tile's cloth from this tile's residual parts
+// (m0, sq0: unwarmed start; m1, sq1: with the prediction).  A single-tile
+// cloth decides alone.  A multi-tile cloth's tiles add their sums and count
+// themselves in; the last one applies the rule, records and publishes it
+// and appends the cloth's later chunks (chunk-major, so that an item waits
+// only on items before it); the others wait for the publication.  Every
+// thread of the workgroup calls this; all get the same answer.
+__device__ __forceinline__ bool cloth_decide(const TileQueue& q, const TileDev& td, const ClothDev& cd, int ncl,
+                                             int* sweeps_out, double m0, double m1, unsigned long long sq0,
+                                             unsigned long long sq1) {
+    __shared__ unsigned long long acc[4];
+    __shared__ int s_flag, s_base;
+    const int c = td.cloth;
+    if (threadIdx.x == 0) acc[0] = acc[1] = acc[2] = acc[3] = 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmax(m0, __shfl_xor(m0, off));
+        m1 = fmax(m1, __shfl_xor(m1, off));
+        sq0 += __shfl_xor(sq0, off);
+        sq1 += __shfl_xor(sq1, off);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        // non-negative doubles order as their bit patterns
+        atomicMax(&acc[0], (unsigned long long)__double_as_longlong(m0));
+        atomicMax(&acc[1], (unsigned long long)__double_as_longlong(m1));
+        atomicAdd(&acc[2], sq0);
+        atomicAdd(&acc[3], sq1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int flag;
+        if (td.whole) {
+            const bool k = q.rule == 1 ? acc[3] <= acc[2]
+                                       : __longlong_as_double((long long)acc[1]) <= __longlong_as_double((long long)acc[0]);
+            record_decision(q, cd, c, k, sweeps_out);
+            flag = k ? 1 : 0;
+        } else {
+            if (q.rule == 1) {
+                atomicAdd(&q.rs[2 * ncl + c], acc[2]);
+                atomicAdd(&q.rs[3 * ncl + c], acc[3]);
+            } else {
+                atomicMax(&q.rs[c], acc[0]);
+                atomicMax(&q.rs[ncl + c], acc[1]);
+            }
+            const int T = cd.tile_end - cd.tile_begin;
+            __builtin_amdgcn_s_waitcnt(0);  // the sums are in before the tile counts itself in
+            const int prev = __hip_atomic_fetch_add(&q.posted[c], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (prev == T - 1) {
+                unsigned long long r[4];
+                for (int k = 0; k < 4; k++)
+                    r[k] = __hip_atomic_load(&q.rs[k * ncl + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                const bool k = q.rule == 1 ? r[3] <= r[2]
+                                           : __longlong_as_double((long long)r[1]) <= __longlong_as_double((long long)r[0]);
+                record_decision(q, cd, c, k, sweeps_out);
+                __hip_atomic_store(&q.decided[c * QC_DECIDED_STRIDE], k ? 2 : 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                flag = (k ? 1 : 0) | 2;
+            } else {
+                int d = 0, spins = 0;
+                while ((d = __hip_atomic_load(&q.decided[c * QC_DECIDED_STRIDE], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_SYSTEM)) == 0) {
+                    if (++spins > DECIDE_SPIN || ((spins & 63) == 0 && queue_aborted(q))) {
+                        queue_give_up(q);
+                        d = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                flag = d - 1;
+            }
+        }
+        s_flag = flag;
+    }
+    __syncthreads();
+    const int flag = s_flag;
+    if (flag & 2) {
+        // this tile completed the decision: append the cloth's later chunks
+        const int T = cd.tile_end - cd.tile_begin;
+        const int b = (flag & 1) ? q.warm_sweeps : q.full_sweeps;
+        const int more = (min(b, q.iterations) + q.ts - 1) / q.ts - 1;
+        const int m = T * max(more, 0);
+        if (m > 0) {
+            if (threadIdx.x == 0) s_base = atomicAdd(&q.ctr[QC_DYN_TAIL], m);
+            __syncthreads();
+            const int base = s_base;
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                __hip_atomic_store(&q.dyn_tile[base + i], cd.tile_begin + i % T, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(&q.dyn_chunk[base + i], 1 + i / T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            __builtin_amdgcn_s_waitcnt(0);  // written through and complete before the count is published
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                // publish in reservation order: the published count is a prefix
+                int spins = 0;
+                while (__hip_atomic_load(&q.ctr[QC_DYN_READY], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != base) {
+                    if (++spins > DECIDE_SPIN || ((spins & 63) == 0 && queue_aborted(q))) {
+                        queue_give_up(q);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __hip_atomic_store(&q.ctr[QC_DYN_READY], base + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(&q.ctr[QC_DECIDED], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    return (flag & 1) != 0;
+}
+
 // The tile kernel's body; BEND = false leaves the bending phases out (a plan
-// with no bending prims).
-template <bool BEND>
-__device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, double* red) {
+// with no bending prims).  DECIDE: a first chunk makes the warm start's keep
+// decision itself (qd, see cloth_decide), and every budget is read from qd
+// (the launch writes it; sweep_cap is unused).  Returns the cloth's sweep
+// budget.  slot: the tile (< 0: none).
+template <bool BEND, bool DECIDE = false>
+__device__ __forceinline__ int tiles_body(TILE_PARAMS, int slot, float4* P, double* red,
+                                          const TileQueue* qd = nullptr) {
     constexpr int PREFETCH = TileFetch<BEND>::dist;
     constexpr int BEND_PREFETCH = TileFetch<BEND>::bend;
     constexpr int VERT_PREFETCH = TileFetch<BEND>::vert;
-    int slot = order[item];
-    if (slot < 0) return;
+    if (slot < 0) return 0;
     const TileDev td = tiles[slot];
     const ClothDev& cd = cloths[td.cloth];
     const int T = blockDim.x, tid = threadIdx.x;
@@ -478,8 +689,17 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, dou
     // per-cloth sweep budget (warm start: short when the prediction was kept);
     // a cloth whose budget is spent is left where it is (its final buffer is
     // tracked per cloth), one that met the tolerance is carried forward
+    const bool deciding = DECIDE && a.chunk == 0;
     int nsweeps = td.whole ? a.whole_nsweeps : a.nsweeps;
-    if (sweep_cap) nsweeps = min(nsweeps, sweep_cap[td.cloth] - a.sweep_base);
+    int budget_c = 0x7fffffff;
+    if (DECIDE && !deciding) {
+        // written earlier in this launch by the cloth's decision
+        budget_c = __hip_atomic_load(&qd->cap[td.cloth], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        nsweeps = min(nsweeps, budget_c - a.sweep_base);
+    } else if (!DECIDE && sweep_cap) {
+        budget_c = sweep_cap[td.cloth];
+        nsweeps = min(nsweeps, budget_c - a.sweep_base);
+    }
     const double tol = cd.tol >= 0.0 ? cd.tol : a.tol;
     bool carry = false;
     if (!td.whole && a.chunk > 0 && tol >= 0.0) {
@@ -488,7 +708,7 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, dou
     }
     if (!td.whole && nsweeps <= 0) {
         if (a.copy_spent) carry = true;
-        else return;
+        else return budget_c;
     }
     // Gathers and scatters below go in batches of VERT_PREFETCH per thread:
     // indices first, then the dependent accesses, so each batch waits for
@@ -505,7 +725,7 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, dou
             for (int u = 0; u < VERT_PREFETCH; u++)
                 if (g[u] >= 0) p_out[g[u]] = q[u];
         }
-        return;
+        return budget_c;
     }
     const int C = td.ncolors;
     const int* dc = cols + td.col_off;
@@ -517,11 +737,51 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, dou
     const int* dn = bc + C + 1;
     const int* bn = dn + td.plan_sweeps * C;
     const int* ln = bn + td.plan_sweeps * C;
+    auto cone = [&](int n) { return (td.plan_sweeps && n < td.plan_sweeps) ? ln[n - 1] : td.n_load; };
+    bool loaded = false;
+    if (deciding) {
+        // the unwarmed start of the kept budget's cone, its residuals, the
+        // prediction added in LDS, the residuals again; then the decision
+        const TileQueue& q = *qd;
+        const int nk = min(nsweeps, q.warm_sweeps), nd = min(nsweeps, q.full_sweeps);
+        const int lk = cone(nk);
+        tile_load_p<VERT_PREFETCH>(P, vl, p_in, lk);
+        __syncthreads();
+        double m0 = 0.0, m1 = 0.0;
+        unsigned long long sq0 = 0ull, sq1 = 0ull;
+        own_resid_parts<PREFETCH>(P, own + td.own_off, own_rest + td.own_off, td.n_own, &m0, &sq0);
+        __syncthreads();  // every read of the unwarmed P done before the prediction goes in
+        for (int i0 = tid; i0 < lk; i0 += VERT_PREFETCH * T) {
+            int g[VERT_PREFETCH];
+            float4 dq[VERT_PREFETCH];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < lk ? vl[i0 + u * T] : -1;
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) dq[u] = a.delta[max(g[u], 0)];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) {
+                    const float4 q0 = P[i0 + u * T];
+                    P[i0 + u * T] = make_float4(q0.x + dq[u].x, q0.y + dq[u].y, q0.z + dq[u].z, q0.w);
+                }
+        }
+        __syncthreads();
+        own_resid_parts<PREFETCH>(P, own + td.own_off, own_rest + td.own_off, td.n_own, &m1, &sq1);
+        // (its barriers come after every read of P above)
+        const bool k = cloth_decide(q, td, cd, ncl, sweeps_out, m0, m1, sq0, sq1);
+        budget_c = k ? q.warm_sweeps : q.full_sweeps;
+        nsweeps = k ? nk : nd;
+        if (!k) {
+            // discarded: the unwarmed start again, over the full budget's cone
+            tile_load_p<VERT_PREFETCH>(P, vl, p_in, cone(nd));
+        }
+        loaded = true;
+    }
     const int shift = max(0, td.plan_sweeps - nsweeps);
-    const int n_load = (td.plan_sweeps && nsweeps < td.plan_sweeps) ? ln[nsweeps - 1] : td.n_load;
+    const int n_load = cone(nsweeps);
     // the kept warm-start prediction enters here, on the substep's first load
-    const bool add_delta = a.delta != nullptr && a.keep[td.cloth] != 0;
-    for (int i0 = tid; i0 < n_load; i0 += VERT_PREFETCH * T) {
+    const bool add_delta = !loaded && a.delta != nullptr && a.keep[td.cloth] != 0;
+    for (int i0 = tid; !loaded && i0 < n_load; i0 += VERT_PREFETCH * T) {
         int g[VERT_PREFETCH];
 #pragma unroll
         for (int u = 0; u < VERT_PREFETCH; u++) g[u] = i0 + u * T < n_load ? vl[i0 + u * T] : -1;
@@ -548,6 +808,13 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, dou
 #pragma unroll
             for (int u = 0; u < VERT_PREFETCH; u++)
                 if (g[u] >= 0) P[i0 + u * T] = make_float4(q[u].x + dq[u].x, q[u].y + dq[u].y, q[u].z + dq[u].z, q[u].w);
+        } else if (a.coherent_load) {
+            float4 q[VERT_PREFETCH];
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++) q[u] = load_coherent(p_in + max(g[u], 0));
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) P[i0 + u * T] = q[u];
         } else {
             float4 q[VERT_PREFETCH];
 #pragma unroll
@@ -647,8 +914,10 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, dou
     }
     // integrate on store: a fused whole cloth, or the launch holding this
     // cloth's last sweep when the caller allows it (fixed budgets only)
-    const int budget = sweep_cap ? min(sweep_cap[td.cloth], a.whole_nsweeps) : a.whole_nsweeps;
+    const int budget = min(budget_c, a.whole_nsweeps);
     const bool integ = (td.whole && a.fused) || (a.integrate_end && a.sweep_base + nsweeps >= budget);
+    // a queued chunk with a further chunk of its cloth after it hands its stores on
+    const bool handoff = a.queue_ts > 0 && !td.whole && budget_c > a.sweep_base + a.queue_ts;
     for (int i0 = tid; i0 < td.n_int; i0 += VERT_PREFETCH * T) {
         int g[VERT_PREFETCH];
 #pragma unroll
@@ -665,12 +934,17 @@ __device__ __forceinline__ void tiles_body(TILE_PARAMS, int item, float4* P, dou
                     x[g[u]] = make_float4(Q.x, Q.y, Q.z, X[u].w);
                     p_out[g[u]] = Q;
                 }
+        } else if (handoff) {
+#pragma unroll
+            for (int u = 0; u < VERT_PREFETCH; u++)
+                if (g[u] >= 0) store_coherent(p_out + g[u], P[i0 + u * T]);
         } else {
 #pragma unroll
             for (int u = 0; u < VERT_PREFETCH; u++)
                 if (g[u] >= 0) p_out[g[u]] = P[i0 + u * T];
         }
     }
+    return budget_c;
 }
 
 // One tile per workgroup (n_items == nullptr: order[blockIdx.x]), or a
@@ -683,7 +957,7 @@ __device__ __forceinline__ void tiles_walk(TILE_PARAMS, const int* __restrict__ 
     // one loop for both forms, so the body is inlined once
     const int n = n_items ? *n_items : (int)gridDim.x;
     for (int it = blockIdx.x; it < n; it += gridDim.x) {
-        tiles_body<BEND>(TILE_ARGS, it, P, red);
+        tiles_body<BEND>(TILE_ARGS, order[it], P, red);
         __syncthreads();  // the tile's stores have read LDS before the next tile loads
     }
 }
@@ -773,10 +1047,10 @@ __global__ __launch_bounds__(512) void k_follow_list(const int* __restrict__ ord
 // whatever the number of resident workgroups: an item waits only for items
 // taken before it, and a taken item is held by a running workgroup, so the
 // oldest unfinished item can always proceed.  A tile whose cloth has a
-// further chunk publishes its chunk after its stores (release at agent
-// scope, so that workgroups on other XCDs see them).  A wait gives up after
-// QUEUE_SPIN polls and counts a timeout instead of hanging.
-constexpr int QUEUE_SPIN = 1 << 22;
+// further chunk hands its chunk on as load_coherent describes (write-through
+// stores, then the flag; the next chunk polls the flags and loads past the
+// caches).  A wait gives up after QUEUE_SPIN polls and counts a timeout
+// instead of hanging.
 
 template <bool BEND>
 __device__ __forceinline__ void tiles_queue(TILE_PARAMS, const TileQueue& q, float4* P, double* red) {
@@ -796,11 +1070,11 @@ __device__ __forceinline__ void tiles_queue(TILE_PARAMS, const TileQueue& q, flo
             continue;
         }
         if (c > 0) {
-            // poll relaxed (no cache invalidation per poll), then one acquire
+            // poll relaxed; the chunk then loads its cone past the caches
             const int e = q.nbr_off[slot + 1];
             for (int k = q.nbr_off[slot] + (int)threadIdx.x; k < e; k += blockDim.x) {
                 int spins = 0;
-                while (__hip_atomic_load(&q.done[q.nbr[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < c - 1) {
+                while (__hip_atomic_load(&q.done[q.nbr[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < c - 1) {
                     if (++spins > QUEUE_SPIN) {
                         atomicAdd(q.timeouts, 1);
                         break;
@@ -809,28 +1083,27 @@ __device__ __forceinline__ void tiles_queue(TILE_PARAMS, const TileQueue& q, flo
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // invalidates the CU's caches
-            __syncthreads();
         }
         SolveArgs b = a;
         b.sweep_base = c * q.ts;
         b.nsweeps = min(q.ts, q.iterations - b.sweep_base);
         b.chunk = c;
+        b.queue_ts = q.ts;
         if (c > 0) {
             b.delta = nullptr;
             b.keep = nullptr;
             b.advect_load = 0;
+            b.coherent_load = 1;
         }
         const bool odd = ((q.start ^ c) & 1) != 0;
         tiles_body<BEND>(q.item_tile, tiles, verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths,
                          ktab_d, ktab_b, x, v, odd ? q.pbuf1 : q.pbuf0, odd ? q.pbuf0 : q.pbuf1, res_slots, ncl,
-                         sweeps_out, sweep_cap, b, idx, P, red);
+                         sweeps_out, sweep_cap, b, slot, P, red);
         const bool more = !td.whole && sweep_cap && sweep_cap[td.cloth] > (c + 1) * q.ts;
-        // every thread's stores complete (s_waitcnt 0), then one device-wide
-        // release for all of them (the L2 write-back is cache-wide)
+        // every wave's write-through stores complete, then one thread publishes
         if (more) __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();  // also: s_item read by all before the next take
-        if (more && threadIdx.x == 0) __hip_atomic_store(&q.done[slot], c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (more && threadIdx.x == 0) __hip_atomic_store(&q.done[slot], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -844,6 +1117,148 @@ __global__ __launch_bounds__(1024) void k_tiles_queue_bend(TILE_PARAMS, TileQueu
     extern __shared__ float4 P[];
     __shared__ double red[16];
     tiles_queue<true>(TILE_ARGS, q, P, red);
+}
+
+// The queued launch that decides (see cloth_decide): a workgroup takes an
+// appended later chunk when one is published, else the next fixed first
+// chunk; with neither left it waits until every multi-tile cloth is decided
+// and all appended items are taken.  Deadlock-free when every multi-tile
+// cloth has fewer tiles than the resident workgroups: first chunks are
+// taken in cloth order, so at most one cloth is partly taken, its waiting
+// tiles leave a workgroup free for its remaining ones, and a later chunk
+// waits only for items taken before it.
+template <bool BEND>
+__device__ __forceinline__ void tiles_decide_walk(TILE_PARAMS, const TileQueue& q, float4* P, double* red) {
+    __shared__ int s_slot, s_chunk;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(q.launches, 1);
+    for (;;) {
+        if (threadIdx.x < 64) {
+            // wave 0 takes the item; lane 0 touches the counters
+            const int lane = threadIdx.x;
+            int slot = -1, ch = 0, spins = 0;
+            for (;;) {
+                int h = 0, r = 0, st = 0;
+                if (lane == 0) {
+                    h = __hip_atomic_load(&q.ctr[QC_DYN_HEAD], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    r = __hip_atomic_load(&q.ctr[QC_DYN_READY], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    st = __hip_atomic_load(&q.ctr[QC_STAT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+                h = __shfl(h, 0);
+                r = __shfl(r, 0);
+                const bool stat_left = __shfl(st, 0) < q.n_static;
+                if (h < r) {
+                    // the next appended item (written through before its count);
+                    // while first chunks remain it is taken only if it can run
+                    // at once, so that no workgroup sits on a chain meanwhile
+                    const int hs = __hip_atomic_load(&q.dyn_tile[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    const int hc = __hip_atomic_load(&q.dyn_chunk[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    bool ok = true;
+                    if (stat_left) {
+                        const int e = q.nbr_off[hs + 1];
+                        for (int k = q.nbr_off[hs] + lane; k < e; k += 64)
+                            ok = ok && __hip_atomic_load(&q.finished[q.nbr[k]], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_SYSTEM) >= hc;
+                        ok = __all(ok);
+                    }
+                    if (ok) {
+                        int got = 0;
+                        if (lane == 0) got = atomicCAS(&q.ctr[QC_DYN_HEAD], h, h + 1) == h;
+                        if (__shfl(got, 0)) {
+                            slot = hs;
+                            ch = hc;
+                            break;
+                        }
+                        continue;
+                    }
+                }
+                if (stat_left) {
+                    int k = 0;
+                    if (lane == 0) k = atomicAdd(&q.ctr[QC_STAT], 1);
+                    k = __shfl(k, 0);
+                    if (k < q.n_static) {
+                        slot = q.stat_item[k];
+                        break;
+                    }
+                    continue;
+                }
+                // nothing to take now; done once every multi-tile cloth has
+                // decided (then every appended item is published) and taken
+                int fin = 0;
+                if (lane == 0 &&
+                    __hip_atomic_load(&q.ctr[QC_DECIDED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == q.n_multi)
+                    fin = __hip_atomic_load(&q.ctr[QC_DYN_HEAD], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >=
+                                  __hip_atomic_load(&q.ctr[QC_DYN_READY], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
+                              ? 1
+                              : 2;
+                fin = __shfl(fin, 0);
+                if (fin == 1) break;
+                if (fin == 2) continue;
+                int stop = 0;
+                if (lane == 0 && (++spins > QUEUE_SPIN || ((spins & 63) == 0 && queue_aborted(q)))) {
+                    queue_give_up(q);
+                    stop = 1;
+                }
+                if (__shfl(stop, 0)) break;
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (lane == 0) {
+                s_slot = slot;
+                s_chunk = ch;
+            }
+        }
+        __syncthreads();
+        const int slot = s_slot, c = s_chunk;
+        if (slot < 0) break;
+        if (c > 0) {
+            // the neighbour tiles' previous chunk (polled relaxed; the chunk then
+            // loads its cone past the caches)
+            const int e = q.nbr_off[slot + 1];
+            for (int k = q.nbr_off[slot] + (int)threadIdx.x; k < e; k += blockDim.x) {
+                int spins = 0;
+                while (__hip_atomic_load(&q.finished[q.nbr[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < c) {
+                    if (++spins > QUEUE_SPIN || ((spins & 63) == 0 && queue_aborted(q))) {
+                        queue_give_up(q);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __syncthreads();
+        }
+        SolveArgs b = a;
+        b.sweep_base = c * q.ts;
+        b.nsweeps = min(q.ts, q.iterations - b.sweep_base);
+        b.chunk = c;
+        b.queue_ts = q.ts;
+        if (c > 0) {
+            b.delta = nullptr;
+            b.keep = nullptr;
+            b.advect_load = 0;
+            b.coherent_load = 1;
+        }
+        const bool odd = ((q.start ^ c) & 1) != 0;
+        const int budget = tiles_body<BEND, true>(order, tiles, verts, dist, dist_rest, bend, bend_rest, own,
+                                                  own_rest, cols, cloths, ktab_d, ktab_b, x, v,
+                                                  odd ? q.pbuf1 : q.pbuf0, odd ? q.pbuf0 : q.pbuf1, res_slots, ncl,
+                                                  sweeps_out, nullptr, b, slot, P, red, &q);
+        const bool more = !tiles[slot].whole && budget > (c + 1) * q.ts;
+        if (more) __builtin_amdgcn_s_waitcnt(0);  // every wave's write-through stores complete
+        __syncthreads();  // also: s_slot, s_chunk read by all before the next take
+        if (more && threadIdx.x == 0)
+            __hip_atomic_store(&q.finished[slot], c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+__global__ __launch_bounds__(1024, 8) void k_tiles_decide(TILE_PARAMS, TileQueue q) {
+    extern __shared__ float4 P[];
+    __shared__ double red[16];
+    tiles_decide_walk<false>(TILE_ARGS, q, P, red);
+}
+
+__global__ __launch_bounds__(1024) void k_tiles_decide_bend(TILE_PARAMS, TileQueue q) {
+    extern __shared__ float4 P[];
+    __shared__ double red[16];
+    tiles_decide_walk<true>(TILE_ARGS, q, P, red);
 }
 
 // Block-wide exclusive prefix of a 0/1 flag over up to 1024 threads (16
@@ -956,7 +1371,8 @@ __global__ __launch_bounds__(1024) void k_queue_schedule_decide(
 size_t tiles_lds_optin(size_t bytes) {
     if (bytes <= 65536) return bytes;
     for (const void* k : {(const void*)k_tiles, (const void*)k_tiles_bend, (const void*)k_tiles_queue,
-                          (const void*)k_tiles_queue_bend}) {
+                          (const void*)k_tiles_queue_bend, (const void*)k_tiles_decide,
+                          (const void*)k_tiles_decide_bend}) {
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();  // the refusal is handled by the smaller limit
@@ -990,6 +1406,22 @@ void launch_tiles_queue(hipStream_t s, int nblocks, const TileDev* tiles, const 
     // queue carries the items and the two buffers
     hipLaunchKernelGGL(with_bend ? k_tiles_queue_bend : k_tiles_queue, dim3(nblocks), dim3(threads), lds_bytes, s,
                        q.item_tile, tiles, verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths,
+                       ktab_d, ktab_b, x, v, (const float4*)q.pbuf0, q.pbuf1, res_slots, ncl, sweeps_out, sweep_cap,
+                       a, q);
+    PBD_HIP(hipGetLastError());
+}
+
+void launch_tiles_decide(hipStream_t s, int nblocks, const TileDev* tiles, const int* verts, const uint32_t* dist,
+                         const float* dist_rest, const uint32_t* bend, const float* bend_rest, const uint32_t* own,
+                         const float* own_rest, const int* cols, const ClothDev* cloths, const float* ktab_d,
+                         const float* ktab_b, float4* x, float4* v, unsigned long long* res_slots, int ncl,
+                         int* sweeps_out, const int* sweep_cap, const SolveArgs& a, const TileQueue& q, int threads,
+                         size_t lds_bytes, bool with_bend) {
+    if (nblocks <= 0) return;
+    // order, p_in and p_out of the common parameter list are unused: the
+    // queue carries the items and the two buffers
+    hipLaunchKernelGGL(with_bend ? k_tiles_decide_bend : k_tiles_decide, dim3(nblocks), dim3(threads), lds_bytes, s,
+                       q.stat_item, tiles, verts, dist, dist_rest, bend, bend_rest, own, own_rest, cols, cloths,
                        ktab_d, ktab_b, x, v, (const float4*)q.pbuf0, q.pbuf1, res_slots, ncl, sweeps_out, sweep_cap,
                        a, q);
     PBD_HIP(hipGetLastError());

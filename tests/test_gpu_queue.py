@@ -1,12 +1,15 @@
-"""The warm start's one-launch tile work (per_chunk_launches = 0: workgroups
-walk a device-built queue of (tile, chunk) items, a tile's later chunk
-waiting only for its neighbour tiles' previous chunk) against the same
-substeps run as one launch per chunk: bit-identical states on an uneven
-batch with kept and discarded multi-tile cloths, at several chunk lengths
-and warm budgets (including a budget longer than one chunk, so that kept
-cloths also have later chunks), graph replay and eager, and no wait of the
-queued launch ending on its bounded spin instead of the neighbour's
-chunk."""
+"""The warm start's one-launch tile work against the same substeps run as
+one launch per chunk (per_chunk_launches = 1).  Two queued forms: 0, where
+a separate residual kernel decides and a device-built queue follows, and 2,
+where the launch itself computes the fallback residuals and keeps or
+discards each cloth's prediction (fixed first-chunk items, later chunks
+appended by the tile that completes its cloth's decision).  In both, workgroups take
+(tile, chunk) items and a tile's later chunk waits only for its neighbour
+tiles' previous chunk.  Checked: bit-identical states and keep decisions on
+an uneven batch with kept and discarded multi-tile cloths, with and without
+bending prims, at several chunk lengths and warm budgets (including a budget
+longer than one chunk, so that kept cloths also have later chunks), graph
+replay and eager; the deciding form ran; no bounded wait gave up."""
 import numpy as np
 import pytest
 
@@ -54,13 +57,18 @@ def run(ctx, per_chunk, ts, warm, graph, max_verts, steps=6):
 
 @pytest.mark.parametrize("ts,warm,max_verts", [(2, 1, 2048), (1, 1, 1024), (2, 3, 2048), (3, 1, 0)])
 @pytest.mark.parametrize("graph", [1, 0])
-def test_queued_launch_equals_per_chunk_launches(batch, ts, warm, max_verts, graph):
+@pytest.mark.parametrize("form", [0, 2])
+def test_queued_launch_equals_per_chunk_launches(batch, form, ts, warm, max_verts, graph):
     max_verts *= batch.budget_scale
-    print(f"queue vs per-chunk: ts {ts} warm {warm} max_verts {max_verts} graph {graph}", flush=True)
+    print(f"queue form {form} vs per-chunk: ts {ts} warm {warm} max_verts {max_verts} graph {graph}", flush=True)
     x0, v0, k0 = run(batch, 1, ts, warm, graph, max_verts)
-    x1, v1, k1 = run(batch, 0, ts, warm, graph, max_verts)
-    prof = batch.profile_read(reset=False)  # the timeout count is read from the device either way
-    print(f"  kept per substep {k1}, queue wait timeouts {prof.queue_wait_timeouts}", flush=True)
+    before = batch.profile_read(reset=False).queue_deciding_launches  # counted on the device
+    x1, v1, k1 = run(batch, form, ts, warm, graph, max_verts)
+    prof = batch.profile_read(reset=False)
+    deciding = prof.queue_deciding_launches - before
+    print(f"  kept per substep {k1}, deciding launches {deciding}, queue wait timeouts {prof.queue_wait_timeouts}",
+          flush=True)
+    assert deciding == (6 if form == 2 else 0)
     assert k0 == k1
     assert 0 < min(k1) and max(k1) < 40  # both kept and discarded cloths
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
