@@ -544,6 +544,15 @@ __device__ __forceinline__ bool queue_aborted(const TileQueue& q) {
     return __hip_atomic_load(&q.ctr[QC_ABORT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
 }
 
+// Pause between two polls of a flag that bypass the caches, doubling from
+// about 0.25 to 4 microseconds: with hundreds of workgroups waiting at once,
+// polling flat out floods the memory lines that hold the flags and slows
+// every other access of the launch.
+__device__ __forceinline__ void poll_pause(int* backoff) {
+    for (int z = 0; z < *backoff; z++) __builtin_amdgcn_s_sleep(8);
+    *backoff = min(*backoff * 2, 16);
+}
+
 // The cloth's budget, keep flag, final p buffer and a zeroed sweep count,
 // written through to memory by the tile that decides and complete before it
 // publishes the decision (other tiles of the launch read the budget and add
@@ -617,7 +626,7 @@ __device__ __forceinline__ bool cloth_decide(const TileQueue& q, const TileDev& 
                 __hip_atomic_store(&q.decided[c * QC_DECIDED_STRIDE], k ? 2 : 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 flag = (k ? 1 : 0) | 2;
             } else {
-                int d = 0, spins = 0;
+                int d = 0, spins = 0, backoff = 1;
                 while ((d = __hip_atomic_load(&q.decided[c * QC_DECIDED_STRIDE], __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_SYSTEM)) == 0) {
                     if (++spins > DECIDE_SPIN || ((spins & 63) == 0 && queue_aborted(q))) {
@@ -625,7 +634,7 @@ __device__ __forceinline__ bool cloth_decide(const TileQueue& q, const TileDev& 
                         d = 1;
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(2);
+                    poll_pause(&backoff);
                 }
                 flag = d - 1;
             }
@@ -1135,7 +1144,7 @@ __device__ __forceinline__ void tiles_decide_walk(TILE_PARAMS, const TileQueue& 
         if (threadIdx.x < 64) {
             // wave 0 takes the item; lane 0 touches the counters
             const int lane = threadIdx.x;
-            int slot = -1, ch = 0, spins = 0;
+            int slot = -1, ch = 0, spins = 0, backoff = 1;
             for (;;) {
                 int h = 0, r = 0, st = 0;
                 if (lane == 0) {
@@ -1199,7 +1208,7 @@ __device__ __forceinline__ void tiles_decide_walk(TILE_PARAMS, const TileQueue& 
                     stop = 1;
                 }
                 if (__shfl(stop, 0)) break;
-                __builtin_amdgcn_s_sleep(2);
+                poll_pause(&backoff);
             }
             if (lane == 0) {
                 s_slot = slot;
@@ -1210,17 +1219,19 @@ __device__ __forceinline__ void tiles_decide_walk(TILE_PARAMS, const TileQueue& 
         const int slot = s_slot, c = s_chunk;
         if (slot < 0) break;
         if (c > 0) {
-            // the neighbour tiles' previous chunk (polled relaxed; the chunk then
-            // loads its cone past the caches)
-            const int e = q.nbr_off[slot + 1];
-            for (int k = q.nbr_off[slot] + (int)threadIdx.x; k < e; k += blockDim.x) {
-                int spins = 0;
-                while (__hip_atomic_load(&q.finished[q.nbr[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < c) {
-                    if (++spins > QUEUE_SPIN || ((spins & 63) == 0 && queue_aborted(q))) {
-                        queue_give_up(q);
-                        break;
+            // the neighbour tiles' previous chunk, polled by one thread in turn
+            // (relaxed; the chunk then loads its cone past the caches)
+            if (threadIdx.x == 0) {
+                const int e = q.nbr_off[slot + 1];
+                for (int k = q.nbr_off[slot]; k < e; k++) {
+                    int spins = 0, backoff = 1;
+                    while (__hip_atomic_load(&q.finished[q.nbr[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < c) {
+                        if (++spins > QUEUE_SPIN || ((spins & 63) == 0 && queue_aborted(q))) {
+                            queue_give_up(q);
+                            break;
+                        }
+                        poll_pause(&backoff);
                     }
-                    __builtin_amdgcn_s_sleep(2);
                 }
             }
             __syncthreads();
