@@ -741,9 +741,12 @@ bool queue_folds(pbd_ctx* c, int iterations) {
 
 // Whether that launch also computes the residuals and decides
 // (per_chunk_launches 2), when every multi-tile cloth's tiles fit the GPU's
-// resident workgroups at once (they wait for each other).
+// resident workgroups at once (they wait for each other): fewer tiles than
+// resident slots, the grid's rounding up to a multiple of 8 not counted.
 bool queue_decides(pbd_ctx* c, int iterations) {
-    return queue_folds(c, iterations) && c->sp.per_chunk_launches == 2 && c->q_tiles_max < walk_grid(c);
+    if (!queue_folds(c, iterations) || c->sp.per_chunk_launches != 2) return false;
+    (void)walk_grid(c);  // sets walk_slots for the current launch shape
+    return c->q_tiles_max < c->walk_slots;
 }
 
 // Predicts the correction and decides, per cloth, whether to keep it (its
